@@ -1,0 +1,1101 @@
+// hh_api.cpp — C ABI and pass orchestration of the gfx950 WIDIM backend.
+//
+// Implements include/hammerhead_mi355x.h. The orchestration of hh_run_piv
+// follows run_piv / piv_pass (src/pipeline.jl:361-449,655-832): per pass the
+// predictor is built from the previous result (build_predictor,
+// src/pipeline.jl:454-464), every sweep deforms the prefiltered pair by it,
+// correlates all windows, adds the predictor back, validates and replaces,
+// and iterating passes rebuild their own predictor between sweeps; the
+// Wieneke uncertainty runs once on the final warp. Everything stays on the
+// context's HIP stream; with convergence_tol = 0 there is no host
+// synchronization inside the schedule.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "hh_internal.h"
+
+using namespace hh;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *ensure(size_t m)
+    {
+        if (m > n) {
+            release();
+            if (m > 0) {
+                hipError_t e = hipMalloc(&p, sizeof(T) * m);
+                if (e != hipSuccess) {
+                    p = nullptr;
+                    throw Error(HH_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+                }
+            }
+            n = m;
+        }
+        return p;
+    }
+};
+
+// Per window-configuration device constants (apodization, overlap gain,
+// phase filter), the engine pool of piv_correlation_engines
+// (src/pipeline.jl:104-137).
+struct ConfigBufs {
+    DBuf<float> apod, gain, wfilt;
+    int box_lo = 0, box_hi = 0;
+};
+
+struct TimedLaunch {
+    std::string family;
+    int w;
+    hipEvent_t a, b;
+};
+
+struct PassGrid {
+    int ny = 0, nx = 0;
+    std::vector<float> x, y;
+    std::vector<int32_t> jobs;
+};
+
+}  // namespace
+
+struct hh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int form = HH_FORM_FFT32;
+    int nr = 0, nc = 0;
+    const float *A = nullptr, *B = nullptr;  // current pair (owned or borrowed)
+    DBuf<float> ownA, ownB;
+    DBuf<float> coefA, coefB, ptmp;
+    bool coefs_valid = false;
+    DBuf<float> warpA, warpB;
+    bool warp_valid = false;
+    // predictor grid (device) + knots
+    DBuf<float> pu, pv;
+    int gny = 0, gnx = 0;
+    float y0 = 0, ysp = 1, x0 = 0, xsp = 1;
+    std::map<std::tuple<int, int, int, int>, std::unique_ptr<ConfigBufs>> configs;
+    // job lists per (w, overlap, nr, nc)
+    std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<DBuf<int32_t>>> jobcache;
+    // chain buffers
+    DBuf<float> gu, gv, gpr, gcm, galtu, galtv, guu, guv, gu0, gv0, gmu, gmv, gsu, gsv, gpx, gpy, gprevu, gprevv;
+    DBuf<uint8_t> gout, gsflag;
+    DBuf<int> gcount;
+    DBuf<int32_t> hookjobs;
+    DBuf<float> hk[12];
+    DBuf<double> hkd;
+    DBuf<int32_t> hki[2];
+    // final result of hh_run_piv
+    PassGrid final_grid;
+    bool have_result = false;
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::pair<std::string, int>, std::pair<double, int64_t>> ktime;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+hipEvent_t get_event(hh_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HH_HIP(hipEventCreate(&e));
+    return e;
+}
+
+// Run a launch, bracketed by events when timing is on.
+template <typename F>
+void timed(hh_ctx *c, const char *family, int w, F &&f)
+{
+    if (!c->timing) {
+        f();
+        return;
+    }
+    TimedLaunch t{family, w, get_event(c), get_event(c)};
+    HH_HIP(hipEventRecord(t.a, c->stream));
+    f();
+    HH_HIP(hipEventRecord(t.b, c->stream));
+    c->pending.push_back(t);
+}
+
+void drain_timing(hh_ctx *c)
+{
+    if (c->pending.empty()) return;
+    HH_HIP(hipStreamSynchronize(c->stream));
+    for (auto &t : c->pending) {
+        float ms = 0.f;
+        HH_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+        auto &acc = c->ktime[{t.family, t.w}];
+        acc.first += ms;
+        acc.second += 1;
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+int fail(hh_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(hh_ctx *c, F &&f)
+{
+    if (!c) return HH_EINVAL;
+    c->err.clear();
+    try {
+        HH_HIP(hipSetDevice(c->device));
+        f();
+        return HH_OK;
+    } catch (const Error &e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        c->err = "host allocation failed";
+        return HH_ENOMEM;
+    }
+}
+
+// ---- parameter checks ---------------------------------------------------------
+
+void check_pass_valid(const hh_pass &p)
+{
+    auto bad = [](const std::string &m) { throw Error(HH_EINVAL, m); };
+    if (p.window_size[0] < 4 || p.window_size[1] < 4) bad("window_size must be at least 4 in each dimension");
+    for (int d = 0; d < 2; ++d) {
+        if (p.search_area_size[d] < p.window_size[d])
+            bad("search_area_size must be at least window_size in each dimension");
+        if ((p.search_area_size[d] - p.window_size[d]) % 2)
+            bad("search_area_size - window_size must be even in each dimension");
+        if (!(0 <= p.overlap[d] && p.overlap[d] < p.window_size[d]))
+            bad("overlap must satisfy 0 <= overlap < window_size");
+    }
+    if (p.correlation_method != HH_CROSS && p.correlation_method != HH_PHASE)
+        bad("correlation_method must be :cross or :phase");
+    if (p.apodization != HH_APOD_NONE && p.apodization != HH_APOD_GAUSS) bad("apodization must be :none or :gauss");
+    if (p.subpixel_method < HH_GAUSS3 || p.subpixel_method > HH_SUBPIX_NONE) bad("unknown subpixel_method");
+    if (p.n_peaks < 1) bad("n_peaks must be at least 1");
+    if (p.peak_finder != HH_REGIONALMAX && p.peak_finder != HH_EXCLUSION)
+        bad("peak_finder must be :exclusion or :regionalmax");
+    if (!(p.uod_threshold > 0)) bad("uod_threshold must be positive");
+    if (p.uod_neighborhood < 1) bad("uod_neighborhood must be at least 1");
+    if (!(p.min_peak_ratio >= 0)) bad("min_peak_ratio must be non-negative");
+    if (p.max_iterations < 1) bad("max_iterations must be at least 1");
+    if (!(p.convergence_tol >= 0)) bad("convergence_tol must be non-negative");
+    if (p.n_validators < 0 || p.n_validators > HH_MAX_VALIDATORS) bad("at most 4 extra validators");
+}
+
+// The device engine's scope (the counterpart of _ka_scope_check,
+// src/ka_backend.jl:13-29): rejected up front with HH_EUNSUPPORTED.
+void check_pass_device(const hh_pass &p, bool for_schedule)
+{
+    check_pass_valid(p);
+    auto uns = [](const std::string &m) { throw Error(HH_EUNSUPPORTED, "backend :mi355x " + m); };
+    if (p.search_area_size[0] != p.window_size[0] || p.search_area_size[1] != p.window_size[1])
+        uns("does not support enlarged search_area_size yet; use backend = :cpu");
+    if (p.window_size[0] != p.window_size[1]) uns("supports square windows only");
+    const int w = p.window_size[0];
+    const int M = p.padding ? 2 * w : w;
+    if (!(M == 16 || M == 32 || M == 64 || M == 128))
+        uns("supports FFT sides 16, 32, 64 and 128 (window 8-64 padded, 16-128 unpadded)");
+    if (p.subpixel_method == HH_GAUSS2D) uns("supports subpixel_method :gauss3, :gauss9 or :none only");
+    if (p.n_peaks > kMaxPeaks) uns("supports n_peaks up to 8");
+    if (p.uod_neighborhood > 3) uns("supports uod_neighborhood 1, 2 or 3");
+    for (int q = 0; q < p.n_validators; ++q) {
+        const hh_validator &v = p.validators[q];
+        if (v.kind < HH_V_PEAK_RATIO || v.kind > HH_V_UOD) throw Error(HH_EINVAL, "unknown validator kind");
+        if (v.kind == HH_V_UOD && (v.neighborhood < 1 || v.neighborhood > 3))
+            uns("supports UOD validators with neighborhood 1, 2 or 3");
+        if (v.kind == HH_V_UOD && !(v.b > 0)) throw Error(HH_EINVAL, "epsilon must be positive");
+    }
+    if (p.uncertainty && w > 64) uns("supports uncertainty for windows up to 64 px");
+    if (for_schedule && p.keep_correlation_planes)
+        uns("does not support keep_correlation_planes in run_piv; use hh_correlation_planes");
+}
+
+PassGrid make_grid(const hh_pass &p, int nr, int nc)
+{
+    const int wr = p.window_size[0], wc = p.window_size[1];
+    const int sr = p.search_area_size[0], sc = p.search_area_size[1];
+    const int mr = (sr - wr) / 2, mc = (sc - wc) / 2;
+    if (sr > nr || sc > nc) throw Error(HH_EINVAL, "search area exceeds image size");
+    const int r0 = 1 + mr, rstep = wr - p.overlap[0], rstop = nr - wr - mr + 1;
+    const int c0 = 1 + mc, cstep = wc - p.overlap[1], cstop = nc - wc - mc + 1;
+    PassGrid g;
+    g.ny = rstop >= r0 ? (rstop - r0) / rstep + 1 : 0;
+    g.nx = cstop >= c0 ? (cstop - c0) / cstep + 1 : 0;
+    g.x.resize(g.nx);
+    g.y.resize(g.ny);
+    for (int j = 0; j < g.nx; ++j) g.x[j] = (float)(c0 + j * cstep) + (float)((wc - 1) / 2.0);
+    for (int i = 0; i < g.ny; ++i) g.y[i] = (float)(r0 + i * rstep) + (float)((wr - 1) / 2.0);
+    g.jobs.resize((size_t)4 * g.ny * g.nx);
+    size_t k = 0;
+    for (int j = 0; j < g.nx; ++j)
+        for (int i = 0; i < g.ny; ++i) {
+            g.jobs[4 * k] = i + 1;
+            g.jobs[4 * k + 1] = j + 1;
+            g.jobs[4 * k + 2] = r0 + i * rstep;
+            g.jobs[4 * k + 3] = c0 + j * cstep;
+            ++k;
+        }
+    return g;
+}
+
+// apodization_window (src/correlators.jl:18-25), Float64 then float32.
+std::vector<float> host_apod(int w, int apod)
+{
+    std::vector<float> a((size_t)w * w);
+    const double c = (w + 1) / 2.0, s = w / 4.0;
+    for (int j = 1; j <= w; ++j)
+        for (int i = 1; i <= w; ++i)
+            a[(i - 1) + (size_t)(j - 1) * w] =
+                apod == HH_APOD_GAUSS
+                    ? (float)std::exp(-((i - c) * (i - c) / (2 * s * s) + (j - c) * (j - c) / (2 * s * s)))
+                    : 1.0f;
+    return a;
+}
+
+ConfigBufs &get_config(hh_ctx *c, const hh_pass &p)
+{
+    const int w = p.window_size[0];
+    auto key = std::make_tuple(w, (int)p.padding, (int)p.apodization, (int)p.correlation_method);
+    auto it = c->configs.find(key);
+    if (it != c->configs.end()) return *it->second;
+    auto cb = std::make_unique<ConfigBufs>();
+    const int M = p.padding ? 2 * w : w;
+    std::vector<float> apod = host_apod(w, p.apodization);
+    cb->apod.ensure(apod.size());
+    HH_HIP(hipMemcpy(cb->apod.p, apod.data(), sizeof(float) * apod.size(), hipMemcpyHostToDevice));
+    cb->box_lo = 0;
+    cb->box_hi = M - 1;
+    if (p.padding) {
+        // overlap_gain! (src/correlators.jl:170-181): the autocorrelation of the
+        // apodization (direct sums in Float64 of the float32 window), in the
+        // fftshifted layout; wmax / w where w >= wmax / 2, else 0.
+        std::vector<double> ac((size_t)M * M, 0.0);
+        const int H = M / 2;
+        for (int l2 = -(w - 1); l2 <= w - 1; ++l2)
+            for (int l1 = -(w - 1); l1 <= w - 1; ++l1) {
+                double s = 0.0;
+                for (int j = std::max(0, -l2); j < std::min(w, w - l2); ++j)
+                    for (int i = std::max(0, -l1); i < std::min(w, w - l1); ++i)
+                        s += (double)apod[i + (size_t)j * w] * (double)apod[(i + l1) + (size_t)(j + l2) * w];
+                ac[(l1 + H) + (size_t)(l2 + H) * M] = s;
+            }
+        double wmax = 0.0;
+        for (double v : ac) wmax = std::max(wmax, v);
+        std::vector<float> gain((size_t)M * M);
+        int lo = M, hi = -1;
+        for (int jp = 0; jp < M; ++jp)
+            for (int ip = 0; ip < M; ++ip) {
+                const double v = ac[ip + (size_t)jp * M];
+                const bool on = v >= wmax / 2;
+                gain[ip + (size_t)jp * M] = on ? (float)(wmax / v) : 0.0f;
+                if (on) {
+                    lo = std::min(lo, std::min(ip, jp));
+                    hi = std::max(hi, std::max(ip, jp));
+                }
+            }
+        cb->box_lo = std::max(0, lo - 1);
+        cb->box_hi = std::min(M - 1, hi + 1);
+        cb->gain.ensure(gain.size());
+        HH_HIP(hipMemcpy(cb->gain.p, gain.data(), sizeof(float) * gain.size(), hipMemcpyHostToDevice));
+    }
+    if (p.correlation_method == HH_PHASE) {
+        const double sigma = M / 8.0;
+        std::vector<float> wf((size_t)M * M);
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < M; ++i) {
+                const double fi = i < (M + 1) / 2 ? i : i - M, fj = j < (M + 1) / 2 ? j : j - M;
+                wf[i + (size_t)j * M] = (float)std::exp(-(fi * fi + fj * fj) / (2 * sigma * sigma));
+            }
+        cb->wfilt.ensure(wf.size());
+        HH_HIP(hipMemcpy(cb->wfilt.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice));
+    }
+    ConfigBufs &ref = *cb;
+    c->configs.emplace(key, std::move(cb));
+    return ref;
+}
+
+CorrConfig make_corr(hh_ctx *c, const hh_pass &p)
+{
+    ConfigBufs &cb = get_config(c, p);
+    CorrConfig cfg{};
+    cfg.w = p.window_size[0];
+    cfg.M = p.padding ? 2 * cfg.w : cfg.w;
+    cfg.padded = p.padding;
+    cfg.method = p.correlation_method;
+    cfg.subpixel = p.subpixel_method;
+    cfg.finder = p.peak_finder;
+    cfg.n_peaks = p.n_peaks;
+    cfg.K = std::max(p.n_peaks, 2);
+    cfg.box_lo = cb.box_lo;
+    cfg.box_hi = cb.box_hi;
+    cfg.apod = cb.apod.p;
+    cfg.gain = p.padding ? cb.gain.p : nullptr;
+    cfg.wfilt = p.correlation_method == HH_PHASE ? cb.wfilt.p : nullptr;
+    return cfg;
+}
+
+const int32_t *device_jobs(hh_ctx *c, const hh_pass &p, const PassGrid &g)
+{
+    auto key = std::make_tuple(p.window_size[0], p.window_size[1], p.overlap[0], p.overlap[1], c->nr, c->nc);
+    auto it = c->jobcache.find(key);
+    if (it != c->jobcache.end()) return it->second->p;
+    auto b = std::make_unique<DBuf<int32_t>>();
+    b->ensure(std::max<size_t>(g.jobs.size(), 4));
+    if (!g.jobs.empty())
+        HH_HIP(hipMemcpy(b->p, g.jobs.data(), sizeof(int32_t) * g.jobs.size(), hipMemcpyHostToDevice));
+    const int32_t *ptr = b->p;
+    c->jobcache.emplace(key, std::move(b));
+    return ptr;
+}
+
+void need_pair(hh_ctx *c)
+{
+    if (!c->A) throw Error(HH_ESTATE, "no image pair: call hh_set_pair first");
+}
+
+const float *src_image(hh_ctx *c, int src, bool a)
+{
+    need_pair(c);
+    if (src == HH_SRC_WARP) {
+        if (!c->warp_valid) throw Error(HH_ESTATE, "no warped pair: call hh_deform first");
+        return a ? c->warpA.p : c->warpB.p;
+    }
+    if (src != HH_SRC_RAW) throw Error(HH_EINVAL, "src must be HH_SRC_RAW or HH_SRC_WARP");
+    return a ? c->A : c->B;
+}
+
+void do_prefilter(hh_ctx *c)
+{
+    need_pair(c);
+    const size_t n = (size_t)(c->nr + 2) * (c->nc + 2);
+    c->coefA.ensure(n);
+    c->coefB.ensure(n);
+    c->ptmp.ensure((size_t)(c->nr + 2) * c->nc);
+    timed(c, "prefilter", 0, [&] {
+        launch_prefilter(c->A, c->nr, c->nc, c->coefA.p, c->ptmp.p, nullptr, 0, c->stream);
+        launch_prefilter(c->B, c->nr, c->nc, c->coefB.p, c->ptmp.p, nullptr, 0, c->stream);
+    });
+    c->coefs_valid = true;
+}
+
+void do_deform(hh_ctx *c)
+{
+    const size_t n = (size_t)c->nr * c->nc;
+    c->warpA.ensure(n);
+    c->warpB.ensure(n);
+    timed(c, "deform", 0, [&] {
+        launch_deform(c->coefA.p, c->coefB.p, c->nr, c->nc, c->pu.p, c->pv.p, c->gny, c->gnx, c->y0, c->ysp, c->x0,
+                      c->xsp, c->warpA.p, c->warpB.p, c->stream);
+    });
+    c->warp_valid = true;
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = v;
+}
+
+__global__ void k_restore(float *u, float *v, const float *mu, const float *mv, const uint8_t *out, int n)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n && out[e]) {
+        u[e] = mu[e];
+        v[e] = mv[e];
+    }
+}
+
+// max(|du|, |dv|) per unmasked cell for field_change (src/pipeline.jl:849-866);
+// NaN/finite flips give +Inf, NaN in both sweeps gives NaN (dropped on host).
+__global__ void k_change(const float *u, const float *v, const float *pu, const float *pv, int n, float *out)
+{
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float du = fabsf(u[e] - pu[e]), dv = fabsf(v[e] - pv[e]);
+    if (isnan(du) || isnan(dv)) {
+        const bool same = (isnan(u[e]) == isnan(pu[e])) && (isnan(v[e]) == isnan(pv[e]));
+        out[e] = same ? NAN : INFINITY;
+        return;
+    }
+    out[e] = du > dv ? du : dv;
+}
+
+void fill(hh_ctx *c, float *p, size_t n, float v)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p, n, v);
+    HH_HIP(hipGetLastError());
+}
+
+// quantile!(buf, 0.95), type 7 (Statistics.jl semantics).
+float quantile95(std::vector<float> &v)
+{
+    std::sort(v.begin(), v.end());
+    const size_t n = v.size();
+    const double p = 0.95;
+    const double aleph = n * p + (1.0 - p);
+    long j = (long)aleph;
+    if (j > (long)n - 1) j = (long)n - 1;
+    if (j < 1) j = 1;
+    float gamma = (float)(aleph - (double)j);
+    gamma = std::min(1.f, std::max(0.f, gamma));
+    float a, b;
+    if (n == 1) {
+        a = b = v[0];
+    } else {
+        a = v[j - 1];
+        b = v[j];
+    }
+    if (std::isfinite(a) && std::isfinite(b)) return a + gamma * (b - a);
+    return (1 - gamma) * a + gamma * b;
+}
+
+void set_predictor_from_grid(hh_ctx *c, const PassGrid &g, const float *u, const float *v, bool smoothing)
+{
+    const size_t n = (size_t)g.ny * g.nx;
+    c->pu.ensure(n);
+    c->pv.ensure(n);
+    if (smoothing) {
+        launch_smooth(u, g.ny, g.nx, c->pu.p, c->stream);
+        launch_smooth(v, g.ny, g.nx, c->pv.p, c->stream);
+    } else {
+        HH_HIP(hipMemcpyAsync(c->pu.p, u, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+        HH_HIP(hipMemcpyAsync(c->pv.p, v, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->gny = g.ny;
+    c->gnx = g.nx;
+    c->y0 = g.y[0];
+    c->x0 = g.x[0];
+    c->ysp = g.ny > 1 ? g.y[1] - g.y[0] : 1.f;
+    c->xsp = g.nx > 1 ? g.x[1] - g.x[0] : 1.f;
+}
+
+void run_schedule(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
+{
+    need_pair(c);
+    if (np < 1) throw Error(HH_EINVAL, "at least one pass is required");
+    for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
+    bool deforms = np > 1;
+    for (int k = 0; k < np; ++k) deforms |= passes[k].max_iterations > 1;
+    // size every chain buffer for the largest pass up front: no buffer is
+    // reallocated while queued work may still read it
+    size_t maxng = 0, maxalt = 0;
+    int maxny = 0, maxnx = 0;
+    for (int k = 0; k < np; ++k) {
+        int ny = 0, nx = 0;
+        hh_grid_size(&passes[k], c->nr, c->nc, &ny, &nx);
+        maxng = std::max(maxng, (size_t)ny * nx);
+        maxalt = std::max(maxalt, (size_t)ny * nx * std::max(passes[k].n_peaks - 1, 0));
+        maxny = std::max(maxny, ny);
+        maxnx = std::max(maxnx, nx);
+    }
+    for (DBuf<float> *b : {&c->gu, &c->gv, &c->gpr, &c->gcm, &c->guu, &c->guv, &c->gu0, &c->gv0, &c->gsu, &c->gsv,
+                           &c->gmu, &c->gmv, &c->gprevu, &c->gprevv, &c->pu, &c->pv})
+        b->ensure(std::max<size_t>(maxng, 1));
+    c->galtu.ensure(std::max<size_t>(maxalt, 1));
+    c->galtv.ensure(std::max<size_t>(maxalt, 1));
+    c->gout.ensure(std::max<size_t>(maxng, 1));
+    c->gsflag.ensure(std::max<size_t>(maxng, 1));
+    c->gpx.ensure(std::max(maxnx, 1));
+    c->gpy.ensure(std::max(maxny, 1));
+    if (deforms) do_prefilter(c);
+    bool have_pred = false;
+    PassGrid prev;
+    c->have_result = false;
+    for (int k = 0; k < np; ++k) {
+        const hh_pass &p = passes[k];
+        const bool force_replace = k < np - 1;
+        PassGrid g = make_grid(p, c->nr, c->nc);
+        const int ny = g.ny, nx = g.nx;
+        const size_t ng = (size_t)ny * nx;
+        if (ng == 0) throw Error(HH_EINVAL, "the pass grid is empty");
+        if (k > 0) {
+            // build_predictor from the previous pass's result (unmasked)
+            set_predictor_from_grid(c, prev, c->gu.p, c->gv.p, smoothing);
+            have_pred = true;
+        }
+        const int32_t *djobs = device_jobs(c, p, g);
+        CorrConfig cfg = make_corr(c, p);
+        const int nalt = p.n_peaks - 1;
+        float *u = c->gu.ensure(ng), *v = c->gv.ensure(ng);
+        float *pr = c->gpr.ensure(ng), *cm = c->gcm.ensure(ng);
+        float *au = nalt > 0 ? c->galtu.ensure(ng * nalt) : nullptr;
+        float *av = nalt > 0 ? c->galtv.ensure(ng * nalt) : nullptr;
+        float *uu = c->guu.ensure(ng), *uv = c->guv.ensure(ng);
+        float *u0 = c->gu0.ensure(ng), *v0 = c->gv0.ensure(ng);
+        uint8_t *outl = c->gout.ensure(ng);
+        float *su = c->gsu.ensure(ng), *sv = c->gsv.ensure(ng);
+        uint8_t *sflag = c->gsflag.ensure(ng);
+        const int maxiter = p.max_iterations;
+        const bool unc = p.uncertainty && !force_replace;
+        const bool keep_measured = maxiter > 1 && !(force_replace || p.replace_outliers);
+        float *mu = keep_measured ? c->gmu.ensure(ng) : nullptr;
+        float *mv = keep_measured ? c->gmv.ensure(ng) : nullptr;
+        bool have_prev = false;
+        fill(c, uu, ng, NAN);
+        fill(c, uv, ng, NAN);
+        const float *srcA = c->A, *srcB = c->B;
+        for (int it = 1; it <= maxiter; ++it) {
+            if (have_pred) {
+                do_deform(c);
+                srcA = c->warpA.p;
+                srcB = c->warpB.p;
+                float *gx = c->gpx.ensure(nx), *gy = c->gpy.ensure(ny);
+                HH_HIP(hipMemcpyAsync(gx, g.x.data(), sizeof(float) * nx, hipMemcpyHostToDevice, c->stream));
+                HH_HIP(hipMemcpyAsync(gy, g.y.data(), sizeof(float) * ny, hipMemcpyHostToDevice, c->stream));
+                launch_node_values(c->pu.p, c->pv.p, c->gny, c->gnx, c->y0, c->ysp, c->x0, c->xsp, gy, ny, gx, nx,
+                                   u0, v0, c->stream);
+            } else {
+                fill(c, u0, ng, 0.f);
+                fill(c, v0, ng, 0.f);
+                srcA = c->A;
+                srcB = c->B;
+            }
+            CorrOut out{};
+            out.u = u;
+            out.v = v;
+            out.pr = pr;
+            out.cm = cm;
+            out.alt_u = au;
+            out.alt_v = av;
+            out.u0 = u0;
+            out.v0 = v0;
+            out.ny = ny;
+            out.nx = nx;
+            timed(c, "correlate", cfg.w,
+                  [&] { launch_correlate(cfg, srcA, srcB, c->nr, c->nc, djobs, (int)ng, out, c->stream); });
+            if (keep_measured) {
+                HH_HIP(hipMemcpyAsync(mu, u, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
+                HH_HIP(hipMemcpyAsync(mv, v, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
+            }
+            ValidateArgs va{};
+            va.ny = ny;
+            va.nx = nx;
+            va.pass = &p;
+            va.force_replace = force_replace || maxiter > 1;
+            va.u = u;
+            va.v = v;
+            va.pr = pr;
+            va.cm = cm;
+            va.alt_u = au;
+            va.alt_v = av;
+            va.outliers = outl;
+            va.su = su;
+            va.sv = sv;
+            va.sflag = sflag;
+            va.count = c->gcount.ensure(1);
+            timed(c, "validate", cfg.w, [&] { launch_validate(va, c->stream); });
+            if (it == maxiter) break;
+            if (p.convergence_tol > 0) {
+                float *pu_ = c->gprevu.ensure(ng), *pv_ = c->gprevv.ensure(ng);
+                if (!have_prev) {
+                    have_prev = true;
+                } else {
+                    float *ch = c->gsu.p;  // scratch reuse (validation is done)
+                    hipLaunchKernelGGL(k_change, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream, u, v,
+                                       pu_, pv_, (int)ng, ch);
+                    std::vector<float> h(ng);
+                    HH_HIP(hipMemcpyAsync(h.data(), ch, sizeof(float) * ng, hipMemcpyDeviceToHost, c->stream));
+                    HH_HIP(hipStreamSynchronize(c->stream));
+                    std::vector<float> buf;
+                    buf.reserve(ng);
+                    for (float x : h)
+                        if (!std::isnan(x)) buf.push_back(x);
+                    const float change = buf.empty() ? 0.f : quantile95(buf);
+                    if (change < (float)p.convergence_tol) break;
+                }
+                HH_HIP(hipMemcpyAsync(pu_, u, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
+                HH_HIP(hipMemcpyAsync(pv_, v, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
+            }
+            set_predictor_from_grid(c, g, u, v, smoothing);
+            have_pred = true;
+        }
+        if (unc) {
+            const ConfigBufs &cb = get_config(c, p);
+            timed(c, "uq", cfg.w, [&] {
+                launch_uq(cfg.w, cb.apod.p, srcA, srcB, c->nr, djobs, (int)ng, ny, uu, uv, nullptr, c->stream);
+            });
+        }
+        if (keep_measured)
+            hipLaunchKernelGGL(k_restore, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream, u, v, mu, mv,
+                               outl, (int)ng);
+        prev = std::move(g);
+    }
+    HH_HIP(hipGetLastError());
+    c->final_grid = prev;
+    c->have_result = true;
+}
+
+void fetch(hh_ctx *c, hh_field *r)
+{
+    if (!c->have_result) throw Error(HH_ESTATE, "no result: call hh_run_piv first");
+    const PassGrid &g = c->final_grid;
+    if (r->ny != g.ny || r->nx != g.nx)
+        throw Error(HH_EDIM, "result grid is " + std::to_string(g.ny) + "x" + std::to_string(g.nx) + ", got " +
+                                 std::to_string(r->ny) + "x" + std::to_string(r->nx));
+    const size_t ng = (size_t)g.ny * g.nx;
+    auto d2h = [&](void *dst, const void *src, size_t bytes) {
+        if (dst) HH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    };
+    d2h(r->u, c->gu.p, sizeof(float) * ng);
+    d2h(r->v, c->gv.p, sizeof(float) * ng);
+    d2h(r->peak_ratio, c->gpr.p, sizeof(float) * ng);
+    d2h(r->correlation_moment, c->gcm.p, sizeof(float) * ng);
+    d2h(r->uncertainty_u, c->guu.p, sizeof(float) * ng);
+    d2h(r->uncertainty_v, c->guv.p, sizeof(float) * ng);
+    d2h(r->outliers, c->gout.p, ng);
+    HH_HIP(hipStreamSynchronize(c->stream));
+    if (r->x) std::memcpy(r->x, g.x.data(), sizeof(float) * g.nx);
+    if (r->y) std::memcpy(r->y, g.y.data(), sizeof(float) * g.ny);
+    if (r->mask) std::memset(r->mask, 0, ng);
+}
+
+template <typename T>
+T *upload(hh_ctx *c, DBuf<T> &b, const T *h, size_t n)
+{
+    T *p = b.ensure(std::max<size_t>(n, 1));
+    if (h && n) HH_HIP(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+    return p;
+}
+
+template <typename T>
+void download(hh_ctx *c, T *h, const T *d, size_t n)
+{
+    if (h && n) HH_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char *hh_version(void) { return "hammerhead-mi355x 0.1 (gfx950)"; }
+
+int hh_create(hh_ctx **out, int device)
+{
+    g_create_err.clear();
+    if (!out) return HH_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_create_err = "no HIP device available";
+        return HH_EDEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_err = "device index out of range";
+        return HH_EINVAL;
+    }
+    auto *c = new (std::nothrow) hh_ctx();
+    if (!c) return HH_ENOMEM;
+    c->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_err = std::string("stream creation failed: ") + hipGetErrorString(e);
+        delete c;
+        return HH_EDEVICE;
+    }
+    *out = c;
+    return HH_OK;
+}
+
+int hh_destroy(hh_ctx *c)
+{
+    if (!c) return HH_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &t : c->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HH_OK;
+}
+
+const char *hh_last_error(const hh_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int hh_set_form(hh_ctx *c, int form)
+{
+    return guarded(c, [&] {
+        if (form != HH_FORM_FFT32 && form != HH_FORM_MFMA16) throw Error(HH_EINVAL, "unknown correlation form");
+        if (form == HH_FORM_MFMA16) throw Error(HH_EUNSUPPORTED, "the split-FP16 MFMA form is not built yet");
+        c->form = form;
+    });
+}
+
+int hh_check_passes(hh_ctx *c, const hh_pass *passes, int npasses)
+{
+    return guarded(c, [&] {
+        if (npasses < 1) throw Error(HH_EINVAL, "at least one pass is required");
+        for (int k = 0; k < npasses; ++k) check_pass_device(passes[k], true);
+    });
+}
+
+int hh_grid_size(const hh_pass *p, int nr, int nc, int *ny, int *nx)
+{
+    if (!p || !ny || !nx) return HH_EINVAL;
+    try {
+        check_pass_valid(*p);
+        PassGrid g = make_grid(*p, nr, nc);
+        *ny = g.ny;
+        *nx = g.nx;
+        return HH_OK;
+    } catch (const Error &e) {
+        return e.code;
+    }
+}
+
+int hh_set_pair(hh_ctx *c, const float *A, const float *B, int nr, int nc)
+{
+    return guarded(c, [&] {
+        if (!A || !B) throw Error(HH_EINVAL, "null image");
+        if (nr < 2 || nc < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
+        const size_t n = (size_t)nr * nc;
+        c->ownA.ensure(n);
+        c->ownB.ensure(n);
+        HH_HIP(hipMemcpyAsync(c->ownA.p, A, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipMemcpyAsync(c->ownB.p, B, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipStreamSynchronize(c->stream));
+        c->A = c->ownA.p;
+        c->B = c->ownB.p;
+        if (nr != c->nr || nc != c->nc) c->jobcache.clear();
+        c->nr = nr;
+        c->nc = nc;
+        c->coefs_valid = c->warp_valid = c->have_result = false;
+    });
+}
+
+int hh_set_pair_device(hh_ctx *c, const float *dA, const float *dB, int nr, int nc)
+{
+    return guarded(c, [&] {
+        if (!dA || !dB) throw Error(HH_EINVAL, "null image");
+        if (nr < 2 || nc < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
+        c->A = dA;
+        c->B = dB;
+        if (nr != c->nr || nc != c->nc) c->jobcache.clear();
+        c->nr = nr;
+        c->nc = nc;
+        c->coefs_valid = c->warp_valid = c->have_result = false;
+    });
+}
+
+int hh_prefilter(hh_ctx *c)
+{
+    return guarded(c, [&] { do_prefilter(c); });
+}
+
+int hh_deform(hh_ctx *c, const float *py, int gny, const float *px, int gnx, const float *pu, const float *pv)
+{
+    return guarded(c, [&] {
+        need_pair(c);
+        if (!c->coefs_valid) throw Error(HH_ESTATE, "no interpolant: call hh_prefilter first");
+        if (gny < 1 || gnx < 1 || !py || !px || !pu || !pv) throw Error(HH_EINVAL, "empty predictor grid");
+        const size_t n = (size_t)gny * gnx;
+        upload(c, c->pu, pu, n);
+        upload(c, c->pv, pv, n);
+        c->gny = gny;
+        c->gnx = gnx;
+        c->y0 = py[0];
+        c->x0 = px[0];
+        c->ysp = gny > 1 ? py[1] - py[0] : 1.f;
+        c->xsp = gnx > 1 ? px[1] - px[0] : 1.f;
+        do_deform(c);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_get_images(hh_ctx *c, int src, float *A, float *B)
+{
+    return guarded(c, [&] {
+        const size_t n = (size_t)c->nr * c->nc;
+        download(c, A, src_image(c, src, true), n);
+        download(c, B, src_image(c, src, false), n);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_get_coefs(hh_ctx *c, float *cA, float *cB)
+{
+    return guarded(c, [&] {
+        if (!c->coefs_valid) throw Error(HH_ESTATE, "no interpolant: call hh_prefilter first");
+        const size_t n = (size_t)(c->nr + 2) * (c->nc + 2);
+        download(c, cA, c->coefA.p, n);
+        download(c, cB, c->coefB.p, n);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *jobs, int njobs, int ny, int nx,
+                       float *u, float *v, float *peak_ratio, float *correlation_moment, float *alt_u, float *alt_v,
+                       float *unc_u, float *unc_v, int32_t *peak_locs, float *peak_vals, int32_t *found)
+{
+    return guarded(c, [&] {
+        if (!pass) throw Error(HH_EINVAL, "null pass");
+        check_pass_device(*pass, false);
+        if (njobs < 0 || (njobs > 0 && !jobs)) throw Error(HH_EINVAL, "invalid job list");
+        if (!u || !v || !peak_ratio || !correlation_moment) throw Error(HH_EINVAL, "null output grid");
+        const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
+        const int w = pass->window_size[0];
+        for (int k = 0; k < njobs; ++k) {
+            const int gi = jobs[4 * k], gj = jobs[4 * k + 1], rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
+            if (gi < 1 || gi > ny || gj < 1 || gj > nx) throw Error(HH_EDIM, "job grid index out of range");
+            if (rs < 1 || cs < 1 || rs + w - 1 > c->nr || cs + w - 1 > c->nc)
+                throw Error(HH_EDIM, "window outside the image");
+        }
+        if (njobs == 0) return;
+        const size_t ng = (size_t)ny * nx;
+        const int nalt = pass->n_peaks - 1;
+        const int K = std::max(pass->n_peaks, 2);
+        CorrConfig cfg = make_corr(c, *pass);
+        const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
+        CorrOut out{};
+        out.u = upload(c, c->hk[0], u, ng);
+        out.v = upload(c, c->hk[1], v, ng);
+        out.pr = upload(c, c->hk[2], peak_ratio, ng);
+        out.cm = upload(c, c->hk[3], correlation_moment, ng);
+        if (alt_u && alt_v && nalt > 0) {
+            out.alt_u = upload(c, c->hk[4], alt_u, ng * nalt);
+            out.alt_v = upload(c, c->hk[5], alt_v, ng * nalt);
+        }
+        if (peak_locs) out.locs = upload<int32_t>(c, c->hki[0], nullptr, (size_t)2 * K * njobs);
+        if (peak_vals) out.vals = upload<float>(c, c->hk[6], nullptr, (size_t)K * njobs);
+        if (found) out.found = upload<int32_t>(c, c->hki[1], nullptr, (size_t)njobs);
+        out.ny = ny;
+        out.nx = nx;
+        timed(c, "correlate", w, [&] { launch_correlate(cfg, dA, dB, c->nr, c->nc, dj, njobs, out, c->stream); });
+        float *duu = nullptr, *duv = nullptr;
+        if (unc_u || unc_v) {
+            duu = upload(c, c->hk[7], unc_u, ng);
+            duv = upload(c, c->hk[8], unc_v, ng);
+            const ConfigBufs &cb = get_config(c, *pass);
+            timed(c, "uq", w, [&] {
+                launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr,
+                          nullptr, c->stream);
+            });
+        }
+        download(c, u, out.u, ng);
+        download(c, v, out.v, ng);
+        download(c, peak_ratio, out.pr, ng);
+        download(c, correlation_moment, out.cm, ng);
+        if (out.alt_u) {
+            download(c, alt_u, out.alt_u, ng * nalt);
+            download(c, alt_v, out.alt_v, ng * nalt);
+        }
+        if (unc_u) download(c, unc_u, duu, ng);
+        if (unc_v) download(c, unc_v, duv, ng);
+        if (peak_locs) download(c, peak_locs, out.locs, (size_t)2 * K * njobs);
+        if (peak_vals) download(c, peak_vals, out.vals, (size_t)K * njobs);
+        if (found) download(c, found, out.found, (size_t)njobs);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_correlation_planes(hh_ctx *c, const hh_pass *pass, int src, const int32_t *jobs, int njobs, float *planes)
+{
+    return guarded(c, [&] {
+        if (!pass || !planes) throw Error(HH_EINVAL, "null argument");
+        check_pass_device(*pass, false);
+        const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
+        const int w = pass->window_size[0];
+        const int M = pass->padding ? 2 * w : w;
+        int maxgi = 1, maxgj = 1;
+        for (int k = 0; k < njobs; ++k) {
+            const int rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
+            if (rs < 1 || cs < 1 || rs + w - 1 > c->nr || cs + w - 1 > c->nc)
+                throw Error(HH_EDIM, "window outside the image");
+            maxgi = std::max(maxgi, jobs[4 * k]);
+            maxgj = std::max(maxgj, jobs[4 * k + 1]);
+        }
+        if (njobs <= 0) return;
+        CorrConfig cfg = make_corr(c, *pass);
+        const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
+        const size_t ng = (size_t)maxgi * maxgj;
+        CorrOut out{};
+        out.u = upload<float>(c, c->hk[0], nullptr, ng);
+        out.v = upload<float>(c, c->hk[1], nullptr, ng);
+        out.pr = upload<float>(c, c->hk[2], nullptr, ng);
+        out.cm = upload<float>(c, c->hk[3], nullptr, ng);
+        out.planes = upload<float>(c, c->hk[9], nullptr, (size_t)M * M * njobs);
+        out.ny = maxgi;
+        out.nx = maxgj;
+        launch_correlate(cfg, dA, dB, c->nr, c->nc, dj, njobs, out, c->stream);
+        download(c, planes, out.planes, (size_t)M * M * njobs);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_uncertainty_sweep(hh_ctx *c, const hh_pass *pass, int src, const int32_t *jobs, int njobs, int ny, int nx,
+                         float *unc_u, float *unc_v, double *stats)
+{
+    return guarded(c, [&] {
+        if (!pass) throw Error(HH_EINVAL, "null pass");
+        check_pass_device(*pass, false);
+        if (pass->window_size[0] > 64) throw Error(HH_EUNSUPPORTED, "uncertainty supports windows up to 64 px");
+        const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
+        const int w = pass->window_size[0];
+        for (int k = 0; k < njobs; ++k) {
+            const int gi = jobs[4 * k], gj = jobs[4 * k + 1], rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
+            if (gi < 1 || gi > ny || gj < 1 || gj > nx) throw Error(HH_EDIM, "job grid index out of range");
+            if (rs < 1 || cs < 1 || rs + w - 1 > c->nr || cs + w - 1 > c->nc)
+                throw Error(HH_EDIM, "window outside the image");
+        }
+        if (njobs <= 0) return;
+        const size_t ng = (size_t)ny * nx;
+        const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
+        float *duu = upload(c, c->hk[7], unc_u, ng);
+        float *duv = upload(c, c->hk[8], unc_v, ng);
+        double *dst = stats ? c->hkd.ensure((size_t)2 * HH_UQ_NSTATS * njobs) : nullptr;
+        const ConfigBufs &cb = get_config(c, *pass);
+        timed(c, "uq", w, [&] {
+            launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr, dst,
+                      c->stream);
+        });
+        if (unc_u) download(c, unc_u, duu, ng);
+        if (unc_v) download(c, unc_v, duv, ng);
+        if (stats) download(c, stats, dst, (size_t)2 * HH_UQ_NSTATS * njobs);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_validate_and_replace(hh_ctx *c, const hh_pass *pass, int ny, int nx, int force_replace, float *u, float *v,
+                            const float *peak_ratio, const float *correlation_moment, const float *alt_u,
+                            const float *alt_v, const uint8_t *mask, uint8_t *outliers)
+{
+    return guarded(c, [&] {
+        if (!pass || !u || !v || !peak_ratio || !correlation_moment || !outliers)
+            throw Error(HH_EINVAL, "null argument");
+        check_pass_valid(*pass);
+        if (pass->uod_neighborhood > 3) throw Error(HH_EUNSUPPORTED, "uod_neighborhood up to 3 on the device");
+        if (ny < 1 || nx < 1) throw Error(HH_EDIM, "empty grid");
+        const size_t ng = (size_t)ny * nx;
+        if (mask)
+            for (size_t k = 0; k < ng; ++k)
+                if (mask[k]) throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks yet");
+        const int nalt = pass->n_peaks - 1;
+        ValidateArgs va{};
+        va.ny = ny;
+        va.nx = nx;
+        va.pass = pass;
+        va.force_replace = force_replace;
+        va.u = upload(c, c->hk[0], u, ng);
+        va.v = upload(c, c->hk[1], v, ng);
+        va.pr = upload(c, c->hk[2], peak_ratio, ng);
+        va.cm = upload(c, c->hk[3], correlation_moment, ng);
+        if (alt_u && alt_v && nalt > 0) {
+            va.alt_u = upload(c, c->hk[4], alt_u, ng * nalt);
+            va.alt_v = upload(c, c->hk[5], alt_v, ng * nalt);
+        }
+        va.outliers = c->gout.ensure(ng);
+        va.su = c->gsu.ensure(ng);
+        va.sv = c->gsv.ensure(ng);
+        va.sflag = c->gsflag.ensure(ng);
+        va.count = c->gcount.ensure(1);
+        c->have_result = false;
+        timed(c, "validate", pass->window_size[0], [&] { launch_validate(va, c->stream); });
+        download(c, u, va.u, ng);
+        download(c, v, va.v, ng);
+        download(c, outliers, va.outliers, ng);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_smooth_field(hh_ctx *c, int ny, int nx, const float *f, float *out)
+{
+    return guarded(c, [&] {
+        if (!f || !out) throw Error(HH_EINVAL, "null argument");
+        if (ny < 1 || nx < 1) throw Error(HH_EDIM, "empty grid");
+        const size_t n = (size_t)ny * nx;
+        float *df = upload(c, c->hk[10], f, n);
+        float *dout = c->hk[11].ensure(n);
+        launch_smooth(df, ny, nx, dout, c->stream);
+        download(c, out, (const float *)dout, n);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_run_piv(hh_ctx *c, const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result)
+{
+    return guarded(c, [&] {
+        if (!passes) throw Error(HH_EINVAL, "null pass schedule");
+        run_schedule(c, passes, npasses, predictor_smoothing);
+        if (result) fetch(c, result);
+    });
+}
+
+int hh_fetch_result(hh_ctx *c, hh_field *result)
+{
+    return guarded(c, [&] {
+        if (!result) throw Error(HH_EINVAL, "null result");
+        fetch(c, result);
+    });
+}
+
+void *hh_stream(hh_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int hh_synchronize(hh_ctx *c)
+{
+    return guarded(c, [&] { HH_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+int hh_set_timing(hh_ctx *c, int enable)
+{
+    return guarded(c, [&] {
+        drain_timing(c);
+        c->timing = enable != 0;
+        c->ktime.clear();
+    });
+}
+
+int hh_kernel_time(hh_ctx *c, const char *family, int window, double *ms_total, int64_t *launches)
+{
+    return guarded(c, [&] {
+        if (!family || !ms_total || !launches) throw Error(HH_EINVAL, "null argument");
+        drain_timing(c);
+        auto it = c->ktime.find({std::string(family), window});
+        *ms_total = it == c->ktime.end() ? 0.0 : it->second.first;
+        *launches = it == c->ktime.end() ? 0 : it->second.second;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// hh_internal.h — shared definitions of the gfx950 WIDIM backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hammerhead_mi355x.h"
+
+namespace hh {
+
+// Error carrier: every launcher returns HH_OK or sets a message.
+struct Status {
+    int code = HH_OK;
+    std::string msg;
+};
+
+#define HH_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            throw ::hh::Error(HH_EDEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
+        }                                                                               \
+    } while (0)
+
+struct Error {
+    int code;
+    std::string msg;
+    Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// ---- per-pass correlation configuration ------------------------------------
+constexpr int kMaxPeaks = 8;   // max(n_peaks, 2) supported on the device
+constexpr int kMaxM = 128;     // largest FFT side
+
+struct CorrConfig {
+    int w;          // square window side
+    int M;          // FFT side (2w when padded)
+    int padded;
+    int method;     // HH_CROSS / HH_PHASE
+    int subpixel;   // HH_GAUSS3 / HH_GAUSS9 / HH_SUBPIX_NONE
+    int finder;     // HH_REGIONALMAX / HH_EXCLUSION
+    int n_peaks;
+    int K;          // max(n_peaks, 2)
+    // scan box of the analysis in fftshifted plane coordinates (0-based,
+    // inclusive): outside the overlap-gain support the plane is exactly 0
+    int box_lo, box_hi;
+    const float *apod;  // w*w device
+    const float *gain;  // M*M device (fftshifted), nullptr if not padded
+    const float *wfilt; // M*M device phase filter (unshifted), or nullptr
+};
+
+// Outputs of a correlation launch; any pointer may be null.
+struct CorrOut {
+    float *u, *v, *pr, *cm;          // ny*nx grids
+    float *alt_u, *alt_v;            // ny*nx*(n_peaks-1)
+    const float *u0, *v0;            // predictor node values added (nullable)
+    int32_t *locs;                   // 2*K*njobs
+    float *vals;                     // K*njobs
+    int32_t *found;                  // njobs
+    float *planes;                   // M*M*njobs
+    int ny, nx;
+};
+
+// ---- launchers (k_*.hip) ----------------------------------------------------
+void launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
+                      int njobs, const CorrOut &out, hipStream_t s);
+
+void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp, const float *cp, int ncp,
+                      hipStream_t s);
+void launch_deform(const float *cA, const float *cB, int nr, int nc, const float *pu, const float *pv, int gny,
+                   int gnx, float y0, float ysp, float x0, float xsp, float *wA, float *wB, hipStream_t s);
+void launch_node_values(const float *pu, const float *pv, int gny, int gnx, float y0, float ysp, float x0, float xsp,
+                        const float *y, int ny, const float *x, int nx, float *u, float *v, hipStream_t s);
+
+struct ValidateArgs {
+    int ny, nx;
+    const hh_pass *pass;     // host copy (scalars are passed by value)
+    int force_replace;
+    float *u, *v;            // in/out
+    const float *pr, *cm;
+    const float *alt_u, *alt_v;  // nullable, ny*nx*(n_peaks-1)
+    uint8_t *outliers;       // out
+    // scratch (device): ny*nx each
+    float *su, *sv;
+    uint8_t *sflag;
+    int *count;  // one device int
+};
+void launch_validate(const ValidateArgs &a, hipStream_t s);
+void launch_smooth(const float *f, int ny, int nx, float *out, hipStream_t s);
+
+void launch_uq(int w, const float *apod, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+               int ny, float *unc_u, float *unc_v, double *stats, hipStream_t s);
+
+}  // namespace hh

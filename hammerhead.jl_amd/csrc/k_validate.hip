@@ -1,0 +1,368 @@
+// k_validate.hip — vector validation, peak substitution, local-median
+// replacement and predictor smoothing on the vector grid.
+//
+// validate_and_replace! (src/pipeline.jl:617-645):
+//   1. universal_outlier_detection (src/quality.jl:244-293): neighbours of a
+//      (2n+1)^2 box in column-major order, centre and excluded cells left
+//      out; medians (Statistics.median!: NaN if any NaN, mean of the middle
+//      pair as a/2 + b/2) and residuals in T; the normalized residual
+//      T / (Float64(median) + epsilon) in Float64, flagged when > threshold;
+//   2. PeakRatioValidator when min_peak_ratio > 1, then the extra validators
+//      (src/quality.jl:479-503);
+//   3. substitute_alternatives! (src/quality.jl:307-351): Float64 median and
+//      MAD of the valid neighbours of a snapshot; first alternative within
+//      threshold wins and is unflagged;
+//   4. replace_vectors! (src/quality.jl:85-114): Float64 median of the valid
+//      cells of growing squares until >= 3, from a snapshot, stored as T.
+// smooth_field (src/quality.jl:123-136): separable [1 2 1]/4 with
+// replicated edges, in T, same operation order.
+// All kernels read a snapshot and write a separate buffer, so results do not
+// depend on thread order. Floating-point contraction is off: flags are
+// bit-exact functions of the inputs.
+
+#include "hh_internal.h"
+
+#include <cfloat>
+
+namespace hh {
+
+namespace {
+
+#pragma clang fp contract(off)
+
+// Odd-even transposition sort of N values in registers (fully unrolled).
+template <typename T, int N>
+__device__ __forceinline__ void sort_net(T (&x)[N])
+{
+#pragma unroll
+    for (int round = 0; round < N; ++round) {
+#pragma unroll
+        for (int i = round & 1; i + 1 < N; i += 2) {
+            const T a = x[i], b = x[i + 1];
+            x[i] = a < b ? a : b;
+            x[i + 1] = a < b ? b : a;
+        }
+    }
+}
+
+// median! of the first n of N sorted-with-padding registers (padding = +inf
+// placed after the valid values). hasnan forces NaN.
+template <typename T, int N>
+__device__ __forceinline__ T median_sorted(const T (&x)[N], int n, bool hasnan)
+{
+    if (hasnan) return (T)NAN;
+    T lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (i == (n - 1) / 2) lo = x[i];
+        if (i == n / 2) hi = x[i];
+    }
+    if (n & 1) return lo;
+    return lo / (T)2 + hi / (T)2;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const float *__restrict__ v, int ny, int nx,
+                                             double thr, double eps, const uint8_t *__restrict__ exclude,
+                                             uint8_t *__restrict__ flags)
+{
+    constexpr int W = 2 * NB + 1;
+    constexpr int N = W * W - 1;  // neighbours; the box centre is linear index N/2
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ny * nx) return;
+    if (exclude && exclude[e]) return;
+    const int r = e % ny, c = e / ny;
+    float nu[N], nv[N], su[N], sv[N];
+    bool ok[N];
+    int n = 0;
+    bool nanu = false, nanv = false;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int lin = q < N / 2 ? q : q + 1;
+        const int dc = lin / W - NB, dr = lin % W - NB;
+        const int r2 = r + dr, c2 = c + dc;
+        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !(exclude && exclude[r2 + c2 * ny]);
+        nu[q] = ok[q] ? u[r2 + c2 * ny] : INFINITY;
+        nv[q] = ok[q] ? v[r2 + c2 * ny] : INFINITY;
+        nanu |= ok[q] && isnan(nu[q]);
+        nanv |= ok[q] && isnan(nv[q]);
+        n += ok[q];
+        su[q] = nu[q];
+        sv[q] = nv[q];
+    }
+    if (n == 0) return;
+    sort_net<float, N>(su);
+    sort_net<float, N>(sv);
+    const float umed = median_sorted<float, N>(su, n, nanu);
+    const float vmed = median_sorted<float, N>(sv, n, nanv);
+    bool nanru = false, nanrv = false;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        su[q] = ok[q] ? fabsf(nu[q] - umed) : INFINITY;
+        sv[q] = ok[q] ? fabsf(nv[q] - vmed) : INFINITY;
+        nanru |= ok[q] && isnan(su[q]);
+        nanrv |= ok[q] && isnan(sv[q]);
+    }
+    sort_net<float, N>(su);
+    sort_net<float, N>(sv);
+    const float madu = median_sorted<float, N>(su, n, nanru);
+    const float madv = median_sorted<float, N>(sv, n, nanrv);
+    const float du = fabsf(u[e] - umed), dv = fabsf(v[e] - vmed);
+    const double un = (double)du / ((double)madu + eps);
+    const double vn = (double)dv / ((double)madv + eps);
+    if (un > thr || vn > thr) flags[e] = 1;
+}
+
+// Local validators (src/quality.jl:479-496): OR into flags.
+__global__ __launch_bounds__(256) void k_local_validators(const float *__restrict__ u, const float *__restrict__ v,
+                                                          const float *__restrict__ pr, const float *__restrict__ cm,
+                                                          int n, int kind, double a, double b,
+                                                          uint8_t *__restrict__ flags)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    bool bad = false;
+    if (kind == HH_V_PEAK_RATIO)
+        bad = !((double)pr[e] >= a);
+    else if (kind == HH_V_MOMENT)
+        bad = !((double)cm[e] <= a);
+    else if (kind == HH_V_VELMAG) {
+        const double mag = (double)hypotf(u[e], v[e]);
+        bad = !(a <= mag && mag <= b);
+    }
+    if (bad) flags[e] = 1;
+}
+
+// k-th smallest (1-based) of the valid entries of a box by rank counting.
+template <typename Get>
+__device__ double kth_box(const Get &get, int r0, int r1, int c0, int c1, int k)
+{
+    for (int c = c0; c <= c1; ++c)
+        for (int r = r0; r <= r1; ++r) {
+            double x;
+            if (!get(r, c, x)) continue;
+            int less = 0, leq = 0;
+            for (int c2 = c0; c2 <= c1; ++c2)
+                for (int r2 = r0; r2 <= r1; ++r2) {
+                    double y;
+                    if (!get(r2, c2, y)) continue;
+                    less += y < x;
+                    leq += y <= x;
+                }
+            if (less < k && k <= leq) return x;
+        }
+    return NAN;
+}
+
+template <typename Get>
+__device__ double median_box(const Get &get, int r0, int r1, int c0, int c1, int n)
+{
+    // NaN if any valid entry is NaN
+    for (int c = c0; c <= c1; ++c)
+        for (int r = r0; r <= r1; ++r) {
+            double x;
+            if (get(r, c, x) && isnan(x)) return NAN;
+        }
+    if (n & 1) return kth_box(get, r0, r1, c0, c1, (n + 1) / 2);
+    const double a = kth_box(get, r0, r1, c0, c1, n / 2);
+    const double b = kth_box(get, r0, r1, c0, c1, n / 2 + 1);
+    return a / 2 + b / 2;
+}
+
+// substitute_alternatives! (src/quality.jl:307-351).
+__global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ uref, const float *__restrict__ vref,
+                                                    const uint8_t *__restrict__ flags_in, int ny, int nx, int er,
+                                                    double thr, const float *__restrict__ alt_u,
+                                                    const float *__restrict__ alt_v, int nalt,
+                                                    float *__restrict__ u_out, float *__restrict__ v_out,
+                                                    uint8_t *__restrict__ flags_out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ny * nx) return;
+    u_out[e] = uref[e];
+    v_out[e] = vref[e];
+    flags_out[e] = flags_in[e];
+    if (!flags_in[e]) return;
+    const int r = e % ny, c = e / ny;
+    const int r0 = max(0, r - er), r1 = min(ny - 1, r + er), c0 = max(0, c - er), c1 = min(nx - 1, c + er);
+    int n = 0;
+    for (int c2 = c0; c2 <= c1; ++c2)
+        for (int r2 = r0; r2 <= r1; ++r2)
+            if (!(r2 == r && c2 == c) && !flags_in[r2 + c2 * ny]) ++n;
+    if (n < 3) return;
+    auto gu = [&](int r2, int c2, double &x) {
+        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
+        x = (double)uref[r2 + c2 * ny];
+        return true;
+    };
+    auto gv = [&](int r2, int c2, double &x) {
+        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
+        x = (double)vref[r2 + c2 * ny];
+        return true;
+    };
+    const double mu = median_box(gu, r0, r1, c0, c1, n);
+    const double mv = median_box(gv, r0, r1, c0, c1, n);
+    auto gau = [&](int r2, int c2, double &x) {
+        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
+        x = fabs((double)uref[r2 + c2 * ny] - mu);
+        return true;
+    };
+    auto gav = [&](int r2, int c2, double &x) {
+        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
+        x = fabs((double)vref[r2 + c2 * ny] - mv);
+        return true;
+    };
+    const double madu = median_box(gau, r0, r1, c0, c1, n);
+    const double madv = median_box(gav, r0, r1, c0, c1, n);
+    const size_t ng = (size_t)ny * nx;
+    for (int q = 0; q < nalt; ++q) {
+        const float au = alt_u[e + q * ng], av = alt_v[e + q * ng];
+        if (!(isfinite(au) && isfinite(av))) break;
+        if (fabs((double)au - mu) / (madu + 0.1) <= thr && fabs((double)av - mv) / (madv + 0.1) <= thr) {
+            u_out[e] = au;
+            v_out[e] = av;
+            flags_out[e] = 0;
+            break;
+        }
+    }
+}
+
+// replace_vectors! (src/quality.jl:85-114).
+// Number of valid cells (atomic per block): with fewer than 3 in the whole
+// grid no vector can be replaced and k_replace returns at once.
+__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, int n, int *__restrict__ count)
+{
+    __shared__ int part[4];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int x = e < n && !invalid[e] ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
+}
+
+__global__ void k_zero_i32(int *p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+__global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref, const float *__restrict__ vref,
+                                                 const uint8_t *__restrict__ invalid, int ny, int nx,
+                                                 const int *__restrict__ nvalid, float *__restrict__ u_out,
+                                                 float *__restrict__ v_out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ny * nx) return;
+    u_out[e] = uref[e];
+    v_out[e] = vref[e];
+    if (!invalid[e] || *nvalid < 3) return;
+    const int r = e % ny, c = e / ny;
+    const int maxr = max(ny, nx) - 1;
+    for (int rad = 1; rad <= maxr; ++rad) {
+        const int r0 = max(0, r - rad), r1 = min(ny - 1, r + rad), c0 = max(0, c - rad), c1 = min(nx - 1, c + rad);
+        int n = 0;
+        for (int c2 = c0; c2 <= c1; ++c2)
+            for (int r2 = r0; r2 <= r1; ++r2) n += !invalid[r2 + c2 * ny];
+        if (n < 3) continue;
+        auto gu = [&](int r2, int c2, double &x) {
+            if (invalid[r2 + c2 * ny]) return false;
+            x = (double)uref[r2 + c2 * ny];
+            return true;
+        };
+        auto gv = [&](int r2, int c2, double &x) {
+            if (invalid[r2 + c2 * ny]) return false;
+            x = (double)vref[r2 + c2 * ny];
+            return true;
+        };
+        u_out[e] = (float)median_box(gu, r0, r1, c0, c1, n);
+        v_out[e] = (float)median_box(gv, r0, r1, c0, c1, n);
+        return;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f, int ny, int nx, float *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ny * nx) return;
+    const int r = e % ny, c = e / ny;
+    const int cl = c > 0 ? c - 1 : 0, cr = c < nx - 1 ? c + 1 : nx - 1;
+    const int rl = r > 0 ? r - 1 : 0, rr = r < ny - 1 ? r + 1 : ny - 1;
+    const float q = 0.25f;
+    auto tmp = [&](int rw) {
+        return q * ((f[rw + cl * ny] + 2.f * f[rw + c * ny]) + f[rw + cr * ny]);
+    };
+    out[e] = q * ((tmp(rl) + 2.f * tmp(r)) + tmp(rr));
+}
+
+__global__ void k_fill_u8(uint8_t *p, int n, uint8_t v)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = v;
+}
+
+template <int NB>
+void uod_launch(const ValidateArgs &a, double thr, double eps, hipStream_t s)
+{
+    const int n = a.ny * a.nx;
+    hipLaunchKernelGGL(k_uod<NB>, dim3((n + 255) / 256), dim3(256), 0, s, a.u, a.v, a.ny, a.nx, thr, eps,
+                       (const uint8_t *)nullptr, a.outliers);
+}
+
+void uod(const ValidateArgs &a, int nb, double thr, double eps, hipStream_t s)
+{
+    switch (nb) {
+    case 1: uod_launch<1>(a, thr, eps, s); break;
+    case 2: uod_launch<2>(a, thr, eps, s); break;
+    case 3: uod_launch<3>(a, thr, eps, s); break;
+    default: throw Error(HH_EUNSUPPORTED, "UOD neighborhood sizes 1, 2 and 3 are supported on the device");
+    }
+}
+
+}  // namespace
+
+void launch_validate(const ValidateArgs &a, hipStream_t s)
+{
+    const hh_pass &p = *a.pass;
+    const int n = a.ny * a.nx;
+    if (n <= 0) return;
+    const dim3 g((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_fill_u8, g, b, 0, s, a.outliers, n, (uint8_t)0);
+    if (p.uod_enable) uod(a, p.uod_neighborhood, p.uod_threshold, 0.1, s);
+    if (p.min_peak_ratio > 1)
+        hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, n, (int)HH_V_PEAK_RATIO,
+                           p.min_peak_ratio, 0.0, a.outliers);
+    for (int q = 0; q < p.n_validators; ++q) {
+        const hh_validator &vd = p.validators[q];
+        if (vd.kind == HH_V_UOD)
+            uod(a, vd.neighborhood, vd.a, vd.b, s);
+        else
+            hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, n, (int)vd.kind, vd.a, vd.b,
+                               a.outliers);
+    }
+    // no mask on the device path: outliers &= !mask is the identity
+    if (a.alt_u && p.n_peaks > 1) {
+        hipLaunchKernelGGL(k_substitute, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_neighborhood,
+                           p.uod_threshold, a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+        HH_HIP(hipMemcpyAsync(a.u, a.su, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        HH_HIP(hipMemcpyAsync(a.v, a.sv, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        HH_HIP(hipMemcpyAsync(a.outliers, a.sflag, n, hipMemcpyDeviceToDevice, s));
+    }
+    if (a.force_replace || p.replace_outliers) {
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, a.count);
+        hipLaunchKernelGGL(k_count_valid, g, b, 0, s, a.outliers, n, a.count);
+        hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, (const int *)a.count, a.su, a.sv);
+        HH_HIP(hipMemcpyAsync(a.u, a.su, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        HH_HIP(hipMemcpyAsync(a.v, a.sv, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    }
+    HH_HIP(hipGetLastError());
+}
+
+void launch_smooth(const float *f, int ny, int nx, float *out, hipStream_t s)
+{
+    const int n = ny * nx;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_smooth, dim3((n + 255) / 256), dim3(256), 0, s, f, ny, nx, out);
+    HH_HIP(hipGetLastError());
+}
+
+}  // namespace hh

@@ -1,0 +1,128 @@
+"""run_piv and the backend seam of the host mirror.
+
+``run_piv(imgA, imgB, passes; backend="mi355x")`` mirrors the reference's
+driver (src/pipeline.jl:361-480) and returns a :class:`PIVResult` with the
+reference's fields (src/types.jl:359-373). The whole schedule — prefilter,
+deformation, correlation, validation, predictor smoothing and uncertainty —
+runs on the GPU through ``hh_run_piv``. There is no CPU fallback: backend
+``"cpu"`` is the reference itself and is not provided here.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .params import ArgumentError, DimensionMismatch, PIVParameters, pass_grid_counts
+
+
+@dataclass
+class PIVResult:
+    """PIVResult{Float32} (src/types.jl:359-373)."""
+
+    x: np.ndarray
+    y: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    peak_ratio: np.ndarray
+    correlation_moment: np.ndarray
+    uncertainty_u: np.ndarray
+    uncertainty_v: np.ndarray
+    outliers: np.ndarray
+    mask: np.ndarray
+    parameters: PIVParameters
+    correlation_planes: Optional[np.ndarray] = None
+    scale: object = None
+
+    def __repr__(self):
+        ny, nx = self.u.shape
+        return f"PIVResult{{Float32}}({nx}×{ny} grid, {int(self.outliers.sum())} outliers)"
+
+
+_BACKENDS = {"mi355x"}
+
+
+def _resolve_backend(backend: str) -> str:
+    """_resolve_backend(::Val{sym}) (src/types.jl:79-86)."""
+    if backend in _BACKENDS:
+        return backend
+    raise ArgumentError(f"unsupported Hammerhead backend :{backend}; this package provides backend = :mi355x "
+                        "(the gfx950 device path). The CPU path is the reference package itself.")
+
+
+class PIVWorkspace:
+    """piv_workspace(; backend) (src/pipeline.jl:21-47): one device context
+    whose buffers, prefiltered coefficients and engine pool are reused across
+    run_piv calls on equally sized pairs. Not for concurrent use."""
+
+    def __init__(self, backend: str = "mi355x", device: int = 0):
+        self.backend = _resolve_backend(backend)
+        self.ctx = _lib.Context(device)
+
+    def close(self):
+        self.ctx.close()
+
+
+def piv_workspace(backend: str = "mi355x", device: int = 0) -> PIVWorkspace:
+    return PIVWorkspace(backend, device)
+
+
+def pass_grid(p: PIVParameters, nr: int, nc: int):
+    """pass_grid (src/pipeline.jl:529-555), unmasked: x, y, jobs (4 x n int32)."""
+    rows, cols = pass_grid_counts(p, nr, nc)
+    wr, wc = p.window_size
+    x = np.array([cs + (wc - 1) / 2 for cs in cols], dtype=np.float32)
+    y = np.array([rs + (wr - 1) / 2 for rs in rows], dtype=np.float32)
+    jobs = np.array([(gi + 1, gj + 1, rs, cs) for gj, cs in enumerate(cols) for gi, rs in enumerate(rows)],
+                    dtype=np.int32).reshape(-1, 4).T
+    return x, y, np.asfortranarray(jobs)
+
+
+def run_piv(imgA, imgB, passes, *, effort=None, backend: str = "mi355x", predictor_smoothing: bool = True,
+            mask=None, workspace: Optional[PIVWorkspace] = None, form: int = _lib.HH_FORM_FFT32, **kwargs):
+    """run_piv(imgA, imgB, passes; backend) -> PIVResult (src/pipeline.jl:361-449).
+
+    ``passes`` is a PIVParameters or a list of them. The computation runs in
+    float32 on the GPU (the reference's Float32 pipeline)."""
+    if effort is not None:
+        raise ArgumentError("effort cannot be combined with explicit PIVParameters or pass schedules")
+    unknown = set(kwargs) - {"threaded", "uncertainty_backend", "mask_threshold", "scale", "roi"}
+    if unknown:
+        raise ArgumentError(f"unsupported run_piv keyword(s): {', '.join(sorted(unknown))}")
+    if kwargs.get("roi") is not None:
+        raise ArgumentError("backend :mi355x does not support roi yet")
+    be = _resolve_backend(backend)
+    if workspace is not None and workspace.backend != be:
+        raise ArgumentError(f"workspace backend {workspace.backend} does not match run backend {be}")
+    if isinstance(passes, PIVParameters):
+        passes = [passes]
+    passes = list(passes)
+    if not passes:
+        raise ArgumentError("at least one pass is required")
+    A = np.asarray(imgA)
+    B = np.asarray(imgB)
+    if A.shape != B.shape:
+        raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
+    if mask is not None:
+        m = np.asarray(mask, dtype=bool)
+        if m.shape != A.shape:
+            raise DimensionMismatch(f"mask must have the same size as the images, got {m.shape}")
+        if m.any():
+            raise ArgumentError("backend :mi355x does not support masks yet")
+    ws = workspace or PIVWorkspace(be)
+    try:
+        ctx = ws.ctx
+        ctx.check_passes(passes)
+        for p in passes:
+            pass_grid_counts(p, *A.shape)
+        if form != _lib.HH_FORM_FFT32:
+            ctx.set_form(form)
+        ctx.set_pair(A, B)
+        r = ctx.run_piv(passes, predictor_smoothing)
+    finally:
+        if workspace is None:
+            ws.close()
+    return PIVResult(r["x"], r["y"], r["u"], r["v"], r["peak_ratio"], r["correlation_moment"], r["uncertainty_u"],
+                     r["uncertainty_v"], r["outliers"], r["mask"], passes[-1], None, kwargs.get("scale"))

@@ -1,0 +1,237 @@
+/*
+ * hammerhead_mi355x.h — C ABI of the MI355X (gfx950) WIDIM correlation backend.
+ *
+ * This library replaces the device-engine hooks of Hammerhead.jl's backend seam
+ * (a private multiple-dispatch hook set resolved from a public Symbol,
+ * docs/src/reference/backends.md:40-66, src/types.jl:61-111). A Julia shim that
+ * registers `backend = :mi355x` binds every entry point below with `ccall`;
+ * the Python host package binds them with ctypes. Plain pointers and sizes
+ * only; no HIP, torch or Julia types cross the boundary.
+ *
+ * Conventions (identical to the reference):
+ *   - images, vector grids and correlation planes are COLUMN-MAJOR;
+ *   - window origins, grid indices and peak locations are 1-BASED;
+ *   - u is the column (x) displacement, v the row (y) displacement; a particle
+ *     at (row, col) in A is found at (row + v, col + u) in B
+ *     (src/correlators.jl:4-7, src/types.jl:322-325).
+ *   - arithmetic type T = float32 (the reference's Float32 pipeline,
+ *     src/types.jl:314-317) with float64 islands where the reference keeps them
+ *     (correlation-moment sums, uncertainty sums, substitution/replacement
+ *     medians, the UOD residual division).
+ *
+ * Every call returns an int status (HH_OK = 0). On error, hh_last_error(ctx)
+ * returns a message. Options the device engine does not implement are rejected
+ * before any work with HH_EUNSUPPORTED, mirroring the reference's up-front
+ * ArgumentError (src/ka_backend.jl:13-29); shape errors return HH_EDIM
+ * (DimensionMismatch). A context owns one HIP stream; it is not thread-safe,
+ * distinct contexts are.
+ */
+#ifndef HAMMERHEAD_MI355X_H
+#define HAMMERHEAD_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    HH_OK = 0,
+    HH_EINVAL = 1,       /* ArgumentError: invalid parameter value            */
+    HH_EDIM = 2,         /* DimensionMismatch                                 */
+    HH_EUNSUPPORTED = 3, /* ArgumentError: option not implemented on device   */
+    HH_EDEVICE = 4,      /* HIP runtime error                                 */
+    HH_ENOMEM = 5,       /* device allocation failed                          */
+    HH_ESTATE = 6        /* call-order error (e.g. hh_deform before a pair)   */
+};
+
+/* ---- enumerations mirroring PIVParameters symbols (src/types.jl:212-231) */
+enum { HH_CROSS = 0, HH_PHASE = 1 };                   /* correlation_method */
+enum { HH_APOD_NONE = 0, HH_APOD_GAUSS = 1 };           /* apodization        */
+enum { HH_GAUSS3 = 0, HH_GAUSS9 = 1, HH_GAUSS2D = 2, HH_SUBPIX_NONE = 3 };
+enum { HH_REGIONALMAX = 0, HH_EXCLUSION = 1 };          /* peak_finder        */
+/* extra validators of PIVParameters.validation (src/quality.jl:383-428) */
+enum { HH_V_PEAK_RATIO = 1, HH_V_MOMENT = 2, HH_V_VELMAG = 3, HH_V_UOD = 4 };
+/* correlation arithmetic form */
+enum { HH_FORM_FFT32 = 0, HH_FORM_MFMA16 = 1 };
+/* window source of a correlation call */
+enum { HH_SRC_RAW = 0, HH_SRC_WARP = 1 };
+
+#define HH_MAX_VALIDATORS 4
+#define HH_UQ_NSTATS 44 /* UQ_NSTATS, src/uncertainty.jl:35 */
+
+/* One entry of PIVParameters.validation (src/quality.jl:383-428):
+ *   HH_V_PEAK_RATIO  a = threshold
+ *   HH_V_MOMENT      a = threshold
+ *   HH_V_VELMAG      a = min, b = max
+ *   HH_V_UOD         a = threshold, b = epsilon, neighborhood = layers       */
+typedef struct hh_validator {
+    int32_t kind;
+    int32_t neighborhood;
+    double a;
+    double b;
+} hh_validator;
+
+/* PIVParameters (src/types.jl:212-291), field for field. */
+typedef struct hh_pass {
+    int32_t window_size[2];      /* (rows, cols)                           */
+    int32_t search_area_size[2]; /* (rows, cols)                           */
+    int32_t overlap[2];          /* (rows, cols)                           */
+    int32_t correlation_method;  /* HH_CROSS | HH_PHASE                    */
+    int32_t padding;             /* bool                                   */
+    int32_t apodization;         /* HH_APOD_*                              */
+    int32_t subpixel_method;     /* HH_GAUSS3 | HH_GAUSS9 | ...            */
+    int32_t n_peaks;
+    int32_t peak_finder;         /* HH_REGIONALMAX | HH_EXCLUSION          */
+    int32_t uncertainty;         /* bool                                   */
+    int32_t uod_enable;          /* bool                                   */
+    double uod_threshold;
+    int32_t uod_neighborhood;
+    int32_t replace_outliers;    /* bool                                   */
+    double min_peak_ratio;
+    int32_t max_iterations;
+    int32_t keep_correlation_planes; /* bool (rejected on device)          */
+    double convergence_tol;
+    int32_t n_validators;
+    int32_t reserved;
+    hh_validator validators[HH_MAX_VALIDATORS];
+} hh_pass;
+
+/* PIVResult{Float32} (src/types.jl:359-373) as caller-owned host arrays.
+ * Grids are ny x nx column-major; x has nx entries, y has ny. A NULL array
+ * is skipped. outliers/mask are one byte per cell (0/1).                     */
+typedef struct hh_field {
+    int32_t ny, nx;
+    float *x, *y;
+    float *u, *v, *peak_ratio, *correlation_moment;
+    float *uncertainty_u, *uncertainty_v;
+    uint8_t *outliers, *mask;
+} hh_field;
+
+typedef struct hh_ctx hh_ctx;
+
+/* ---- lifecycle ------------------------------------------------------- */
+
+/* Create a context on HIP device `device` (piv_workspace(; backend),
+ * src/pipeline.jl:44-47). */
+int hh_create(hh_ctx **out, int device);
+/* Release all device memory (release_piv_engines!, src/pipeline.jl:121). */
+int hh_destroy(hh_ctx *ctx);
+/* Message of the last failing call on ctx ("" if none); valid until the
+ * next call on ctx. ctx may be NULL (reports hh_create failures). */
+const char *hh_last_error(const hh_ctx *ctx);
+const char *hh_version(void);
+
+/* Select the correlation arithmetic: HH_FORM_FFT32 (FP32 radix FFT) or
+ * HH_FORM_MFMA16 (split-FP16 DFT products on MFMA). */
+int hh_set_form(hh_ctx *ctx, int form);
+
+/* ---- parameter checks (no device work) --------------------------------- */
+
+/* _check_backend_params(b, passes) (src/types.jl:110, src/ka_backend.jl:13-29):
+ * HH_OK, or HH_EINVAL / HH_EUNSUPPORTED with a message, before any work. */
+int hh_check_passes(hh_ctx *ctx, const hh_pass *passes, int npasses);
+/* Grid counts of pass_grid (src/pipeline.jl:529-555); *ny = #row starts. */
+int hh_grid_size(const hh_pass *pass, int nr, int nc, int *ny, int *nx);
+
+/* ---- image pair -------------------------------------------------------- */
+
+/* Copy a host column-major float32 pair (nr x nc) to the device
+ * (the engine staging of ext/HammerheadAMDGPUExt.jl:375-396). */
+int hh_set_pair(hh_ctx *ctx, const float *A, const float *B, int nr, int nc);
+/* Borrow a pair already resident in device memory (column-major float32,
+ * nr x nc). The pointers must stay valid while ctx uses them. */
+int hh_set_pair_device(hh_ctx *ctx, const float *dA, const float *dB, int nr, int nc);
+
+/* ---- deformation (A10) ------------------------------------------------- */
+
+/* Cubic B-spline prefilter of both images on the device: the coefficients of
+ * Interpolations' BSpline(Cubic(Line(OnGrid()))) (image_interpolant,
+ * src/pipeline.jl:909-915), padded (nr+2) x (nc+2). */
+int hh_prefilter(hh_ctx *ctx);
+/* Symmetric deformation by a predictor grid (apply_predictor/deform_images,
+ * src/pipeline.jl:575-600,936-965; _ka_deform!, src/ka_backend.jl:293-312).
+ * py (gny knots, uniform), px (gnx knots, uniform), pu/pv gny x gnx host
+ * column-major. Writes the device-resident warped pair (HH_SRC_WARP). */
+int hh_deform(hh_ctx *ctx, const float *py, int gny, const float *px, int gnx,
+              const float *pu, const float *pv);
+/* Copy the current warped pair (or raw pair for src = HH_SRC_RAW) to host. */
+int hh_get_images(hh_ctx *ctx, int src, float *A, float *B);
+/* Copy the prefiltered coefficients ((nr+2) x (nc+2) each) to host. */
+int hh_get_coefs(hh_ctx *ctx, float *cA, float *cB);
+
+/* ---- per-window correlation (A4-A8) ------------------------------------ */
+
+/* process_windows! (src/pipeline.jl:972-1027; device engine
+ * ext/HammerheadAMDGPUExt.jl:398-493). jobs: int32 4 x njobs column-major
+ * (gi, gj, rs, cs), 1-based, as pass_grid builds them. Writes RESIDUAL
+ * displacements into u, v and peak_ratio, correlation_moment (ny x nx) and
+ * the gauss3 alternatives into alt_u/alt_v (ny x nx x (n_peaks-1), entries
+ * of absent peaks untouched). unc_u/unc_v (nullable) receive the fused
+ * Wieneke uncertainty. Debug outputs (nullable): peak_locs int32
+ * 2 x K x njobs (1-based plane row, col), peak_vals float K x njobs, found
+ * int32 njobs, with K = max(n_peaks, 2). */
+int hh_process_windows(hh_ctx *ctx, const hh_pass *pass, int src,
+                       const int32_t *jobs, int njobs, int ny, int nx,
+                       float *u, float *v, float *peak_ratio, float *correlation_moment,
+                       float *alt_u, float *alt_v, float *unc_u, float *unc_v,
+                       int32_t *peak_locs, float *peak_vals, int32_t *found);
+
+/* correlation_plane! (src/correlators.jl:326-341) for each job: the
+ * fftshifted, overlap-normalized plane, M x M x njobs float32 (M = 2w when
+ * padded). The keep_correlation_planes inspection path. */
+int hh_correlation_planes(hh_ctx *ctx, const hh_pass *pass, int src,
+                          const int32_t *jobs, int njobs, float *planes);
+
+/* uncertainty_sweep! (src/pipeline.jl:873-899; src/uncertainty.jl:42-179):
+ * sigma_u/sigma_v per job; stats (nullable) float64 2 x 44 x njobs. */
+int hh_uncertainty_sweep(hh_ctx *ctx, const hh_pass *pass, int src,
+                         const int32_t *jobs, int njobs, int ny, int nx,
+                         float *unc_u, float *unc_v, double *stats);
+
+/* ---- validation and predictor (A9, A11) -------------------------------- */
+
+/* validate_and_replace! (src/pipeline.jl:617-645): UOD, peak-ratio and
+ * extra validators, mask exclusion, peak substitution (alt_u/alt_v nullable:
+ * ny x nx x (n_peaks-1), NaN where absent) and local-median replacement
+ * when force_replace || replace_outliers. u, v edited in place; outliers
+ * written. mask nullable (all false). */
+int hh_validate_and_replace(hh_ctx *ctx, const hh_pass *pass, int ny, int nx,
+                            int force_replace, float *u, float *v,
+                            const float *peak_ratio, const float *correlation_moment,
+                            const float *alt_u, const float *alt_v,
+                            const uint8_t *mask, uint8_t *outliers);
+/* smooth_field (src/quality.jl:123-136). */
+int hh_smooth_field(hh_ctx *ctx, int ny, int nx, const float *f, float *out);
+
+/* ---- whole schedule ---------------------------------------------------- */
+
+/* run_piv(imgA, imgB, passes; predictor_smoothing) (src/pipeline.jl:361-449)
+ * on the current pair, entirely on the device. If result != NULL the final
+ * PIVResult is copied to its host arrays (result->ny/nx must match the final
+ * pass grid); with NULL the result stays device-resident (hh_fetch_result). */
+int hh_run_piv(hh_ctx *ctx, const hh_pass *passes, int npasses,
+               int predictor_smoothing, hh_field *result);
+/* Copy the device-resident result of the last hh_run_piv to host. */
+int hh_fetch_result(hh_ctx *ctx, hh_field *result);
+
+/* ---- stream and timing ------------------------------------------------- */
+
+/* The context's HIP stream (hipStream_t), for callers that time or order
+ * work around the library. */
+void *hh_stream(hh_ctx *ctx);
+int hh_synchronize(hh_ctx *ctx);
+/* Per-kernel timing with HIP events on the context stream: enable, then read
+ * total milliseconds and launch count of a kernel family by name
+ * ("correlate", "deform", "prefilter", "validate", "uq"). */
+int hh_set_timing(hh_ctx *ctx, int enable);
+int hh_kernel_time(hh_ctx *ctx, const char *family, int window, double *ms_total,
+                   int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAMMERHEAD_MI355X_H */

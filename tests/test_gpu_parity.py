@@ -1,0 +1,231 @@
+"""GPU parity: the gfx950 library, called through its C ABI, against the
+oracle on the same seeded inputs.
+
+Bars: integer peak locations and outlier flags bit-exact (peak mismatches
+only where the FP64 plane has a near-tie, reported separately); u, v within
+1e-3 px of the FP64 oracle; sigma within 1 %.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def P(hh, **kw):
+    return hh.PIVParameters(**kw)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c1(hh):
+    return hh.synthetic.config_uniform_256()
+
+
+def compare_windows(orc, hh, ctx, A, B, p, src=None, tol_px=1e-3, tie_rel=1e-5):
+    """Run process_windows on GPU and FP64 oracle; returns stats dict and
+    asserts the parity bars."""
+    x, y, jobs = hh.pass_grid(p, *A.shape)
+    ny, nx = y.size, x.size
+    st = p.to_struct()
+    g = ctx.process_windows(p, jobs, ny, nx, src=hh.HH_SRC_RAW if src is None else src, debug=True)
+    o = orc.process_windows(st, A, B, jobs, ny, nx, debug=True, planes=True)
+    K = max(p.n_peaks, 2)
+    nj = jobs.shape[1]
+    mism = ties = 0
+    same_primary = np.zeros(nj, bool)
+    M = o["planes"].shape[0]
+    for k in range(nj):
+        fg, fo = g["found"][k], o["found"][k]
+        lg = [tuple(g["locs"][:, m, k]) for m in range(fg)]
+        lo = [tuple(o["locs"][:, m, k]) for m in range(fo)]
+        same_primary[k] = fg > 0 and fo > 0 and lg[0] == lo[0]
+        if lg == lo:
+            continue
+        mism += 1
+        R = o["planes"][:, :, k]
+        scale = max(R.max(), 1e-300)
+        # explained when each differing pair of candidates is a near tie in
+        # the FP64 plane (or a near-zero secondary cut by the positivity rule)
+        ok = True
+        for m in range(max(len(lg), len(lo))):
+            a = lg[m] if m < len(lg) else None
+            b = lo[m] if m < len(lo) else None
+            if a == b:
+                continue
+            va = R[a[0] - 1, a[1] - 1] if a else 0.0
+            vb = R[b[0] - 1, b[1] - 1] if b else 0.0
+            if abs(va - vb) > tie_rel * scale:
+                ok = False
+        ties += ok
+    grid = lambda a: a.ravel(order="F")
+    jg = (jobs[0] - 1) + (jobs[1] - 1) * ny
+    du = np.abs(grid(g["u"])[jg] - grid(o["u"])[jg])[same_primary]
+    dv = np.abs(grid(g["v"])[jg] - grid(o["v"])[jg])[same_primary]
+    stats = dict(windows=nj, peak_mismatch=mism, near_ties=ties, max_du=float(du.max(initial=0)),
+                 max_dv=float(dv.max(initial=0)), n_over_tol=int(((du > tol_px) | (dv > tol_px)).sum()))
+    return stats, g, o
+
+
+@pytest.mark.parametrize("w,padding,apod", [(32, True, "gauss"), (16, True, "gauss"), (32, False, "none"),
+                                            (64, True, "gauss"), (16, True, "none")])
+def test_correlate_vs_oracle(hh, orc, ctx, c1, w, padding, apod):
+    A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
+    ctx.set_pair(A, B)
+    p = P(hh, window_size=w, overlap=w // 2, padding=padding, apodization=apod)
+    say("correlate", w, padding, apod)
+    s, g, o = compare_windows(orc, hh, ctx, A.astype(np.float64), B.astype(np.float64), p)
+    say(s)
+    assert s["peak_mismatch"] == s["near_ties"], s
+    assert s["n_over_tol"] == 0, s
+
+
+def test_planes_vs_oracle(hh, orc, ctx, c1):
+    A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
+    ctx.set_pair(A, B)
+    for w in (16, 32, 64):
+        p = P(hh, window_size=w, overlap=w // 2, padding=True, apodization="gauss")
+        x, y, jobs = hh.pass_grid(p, *A.shape)
+        jobs = jobs[:, :40]
+        say("planes", w)
+        pg = ctx.correlation_planes(p, jobs)
+        po = orc.process_windows(p.to_struct(), A.astype(np.float64), B.astype(np.float64), jobs, y.size, x.size,
+                                 planes=True)["planes"]
+        err = np.abs(pg - po).max(axis=(0, 1)) / po.max(axis=(0, 1))
+        say("max rel plane error", err.max())
+        assert err.max() < 2e-5
+
+
+def test_phase_and_exclusion_vs_oracle(hh, orc, ctx, c1):
+    A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
+    ctx.set_pair(A, B)
+    for kw in (dict(correlation_method="phase", padding=True, apodization="gauss"),
+               dict(peak_finder="exclusion", padding=True, apodization="gauss"),
+               dict(subpixel_method="gauss9", padding=True, apodization="gauss")):
+        p = P(hh, window_size=32, overlap=16, **kw)
+        say("variant", kw)
+        s, _, _ = compare_windows(orc, hh, ctx, A.astype(np.float64), B.astype(np.float64), p)
+        say(s)
+        assert s["peak_mismatch"] == s["near_ties"], s
+        assert s["n_over_tol"] == 0, s
+
+
+def test_prefilter_and_deform_vs_oracle(hh, orc, ctx):
+    pair = hh.synthetic.particle_pair(200, 180, hh.synthetic.uniform_flow(1.3, -0.7), seed=3)
+    ctx.set_pair(pair.A, pair.B)
+    say("prefilter")
+    ctx.prefilter()
+    cA, cB = ctx.coefs()
+    oA = orc.prefilter(pair.A.astype(np.float64))
+    err = np.abs(cA - oA).max() / np.abs(oA).max()
+    say("prefilter rel err", err)
+    assert err < 5e-6
+    py = np.linspace(16.5, 184.5, 11)
+    px = np.linspace(16.5, 164.5, 10)
+    rng = np.random.default_rng(0)
+    pu = rng.uniform(-3, 3, (11, 10))
+    pv = rng.uniform(-3, 3, (11, 10))
+    say("deform")
+    ctx.deform(py, px, pu, pv)
+    wA, wB = ctx.images()
+    oB = orc.prefilter(pair.B.astype(np.float64))
+    rA, rB = orc.deform(oA, oB, 200, 180, py, px, pu, pv)
+    e = max(np.abs(wA - rA).max(), np.abs(wB - rB).max()) / max(np.abs(rA).max(), 1e-30)
+    say("deform rel err", e)
+    assert e < 5e-6
+
+
+def test_validate_bit_exact(hh, orc, ctx):
+    rng = np.random.default_rng(8)
+    for trial, nb in enumerate((1, 2, 3, 2)):
+        ny, nx = 37, 41
+        u = (2.0 + 0.1 * rng.standard_normal((ny, nx))).astype(np.float32)
+        v = (-1.0 + 0.1 * rng.standard_normal((ny, nx))).astype(np.float32)
+        bad = rng.random((ny, nx)) < 0.08
+        u[bad] += rng.uniform(-5, 5, bad.sum()).astype(np.float32)
+        pr = rng.uniform(1.0, 3.0, (ny, nx)).astype(np.float32)
+        cm = rng.uniform(0.1, 1.0, (ny, nx)).astype(np.float32)
+        au = np.full((ny, nx, 2), np.nan, np.float32)
+        av = np.full((ny, nx, 2), np.nan, np.float32)
+        au[..., 0] = u + rng.uniform(-3, 3, (ny, nx)).astype(np.float32)
+        av[..., 0] = v + rng.uniform(-0.2, 0.2, (ny, nx)).astype(np.float32)
+        au[..., 1] = (2.0 + 0.05 * rng.standard_normal((ny, nx))).astype(np.float32)
+        av[..., 1] = (-1.0 + 0.05 * rng.standard_normal((ny, nx))).astype(np.float32)
+        p = P(hh, uod_neighborhood=nb, min_peak_ratio=1.2 if trial == 3 else 1.0,
+              validation=(("correlation_moment", 0.9),) if trial == 3 else ())
+        say("validate", trial, nb)
+        gu, gv, gf = ctx.validate_and_replace(p, u, v, pr, cm, au, av, force_replace=True)
+        ou, ov, of = orc.validate_and_replace(p.to_struct(), u, v, pr, cm, au, av, force_replace=True, prec="f32")
+        say("flags", gf.sum(), of.sum())
+        assert np.array_equal(gf, of)
+        assert np.array_equal(gu, ou) and np.array_equal(gv, ov)
+
+
+def test_smooth_bit_exact(hh, orc, ctx):
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((31, 17)).astype(np.float32)
+    assert np.array_equal(ctx.smooth_field(f), orc.smooth_field(f, prec="f32"))
+
+
+def test_uq_vs_oracle(hh, orc, ctx):
+    pair = hh.synthetic.particle_pair(160, 160, hh.synthetic.uniform_flow(0.3, 0.2), seed=12, noise=0.05)
+    ctx.set_pair(pair.A, pair.B)
+    for w in (16, 32):
+        p = P(hh, window_size=w, overlap=w // 2, padding=True, apodization="gauss", uncertainty=True)
+        x, y, jobs = hh.pass_grid(p, 160, 160)
+        say("uq", w)
+        gu, gv, gs = ctx.uncertainty_sweep(p, jobs, y.size, x.size, src=hh.HH_SRC_RAW, stats=True)
+        ou, ov, os_ = orc.uncertainty_sweep(p.to_struct(), pair.A.astype(np.float64), pair.B.astype(np.float64),
+                                            jobs, y.size, x.size, stats=True)
+        fin = np.isfinite(ou) & np.isfinite(gu)
+        assert np.array_equal(np.isfinite(ou), np.isfinite(gu))
+        rel = np.abs(gu[fin] - ou[fin]) / np.abs(ou[fin])
+        relv = np.abs(gv[fin] - ov[fin]) / np.abs(ov[fin])
+        say("uq max rel", rel.max(), relv.max())
+        assert rel.max() < 0.01 and relv.max() < 0.01
+
+
+def run_both(hh, orc, ctx, A, B, passes, prec="f64"):
+    ctx.set_pair(A, B)
+    g = ctx.run_piv(passes)
+    o = orc.run_piv(A.astype(np.float64 if prec == "f64" else np.float32),
+                    B.astype(np.float64 if prec == "f64" else np.float32), [p.to_struct() for p in passes],
+                    prec=prec)
+    return g, o
+
+
+def test_run_piv_single_pass(hh, orc, ctx, c1):
+    p = P(hh, window_size=32, overlap=16, padding=True, apodization="gauss")
+    say("run_piv C1")
+    g, o = run_both(hh, orc, ctx, c1.A.astype(np.float32), c1.B.astype(np.float32), [p])
+    assert np.array_equal(g["outliers"], o["outliers"])
+    assert np.abs(g["u"] - o["u"]).max() < 1e-3 and np.abs(g["v"] - o["v"]).max() < 1e-3
+    assert abs(np.median(g["u"]) - 2.3) < 0.05 and abs(np.median(g["v"]) + 1.7) < 0.05
+
+
+def test_run_piv_multipass_vortex(hh, orc, ctx):
+    pair = hh.synthetic.config_vortex_1024(n=512)
+    passes = hh.multipass_parameters([64, 32, 16], padding=True, apodization="gauss",
+                                     final=dict(overlap=12, max_iterations=3, convergence_tol=0.0,
+                                                uncertainty=True))
+    passes[-1] = passes[-1].replace(overlap=(12, 12))
+    say("run_piv vortex 512")
+    g, o = run_both(hh, orc, ctx, pair.A, pair.B, passes)
+    dflag = int((g["outliers"] != o["outliers"]).sum())
+    same = (~g["outliers"]) & (~o["outliers"])
+    du = np.abs(g["u"] - o["u"])[same]
+    dv = np.abs(g["v"] - o["v"])[same]
+    rms = math.sqrt(np.mean(np.concatenate([du, dv]) ** 2))
+    say("flag diffs", dflag, "of", g["outliers"].size, "rms dev", rms, "max", du.max(), dv.max())
+    fin = np.isfinite(g["uncertainty_u"]) & np.isfinite(o["uncertainty_u"])
+    srel = np.median(np.abs(g["uncertainty_u"][fin] - o["uncertainty_u"][fin]) / o["uncertainty_u"][fin])
+    say("sigma median rel dev", srel)
+    assert rms < 5e-3
+    assert dflag <= 0.002 * g["outliers"].size + 2
+    assert srel < 0.01
