@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of the WIDIM correlation path on one MI355X.
+
+Workload (one "step" = one vector field): a seeded 2048 x 2048 float32
+particle-image pair (0.03 ppp, d = 2.5 px, noise 0.02; solid-body rotation of
+8 px at the corners plus a global U[-0.5, 0.5] px shift) through the schedule
+64/32 -> 32/16 -> 16/12 (75 % final overlap, 3 final sweeps, convergence_tol
+= 0, Wieneke uncertainty on the final pass), padding + Gaussian apodization
+on every pass: 3969 + 16129 + 3 x 259081 = 797341 window correlations.
+The pair is uploaded to HBM before the timed region; results stay on the
+device. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fft32|mfma16]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+METRIC = "vector fields/s (4MP, 64→32→16 WIDIM), split-FP16 MFMA vs FP32 GPU at ≤0.005px"
+
+
+def workload(hh):
+    pair = hh.synthetic.config_rotation_4mp()
+    passes = hh.multipass_parameters([64, 32, 16], padding=True, apodization="gauss",
+                                     final=dict(overlap=12, max_iterations=3, convergence_tol=0.0, uncertainty=True))
+    return pair, passes
+
+
+def fft_flops_per_window(M: int) -> float:
+    """FFT-form convention: two forward and one inverse M x M FFT
+    (5 M^2 log2 M^2 each) plus 12 M^2 for the spectrum, magnitude and gain."""
+    return 3 * 5 * M * M * math.log2(M * M) + 12 * M * M
+
+
+def read_traffic(path: pathlib.Path):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 counter summary (line 'traffic_bytes_per_launch: N')."""
+    try:
+        for line in path.read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(pair, passes, threads: int):
+    """The oracle's Float32 restatement of the reference run_piv, timed on the
+    host for one vector field of the same workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # noqa: E402  (checker only; never the measured GPU path)
+
+    oracle.lib()
+    t0 = time.perf_counter()
+    oracle.run_piv(pair.A, pair.B, [p.to_struct() for p in passes], nthreads=threads, prec="f32")
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "vector fields/s", "cores": threads, "kind": "port",
+            "sample": f"one full 2048x2048 vector field (797341 window correlations + UQ) on {threads} threads, "
+                      f"{dt:.1f} s; C restatement of the reference's Float32 run_piv"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--form", choices=["fft32", "mfma16"], default="fft32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "correlate_w16_traffic.txt"))
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run device copies of this size (HBM counter calibration) and exit")
+    args = ap.parse_args(argv)
+    if args.calibrate_copy_mib:
+        import torch
+
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes_read": n * 4, "calibration_copy_bytes_written": n * 4}))
+        return 0
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding')",
+              file=sys.stderr)
+        return 2
+
+    import __graft_entry__ as ge
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    hh = ge.load_package()
+    pair, passes = workload(hh)
+    log("workload ready")
+    nr, nc = pair.A.shape
+    ctx = hh.Context(0)
+    if args.form == "mfma16":
+        ctx.set_form(hh.HH_FORM_MFMA16)
+    ctx.set_pair(pair.A, pair.B)
+
+    try:
+        import torch
+
+        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except ImportError:  # pragma: no cover
+        sync_torch = lambda: None
+
+    for _ in range(args.warmup):
+        ctx.run_piv(passes, fetch=False)
+    ctx.synchronize()
+    log("warmup done")
+
+    ctx.set_timing(True)
+    sync_torch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.run_piv(passes, fetch=False)
+    ctx.synchronize()
+    sync_torch()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
+
+    kernels = {}
+    for fam in ("prefilter", "deform", "correlate", "validate", "uq"):
+        for w in (0, 16, 32, 64):
+            ms, n = ctx.kernel_time(fam, w)
+            if n:
+                kernels[f"{fam}_w{w}" if w else fam] = {"avg_ms": ms / n, "launches_per_step": n / args.steps}
+    ctx.set_timing(False)
+
+    res = ctx.run_piv(passes, fetch=True)
+    rms = hh.synthetic.rms_error(res, pair.flow, midpoint=True)
+
+    # dominant kernel: the 16-px-window correlation (M = 32), 259081 windows
+    ny3, nx3 = hh.grid_size(passes[-1], nr, nc)
+    nwin = ny3 * nx3
+    ms3, n3 = kernels["correlate_w16"]["avg_ms"], kernels["correlate_w16"]["launches_per_step"]
+    flops = nwin * fft_flops_per_window(32)
+    achieved = flops / (ms3 * 1e-3) / 1e12
+    traffic = read_traffic(pathlib.Path(args.traffic_file))
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3),
+                "peak": PEAK_FP32_TFLOPS if args.form == "fft32" else PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                "frac": None, "traffic": traffic, "kernel": "correlate_w16",
+                "algorithmic_flops_per_launch": flops}
+    roofline["frac"] = round(achieved / roofline["peak"], 5)
+
+    ms_per_step = elapsed / args.steps * 1e3
+    line = {
+        "metric": METRIC,
+        "value": round(args.steps / elapsed, 4),
+        "unit": "vector fields/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.form == "fft32" else "fp16x2-split/f32",
+        "data": "synthetic (seeded particle images; device-resident pair)",
+        "config": {"workload": "2048x2048 float32 pair, 64/32->32/16->16/12 x3 sweeps, pad+gauss, UQ",
+                   "form": args.form, "window_correlations": 3969 + 16129 + 3 * nwin},
+        "roofline": roofline,
+        "kernels": kernels,
+        "accuracy": {"rms_px_vs_midpoint_truth": round(rms, 5),
+                     "outliers": int(res["outliers"].sum())},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(pair, passes, args.cpu_threads)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -220,7 +220,10 @@ def multipass_parameters(window_sizes: Sequence, overlap_fraction: float = 0.5, 
         if i == n - 1:
             extra.update(final)
         ov = tuple(int(math.floor(s * overlap_fraction)) for s in ws)
-        out.append(PIVParameters(window_size=ws, overlap=ov, **extra))
+        # Julia keyword splatting: later keywords (kwargs, final) override
+        kw = dict(window_size=ws, overlap=ov)
+        kw.update(extra)
+        out.append(PIVParameters(**kw))
     return out
 
 

@@ -144,3 +144,28 @@ def config_rotation_4mp(seed: int = 103, n: int = 2048) -> SyntheticPair:
 def config_sine_shear_1024(seed: int = 105, n: int = 1024) -> SyntheticPair:
     """1024 x 1024 Float32 pair, 0.03 ppp: u = 3 sin(2 pi y / 256), v = 0."""
     return particle_pair(n, n, sine_shear_flow(3.0, 256.0), seed, ppp=0.03, noise=0.02)
+
+
+def midpoint_truth(flow, x, y, iters: int = 8):
+    """Displacement whose midpoint lies at (x, y): the truth a symmetric
+    deformation measures (bench/run_benchmarks.jl:199-207)."""
+    x = np.asarray(x, float)
+    y = np.asarray(y, float)
+    x0, y0 = x.copy(), y.copy()
+    for _ in range(iters):
+        u, v = flow(x0, y0)
+        x0 = x - 0.5 * u
+        y0 = y - 0.5 * v
+    return flow(x0, y0)
+
+
+def rms_error(result, flow, midpoint: bool = True) -> float:
+    """RMS of the vector error over valid cells, sqrt(mean(du^2 + dv^2))
+    (bench/run_benchmarks.jl:209-220)."""
+    X, Y = np.meshgrid(np.asarray(result["x"], float), np.asarray(result["y"], float))
+    tu, tv = midpoint_truth(flow, X, Y) if midpoint else flow(X, Y)
+    ok = ~np.asarray(result["outliers"], bool)
+    if "mask" in result:
+        ok &= ~np.asarray(result["mask"], bool)
+    e2 = (np.asarray(result["u"], float) - tu) ** 2 + (np.asarray(result["v"], float) - tv) ** 2
+    return float(np.sqrt(np.mean(e2[ok])))

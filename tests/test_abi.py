@@ -76,6 +76,8 @@ def test_multipass_parameters(hh):
     assert all(p.padding for p in fp) and fp[0].n_peaks == 3 and fp[2].n_peaks == 2
     assert fp[2].keep_correlation_planes and not fp[0].keep_correlation_planes
     assert hh.multipass_parameters([32], final=dict(n_peaks=1))[0].n_peaks == 1
+    # a final override of a computed keyword wins, as with Julia's splatting
+    assert hh.multipass_parameters([64, 16], final=dict(overlap=12))[1].overlap == (12, 12)
 
 
 def test_pass_grid(hh):
