@@ -68,6 +68,10 @@ struct CorrOut {
 void launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
                       int njobs, const CorrOut &out, hipStream_t s);
 
+// Specialized kernels; return false when the configuration is not covered.
+bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
+                           int njobs, const CorrOut &out, hipStream_t s);
+
 void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp, const float *cp, int ncp,
                       hipStream_t s);
 void launch_deform(const float *cA, const float *cB, int nr, int nc, const float *pu, const float *pv, int gny,

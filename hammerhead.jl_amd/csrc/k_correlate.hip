@@ -487,6 +487,7 @@ void launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
 {
     (void)nc;
     if (njobs <= 0) return;
+    if (launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s)) return;
     const float2 *tw = twiddle_table(cfg.M, s);
     switch (cfg.M) {
     case 16: hipLaunchKernelGGL(k_corr_lds<16>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;

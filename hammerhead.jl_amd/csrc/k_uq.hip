@@ -64,9 +64,12 @@ __device__ __forceinline__ float wave_sumf(float x)
 }
 
 // Statistics of one component: shift along the second index of the
-// (nr x ncl) view; view(A, r, c) = A[r*rs + c*cs].
+// (nr x ncl) view; view(A, r, c) = A[r*rs + c*cs]. dC/dCs: float scratch of
+// nr*(ncl-1); d: double scratch of the same size. Lane k < 41 accumulates
+// lag k (0 = S00, k = offset k-1) over the whole field: one Float64
+// accumulator per lane, no cross-lane reduction of the 41 sums.
 __device__ void uq_component(double *st, const float *wA, const float *wB, int nr, int ncl, int rs, int cs, float *dC,
-                             float *dCs, const Offsets &off)
+                             float *dCs, double *d, const Offsets &off)
 {
     const int lane = threadIdx.x & 63;
     const int m = ncl - 1;
@@ -97,29 +100,26 @@ __device__ void uq_component(double *st, const float *wA, const float *wB, int n
     Cm = wave_sum(Cm);
     mu = wave_sum(mu) / (double)ne;
     __syncthreads();
-    // S00 and the 40 offset sums: lanes stride over (offset, column) pairs
-    double S[kNoff + 1];
-#pragma unroll
-    for (int k = 0; k <= kNoff; ++k) S[k] = 0.0;
-    for (int e = lane; e < ne; e += 64) {
-        const int r = e % nr, c = e / nr;
-        const double d = (double)dCs[e] - mu;
-        S[0] += d * d;
-#pragma unroll
-        for (int k = 0; k < kNoff; ++k) {
-            const int r2 = r + off.dr[k], c2 = c + off.dc[k];
-            if (r2 >= 0 && r2 < nr && c2 >= 0 && c2 < m) S[k + 1] += d * ((double)dCs[r2 + c2 * nr] - mu);
+    for (int e = lane; e < ne; e += 64) d[e] = (double)dCs[e] - mu;
+    __syncthreads();
+    double S = 0.0;
+    if (lane <= kNoff) {
+        const int dr = lane == 0 ? 0 : off.dr[lane - 1];
+        const int dc = lane == 0 ? 0 : off.dc[lane - 1];
+        const int c_lo = dc < 0 ? -dc : 0, c_hi = dc > 0 ? m - dc : m;  // c + dc in [0, m)
+        const int r_lo = dr < 0 ? -dr : 0, r_hi = dr > 0 ? nr - dr : nr;
+        const int shift = dr + dc * nr;
+        for (int c = c_lo; c < c_hi; ++c) {
+            const double *col = d + c * nr;
+            for (int r = r_lo; r < r_hi; ++r) S += col[r] * col[r + shift];
         }
     }
-#pragma unroll
-    for (int k = 0; k <= kNoff; ++k) S[k] = wave_sum(S[k]);
     if (lane == 0) {
         st[0] += C0;
         st[1] += Cp;
         st[2] += Cm;
-#pragma unroll
-        for (int k = 0; k <= kNoff; ++k) st[3 + k] += S[k];
     }
+    if (lane <= kNoff) st[3 + lane] += S;
     __syncthreads();
 }
 
@@ -150,6 +150,7 @@ __global__ __launch_bounds__(64) void k_uq(int w, const float *__restrict__ apod
 {
     __shared__ float wA[W * W], wB[W * W];
     __shared__ float dC[W * W], dCs[W * W];
+    __shared__ double dd[W * W];
     __shared__ double st[2 * HH_UQ_NSTATS];
     const int lane = threadIdx.x;
     const int job = blockIdx.x;
@@ -177,8 +178,8 @@ __global__ __launch_bounds__(64) void k_uq(int w, const float *__restrict__ apod
     }
     __syncthreads();
     // u: shift along columns (element (r, c) at r + c*w); v: transposed views
-    uq_component(st, wA, wB, w, w, 1, w, dC, dCs, off);
-    uq_component(st + HH_UQ_NSTATS, wA, wB, w, w, w, 1, dC, dCs, off);
+    uq_component(st, wA, wB, w, w, 1, w, dC, dCs, dd, off);
+    uq_component(st + HH_UQ_NSTATS, wA, wB, w, w, w, 1, dC, dCs, dd, off);
     if (lane == 0) {
         const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * ny;
         if (unc_u) unc_u[g] = uq_finalize(st, off);

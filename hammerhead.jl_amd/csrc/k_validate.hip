@@ -169,14 +169,18 @@ __device__ double median_box(const Get &get, int r0, int r1, int c0, int c1, int
     return a / 2 + b / 2;
 }
 
-// substitute_alternatives! (src/quality.jl:307-351).
+// substitute_alternatives! (src/quality.jl:307-351). The valid neighbours
+// of a flagged cell are gathered into registers (+inf padding) and sorted
+// by a network, so the Float64 median and MAD cost no memory round trips.
+template <int NB>
 __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ uref, const float *__restrict__ vref,
-                                                    const uint8_t *__restrict__ flags_in, int ny, int nx, int er,
-                                                    double thr, const float *__restrict__ alt_u,
-                                                    const float *__restrict__ alt_v, int nalt,
-                                                    float *__restrict__ u_out, float *__restrict__ v_out,
+                                                    const uint8_t *__restrict__ flags_in, int ny, int nx, double thr,
+                                                    const float *__restrict__ alt_u, const float *__restrict__ alt_v,
+                                                    int nalt, float *__restrict__ u_out, float *__restrict__ v_out,
                                                     uint8_t *__restrict__ flags_out)
 {
+    constexpr int W = 2 * NB + 1;
+    constexpr int N = W * W - 1;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ny * nx) return;
     u_out[e] = uref[e];
@@ -184,36 +188,41 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
     flags_out[e] = flags_in[e];
     if (!flags_in[e]) return;
     const int r = e % ny, c = e / ny;
-    const int r0 = max(0, r - er), r1 = min(ny - 1, r + er), c0 = max(0, c - er), c1 = min(nx - 1, c + er);
+    double nu[N], nv[N], su[N], sv[N];
+    bool ok[N];
     int n = 0;
-    for (int c2 = c0; c2 <= c1; ++c2)
-        for (int r2 = r0; r2 <= r1; ++r2)
-            if (!(r2 == r && c2 == c) && !flags_in[r2 + c2 * ny]) ++n;
+    bool nanu = false, nanv = false;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int lin = q < N / 2 ? q : q + 1;
+        const int dc = lin / W - NB, dr = lin % W - NB;
+        const int r2 = r + dr, c2 = c + dc;
+        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !flags_in[r2 + c2 * ny];
+        nu[q] = ok[q] ? (double)uref[r2 + c2 * ny] : INFINITY;
+        nv[q] = ok[q] ? (double)vref[r2 + c2 * ny] : INFINITY;
+        nanu |= ok[q] && isnan(nu[q]);
+        nanv |= ok[q] && isnan(nv[q]);
+        n += ok[q];
+        su[q] = nu[q];
+        sv[q] = nv[q];
+    }
     if (n < 3) return;
-    auto gu = [&](int r2, int c2, double &x) {
-        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
-        x = (double)uref[r2 + c2 * ny];
-        return true;
-    };
-    auto gv = [&](int r2, int c2, double &x) {
-        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
-        x = (double)vref[r2 + c2 * ny];
-        return true;
-    };
-    const double mu = median_box(gu, r0, r1, c0, c1, n);
-    const double mv = median_box(gv, r0, r1, c0, c1, n);
-    auto gau = [&](int r2, int c2, double &x) {
-        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
-        x = fabs((double)uref[r2 + c2 * ny] - mu);
-        return true;
-    };
-    auto gav = [&](int r2, int c2, double &x) {
-        if ((r2 == r && c2 == c) || flags_in[r2 + c2 * ny]) return false;
-        x = fabs((double)vref[r2 + c2 * ny] - mv);
-        return true;
-    };
-    const double madu = median_box(gau, r0, r1, c0, c1, n);
-    const double madv = median_box(gav, r0, r1, c0, c1, n);
+    sort_net<double, N>(su);
+    sort_net<double, N>(sv);
+    const double mu = median_sorted<double, N>(su, n, nanu);
+    const double mv = median_sorted<double, N>(sv, n, nanv);
+    bool nanru = false, nanrv = false;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        su[q] = ok[q] ? fabs(nu[q] - mu) : INFINITY;
+        sv[q] = ok[q] ? fabs(nv[q] - mv) : INFINITY;
+        nanru |= ok[q] && isnan(su[q]);
+        nanrv |= ok[q] && isnan(sv[q]);
+    }
+    sort_net<double, N>(su);
+    sort_net<double, N>(sv);
+    const double madu = median_sorted<double, N>(su, n, nanru);
+    const double madv = median_sorted<double, N>(sv, n, nanrv);
     const size_t ng = (size_t)ny * nx;
     for (int q = 0; q < nalt; ++q) {
         const float au = alt_u[e + q * ng], av = alt_v[e + q * ng];
@@ -227,7 +236,6 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
     }
 }
 
-// replace_vectors! (src/quality.jl:85-114).
 // Number of valid cells (atomic per block): with fewer than 3 in the whole
 // grid no vector can be replaced and k_replace returns at once.
 __global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, int n, int *__restrict__ count)
@@ -257,8 +265,31 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref,
     v_out[e] = vref[e];
     if (!invalid[e] || *nvalid < 3) return;
     const int r = e % ny, c = e / ny;
+    {
+        // radius 1 (the common case) from registers
+        double nu[9], nv[9];
+        int n = 0;
+        bool nanu = false, nanv = false;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int r2 = r + q % 3 - 1, c2 = c + q / 3 - 1;
+            const bool ok = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !invalid[r2 + c2 * ny];
+            nu[q] = ok ? (double)uref[r2 + c2 * ny] : INFINITY;
+            nv[q] = ok ? (double)vref[r2 + c2 * ny] : INFINITY;
+            nanu |= ok && isnan(nu[q]);
+            nanv |= ok && isnan(nv[q]);
+            n += ok;
+        }
+        if (n >= 3) {
+            sort_net<double, 9>(nu);
+            sort_net<double, 9>(nv);
+            u_out[e] = (float)median_sorted<double, 9>(nu, n, nanu);
+            v_out[e] = (float)median_sorted<double, 9>(nv, n, nanv);
+            return;
+        }
+    }
     const int maxr = max(ny, nx) - 1;
-    for (int rad = 1; rad <= maxr; ++rad) {
+    for (int rad = 2; rad <= maxr; ++rad) {
         const int r0 = max(0, r - rad), r1 = min(ny - 1, r + rad), c0 = max(0, c - rad), c1 = min(nx - 1, c + rad);
         int n = 0;
         for (int c2 = c0; c2 <= c1; ++c2)
@@ -341,8 +372,20 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
     }
     // no mask on the device path: outliers &= !mask is the identity
     if (a.alt_u && p.n_peaks > 1) {
-        hipLaunchKernelGGL(k_substitute, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_neighborhood,
-                           p.uod_threshold, a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+        switch (p.uod_neighborhood) {
+        case 1:
+            hipLaunchKernelGGL(k_substitute<1>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_substitute<2>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+            break;
+        default:
+            hipLaunchKernelGGL(k_substitute<3>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+            break;
+        }
         HH_HIP(hipMemcpyAsync(a.u, a.su, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         HH_HIP(hipMemcpyAsync(a.v, a.sv, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         HH_HIP(hipMemcpyAsync(a.outliers, a.sflag, n, hipMemcpyDeviceToDevice, s));

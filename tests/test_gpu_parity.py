@@ -102,13 +102,19 @@ def test_planes_vs_oracle(hh, orc, ctx, c1):
         assert err.max() < 2e-5
 
 
-def test_phase_and_exclusion_vs_oracle(hh, orc, ctx, c1):
+@pytest.mark.parametrize("w", [16, 32])
+def test_phase_and_exclusion_vs_oracle(hh, orc, ctx, c1, w):
     A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
     ctx.set_pair(A, B)
     for kw in (dict(correlation_method="phase", padding=True, apodization="gauss"),
+               dict(correlation_method="phase", padding=True, apodization="none"),
                dict(peak_finder="exclusion", padding=True, apodization="gauss"),
-               dict(subpixel_method="gauss9", padding=True, apodization="gauss")):
-        p = P(hh, window_size=32, overlap=16, **kw)
+               dict(subpixel_method="gauss9", padding=True, apodization="gauss"),
+               dict(subpixel_method="none", padding=True, apodization="gauss"),
+               dict(n_peaks=2, padding=True, apodization="gauss"),
+               dict(n_peaks=1, padding=True, apodization="gauss"),
+               dict(n_peaks=5, padding=True, apodization="gauss")):
+        p = P(hh, window_size=w, overlap=w // 2, **kw)
         say("variant", kw)
         s, _, _ = compare_windows(orc, hh, ctx, A.astype(np.float64), B.astype(np.float64), p)
         say(s)
