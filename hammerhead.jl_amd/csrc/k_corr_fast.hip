@@ -165,10 +165,142 @@ __device__ __forceinline__ float2 phase_norm(float2 p, float w)
     return make_float2(w * p.x / den, w * p.y / den);
 }
 
-struct Cand {
-    float v;
-    int lin;  // column-major plane index, INT_MAX = none
-};
+// ---- cross-lane combination of the L sub-transforms of one line ----------
+// Lanes of a line are adjacent (quad-aligned), so exchanges are DPP
+// quad permutations: full-rate VALU moves, no LDS.
+constexpr int kQX1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kQX2 = 0x4E;   // [2,3,0,1]
+constexpr int kQX3 = 0x1B;   // [3,2,1,0]
+constexpr int kQNEG4 = 0xB4; // [0,1,3,2]
+
+template <int CTRL>
+__device__ __forceinline__ float qp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float2 qp2(float2 v)
+{
+    return make_float2(qp<CTRL>(v.x), qp<CTRL>(v.y));
+}
+
+// value of the lane holding the mirrored block L-1-m (index 32-k, k >= 1)
+template <int L>
+__device__ __forceinline__ float2 mirror(float2 v)
+{
+    if constexpr (L == 1) return v;
+    else if constexpr (L == 2) return qp2<kQX1>(v);
+    else return qp2<kQX3>(v);
+}
+// value of the lane holding block (-m) mod L (index 0)
+template <int L>
+__device__ __forceinline__ float2 neg0(float2 v)
+{
+    if constexpr (L == 4) return qp2<kQNEG4>(v);
+    else return v;
+}
+// block held by lane l after a forward cross-lane transform (bit reversal)
+template <int L>
+__device__ __forceinline__ int block_of(int l)
+{
+    if constexpr (L == 4) return ((l & 1) << 1) | ((l >> 1) & 1);
+    else return l;
+}
+
+__device__ __forceinline__ float2 cadd2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub2(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot4(float2 b)
+{
+    return INV ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
+}
+
+// Forward L-point DFT across the lanes of a line, decimation in frequency:
+// lane l holds input q = l, returns output block block_of(l).
+template <int L>
+__device__ __forceinline__ float2 xdft_fwd(float2 v, int l)
+{
+    if constexpr (L == 1) {
+        return v;
+    } else if constexpr (L == 2) {
+        const float2 o = qp2<kQX1>(v);
+        return (l & 1) ? csub2(o, v) : cadd2(v, o);
+    } else {
+        float2 o = qp2<kQX2>(v);
+        float2 s = (l & 2) ? csub2(o, v) : cadd2(v, o);
+        if ((l & 3) == 3) s = rot4<false>(s);
+        o = qp2<kQX1>(s);
+        return (l & 1) ? csub2(o, s) : cadd2(s, o);
+    }
+}
+// Inverse L-point DFT across lanes, decimation in time: lane l holds input
+// block block_of(l), returns output q = l.
+template <int L>
+__device__ __forceinline__ float2 xdft_inv(float2 v, int l)
+{
+    if constexpr (L == 1) {
+        return v;
+    } else if constexpr (L == 2) {
+        const float2 o = qp2<kQX1>(v);
+        return (l & 1) ? csub2(o, v) : cadd2(v, o);
+    } else {
+        float2 o = qp2<kQX1>(v);
+        const float2 s = (l & 1) ? csub2(o, v) : cadd2(v, o);
+        o = qp2<kQX2>(s);
+        // twiddle (+i)^(l & 1) on the span-2 partner
+        const float2 os = (l & 1) ? rot4<true>(o) : o;
+        const float2 ss = (l & 1) ? rot4<true>(s) : s;
+        return (l & 2) ? csub2(o, ss) : cadd2(s, os);
+    }
+}
+
+// four-step twiddle W_M^{+-q k} for runtime q in [0, L), compile-time k
+template <int L, int K, bool INV>
+__device__ __forceinline__ float2 twq(float2 v, int q)
+{
+    constexpr int M = 32 * L;
+    if constexpr (L == 1 || K == 0) {
+        return v;
+    } else if constexpr (L == 2) {
+        const float2 a = twmul<M, K, INV>(v);
+        return q ? a : v;
+    } else {
+        constexpr float c1 = (float)c_cos(angle(K, M)), c2 = (float)c_cos(angle(2 * K, M)),
+                        c3 = (float)c_cos(angle(3 * K, M));
+        constexpr float s1 = (float)(INV ? 1 : -1) * (float)c_sin(angle(K, M)),
+                        s2 = (float)(INV ? 1 : -1) * (float)c_sin(angle(2 * K, M)),
+                        s3 = (float)(INV ? 1 : -1) * (float)c_sin(angle(3 * K, M));
+        const float c = q == 1 ? c1 : (q == 2 ? c2 : c3);
+        const float s = q == 1 ? s1 : (q == 2 ? s2 : s3);
+        const float2 a = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+        return q == 0 ? v : a;
+    }
+}
+
+// Forward M-point DFT of a line whose lane q holds x[L n + q], n < 16
+// (x[L n + q] = 0 for n >= 16): returns X[k + 32 block_of(l)] in X[k].
+template <int L>
+__device__ __forceinline__ void line_fwd(const float2 (&x)[16], float2 (&X)[32], int l)
+{
+    dft32_half(x, X);
+    sfor<32>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        X[k] = xdft_fwd<L>(twq<L, k, false>(X[k], l), l);
+    });
+}
+// Inverse M-point DFT: lane l holds X[k + 32 block_of(l)] in X[k]; returns
+// x[L n + l] in X[n] (unnormalized).
+template <int L>
+__device__ __forceinline__ void line_inv(float2 (&X)[32], int l)
+{
+    sfor<32>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        X[k] = twq<L, k, true>(xdft_inv<L>(X[k], l), l);
+    });
+    fft<32, true>(X);
+}
+
 __device__ __forceinline__ bool better(float av, int al, float bv, int bl)
 {
     if (al == INT_MAX) return false;
@@ -187,10 +319,9 @@ struct BoxView {
     }
 };
 
-template <typename V>
-__device__ float2 gauss3_b(const V &R, int i, int j)  // 1-based, M = 32
+template <int M, typename V>
+__device__ float2 gauss3_b(const V &R, int i, int j)  // 1-based
 {
-    constexpr int M = 32;
     float di = 0.f, dj = 0.f;
     const float I0 = R(i - 1, j - 1);
     if (1 < i && i < M) {
@@ -210,16 +341,15 @@ __device__ float2 gauss3_b(const V &R, int i, int j)  // 1-based, M = 32
     return make_float2((float)i + di, (float)j + dj);
 }
 
-template <typename V>
+template <int M, typename V>
 __device__ float2 gauss9_b(const V &R, int pr, int pc)
 {
-    constexpr int M = 32;
-    if (!(1 < pr && pr < M && 1 < pc && pc < M)) return gauss3_b(R, pr, pc);
+    if (!(1 < pr && pr < M && 1 < pc && pc < M)) return gauss3_b<M>(R, pr, pc);
     float sL = 0.f, sxL = 0.f, syL = 0.f, sxxL = 0.f, syyL = 0.f, sxyL = 0.f;
     for (int dj = -1; dj <= 1; ++dj)
         for (int di = -1; di <= 1; ++di) {
             const float val = R(pr + di - 1, pc + dj - 1);
-            if (!(val > 0.f)) return gauss3_b(R, pr, pc);
+            if (!(val > 0.f)) return gauss3_b<M>(R, pr, pc);
             const float L = logf(val);
             sL += L;
             sxL += (float)dj * L;
@@ -231,17 +361,16 @@ __device__ float2 gauss9_b(const V &R, int pr, int pc)
     const float a1 = sxL / 6.f, a2 = syL / 6.f, a3 = sxxL / 2.f - sL / 3.f, a4 = sxyL / 4.f,
                 a5 = syyL / 2.f - sL / 3.f;
     const float D = 4.f * a3 * a5 - a4 * a4;
-    if (!(a3 < 0.f && D > 0.f)) return gauss3_b(R, pr, pc);
+    if (!(a3 < 0.f && D > 0.f)) return gauss3_b<M>(R, pr, pc);
     const float dx = (a4 * a2 - 2.f * a5 * a1) / D;
     const float dy = (a4 * a1 - 2.f * a3 * a2) / D;
-    if (!(fabsf(dx) <= 1.f && fabsf(dy) <= 1.f)) return gauss3_b(R, pr, pc);
+    if (!(fabsf(dx) <= 1.f && fabsf(dy) <= 1.f)) return gauss3_b<M>(R, pr, pc);
     return make_float2((float)pr + dy, (float)pc + dx);
 }
 
-template <typename V>
+template <int M, typename V>
 __device__ float moment_b(const V &R, float pr, float pc)
 {
-    constexpr int M = 32;
     if (!(isfinite(pr) && isfinite(pc))) return NAN;
     const int r0 = (int)rintf(pr), c0 = (int)rintf(pc);
     const int rlo = max(1, r0 - 1), rhi = min(M, r0 + 1);
@@ -261,171 +390,251 @@ __device__ float moment_b(const V &R, float pr, float pc)
     return (float)sqrt((sdx + sdy) / sumC);
 }
 
-// HW: half-width of the analysis box (box = [16-HW, 16+HW]^2, which must
-// contain the gain support plus a one-pixel zero margin).
-template <int K, bool PHASE, int HW>
-__global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__restrict__ A,
-                                               const float *__restrict__ B, int nr,
-                                               const int32_t *__restrict__ jobs, int njobs, int nblocks, CorrOut out)
+// Correlation of w = 16 L windows padded to M = 32 L. L lanes per line of the
+// 2D transform, G = 16 L^2 lanes per window. HW: half-width of the analysis
+// box [M/2 - HW, M/2 + HW]^2, which holds the gain support plus a zero margin.
+template <int L, int K, bool PHASE, int HW>
+__global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, const float *__restrict__ A,
+                                                           const float *__restrict__ B, int nr,
+                                                           const int32_t *__restrict__ jobs, int njobs, int nblocks,
+                                                           CorrOut out)
 {
 #pragma clang fp contract(off)
-    constexpr int M = 32;
+    constexpr int M = 32 * L;
+    constexpr int H = M / 2;
+    constexpr int W = 16 * L;                // window side = number of lines
+    constexpr int G = 16 * L * L;            // lanes per window
+    constexpr int WPB = G >= 64 ? 1 : 64 / G;  // windows per block
+    constexpr int NT = G >= 64 ? G : 64;     // threads per block
     constexpr int NB = 2 * HW + 1;
-    constexpr int LO = M / 2 - HW;
-    constexpr int TSZ = 17 * 16;        // float4 per window for the transpose
+    constexpr int LO = H - HW;
+    constexpr int U = W + 1;                 // unit pitch of the kept rows
+    constexpr int TSZ = (H + 1) * W;         // float4 per window
     constexpr int NPAIR = (NB + 1) / 2;
-    static_assert(NB * 17 * 2 + NB * NB <= TSZ * 4, "box does not fit the LDS slice");
-    __shared__ float4 lds[4 * TSZ];
+    constexpr int NWAVE = NT / 64;
+    static_assert(NB * U * 2 + NB * NB <= TSZ * 4, "box does not fit the LDS slice");
+    static_assert(2 * W * (W + 1) <= TSZ * 4, "staging does not fit the LDS slice");
+    __shared__ float4 lds[WPB * TSZ];
+    __shared__ float red[2 * NWAVE];
+    __shared__ float mv[NWAVE];
+    __shared__ int ml[NWAVE];
 
-    const int lane = threadIdx.x;
-    const int grp = lane >> 4;
-    const int t = lane & 15;
-    // XCD-aware order: hardware dispatches block b to XCD b % 8; give each XCD
-    // a contiguous range of tiles
+    const int tid = threadIdx.x;
+    const int slot_w = tid / G;               // window slot in the block
+    const int t = tid % G;                    // lane within the window
+    const int line = t / L, l = t % L;        // line (row / unit) and sub-lane
+    const int wave = tid >> 6;
+    // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
+    // range of tiles so overlapping windows share its L2
     const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
     const int tile = xcd * q8 + min(xcd, r8) + slot;
-    const int win = tile * 4 + grp;
+    const int win = tile * WPB + slot_w;
     const bool active = win < njobs;
     const int jb = active ? win : njobs - 1;
     const int gi = jobs[4 * jb], gj = jobs[4 * jb + 1], rs = jobs[4 * jb + 2], cs = jobs[4 * jb + 3];
+    float4 *T = lds + slot_w * TSZ;
 
-    // ---- 1. load row t of both windows; means; apodize; pack A + iB ----
-    const float *pa = A + (size_t)(cs - 1) * nr + (rs - 1 + t);
-    const float *pb = B + (size_t)(cs - 1) * nr + (rs - 1 + t);
-    float ra[16], rb[16];
-    float sa = 0.f, sb = 0.f;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        ra[j] = pa[(size_t)j * nr];
-        rb[j] = pb[(size_t)j * nr];
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        sa += ra[j];
-        sb += rb[j];
-    }
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-        sa += __shfl_xor(sa, o);
-        sb += __shfl_xor(sb, o);
-    }
-    const float meanA = sa / 256.f, meanB = sb / 256.f;
+    // ---- 1. window rows -> means -> apodized A + iB ----
     float2 x[16];
+    {
+        const float *pa = A + (size_t)(cs - 1) * nr + (rs - 1);
+        const float *pb = B + (size_t)(cs - 1) * nr + (rs - 1);
+        float sa = 0.f, sb = 0.f;
+        if constexpr (L == 1) {
+            float ra[16], rb[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const float a = cfg.apod[t + 16 * j];
-        x[j] = make_float2(a * (ra[j] - meanA), a * (rb[j] - meanB));
+            for (int j = 0; j < 16; ++j) {
+                ra[j] = pa[line + (size_t)j * nr];
+                rb[j] = pb[line + (size_t)j * nr];
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                sa += ra[j];
+                sb += rb[j];
+            }
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                sa += __shfl_xor(sa, o);
+                sb += __shfl_xor(sb, o);
+            }
+            const float meanA = sa / 256.f, meanB = sb / 256.f;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const float a = cfg.apod[line + 16 * j];
+                x[j] = make_float2(a * (ra[j] - meanA), a * (rb[j] - meanB));
+            }
+        } else {
+            // stage the window through LDS (coalesced global reads)
+            float *SA = reinterpret_cast<float *>(T);
+            float *SB = SA + W * (W + 1);
+            for (int e = t; e < W * W; e += G) {
+                const int i = e % W, j = e / W;
+                const float va = pa[i + (size_t)j * nr], vb = pb[i + (size_t)j * nr];
+                SA[i + j * (W + 1)] = va;
+                SB[i + j * (W + 1)] = vb;
+                sa += va;
+                sb += vb;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                sa += __shfl_xor(sa, o);
+                sb += __shfl_xor(sb, o);
+            }
+            if constexpr (NWAVE > 1) {
+                if ((tid & 63) == 0) {
+                    red[2 * wave] = sa;
+                    red[2 * wave + 1] = sb;
+                }
+                __syncthreads();
+                sa = 0.f;
+                sb = 0.f;
+#pragma unroll
+                for (int k = 0; k < NWAVE; ++k) {
+                    sa += red[2 * k];
+                    sb += red[2 * k + 1];
+                }
+            } else {
+                __syncthreads();
+            }
+            const float meanA = sa / (float)(W * W), meanB = sb / (float)(W * W);
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+                const int j = L * n + l;
+                const float a = cfg.apod[line + W * j];
+                x[n] = make_float2(a * (SA[line + j * (W + 1)] - meanA), a * (SB[line + j * (W + 1)] - meanB));
+            }
+            __syncthreads();  // staging is overwritten by the transpose below
+        }
     }
 
-    // ---- 2. row DFT (dim 2) and Hermitian split, k2 = 0..16 ----
-    float4 *T = lds + grp * TSZ;
+    // ---- 2. row DFT (dim 2) and Hermitian split, k2 = 0..H ----
+    const int mblk = block_of<L>(l);
     {
         float2 Y[32];
-        dft32_half(x, Y);
-        sfor<17>([&](auto kk) {
-            constexpr int k2 = decltype(kk)::value;
-            constexpr int km = (32 - k2) & 31;
-            const float2 z = Y[k2], zm = Y[km];
-            T[k2 * 16 + t] = make_float4(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y), 0.5f * (z.y + zm.y),
-                                         -0.5f * (z.x - zm.x));
+        line_fwd<L>(x, Y, l);
+        sfor<32>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const int k2 = k + 32 * mblk;
+            const float2 z = Y[k];
+            float2 zm;
+            if constexpr (k == 0)
+                zm = neg0<L>(Y[0]);
+            else
+                zm = mirror<L>(Y[32 - k]);
+            if (k2 <= H)
+                T[k2 * W + line] = make_float4(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y), 0.5f * (z.y + zm.y),
+                                               -0.5f * (z.x - zm.x));
         });
     }
     __syncthreads();
 
-    // ---- 3. column DFT (dim 1) of column k2 = t; spectrum; inverse dim 1 ----
+    // ---- 3. column DFT (dim 1) of unit u = line; spectrum; inverse dim 1 ----
     float2 P[32];
     {
+        const int u = line;
         float2 fa[16], fb[16];
-        // lane 0 packs the real columns k2 = 0 and 16 as re + i im
+        // unit 0 packs the real columns k2 = 0 and H as re + i im
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const float4 v = T[t * 16 + i], v16 = T[16 * 16 + i];
-            fa[i] = t != 0 ? make_float2(v.x, v.y) : make_float2(v.x, v16.x);
-            fb[i] = t != 0 ? make_float2(v.z, v.w) : make_float2(v.z, v16.z);
+        for (int n = 0; n < 16; ++n) {
+            const int i = L * n + l;
+            const float4 v = T[u * W + i], vh = T[H * W + i];
+            fa[n] = u != 0 ? make_float2(v.x, v.y) : make_float2(v.x, vh.x);
+            fb[n] = u != 0 ? make_float2(v.z, v.w) : make_float2(v.z, vh.z);
         }
         float2 FA[32], FB[32];
-        dft32_half(fa, FA);
-        dft32_half(fb, FB);
-        // conjugate pairs (k1, 32 - k1) together, so FA/FB die as P fills;
-        // lane 0 separates its packed real columns k2 = 0 and 16 and
-        // repacks their (Hermitian-in-k1) spectra as P0 + i P16
-        const bool unit0 = t == 0;
+        line_fwd<L>(fa, FA, l);
+        line_fwd<L>(fb, FB, l);
+        const bool unit0 = u == 0;
         sfor<17>([&](auto kk) {
-            constexpr int k1 = decltype(kk)::value;
-            constexpr int km = (32 - k1) & 31;
-            const float2 ga = FA[k1], gam = FA[km], gb = FB[k1], gbm = FB[km];
-            // general column k2 = t
-            float2 p = conj_mul(ga, gb), pm = conj_mul(gam, gbm);
-            // packed columns 0 and 16
-            const float2 a0 = make_float2(0.5f * (ga.x + gam.x), 0.5f * (ga.y - gam.y));
-            const float2 a16 = make_float2(0.5f * (ga.y + gam.y), -0.5f * (ga.x - gam.x));
-            const float2 b0 = make_float2(0.5f * (gb.x + gbm.x), 0.5f * (gb.y - gbm.y));
-            const float2 b16 = make_float2(0.5f * (gb.y + gbm.y), -0.5f * (gb.x - gbm.x));
-            float2 p0 = conj_mul(a0, b0), p16 = conj_mul(a16, b16);
-            if constexpr (PHASE) {
-                const float w = cfg.wfilt[k1 + t * M], wm = cfg.wfilt[km + t * M];
-                p = phase_norm(p, w);
-                pm = phase_norm(pm, wm);
-                p0 = phase_norm(p0, cfg.wfilt[k1]);
-                p16 = phase_norm(p16, cfg.wfilt[k1 + 16 * M]);
+            constexpr int k = decltype(kk)::value;
+            constexpr int km = (32 - k) & 31;
+            // FA(-k1) for k1 = k + 32 m and for k1' = km + 32 m
+            float2 gam, gam2, gbm, gbm2;
+            if constexpr (k == 0) {
+                gam = neg0<L>(FA[0]);
+                gbm = neg0<L>(FB[0]);
+            } else {
+                gam = mirror<L>(FA[km]);
+                gbm = mirror<L>(FB[km]);
             }
-            // P0(km) = conj(P0(k1)), P16(km) = conj(P16(k1))
-            const float2 u1 = make_float2(p0.x - p16.y, p0.y + p16.x);
-            const float2 um = make_float2(p0.x + p16.y, -p0.y + p16.x);
-            P[k1] = unit0 ? u1 : p;
-            if constexpr (km != k1) P[km] = unit0 ? um : pm;
+            gam2 = mirror<L>(FA[k]);
+            gbm2 = mirror<L>(FB[k]);
+            auto spec = [&](float2 ga, float2 gm, float2 gb, float2 gbmm, int k1) {
+                float2 p = conj_mul(ga, gb);
+                const float2 a0 = make_float2(0.5f * (ga.x + gm.x), 0.5f * (ga.y - gm.y));
+                const float2 aH = make_float2(0.5f * (ga.y + gm.y), -0.5f * (ga.x - gm.x));
+                const float2 b0 = make_float2(0.5f * (gb.x + gbmm.x), 0.5f * (gb.y - gbmm.y));
+                const float2 bH = make_float2(0.5f * (gb.y + gbmm.y), -0.5f * (gb.x - gbmm.x));
+                float2 p0 = conj_mul(a0, b0), pH = conj_mul(aH, bH);
+                if constexpr (PHASE) {
+                    p = phase_norm(p, cfg.wfilt[k1 + u * M]);
+                    p0 = phase_norm(p0, cfg.wfilt[k1]);
+                    pH = phase_norm(pH, cfg.wfilt[k1 + H * M]);
+                }
+                return unit0 ? make_float2(p0.x - pH.y, p0.y + pH.x) : p;  // unit 0: P0 + i PH
+            };
+            P[k] = spec(FA[k], gam, FB[k], gbm, k + 32 * mblk);
+            if constexpr (km != k) P[km] = spec(FA[km], gam2, FB[km], gbm2, km + 32 * mblk);
         });
-        fft<32, true>(P);
+        line_inv<L>(P, l);  // P[n] = Q(L n + l, u)
     }
-    __syncthreads();  // every lane holds its column; the LDS slice is reused
+    __syncthreads();  // the LDS slice is reused for the kept rows
 
-    // keep the box rows: Q(l1, k2), row q <-> l1 = q - HW (mod 32)
-    float2 *Qs = reinterpret_cast<float2 *>(T);      // NB x 17
-    float *Rb = reinterpret_cast<float *>(T) + NB * 17 * 2;  // NB x NB, column-major
-    sfor<NB>([&](auto qq) {
-        constexpr int q = decltype(qq)::value;
-        constexpr int l1 = (32 - HW + q) & 31;
-        Qs[q * 17 + t] = P[l1];
+    float2 *Qs = reinterpret_cast<float2 *>(T);              // NB x U
+    float *Rb = reinterpret_cast<float *>(T) + NB * U * 2;    // NB x NB, column-major
+    sfor<32>([&](auto nn) {
+        constexpr int n = decltype(nn)::value;
+        const int l1 = L * n + l;
+        const int ip = (l1 + H) & (M - 1);
+        if (ip >= LO && ip < LO + NB) Qs[(ip - LO) * U + line] = P[n];
     });
     __syncthreads();
 
-    // ---- 4. complex-to-real inverse along dim 2, two box rows per lane ----
-    if (t < NPAIR) {
-        const int qa = 2 * t, qb = 2 * t + 1;
+    // ---- 4. complex-to-real inverse along dim 2, two box rows per line ----
+    if (t < NPAIR * L) {
+        const int pair = line;
+        const int qa = 2 * pair, qb = 2 * pair + 1;
         const bool hasb = qb < NB;
-        const float2 *Qa = Qs + qa * 17;
-        const float2 *Qb = Qs + (hasb ? qb : qa) * 17;
+        const float2 *Qa = Qs + qa * U;
+        const float2 *Qb = Qs + (hasb ? qb : qa) * U;
         const float zb = hasb ? 1.f : 0.f;
         float2 c[32];
-        {
-            const float2 a0 = Qa[0], b0 = Qb[0];  // (Q(.,0), Q(.,16)), real
-            c[0] = make_float2(a0.x, zb * b0.x);
-            c[16] = make_float2(a0.y, zb * b0.y);
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int k2 = k + 32 * mblk;
+            const int idx = k2 <= H ? k2 : M - k2;
+            const bool real0 = idx == 0 || idx == H;  // packed real columns
+            const float2 a = Qa[real0 ? 0 : idx], bq = Qb[real0 ? 0 : idx];
+            float2 cv;
+            if (real0) {
+                const float ar = idx == 0 ? a.x : a.y, br_ = idx == 0 ? bq.x : bq.y;
+                cv = make_float2(ar, zb * br_);
+            } else if (k2 < H) {
+                cv = make_float2(a.x - zb * bq.y, a.y + zb * bq.x);  // Qa + i Qb
+            } else {
+                cv = make_float2(a.x + zb * bq.y, -a.y + zb * bq.x);  // conj(Qa) + i conj(Qb)
+            }
+            c[k] = cv;
         }
-        sfor<15>([&](auto kk) {
-            constexpr int k2 = decltype(kk)::value + 1;
-            const float2 qa_ = Qa[k2];
-            const float2 qb_ = make_float2(zb * Qb[k2].x, zb * Qb[k2].y);
-            c[k2] = make_float2(qa_.x - qb_.y, qa_.y + qb_.x);           // Qa + i Qb
-            c[32 - k2] = make_float2(qa_.x + qb_.y, -qa_.y + qb_.x);     // conj(Qa) + i conj(Qb)
-        });
-        fft<32, true>(c);
-        const float scale = 1.0f / 1024.0f;
+        line_inv<L>(c, l);  // c[n] = r(qa) + i r(qb) at l2 = L n + l
+        constexpr float scale = 1.0f / (float)(M * M);
         const int ipa = LO + qa, ipb = LO + qb;
-        sfor<NB>([&](auto qq) {
-            constexpr int q2 = decltype(qq)::value;
-            constexpr int l2 = (32 - HW + q2) & 31;
-            constexpr int jp = LO + q2;
-            Rb[qa + q2 * NB] = fabsf(c[l2].x * scale) * cfg.gain[ipa + jp * M];
-            if (hasb) Rb[qb + q2 * NB] = fabsf(c[l2].y * scale) * cfg.gain[ipb + jp * M];
+        sfor<32>([&](auto nn) {
+            constexpr int n = decltype(nn)::value;
+            const int l2 = L * n + l;
+            const int jp = (l2 + H) & (M - 1);
+            if (jp >= LO && jp < LO + NB) {
+                const int q2 = jp - LO;
+                Rb[qa + q2 * NB] = fabsf(c[n].x * scale) * cfg.gain[ipa + jp * M];
+                if (hasb) Rb[qb + q2 * NB] = fabsf(c[n].y * scale) * cfg.gain[ipb + jp * M];
+            }
         });
     }
     __syncthreads();
 
     // ---- 5. regional-max top-K over the box ----
-    const BoxView<NB, LO> R{Rb};
     float tv[K];
     int tl[K];
 #pragma unroll
@@ -433,7 +642,7 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
         tv[m] = 0.f;
         tl[m] = INT_MAX;
     }
-    for (int p = t; p < NB * NB; p += 16) {
+    for (int p = t; p < NB * NB; p += G) {
         const int bi = p % NB, bj = p / NB;
         const float I0 = Rb[p];
         if (!(I0 > 0.f)) continue;  // zero-valued box points are never maxima
@@ -449,23 +658,20 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
                 ok &= earlier ? !(In > I0) : !(In >= I0);
             }
         if (!ok) continue;
-        const int lin = (LO + bi) + (LO + bj) * M;
-        // insert (strictly better moves up; scan order is increasing lin)
         float cv = I0;
-        int cl = lin;
+        int cl = (LO + bi) + (LO + bj) * M;
 #pragma unroll
         for (int m = 0; m < K; ++m) {
             if (better(cv, cl, tv[m], tl[m])) {
-                const float sv = tv[m];
-                const int sl = tl[m];
+                const float sv_ = tv[m];
+                const int sl_ = tl[m];
                 tv[m] = cv;
                 tl[m] = cl;
-                cv = sv;
-                cl = sl;
+                cv = sv_;
+                cl = sl_;
             }
         }
     }
-    // merge the 16 lanes' lists: K rounds of group argmax
     float sv[K];
     int sl[K];
     int nsel = 0;
@@ -473,8 +679,9 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
     for (int m = 0; m < K; ++m) {
         float bv = tv[0];
         int bl = tl[0];
+        constexpr int XMAX = G >= 64 ? 32 : G / 2;
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
+        for (int o = XMAX; o > 0; o >>= 1) {
             const float ov = __shfl_xor(bv, o);
             const int ol = __shfl_xor(bl, o);
             if (better(ov, ol, bv, bl)) {
@@ -482,10 +689,24 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
                 bl = ol;
             }
         }
+        if constexpr (NWAVE > 1) {
+            if ((tid & 63) == 0) {
+                mv[wave] = bv;
+                ml[wave] = bl;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int w2 = 0; w2 < NWAVE; ++w2)
+                if (better(mv[w2], ml[w2], bv, bl)) {
+                    bv = mv[w2];
+                    bl = ml[w2];
+                }
+            __syncthreads();
+        }
         sv[m] = bv;
         sl[m] = bl;
         nsel += bl != INT_MAX;
-        if (tl[0] == bl && bl != INT_MAX) {  // pop the winner's head
+        if (tl[0] == bl && bl != INT_MAX) {
 #pragma unroll
             for (int q = 0; q + 1 < K; ++q) {
                 tv[q] = tv[q + 1];
@@ -509,6 +730,7 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
     for (int m = 1; m < K; ++m)
         if (found == m && m < nsel && sv[m] > 0.f) found = m + 1;
 
+    const BoxView<NB, LO> R{Rb};
     const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
     const size_t ng = (size_t)out.ny * out.nx;
     if (out.found) out.found[jb] = found;
@@ -523,16 +745,16 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
     const int pr = sl[0] % M + 1, pc = sl[0] / M + 1;
     float2 ref;
     if (cfg.subpixel == HH_GAUSS9)
-        ref = gauss9_b(R, pr, pc);
+        ref = gauss9_b<M>(R, pr, pc);
     else if (cfg.subpixel == HH_SUBPIX_NONE)
         ref = make_float2((float)pr, (float)pc);
     else
-        ref = gauss3_b(R, pr, pc);
-    const float cc = (float)(M / 2 + 1);
+        ref = gauss3_b<M>(R, pr, pc);
+    const float cc = (float)(H + 1);
     const float du = ref.y - cc, dv = ref.x - cc;
     const float v1 = sv[0];
     const float ratio = found >= 2 ? v1 / sv[1] : (v1 > 0.f ? INFINITY : NAN);
-    const float mom = moment_b(R, ref.x, ref.y);
+    const float mom = moment_b<M>(R, ref.x, ref.y);
     const float u0 = out.u0 ? out.u0[g] : 0.f, v0 = out.v0 ? out.v0[g] : 0.f;
     out.u[g] = out.u0 ? u0 + du : du;
     out.v[g] = out.v0 ? v0 + dv : dv;
@@ -544,7 +766,7 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
             if (m > cfg.n_peaks) continue;
             float au = NAN, av = NAN;
             if (m <= found) {
-                const float2 a = gauss3_b(R, sl[m - 1] % M + 1, sl[m - 1] / M + 1);
+                const float2 a = gauss3_b<M>(R, sl[m - 1] % M + 1, sl[m - 1] / M + 1);
                 au = a.y - cc;
                 av = a.x - cc;
                 if (out.u0) {
@@ -560,24 +782,45 @@ __global__ __launch_bounds__(64) void k_corr32(CorrConfig cfg, const float *__re
     }
 }
 
-template <int K, bool PHASE, int HW>
-void launch32(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
-              const CorrOut &out, hipStream_t s)
+template <int L, int K, bool PHASE, int HW>
+void launchL(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+             const CorrOut &out, hipStream_t s)
 {
-    const int nblocks = (njobs + 3) / 4;
-    hipLaunchKernelGGL((k_corr32<K, PHASE, HW>), dim3(nblocks), dim3(64), 0, s, cfg, A, B, nr, jobs, njobs, nblocks,
-                       out);
+    constexpr int G = 16 * L * L;
+    constexpr int WPB = G >= 64 ? 1 : 64 / G;
+    constexpr int NT = G >= 64 ? G : 64;
+    const int nblocks = (njobs + WPB - 1) / WPB;
+    hipLaunchKernelGGL((k_corrL<L, K, PHASE, HW>), dim3(nblocks), dim3(NT), 0, s, cfg, A, B, nr, jobs, njobs,
+                       nblocks, out);
 }
 
-template <int K, bool PHASE>
-bool launch32_hw(const CorrConfig &cfg, int hw, const float *A, const float *B, int nr, const int32_t *jobs,
-                 int njobs, const CorrOut &out, hipStream_t s)
+template <int L, int K, bool PHASE>
+bool launch_hw(const CorrConfig &cfg, int hw, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+               const CorrOut &out, hipStream_t s)
 {
-    switch (hw) {
-    case 7: launch32<K, PHASE, 7>(cfg, A, B, nr, jobs, njobs, out, s); return true;
-    case 9: launch32<K, PHASE, 9>(cfg, A, B, nr, jobs, njobs, out, s); return true;
-    default: return false;
+    // half-widths of the Gaussian and of the uniform apodization support
+    constexpr int HG = L == 1 ? 7 : (L == 2 ? 13 : 25);
+    constexpr int HN = L == 1 ? 9 : (L == 2 ? 17 : 33);
+    if (hw == HG) {
+        launchL<L, K, PHASE, HG>(cfg, A, B, nr, jobs, njobs, out, s);
+        return true;
     }
+    if (hw == HN) {
+        launchL<L, K, PHASE, HN>(cfg, A, B, nr, jobs, njobs, out, s);
+        return true;
+    }
+    return false;
+}
+
+template <int L>
+bool launch_L(const CorrConfig &cfg, int hw, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+              const CorrOut &out, hipStream_t s)
+{
+    if (cfg.method == HH_PHASE)
+        return cfg.K == 2 ? launch_hw<L, 2, true>(cfg, hw, A, B, nr, jobs, njobs, out, s)
+                          : launch_hw<L, 3, true>(cfg, hw, A, B, nr, jobs, njobs, out, s);
+    return cfg.K == 2 ? launch_hw<L, 2, false>(cfg, hw, A, B, nr, jobs, njobs, out, s)
+                      : launch_hw<L, 3, false>(cfg, hw, A, B, nr, jobs, njobs, out, s);
 }
 
 }  // namespace
@@ -585,20 +828,19 @@ bool launch32_hw(const CorrConfig &cfg, int hw, const float *A, const float *B, 
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s)
 {
-    if (cfg.M != 32 || cfg.w != 16 || !cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3 ||
-        out.planes)
-        return false;
+    if (!cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3 || out.planes) return false;
+    if (!(cfg.M == 32 || cfg.M == 64 || cfg.M == 128) || cfg.w * 2 != cfg.M) return false;
     // the box must be symmetric about the zero lag with a zero margin
-    const int hw = 16 - cfg.box_lo;
-    if (cfg.box_hi != 16 + hw || cfg.box_lo < 1 || cfg.box_hi > 29) return false;
+    const int H = cfg.M / 2;
+    const int hw = H - cfg.box_lo;
+    if (cfg.box_hi != H + hw || cfg.box_lo < 1 || cfg.box_hi > cfg.M - 3) return false;
     if (njobs <= 0) return true;
-    bool ok;
-    if (cfg.method == HH_PHASE)
-        ok = cfg.K == 2 ? launch32_hw<2, true>(cfg, hw, A, B, nr, jobs, njobs, out, s)
-                        : launch32_hw<3, true>(cfg, hw, A, B, nr, jobs, njobs, out, s);
-    else
-        ok = cfg.K == 2 ? launch32_hw<2, false>(cfg, hw, A, B, nr, jobs, njobs, out, s)
-                        : launch32_hw<3, false>(cfg, hw, A, B, nr, jobs, njobs, out, s);
+    bool ok = false;
+    switch (cfg.M) {
+    case 32: ok = launch_L<1>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
+    case 64: ok = launch_L<2>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
+    case 128: ok = launch_L<4>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
+    }
     if (ok) HH_HIP(hipGetLastError());
     return ok;
 }
