@@ -135,7 +135,7 @@ def main(argv=None):
     log(f"timed {args.steps} steps in {elapsed:.3f} s")
 
     kernels = {}
-    for fam in ("prefilter", "deform", "correlate", "validate", "uq"):
+    for fam in ("prefilter", "deform", "correlate", "correlate_uq", "validate", "uq"):
         for w in (0, 16, 32, 64):
             ms, n = ctx.kernel_time(fam, w)
             if n:

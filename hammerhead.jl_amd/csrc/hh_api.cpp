@@ -568,6 +568,7 @@ void run_schedule(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         float *mu = keep_measured ? c->gmu.ensure(ng) : nullptr;
         float *mv = keep_measured ? c->gmv.ensure(ng) : nullptr;
         bool have_prev = false;
+        bool uq_fused = false;
         fill(c, uu, ng, NAN);
         fill(c, uv, ng, NAN);
         const float *srcA = c->A, *srcB = c->B;
@@ -598,8 +599,17 @@ void run_schedule(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
             out.v0 = v0;
             out.ny = ny;
             out.nx = nx;
-            timed(c, "correlate", cfg.w,
-                  [&] { launch_correlate(cfg, srcA, srcB, c->nr, c->nc, djobs, (int)ng, out, c->stream); });
+            // the uncertainty of an iterating pass is taken once on the final
+            // sweep's warp; with a fixed sweep count that sweep is known in
+            // advance and the statistics are fused into its correlation
+            if (unc && it == maxiter && (maxiter == 1 || p.convergence_tol == 0)) {
+                out.unc_u = uu;
+                out.unc_v = uv;
+            }
+            timed(c, out.unc_u ? "correlate_uq" : "correlate", cfg.w, [&] {
+                uq_fused = launch_correlate(cfg, srcA, srcB, c->nr, c->nc, djobs, (int)ng, out, c->stream) &&
+                           out.unc_u != nullptr;
+            });
             if (keep_measured) {
                 HH_HIP(hipMemcpyAsync(mu, u, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
                 HH_HIP(hipMemcpyAsync(mv, v, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
@@ -646,7 +656,7 @@ void run_schedule(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
             set_predictor_from_grid(c, g, u, v, smoothing);
             have_pred = true;
         }
-        if (unc) {
+        if (unc && !uq_fused) {
             const ConfigBufs &cb = get_config(c, p);
             timed(c, "uq", cfg.w, [&] {
                 launch_uq(cfg.w, cb.apod.p, srcA, srcB, c->nr, djobs, (int)ng, ny, uu, uv, nullptr, c->stream);
@@ -903,11 +913,17 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
         if (found) out.found = upload<int32_t>(c, c->hki[1], nullptr, (size_t)njobs);
         out.ny = ny;
         out.nx = nx;
-        timed(c, "correlate", w, [&] { launch_correlate(cfg, dA, dB, c->nr, c->nc, dj, njobs, out, c->stream); });
         float *duu = nullptr, *duv = nullptr;
         if (unc_u || unc_v) {
             duu = upload(c, c->hk[7], unc_u, ng);
             duv = upload(c, c->hk[8], unc_v, ng);
+            out.unc_u = duu;
+            out.unc_v = duv;
+        }
+        bool fused = false;
+        timed(c, out.unc_u ? "correlate_uq" : "correlate", w,
+              [&] { fused = launch_correlate(cfg, dA, dB, c->nr, c->nc, dj, njobs, out, c->stream); });
+        if ((unc_u || unc_v) && !fused) {
             const ConfigBufs &cb = get_config(c, *pass);
             timed(c, "uq", w, [&] {
                 launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr,

@@ -61,16 +61,19 @@ struct CorrOut {
     float *vals;                     // K*njobs
     int32_t *found;                  // njobs
     float *planes;                   // M*M*njobs
+    float *unc_u, *unc_v;            // fused Wieneke uncertainty (nullable)
     int ny, nx;
 };
 
 // ---- launchers (k_*.hip) ----------------------------------------------------
-void launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
+// Returns true when out.unc_u/unc_v were filled by the same launch (fused
+// uncertainty); otherwise the caller runs launch_uq.
+bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
                       int njobs, const CorrOut &out, hipStream_t s);
 
 // Specialized kernels; return false when the configuration is not covered.
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
-                           int njobs, const CorrOut &out, hipStream_t s);
+                           int njobs, const CorrOut &out, hipStream_t s, bool *uq_done);
 
 void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp, const float *cp, int ncp,
                       hipStream_t s);

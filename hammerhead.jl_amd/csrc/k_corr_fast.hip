@@ -301,6 +301,210 @@ __device__ __forceinline__ void line_inv(float2 (&X)[32], int l)
     fft<32, true>(X);
 }
 
+// ---- fused Wieneke statistics for 16-px windows (one row per lane) -------
+// DPP row operations act within rows of 16 lanes: exactly one window.
+template <int S>
+__device__ __forceinline__ int rowsh_i(int v)  // lane r gets lane r + S of its row, 0 outside
+{
+    if constexpr (S == 0) return v;
+    else if constexpr (S > 0) return __builtin_amdgcn_update_dpp(0, v, 0x100 + S, 0xF, 0xF, false);
+    else return __builtin_amdgcn_update_dpp(0, v, 0x110 - S, 0xF, 0xF, false);
+}
+template <int S>
+__device__ __forceinline__ float rowsh(float v)
+{
+    return __int_as_float(rowsh_i<S>(__float_as_int(v)));
+}
+template <int S>
+__device__ __forceinline__ double rowsh_d(double v)
+{
+    const int lo = rowsh_i<S>(__double2loint(v)), hi = rowsh_i<S>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+template <int R>
+__device__ __forceinline__ double rowror_d(double v)  // rotate within the row by R
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x120 + R, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x120 + R, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row_sum_d(double v)  // sum over the 16 lanes, in every lane
+{
+    v += rowror_d<8>(v);
+    v += rowror_d<4>(v);
+    v += rowror_d<2>(v);
+    v += rowror_d<1>(v);
+    return v;
+}
+
+// Accumulators of one component: C0, C+, C-, S00 and per UQ ring
+// (max(|dr|, |dc|) = 1..4) the maximum and the sum of its covariance sums:
+// everything finalize_uncertainty (src/uncertainty.jl:162-179) reads.
+struct UQAcc {
+    double C0, Cp, Cm, S00;
+    double rmax[4], rsum[4];
+};
+
+__device__ __forceinline__ void uq_add(UQAcc &a, int dr, int dc, double s)
+{
+    if (dr == 0 && dc == 0) {
+        a.S00 = s;
+        return;
+    }
+    const int ring = (dr < 0 ? -dr : dr) > dc ? (dr < 0 ? -dr : dr) : dc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (ring == q + 1) {
+            a.rmax[q] = s > a.rmax[q] ? s : a.rmax[q];
+            a.rsum[q] += s;
+        }
+}
+
+__device__ __forceinline__ float uq_sigma(const UQAcc &a)
+{
+    double s2 = a.S00;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (a.rmax[q] < 0.05 * a.S00) break;
+        s2 += 2.0 * a.rsum[q];
+    }
+    const double sdc = sqrt(s2 > 0.0 ? s2 : 0.0);
+    const double Cpm = (a.Cp + a.Cm) / 2.0;
+    const double lo = Cpm - sdc / 2.0, hi = Cpm + sdc / 2.0;
+    if (!(a.C0 > 0.0 && lo > 0.0)) return NAN;
+    const double denom = 2.0 * log(a.C0) - log(hi) - log(lo);
+    if (!(denom > 0.0)) return NAN;
+    return (float)((log(hi) - log(lo)) / (2.0 * denom));
+}
+
+__device__ __forceinline__ void uq_init(UQAcc &a)
+{
+    a.C0 = a.Cp = a.Cm = a.S00 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        a.rmax[q] = -INFINITY;
+        a.rsum[q] = 0.0;
+    }
+}
+
+// uq_component! (src/uncertainty.jl:104-151) of both components of a 16-px
+// window whose apodized, mean-subtracted rows sit one per lane (x[j] =
+// (wA[r, j], wB[r, j]), r = lane of the row). u: shift along columns
+// (in-lane); v: the transposed windows, shift along rows (across lanes).
+__device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &sig_v)
+{
+#pragma clang fp contract(off)
+    const float q = 0.25f;
+    // ---- u: d(r, c), c = 0..14 in registers; lags dr across lanes ----
+    {
+        UQAcc acc;
+        uq_init(acc);
+        float dC[15];
+        double c0 = 0.0, cp = 0.0, cm = 0.0;
+#pragma unroll
+        for (int c = 0; c < 15; ++c) {
+            const float a0 = x[c].x, a1 = x[c + 1].x, b0 = x[c].y, b1 = x[c + 1].y;
+            const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
+            c0 += (double)p00;
+            cp += (double)p01;
+            cm += (double)p10;
+            dC[c] = p01 - p10;
+        }
+        double d[15];
+        double mu = 0.0;
+#pragma unroll
+        for (int c = 0; c < 15; ++c) {
+            const int cl = c > 0 ? c - 1 : 0, cr = c < 14 ? c + 1 : 14;
+            const float v = q * ((dC[cl] + 2.f * dC[c]) + dC[cr]);
+            d[c] = (double)v;
+            mu += (double)v;
+        }
+        acc.C0 = row_sum_d(c0);
+        acc.Cp = row_sum_d(cp);
+        acc.Cm = row_sum_d(cm);
+        mu = row_sum_d(mu) / 240.0;
+#pragma unroll
+        for (int c = 0; c < 15; ++c) d[c] -= mu;
+        // lag dr across lanes: a runtime loop keeps one shifted row live
+#pragma unroll 1
+        for (int dr = -4; dr <= 4; ++dr) {
+            const bool ok = r + dr >= 0 && r + dr < 16;
+            double sh[15];
+#pragma unroll
+            for (int c = 0; c < 15; ++c) {
+                const double v = __shfl(d[c], r + dr, 16);
+                sh[c] = ok ? v : 0.0;
+            }
+#pragma unroll
+            for (int dc = 0; dc < 5; ++dc) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c + dc < 15; ++c) s += d[c] * sh[c + dc];
+                s = row_sum_d(s);
+                if (dc > 0 || dr >= 0) uq_add(acc, dr, dc, s);
+            }
+        }
+        sig_u = uq_sigma(acc);
+    }
+    // ---- v: transposed windows: d(j; i), i = row = lane 0..14, j = 0..15 ----
+    {
+        UQAcc acc;
+        uq_init(acc);
+        const bool live = r < 15;
+        float dC[16];
+        double c0 = 0.0, cp = 0.0, cm = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const float a0 = x[j].x, b0 = x[j].y;
+            const float a1 = rowsh<1>(x[j].x), b1 = rowsh<1>(x[j].y);
+            const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
+            if (live) {
+                c0 += (double)p00;
+                cp += (double)p01;
+                cm += (double)p10;
+            }
+            dC[j] = live ? p01 - p10 : 0.f;
+        }
+        double d[16];
+        double mu = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const float up = rowsh<-1>(dC[j]), dn = rowsh<1>(dC[j]);
+            const float lft = r > 0 ? up : dC[j];
+            const float rgt = r < 14 ? dn : dC[j];
+            const float v = q * ((lft + 2.f * dC[j]) + rgt);
+            d[j] = live ? (double)v : 0.0;
+            mu += d[j];
+        }
+        acc.C0 = row_sum_d(c0);
+        acc.Cp = row_sum_d(cp);
+        acc.Cm = row_sum_d(cm);
+        mu = row_sum_d(mu) / 240.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) d[j] = live ? d[j] - mu : 0.0;
+#pragma unroll 1
+        for (int dc = 0; dc < 5; ++dc) {
+            const bool ok = r + dc < 16;
+            double sh[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const double v = __shfl(d[j], r + dc, 16);
+                sh[j] = ok ? v : 0.0;
+            }
+#pragma unroll
+            for (int dr = -4; dr <= 4; ++dr) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j + dr >= 0 && j + dr < 16) s += d[j] * sh[j + dr];
+                s = row_sum_d(s);
+                if (dc > 0 || dr >= 0) uq_add(acc, dr, dc, s);
+            }
+        }
+        sig_v = uq_sigma(acc);
+    }
+}
+
 __device__ __forceinline__ bool better(float av, int al, float bv, int bl)
 {
     if (al == INT_MAX) return false;
@@ -393,7 +597,7 @@ __device__ float moment_b(const V &R, float pr, float pc)
 // Correlation of w = 16 L windows padded to M = 32 L. L lanes per line of the
 // 2D transform, G = 16 L^2 lanes per window. HW: half-width of the analysis
 // box [M/2 - HW, M/2 + HW]^2, which holds the gain support plus a zero margin.
-template <int L, int K, bool PHASE, int HW>
+template <int L, int K, bool PHASE, int HW, bool UQ>
 __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, const float *__restrict__ A,
                                                            const float *__restrict__ B, int nr,
                                                            const int32_t *__restrict__ jobs, int njobs, int nblocks,
@@ -508,6 +712,11 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
             __syncthreads();  // staging is overwritten by the transpose below
         }
     }
+
+    // fused uncertainty: the final sweep's windows are the ones the
+    // post-loop uncertainty_sweep! reads (src/pipeline.jl:810-824)
+    float sig_u = NAN, sig_v = NAN;
+    if constexpr (UQ && L == 1) uq_fused16(x, line, sig_u, sig_v);
 
     // ---- 2. row DFT (dim 2) and Hermitian split, k2 = 0..H ----
     const int mblk = block_of<L>(l);
@@ -760,6 +969,10 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     out.v[g] = out.v0 ? v0 + dv : dv;
     out.pr[g] = ratio;
     out.cm[g] = mom;
+    if constexpr (UQ) {
+        out.unc_u[g] = sig_u;
+        out.unc_v[g] = sig_v;
+    }
     if (out.alt_u) {
 #pragma unroll
         for (int m = 2; m <= K; ++m) {
@@ -782,7 +995,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     }
 }
 
-template <int L, int K, bool PHASE, int HW>
+template <int L, int K, bool PHASE, int HW, bool UQ>
 void launchL(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
              const CorrOut &out, hipStream_t s)
 {
@@ -790,11 +1003,11 @@ void launchL(const CorrConfig &cfg, const float *A, const float *B, int nr, cons
     constexpr int WPB = G >= 64 ? 1 : 64 / G;
     constexpr int NT = G >= 64 ? G : 64;
     const int nblocks = (njobs + WPB - 1) / WPB;
-    hipLaunchKernelGGL((k_corrL<L, K, PHASE, HW>), dim3(nblocks), dim3(NT), 0, s, cfg, A, B, nr, jobs, njobs,
+    hipLaunchKernelGGL((k_corrL<L, K, PHASE, HW, UQ>), dim3(nblocks), dim3(NT), 0, s, cfg, A, B, nr, jobs, njobs,
                        nblocks, out);
 }
 
-template <int L, int K, bool PHASE>
+template <int L, int K, bool PHASE, bool UQ>
 bool launch_hw(const CorrConfig &cfg, int hw, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
                const CorrOut &out, hipStream_t s)
 {
@@ -802,46 +1015,57 @@ bool launch_hw(const CorrConfig &cfg, int hw, const float *A, const float *B, in
     constexpr int HG = L == 1 ? 7 : (L == 2 ? 13 : 25);
     constexpr int HN = L == 1 ? 9 : (L == 2 ? 17 : 33);
     if (hw == HG) {
-        launchL<L, K, PHASE, HG>(cfg, A, B, nr, jobs, njobs, out, s);
+        launchL<L, K, PHASE, HG, UQ>(cfg, A, B, nr, jobs, njobs, out, s);
         return true;
     }
     if (hw == HN) {
-        launchL<L, K, PHASE, HN>(cfg, A, B, nr, jobs, njobs, out, s);
+        launchL<L, K, PHASE, HN, UQ>(cfg, A, B, nr, jobs, njobs, out, s);
         return true;
     }
     return false;
 }
 
-template <int L>
+template <int L, bool UQ>
 bool launch_L(const CorrConfig &cfg, int hw, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
               const CorrOut &out, hipStream_t s)
 {
     if (cfg.method == HH_PHASE)
-        return cfg.K == 2 ? launch_hw<L, 2, true>(cfg, hw, A, B, nr, jobs, njobs, out, s)
-                          : launch_hw<L, 3, true>(cfg, hw, A, B, nr, jobs, njobs, out, s);
-    return cfg.K == 2 ? launch_hw<L, 2, false>(cfg, hw, A, B, nr, jobs, njobs, out, s)
-                      : launch_hw<L, 3, false>(cfg, hw, A, B, nr, jobs, njobs, out, s);
+        return cfg.K == 2 ? launch_hw<L, 2, true, UQ>(cfg, hw, A, B, nr, jobs, njobs, out, s)
+                          : launch_hw<L, 3, true, UQ>(cfg, hw, A, B, nr, jobs, njobs, out, s);
+    return cfg.K == 2 ? launch_hw<L, 2, false, UQ>(cfg, hw, A, B, nr, jobs, njobs, out, s)
+                      : launch_hw<L, 3, false, UQ>(cfg, hw, A, B, nr, jobs, njobs, out, s);
 }
 
 }  // namespace
 
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
-                           int njobs, const CorrOut &out, hipStream_t s)
+                           int njobs, const CorrOut &out, hipStream_t s, bool *uq_done)
 {
+    *uq_done = false;
     if (!cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3 || out.planes) return false;
     if (!(cfg.M == 32 || cfg.M == 64 || cfg.M == 128) || cfg.w * 2 != cfg.M) return false;
     // the box must be symmetric about the zero lag with a zero margin
     const int H = cfg.M / 2;
     const int hw = H - cfg.box_lo;
     if (cfg.box_hi != H + hw || cfg.box_lo < 1 || cfg.box_hi > cfg.M - 3) return false;
-    if (njobs <= 0) return true;
+    const bool uq = out.unc_u != nullptr && out.unc_v != nullptr && cfg.M == 32;
+    if (njobs <= 0) {
+        *uq_done = uq;
+        return true;
+    }
     bool ok = false;
     switch (cfg.M) {
-    case 32: ok = launch_L<1>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
-    case 64: ok = launch_L<2>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
-    case 128: ok = launch_L<4>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
+    case 32:
+        ok = uq ? launch_L<1, true>(cfg, hw, A, B, nr, jobs, njobs, out, s)
+                : launch_L<1, false>(cfg, hw, A, B, nr, jobs, njobs, out, s);
+        break;
+    case 64: ok = launch_L<2, false>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
+    case 128: ok = launch_L<4, false>(cfg, hw, A, B, nr, jobs, njobs, out, s); break;
     }
-    if (ok) HH_HIP(hipGetLastError());
+    if (ok) {
+        HH_HIP(hipGetLastError());
+        *uq_done = uq;
+    }
     return ok;
 }
 

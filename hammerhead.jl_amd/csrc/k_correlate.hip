@@ -482,12 +482,13 @@ static float2 *twiddle_table(int M, hipStream_t s)
     return tabs[M];
 }
 
-void launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
+bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
                       int njobs, const CorrOut &out, hipStream_t s)
 {
     (void)nc;
-    if (njobs <= 0) return;
-    if (launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s)) return;
+    if (njobs <= 0) return out.unc_u != nullptr;
+    bool uq_done = false;
+    if (launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
     const float2 *tw = twiddle_table(cfg.M, s);
     switch (cfg.M) {
     case 16: hipLaunchKernelGGL(k_corr_lds<16>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
@@ -497,6 +498,7 @@ void launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     default: throw Error(HH_EUNSUPPORTED, "FFT size must be 16, 32, 64 or 128");
     }
     HH_HIP(hipGetLastError());
+    return false;
 }
 
 }  // namespace hh

@@ -123,6 +123,85 @@ __device__ void uq_component(double *st, const float *wA, const float *wB, int n
     __syncthreads();
 }
 
+// 16-px windows (a 16 x 15 field per component): lane i takes column pair
+// (c, c + dc) number i of the 65 pairs with dc = 0..4, holds both columns
+// in registers and forms the 9 row-lag products (dr = -4..4; 0..4 for
+// dc = 0) without memory traffic; the per-pair partial sums are then added
+// over c for each of the 41 (dr, dc) statistics. Same sums as the generic
+// path, in a different Float64 addition order.
+__device__ void uq_component16(double *st, const float *wA, const float *wB, int rs, int cs, float *dC, float *dCs,
+                               double *d, double *part, const Offsets &off)
+{
+    constexpr int nr = 16, m = 15, ne = nr * m;
+    const int lane = threadIdx.x & 63;
+    double C0 = 0.0, Cp = 0.0, Cm = 0.0;
+    for (int e = lane; e < ne; e += 64) {
+        const int r = e % nr, c = e / nr;
+        const float a0 = wA[r * rs + c * cs], a1 = wA[r * rs + (c + 1) * cs];
+        const float b0 = wB[r * rs + c * cs], b1 = wB[r * rs + (c + 1) * cs];
+        const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
+        C0 += (double)p00;
+        Cp += (double)p01;
+        Cm += (double)p10;
+        dC[e] = p01 - p10;
+    }
+    __syncthreads();
+    const float q = 0.25f;
+    double mu = 0.0;
+    for (int e = lane; e < ne; e += 64) {
+        const int r = e % nr, c = e / nr;
+        const int cl = c > 0 ? c - 1 : 0, cr = c < m - 1 ? c + 1 : m - 1;
+        const float v = q * ((dC[r + cl * nr] + 2.f * dC[r + c * nr]) + dC[r + cr * nr]);
+        dCs[e] = v;
+        mu += (double)v;
+    }
+    C0 = wave_sum(C0);
+    Cp = wave_sum(Cp);
+    Cm = wave_sum(Cm);
+    mu = wave_sum(mu) / (double)ne;
+    __syncthreads();
+    for (int e = lane; e < ne; e += 64) d[e] = (double)dCs[e] - mu;
+    __syncthreads();
+    // column pairs: dc = 0..4 has 15 - dc pairs (65 in all)
+    for (int item = lane; item < 65; item += 64) {
+        int dc = 0, c = item;
+        while (c >= m - dc) {
+            c -= m - dc;
+            ++dc;
+        }
+        double a[nr], bb[nr];
+#pragma unroll
+        for (int r = 0; r < nr; ++r) {
+            a[r] = d[r + c * nr];
+            bb[r] = d[r + (c + dc) * nr];
+        }
+#pragma unroll
+        for (int dr = -4; dr <= 4; ++dr) {
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < nr; ++r)
+                if (r + dr >= 0 && r + dr < nr) sum += a[r] * bb[r + dr];
+            part[item * 9 + dr + 4] = sum;
+        }
+    }
+    __syncthreads();
+    double S = 0.0;
+    if (lane <= kNoff) {
+        const int dr = lane == 0 ? 0 : off.dr[lane - 1];
+        const int dc = lane == 0 ? 0 : off.dc[lane - 1];
+        int base = 0;
+        for (int k = 0; k < dc; ++k) base += m - k;
+        for (int c = 0; c < m - dc; ++c) S += part[(base + c) * 9 + dr + 4];
+    }
+    if (lane == 0) {
+        st[0] += C0;
+        st[1] += Cp;
+        st[2] += Cm;
+    }
+    if (lane <= kNoff) st[3 + lane] += S;
+    __syncthreads();
+}
+
 __device__ float uq_finalize(const double *s, const Offsets &off)
 {
     const double C0 = s[0], Cp = s[1], Cm = s[2], S00 = s[3];
@@ -178,8 +257,19 @@ __global__ __launch_bounds__(64) void k_uq(int w, const float *__restrict__ apod
     }
     __syncthreads();
     // u: shift along columns (element (r, c) at r + c*w); v: transposed views
-    uq_component(st, wA, wB, w, w, 1, w, dC, dCs, dd, off);
-    uq_component(st + HH_UQ_NSTATS, wA, wB, w, w, w, 1, dC, dCs, dd, off);
+    if constexpr (W == 16) {
+                __shared__ double part[65 * 9];
+        if (w == 16) {
+            uq_component16(st, wA, wB, 1, 16, dC, dCs, dd, part, off);
+            uq_component16(st + HH_UQ_NSTATS, wA, wB, 16, 1, dC, dCs, dd, part, off);
+        } else {
+            uq_component(st, wA, wB, w, w, 1, w, dC, dCs, dd, off);
+            uq_component(st + HH_UQ_NSTATS, wA, wB, w, w, w, 1, dC, dCs, dd, off);
+        }
+    } else {
+        uq_component(st, wA, wB, w, w, 1, w, dC, dCs, dd, off);
+        uq_component(st + HH_UQ_NSTATS, wA, wB, w, w, w, 1, dC, dCs, dd, off);
+    }
     if (lane == 0) {
         const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * ny;
         if (unc_u) unc_u[g] = uq_finalize(st, off);

@@ -198,6 +198,14 @@ def test_uq_vs_oracle(hh, orc, ctx):
         relv = np.abs(gv[fin] - ov[fin]) / np.maximum(np.abs(ov[fin]), 1e-6)
         say("uq max rel", rel.max(), relv.max())
         assert rel.max() < 0.01 and relv.max() < 0.01
+        # the same statistics from the correlation launch (fused for 16-px windows)
+        r = ctx.process_windows(p, jobs, y.size, x.size, src=hh.HH_SRC_RAW, uncertainty=True)
+        fu, fv = r["unc_u"], r["unc_v"]
+        assert np.array_equal(np.isfinite(fu), np.isfinite(ou))
+        relf = np.abs(fu[fin] - ou[fin]) / np.maximum(np.abs(ou[fin]), 1e-6)
+        relfv = np.abs(fv[fin] - ov[fin]) / np.maximum(np.abs(ov[fin]), 1e-6)
+        say("uq fused max rel", relf.max(), relfv.max())
+        assert relf.max() < 0.01 and relfv.max() < 0.01
 
 
 def run_both(hh, orc, ctx, A, B, passes, prec="f64"):
