@@ -25,8 +25,9 @@ EXPORTS = (
     "hh_create", "hh_destroy", "hh_last_error", "hh_version", "hh_set_form", "hh_check_passes", "hh_grid_size",
     "hh_set_pair", "hh_set_pair_device", "hh_prefilter", "hh_deform", "hh_get_images", "hh_get_coefs",
     "hh_process_windows", "hh_correlation_planes", "hh_uncertainty_sweep", "hh_validate_and_replace",
-    "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_stream", "hh_synchronize", "hh_set_timing",
-    "hh_kernel_time",
+    "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
+    "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_stream", "hh_synchronize",
+    "hh_set_timing", "hh_kernel_time",
 )
 
 
@@ -78,6 +79,11 @@ def load():
         "hh_smooth_field": (i32, [vp, i32, i32, vp, vp]),
         "hh_run_piv": (i32, [vp, pp, i32, i32, C.POINTER(hh_field)]),
         "hh_fetch_result": (i32, [vp, C.POINTER(hh_field)]),
+        "hh_ensemble_begin": (i32, [vp, pp, vp, i32, i32]),
+        "hh_ensemble_add": (i32, [vp, i32]),
+        "hh_ensemble_analyze": (i32, [vp, i32, i32] + [vp] * 8),
+        "hh_ensemble_planes": (i32, [vp, vp]),
+        "hh_run_piv_ensemble": (i32, [vp, vp, vp, i32, i32, i32, pp, i32, i32, C.POINTER(hh_field)]),
         "hh_stream": (vp, [vp]),
         "hh_synchronize": (i32, [vp]),
         "hh_set_timing": (i32, [vp, i32]),
@@ -275,6 +281,86 @@ class Context:
         f = hh_field(ny, nx, *(_ptr(res[k]) for k in ("x", "y", "u", "v", "peak_ratio", "correlation_moment",
                                                        "uncertainty_u", "uncertainty_v", "outliers", "mask")))
         _check(self.h, load().hh_run_piv(self.h, arr, n, 1 if predictor_smoothing else 0, C.byref(f)), "run_piv")
+        res["outliers"] = res["outliers"].astype(bool)
+        res["mask"] = res["mask"].astype(bool)
+        return res
+
+    def _result_arrays(self, ny, nx):
+        res = dict(x=np.zeros(nx, np.float32), y=np.zeros(ny, np.float32))
+        for k in ("u", "v", "peak_ratio", "correlation_moment", "uncertainty_u", "uncertainty_v"):
+            res[k] = np.zeros((ny, nx), dtype=np.float32, order="F")
+        res["outliers"] = np.zeros((ny, nx), dtype=np.uint8, order="F")
+        res["mask"] = np.zeros((ny, nx), dtype=np.uint8, order="F")
+        f = hh_field(ny, nx, *(_ptr(res[k]) for k in ("x", "y", "u", "v", "peak_ratio", "correlation_moment",
+                                                       "uncertainty_u", "uncertainty_v", "outliers", "mask")))
+        return res, f
+
+    def fetch_result(self, ny, nx):
+        res, f = self._result_arrays(ny, nx)
+        _check(self.h, load().hh_fetch_result(self.h, C.byref(f)), "fetch_result")
+        res["outliers"] = res["outliers"].astype(bool)
+        res["mask"] = res["mask"].astype(bool)
+        return res
+
+    # ---- ensemble ------------------------------------------------------------
+    def ensemble_begin(self, p, jobs, uncertainty=False):
+        st = p.to_struct() if hasattr(p, "to_struct") else p
+        jobs = np.asfortranarray(jobs, dtype=np.int32)
+        self._ens = (st, jobs.shape[1], bool(uncertainty))
+        _check(self.h, load().hh_ensemble_begin(self.h, C.byref(st), _ptr(jobs), jobs.shape[1],
+                                                 1 if uncertainty else 0), "ensemble_begin")
+
+    def ensemble_add(self, src=HH_SRC_RAW):
+        _check(self.h, load().hh_ensemble_add(self.h, src), "ensemble_add")
+
+    def ensemble_analyze(self, ny, nx, u=None, v=None):
+        """ensemble_analyze!: u, v = the shared predictor at the nodes (zeros
+        when None); returns the total displacement and the quality grids."""
+        st, nj, unc = self._ens
+        g = lambda: np.zeros((ny, nx), dtype=np.float32, order="F")
+        res = dict(u=g() if u is None else _f32(u).copy(order="F"), v=g() if v is None else _f32(v).copy(order="F"),
+                   peak_ratio=g(), correlation_moment=g())
+        nalt = max(st.n_peaks - 1, 1)
+        res["alt_u"] = np.full((ny, nx, nalt), np.nan, dtype=np.float32, order="F")
+        res["alt_v"] = np.full((ny, nx, nalt), np.nan, dtype=np.float32, order="F")
+        res["uncertainty_u"] = np.full((ny, nx), np.nan, dtype=np.float32, order="F")
+        res["uncertainty_v"] = np.full((ny, nx), np.nan, dtype=np.float32, order="F")
+        has_alt = st.n_peaks > 1
+        _check(self.h, load().hh_ensemble_analyze(
+            self.h, ny, nx, _ptr(res["u"]), _ptr(res["v"]), _ptr(res["peak_ratio"]), _ptr(res["correlation_moment"]),
+            _ptr(res["alt_u"]) if has_alt else None, _ptr(res["alt_v"]) if has_alt else None,
+            _ptr(res["uncertainty_u"]), _ptr(res["uncertainty_v"])), "ensemble_analyze")
+        return res
+
+    def ensemble_planes(self):
+        st, nj, _ = self._ens
+        M = 2 * st.window_size[0] if st.padding else st.window_size[0]
+        pl = np.zeros((M, M, nj), dtype=np.float64, order="F")
+        _check(self.h, load().hh_ensemble_planes(self.h, _ptr(pl)), "ensemble_planes")
+        return pl
+
+    def run_piv_ensemble(self, pairs, passes, predictor_smoothing=True, fetch=True):
+        """run_piv_ensemble(pairs, passes) with every pass on the device."""
+        As = [_f32(A) for A, _ in pairs]
+        Bs = [_f32(B) for _, B in pairs]
+        if not As:
+            raise ArgumentError("pairs must not be empty")
+        shape = As[0].shape
+        for A, B in zip(As, Bs):
+            if A.shape != shape or B.shape != shape:
+                raise DimensionMismatch(f"all pairs must share the image size {shape}")
+        pa = (C.c_void_p * len(As))(*[A.ctypes.data for A in As])
+        pb = (C.c_void_p * len(Bs))(*[B.ctypes.data for B in Bs])
+        arr, n = pass_array(passes)
+        self.nr, self.nc = shape
+        if not fetch:
+            _check(self.h, load().hh_run_piv_ensemble(self.h, pa, pb, len(As), shape[0], shape[1], arr, n,
+                                                       1 if predictor_smoothing else 0, None), "run_piv_ensemble")
+            return None
+        ny, nx = grid_size(passes[-1], *shape)
+        res, f = self._result_arrays(ny, nx)
+        _check(self.h, load().hh_run_piv_ensemble(self.h, pa, pb, len(As), shape[0], shape[1], arr, n,
+                                                   1 if predictor_smoothing else 0, C.byref(f)), "run_piv_ensemble")
         res["outliers"] = res["outliers"].astype(bool)
         res["mask"] = res["mask"].astype(bool)
         return res

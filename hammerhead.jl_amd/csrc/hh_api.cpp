@@ -107,6 +107,21 @@ struct hh_ctx {
     DBuf<float> hk[12];
     DBuf<double> hkd;
     DBuf<int32_t> hki[2];
+    // ensemble accumulation (run_piv_ensemble): FP64 plane sums per window
+    // in the layout of correlate_box_layout, pooled uncertainty statistics
+    struct {
+        bool active = false;
+        hh_pass pass{};
+        CorrConfig cfg{};
+        bool box = false, unc = false;
+        size_t per = 0;
+        int njobs = 0, npairs = 0;
+        const int32_t *jobs = nullptr;       // device
+        std::vector<int32_t> hjobs;          // host copy for bounds checks
+    } ens;
+    DBuf<int32_t> ensjobs;
+    DBuf<double> ensacc, ensstats;
+    DBuf<float> arena;  // the pairs of hh_run_piv_ensemble, resident in HBM
     // final result of hh_run_piv
     PassGrid final_grid;
     bool have_result = false;
@@ -659,12 +674,207 @@ void run_schedule(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         if (unc && !uq_fused) {
             const ConfigBufs &cb = get_config(c, p);
             timed(c, "uq", cfg.w, [&] {
-                launch_uq(cfg.w, cb.apod.p, srcA, srcB, c->nr, djobs, (int)ng, ny, uu, uv, nullptr, c->stream);
+                launch_uq(cfg.w, cb.apod.p, srcA, srcB, c->nr, djobs, (int)ng, ny, uu, uv, nullptr, false,
+                          c->stream);
             });
         }
         if (keep_measured)
             hipLaunchKernelGGL(k_restore, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream, u, v, mu, mv,
                                outl, (int)ng);
+        prev = std::move(g);
+    }
+    HH_HIP(hipGetLastError());
+    c->final_grid = prev;
+    c->have_result = true;
+}
+
+// ---- ensemble (sum-of-correlation) PIV: src/ensemble.jl:133-343 -----------
+
+__global__ void k_fill_f64(double *p, size_t n, double v)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = v;
+}
+
+void fill64(hh_ctx *c, double *p, size_t n, double v)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p, n, v);
+    HH_HIP(hipGetLastError());
+}
+
+// _plane_accumulator / _uncertainty_accumulator (src/ensemble.jl:165-190):
+// zeroed FP64 plane sums for the jobs (device) of one pass.
+void ens_begin(hh_ctx *c, const hh_pass &p, const int32_t *djobs, int njobs, bool unc)
+{
+    check_pass_device(p, true);
+    if (unc && p.window_size[0] > 64) throw Error(HH_EUNSUPPORTED, "uncertainty supports windows up to 64 px");
+    auto &e = c->ens;
+    e.active = false;
+    e.pass = p;
+    e.cfg = make_corr(c, p);
+    e.box = correlate_box_layout(e.cfg);
+    const int nb = e.cfg.box_hi - e.cfg.box_lo + 1;
+    e.per = e.box ? (size_t)nb * nb : (size_t)e.cfg.M * e.cfg.M;
+    e.njobs = njobs;
+    e.jobs = djobs;
+    e.unc = unc;
+    e.npairs = 0;
+    fill64(c, c->ensacc.ensure(std::max<size_t>(e.per * njobs, 1)), e.per * njobs, 0.0);
+    if (unc) fill64(c, c->ensstats.ensure(std::max<size_t>((size_t)2 * HH_UQ_NSTATS * njobs, 1)),
+                    (size_t)2 * HH_UQ_NSTATS * njobs, 0.0);
+    e.active = true;
+}
+
+// accumulate_planes! (src/ensemble.jl:317-343) of one pair (device images).
+void ens_add(hh_ctx *c, const float *A, const float *B)
+{
+    auto &e = c->ens;
+    if (!e.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
+    CorrOut out{};
+    out.acc = c->ensacc.p;
+    out.ny = out.nx = 1;
+    timed(c, "ensemble_add", e.cfg.w,
+          [&] { launch_correlate(e.cfg, A, B, c->nr, c->nc, e.jobs, e.njobs, out, c->stream); });
+    if (e.unc) {
+        const ConfigBufs &cb = get_config(c, e.pass);
+        timed(c, "uq", e.cfg.w, [&] {
+            launch_uq(e.cfg.w, cb.apod.p, A, B, c->nr, e.jobs, e.njobs, 1, nullptr, nullptr, c->ensstats.p, true,
+                      c->stream);
+        });
+    }
+    ++e.npairs;
+}
+
+// ensemble_analyze! (src/ensemble.jl:284-313) into device grids; u, v hold
+// the shared predictor on entry and the total displacement on exit.
+void ens_analyze(hh_ctx *c, int ny, int nx, float *u, float *v, float *pr, float *cm, float *au, float *av, float *uu,
+                 float *uv)
+{
+    auto &e = c->ens;
+    if (!e.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
+    CorrOut out{};
+    out.u = u;
+    out.v = v;
+    out.u0 = u;
+    out.v0 = v;
+    out.pr = pr;
+    out.cm = cm;
+    out.alt_u = au;
+    out.alt_v = av;
+    out.ny = ny;
+    out.nx = nx;
+    timed(c, "ensemble_analyze", e.cfg.w,
+          [&] { launch_plane_analyze(e.cfg, c->ensacc.p, e.box, e.jobs, e.njobs, out, c->stream); });
+    if (e.unc && (uu || uv)) launch_uq_finalize(c->ensstats.p, e.jobs, e.njobs, ny, uu, uv, c->stream);
+}
+
+// run_piv_ensemble(pairs, passes) (src/ensemble.jl:47-84) with ensemble_pass
+// (src/ensemble.jl:133-265), unmasked, entirely on the device. The pairs are
+// uploaded once and stay resident across the passes.
+void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int npairs, int nr, int nc,
+                  const hh_pass *passes, int np, int smoothing)
+{
+    if (npairs < 1) throw Error(HH_EINVAL, "pairs must not be empty");
+    if (np < 1) throw Error(HH_EINVAL, "at least one pass is required");
+    if (!hA || !hB) throw Error(HH_EINVAL, "null pair list");
+    if (nr < 2 || nc < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
+    for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
+    for (int q = 0; q < npairs; ++q)
+        if (!hA[q] || !hB[q]) throw Error(HH_EINVAL, "null image in the pair list");
+    const size_t npx = (size_t)nr * nc;
+    float *arena = c->arena.ensure(2 * npx * npairs);
+    for (int q = 0; q < npairs; ++q) {
+        HH_HIP(hipMemcpyAsync(arena + 2 * npx * q, hA[q], sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipMemcpyAsync(arena + 2 * npx * q + npx, hB[q], sizeof(float) * npx, hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    if (nr != c->nr || nc != c->nc) c->jobcache.clear();
+    c->nr = nr;
+    c->nc = nc;
+    c->A = arena;
+    c->B = arena + npx;
+    c->coefs_valid = c->warp_valid = c->have_result = false;
+    size_t maxng = 0, maxalt = 0;
+    int maxny = 0, maxnx = 0;
+    for (int k = 0; k < np; ++k) {
+        int ny = 0, nx = 0;
+        hh_grid_size(&passes[k], nr, nc, &ny, &nx);
+        maxng = std::max(maxng, (size_t)ny * nx);
+        maxalt = std::max(maxalt, (size_t)ny * nx * std::max(passes[k].n_peaks - 1, 0));
+        maxny = std::max(maxny, ny);
+        maxnx = std::max(maxnx, nx);
+    }
+    for (DBuf<float> *b : {&c->gu, &c->gv, &c->gpr, &c->gcm, &c->guu, &c->guv, &c->gsu, &c->gsv, &c->pu, &c->pv})
+        b->ensure(std::max<size_t>(maxng, 1));
+    c->galtu.ensure(std::max<size_t>(maxalt, 1));
+    c->galtv.ensure(std::max<size_t>(maxalt, 1));
+    c->gout.ensure(std::max<size_t>(maxng, 1));
+    c->gsflag.ensure(std::max<size_t>(maxng, 1));
+    c->gpx.ensure(std::max(maxnx, 1));
+    c->gpy.ensure(std::max(maxny, 1));
+    PassGrid prev;
+    bool have_pred = false;
+    for (int k = 0; k < np; ++k) {
+        const hh_pass &p = passes[k];
+        const bool force_replace = k < np - 1;
+        PassGrid g = make_grid(p, nr, nc);
+        const int ny = g.ny, nx = g.nx;
+        const size_t ng = (size_t)ny * nx;
+        if (ng == 0) throw Error(HH_EINVAL, "the pass grid is empty");
+        if (k > 0) {
+            set_predictor_from_grid(c, prev, c->gu.p, c->gv.p, smoothing);
+            have_pred = true;
+        }
+        const int32_t *djobs = device_jobs(c, p, g);
+        const bool unc = p.uncertainty && !force_replace;
+        ens_begin(c, p, djobs, (int)ng, unc);
+        for (int q = 0; q < npairs; ++q) {
+            c->A = arena + 2 * npx * q;
+            c->B = arena + 2 * npx * q + npx;
+            if (have_pred) {
+                do_prefilter(c);
+                do_deform(c);
+                ens_add(c, c->warpA.p, c->warpB.p);
+            } else {
+                ens_add(c, c->A, c->B);
+            }
+        }
+        float *u = c->gu.p, *v = c->gv.p;
+        if (have_pred) {
+            float *gx = c->gpx.p, *gy = c->gpy.p;
+            HH_HIP(hipMemcpyAsync(gx, g.x.data(), sizeof(float) * nx, hipMemcpyHostToDevice, c->stream));
+            HH_HIP(hipMemcpyAsync(gy, g.y.data(), sizeof(float) * ny, hipMemcpyHostToDevice, c->stream));
+            launch_node_values(c->pu.p, c->pv.p, c->gny, c->gnx, c->y0, c->ysp, c->x0, c->xsp, gy, ny, gx, nx, u, v,
+                               c->stream);
+            // the copies read host vectors that this scope owns
+            HH_HIP(hipStreamSynchronize(c->stream));
+        } else {
+            fill(c, u, ng, 0.f);
+            fill(c, v, ng, 0.f);
+        }
+        const int nalt = p.n_peaks - 1;
+        float *au = nalt > 0 ? c->galtu.p : nullptr, *av = nalt > 0 ? c->galtv.p : nullptr;
+        fill(c, c->guu.p, ng, NAN);
+        fill(c, c->guv.p, ng, NAN);
+        ens_analyze(c, ny, nx, u, v, c->gpr.p, c->gcm.p, au, av, c->guu.p, c->guv.p);
+        ValidateArgs va{};
+        va.ny = ny;
+        va.nx = nx;
+        va.pass = &p;
+        va.force_replace = force_replace;
+        va.u = u;
+        va.v = v;
+        va.pr = c->gpr.p;
+        va.cm = c->gcm.p;
+        va.alt_u = au;
+        va.alt_v = av;
+        va.outliers = c->gout.p;
+        va.su = c->gsu.p;
+        va.sv = c->gsv.p;
+        va.sflag = c->gsflag.p;
+        va.count = c->gcount.ensure(1);
+        timed(c, "validate", p.window_size[0], [&] { launch_validate(va, c->stream); });
         prev = std::move(g);
     }
     HH_HIP(hipGetLastError());
@@ -927,7 +1137,7 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
             const ConfigBufs &cb = get_config(c, *pass);
             timed(c, "uq", w, [&] {
                 launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr,
-                          nullptr, c->stream);
+                          nullptr, false, c->stream);
             });
         }
         download(c, u, out.u, ng);
@@ -1005,7 +1215,7 @@ int hh_uncertainty_sweep(hh_ctx *c, const hh_pass *pass, int src, const int32_t 
         const ConfigBufs &cb = get_config(c, *pass);
         timed(c, "uq", w, [&] {
             launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr, dst,
-                      c->stream);
+                      false, c->stream);
         });
         if (unc_u) download(c, unc_u, duu, ng);
         if (unc_v) download(c, unc_v, duv, ng);
@@ -1084,6 +1294,112 @@ int hh_fetch_result(hh_ctx *c, hh_field *result)
     return guarded(c, [&] {
         if (!result) throw Error(HH_EINVAL, "null result");
         fetch(c, result);
+    });
+}
+
+int hh_ensemble_begin(hh_ctx *c, const hh_pass *pass, const int32_t *jobs, int njobs, int uncertainty)
+{
+    return guarded(c, [&] {
+        if (!pass) throw Error(HH_EINVAL, "null pass");
+        if (njobs < 0 || (njobs > 0 && !jobs)) throw Error(HH_EINVAL, "invalid job list");
+        for (int k = 0; k < njobs; ++k)
+            if (jobs[4 * k] < 1 || jobs[4 * k + 1] < 1 || jobs[4 * k + 2] < 1 || jobs[4 * k + 3] < 1)
+                throw Error(HH_EDIM, "job indices are 1-based");
+        c->ens.hjobs.assign(jobs, jobs + (size_t)4 * njobs);
+        const int32_t *dj = upload(c, c->ensjobs, jobs, (size_t)4 * njobs);
+        ens_begin(c, *pass, dj, njobs, uncertainty != 0);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_ensemble_add(hh_ctx *c, int src)
+{
+    return guarded(c, [&] {
+        if (!c->ens.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
+        const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
+        const int w = c->ens.pass.window_size[0];
+        for (int k = 0; k < c->ens.njobs; ++k) {
+            const int rs = c->ens.hjobs[4 * k + 2], cs = c->ens.hjobs[4 * k + 3];
+            if (rs + w - 1 > c->nr || cs + w - 1 > c->nc) throw Error(HH_EDIM, "window outside the image");
+        }
+        ens_add(c, dA, dB);
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_ensemble_analyze(hh_ctx *c, int ny, int nx, float *u, float *v, float *peak_ratio, float *correlation_moment,
+                        float *alt_u, float *alt_v, float *unc_u, float *unc_v)
+{
+    return guarded(c, [&] {
+        auto &e = c->ens;
+        if (!e.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
+        if (!u || !v || !peak_ratio || !correlation_moment) throw Error(HH_EINVAL, "null output grid");
+        for (int k = 0; k < e.njobs; ++k)
+            if (e.hjobs[4 * k] > ny || e.hjobs[4 * k + 1] > nx) throw Error(HH_EDIM, "job grid index out of range");
+        if (e.njobs == 0) return;
+        const size_t ng = (size_t)ny * nx;
+        const int nalt = e.pass.n_peaks - 1;
+        float *du = upload(c, c->hk[0], u, ng), *dv = upload(c, c->hk[1], v, ng);
+        float *dpr = upload(c, c->hk[2], peak_ratio, ng), *dcm = upload(c, c->hk[3], correlation_moment, ng);
+        float *dau = nullptr, *dav = nullptr;
+        if (alt_u && alt_v && nalt > 0) {
+            dau = upload(c, c->hk[4], alt_u, ng * nalt);
+            dav = upload(c, c->hk[5], alt_v, ng * nalt);
+        }
+        float *duu = nullptr, *duv = nullptr;
+        if (e.unc) {
+            duu = upload(c, c->hk[7], unc_u, ng);
+            duv = upload(c, c->hk[8], unc_v, ng);
+        }
+        ens_analyze(c, ny, nx, du, dv, dpr, dcm, dau, dav, duu, duv);
+        download(c, u, du, ng);
+        download(c, v, dv, ng);
+        download(c, peak_ratio, dpr, ng);
+        download(c, correlation_moment, dcm, ng);
+        if (dau) {
+            download(c, alt_u, dau, ng * nalt);
+            download(c, alt_v, dav, ng * nalt);
+        }
+        if (e.unc) {
+            download(c, unc_u, duu, ng);
+            download(c, unc_v, duv, ng);
+        }
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_ensemble_planes(hh_ctx *c, double *planes)
+{
+    return guarded(c, [&] {
+        auto &e = c->ens;
+        if (!e.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
+        if (!planes) throw Error(HH_EINVAL, "null argument");
+        const int M = e.cfg.M;
+        std::vector<double> h(e.per * e.njobs);
+        download(c, h.data(), (const double *)c->ensacc.p, h.size());
+        HH_HIP(hipStreamSynchronize(c->stream));
+        const int lo = e.cfg.box_lo, nb = e.cfg.box_hi - e.cfg.box_lo + 1;
+        for (int k = 0; k < e.njobs; ++k) {
+            double *dst = planes + (size_t)M * M * k;
+            const double *src = h.data() + e.per * k;
+            if (!e.box) {
+                std::memcpy(dst, src, sizeof(double) * M * M);
+                continue;
+            }
+            std::fill(dst, dst + (size_t)M * M, 0.0);
+            for (int j = 0; j < nb; ++j)
+                for (int i = 0; i < nb; ++i) dst[(lo + i) + (size_t)(lo + j) * M] = src[i + (size_t)j * nb];
+        }
+    });
+}
+
+int hh_run_piv_ensemble(hh_ctx *c, const float *const *A, const float *const *B, int npairs, int nr, int nc,
+                        const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result)
+{
+    return guarded(c, [&] {
+        if (!passes) throw Error(HH_EINVAL, "null pass schedule");
+        run_ensemble(c, A, B, npairs, nr, nc, passes, npasses, predictor_smoothing);
+        if (result) fetch(c, result);
     });
 }
 

@@ -62,6 +62,10 @@ struct CorrOut {
     int32_t *found;                  // njobs
     float *planes;                   // M*M*njobs
     float *unc_u, *unc_v;            // fused Wieneke uncertainty (nullable)
+    // ensemble accumulation: when set, the launch only adds each window's
+    // plane into its FP64 sum (layout of correlate_box_layout) and writes
+    // nothing else
+    double *acc;
     int ny, nx;
 };
 
@@ -70,6 +74,19 @@ struct CorrOut {
 // uncertainty); otherwise the caller runs launch_uq.
 bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
                       int njobs, const CorrOut &out, hipStream_t s);
+
+// Plane-sum layout of the accumulate mode: true when the specialized kernels
+// take the configuration and add only the analysis box [box_lo, box_hi]^2
+// (box side nb = box_hi - box_lo + 1, column-major, nb^2 doubles per
+// window; the plane is exactly zero outside it), false for the whole
+// M x M plane.
+bool correlate_box_layout(const CorrConfig &cfg);
+
+// ensemble_analyze!: peaks, subpixel, ratio, moment and alternatives of
+// summed planes (acc in the layout above, rounded to float), written like a
+// correlation launch (out.u0/v0 the shared predictor).
+void launch_plane_analyze(const CorrConfig &cfg, const double *acc, bool box, const int32_t *jobs, int njobs,
+                          const CorrOut &out, hipStream_t s);
 
 // Specialized kernels; return false when the configuration is not covered.
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
@@ -98,7 +115,12 @@ struct ValidateArgs {
 void launch_validate(const ValidateArgs &a, hipStream_t s);
 void launch_smooth(const float *f, int ny, int nx, float *out, hipStream_t s);
 
+// stats (nullable): 2 x 44 FP64 per job, overwritten, or added to when
+// accumulate (pooled statistics of the ensemble).
 void launch_uq(int w, const float *apod, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
-               int ny, float *unc_u, float *unc_v, double *stats, hipStream_t s);
+               int ny, float *unc_u, float *unc_v, double *stats, bool accumulate, hipStream_t s);
+// finalize_uncertainty of pooled statistics (2 x 44 per job).
+void launch_uq_finalize(const double *stats, const int32_t *jobs, int njobs, int ny, float *unc_u, float *unc_v,
+                        hipStream_t s);
 
 }  // namespace hh

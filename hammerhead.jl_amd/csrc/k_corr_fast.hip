@@ -843,6 +843,15 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     }
     __syncthreads();
 
+    if (out.acc) {
+        // accumulate_planes!: add the box to the window's FP64 plane sum
+        if (active) {
+            double *dst = out.acc + (size_t)jb * (NB * NB);
+            for (int p = t; p < NB * NB; p += G) dst[p] += (double)Rb[p];
+        }
+        return;
+    }
+
     // ---- 5. regional-max top-K over the box ----
     float tv[K];
     int tl[K];
@@ -1038,17 +1047,31 @@ bool launch_L(const CorrConfig &cfg, int hw, const float *A, const float *B, int
 
 }  // namespace
 
+// Half-width of the analysis box when the register kernels cover cfg, else -1.
+static int fast_halfwidth(const CorrConfig &cfg)
+{
+    if (!cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3) return -1;
+    if (!(cfg.M == 32 || cfg.M == 64 || cfg.M == 128) || cfg.w * 2 != cfg.M) return -1;
+    // the box must be symmetric about the zero lag with a zero margin, and be
+    // one of the two compiled widths (Gaussian or uniform apodization support)
+    const int H = cfg.M / 2;
+    const int hw = H - cfg.box_lo;
+    if (cfg.box_hi != H + hw || cfg.box_lo < 1 || cfg.box_hi > cfg.M - 3) return -1;
+    const int L = cfg.M / 32;
+    const int hg = L == 1 ? 7 : (L == 2 ? 13 : 25), hn = L == 1 ? 9 : (L == 2 ? 17 : 33);
+    return hw == hg || hw == hn ? hw : -1;
+}
+
+bool correlate_box_layout(const CorrConfig &cfg) { return fast_halfwidth(cfg) >= 0; }
+
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s, bool *uq_done)
 {
     *uq_done = false;
-    if (!cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3 || out.planes) return false;
-    if (!(cfg.M == 32 || cfg.M == 64 || cfg.M == 128) || cfg.w * 2 != cfg.M) return false;
-    // the box must be symmetric about the zero lag with a zero margin
-    const int H = cfg.M / 2;
-    const int hw = H - cfg.box_lo;
-    if (cfg.box_hi != H + hw || cfg.box_lo < 1 || cfg.box_hi > cfg.M - 3) return false;
-    const bool uq = out.unc_u != nullptr && out.unc_v != nullptr && cfg.M == 32;
+    if (out.planes) return false;
+    const int hw = fast_halfwidth(cfg);
+    if (hw < 0) return false;
+    const bool uq = out.unc_u != nullptr && out.unc_v != nullptr && cfg.M == 32 && !out.acc;
     if (njobs <= 0) {
         *uq_done = uq;
         return true;

@@ -206,6 +206,10 @@ __device__ float moment(const PlaneView<M> &R, float pr, float pc)
 }
 
 template <int M>
+__device__ void analyze_lds(const CorrConfig &cfg, const PlaneView<M> &R, int job, int gi, int gj,
+                            const CorrOut &out, Cand *scand, Cand *ssel);
+
+template <int M>
 __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const float2 *__restrict__ tw,
                                                         const float *__restrict__ A, const float *__restrict__ B,
                                                         int nr, const int32_t *__restrict__ jobs, CorrOut out)
@@ -338,6 +342,25 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
         float *dst = out.planes + (size_t)job * M * M;
         for (int e = tid; e < M * M; e += kThreads) dst[e] = R(e % M, e / M);
     }
+    if (out.acc) {
+        // accumulate_planes!: add the plane to the window's FP64 sum
+        double *dst = out.acc + (size_t)job * M * M;
+        for (int e = tid; e < M * M; e += kThreads) dst[e] += (double)R(e % M, e / M);
+        return;
+    }
+    analyze_lds<M>(cfg, R, job, gi, gj, out, scand, ssel);
+}
+
+// analyze_plane! (src/correlators.jl:459-468) of the plane in LDS and the
+// per-window outputs of process_windows! (src/pipeline.jl:972-1027); all
+// threads of the block take part.
+template <int M>
+__device__ void analyze_lds(const CorrConfig &cfg, const PlaneView<M> &R, int job, int gi, int gj,
+                            const CorrOut &out, Cand *scand, Cand *ssel)
+{
+#pragma clang fp contract(off)
+    constexpr int H = M / 2;
+    const int tid = threadIdx.x;
 
     // ---- peak finding ----
     const int K = cfg.K;
@@ -463,6 +486,37 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
     }
 }
 
+// ensemble_analyze! (src/ensemble.jl:284-313): one block per window; the
+// FP64 plane sum is rounded to float and analyzed like a single plane.
+template <int M>
+__global__ __launch_bounds__(kThreads) void k_plane_analyze(CorrConfig cfg, const double *__restrict__ acc, int box,
+                                                            const int32_t *__restrict__ jobs, CorrOut out)
+{
+    constexpr int P = M + 1;
+    __shared__ float2 pl[M * P];
+    __shared__ Cand scand[kThreads / 64];
+    __shared__ Cand ssel[kMaxPeaks];
+    const int tid = threadIdx.x;
+    const int job = blockIdx.x;
+    const int gi = jobs[4 * job], gj = jobs[4 * job + 1];
+    const int lo = cfg.box_lo, nb = cfg.box_hi - cfg.box_lo + 1;
+    const double *src = acc + (size_t)job * (box ? nb * nb : M * M);
+    for (int e = tid; e < M * M; e += kThreads) {
+        const int i = e % M, j = e / M;
+        float val;
+        if (box) {
+            const int bi = i - lo, bj = j - lo;
+            val = (bi >= 0 && bi < nb && bj >= 0 && bj < nb) ? (float)src[bi + bj * nb] : 0.f;
+        } else {
+            val = (float)src[e];
+        }
+        pl[i + j * P] = make_float2(0.f, val);
+    }
+    __syncthreads();
+    PlaneView<M> R{pl};
+    analyze_lds<M>(cfg, R, job, gi, gj, out, scand, ssel);
+}
+
 }  // namespace
 
 static float2 *twiddle_table(int M, hipStream_t s)
@@ -499,6 +553,23 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     }
     HH_HIP(hipGetLastError());
     return false;
+}
+
+void launch_plane_analyze(const CorrConfig &cfg, const double *acc, bool box, const int32_t *jobs, int njobs,
+                          const CorrOut &out, hipStream_t s)
+{
+    if (njobs <= 0) return;
+    const int b = box ? 1 : 0;
+    switch (cfg.M) {
+    case 16: hipLaunchKernelGGL(k_plane_analyze<16>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out); break;
+    case 32: hipLaunchKernelGGL(k_plane_analyze<32>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out); break;
+    case 64: hipLaunchKernelGGL(k_plane_analyze<64>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out); break;
+    case 128:
+        hipLaunchKernelGGL(k_plane_analyze<128>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out);
+        break;
+    default: throw Error(HH_EUNSUPPORTED, "FFT size must be 16, 32, 64 or 128");
+    }
+    HH_HIP(hipGetLastError());
 }
 
 }  // namespace hh

@@ -225,7 +225,7 @@ template <int W>
 __global__ __launch_bounds__(64) void k_uq(int w, const float *__restrict__ apod, const float *__restrict__ A,
                                            const float *__restrict__ B, int nr, const int32_t *__restrict__ jobs,
                                            int ny, float *__restrict__ unc_u, float *__restrict__ unc_v,
-                                           double *__restrict__ stats, Offsets off)
+                                           double *__restrict__ stats, int accumulate, Offsets off)
 {
     __shared__ float wA[W * W], wB[W * W];
     __shared__ float dC[W * W], dCs[W * W];
@@ -276,23 +276,50 @@ __global__ __launch_bounds__(64) void k_uq(int w, const float *__restrict__ apod
         if (unc_v) unc_v[g] = uq_finalize(st + HH_UQ_NSTATS, off);
     }
     if (stats)
-        for (int k = lane; k < 2 * HH_UQ_NSTATS; k += 64) stats[(size_t)2 * HH_UQ_NSTATS * job + k] = st[k];
+        for (int k = lane; k < 2 * HH_UQ_NSTATS; k += 64) {
+            double *d = stats + (size_t)2 * HH_UQ_NSTATS * job + k;
+            *d = accumulate ? *d + st[k] : st[k];
+        }
+}
+
+__global__ void k_uq_finalize(const double *__restrict__ stats, const int32_t *__restrict__ jobs, int njobs, int ny,
+                              float *__restrict__ unc_u, float *__restrict__ unc_v, Offsets off)
+{
+    const int job = blockIdx.x * blockDim.x + threadIdx.x;
+    if (job >= njobs) return;
+    const size_t g = (size_t)(jobs[4 * job] - 1) + (size_t)(jobs[4 * job + 1] - 1) * ny;
+    const double *s = stats + (size_t)2 * HH_UQ_NSTATS * job;
+    if (unc_u) unc_u[g] = uq_finalize(s, off);
+    if (unc_v) unc_v[g] = uq_finalize(s + HH_UQ_NSTATS, off);
 }
 
 }  // namespace
 
 void launch_uq(int w, const float *apod, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
-               int ny, float *unc_u, float *unc_v, double *stats, hipStream_t s)
+               int ny, float *unc_u, float *unc_v, double *stats, bool accumulate, hipStream_t s)
 {
     if (njobs <= 0) return;
     if (w > kMaxW) throw Error(HH_EUNSUPPORTED, "uncertainty supports windows up to 64 px on the device");
     static const Offsets off = make_offsets();
     if (w <= 16)
-        hipLaunchKernelGGL(k_uq<16>, dim3(njobs), dim3(64), 0, s, w, apod, A, B, nr, jobs, ny, unc_u, unc_v, stats, off);
+        hipLaunchKernelGGL(k_uq<16>, dim3(njobs), dim3(64), 0, s, w, apod, A, B, nr, jobs, ny, unc_u, unc_v, stats,
+                           (int)accumulate, off);
     else if (w <= 32)
-        hipLaunchKernelGGL(k_uq<32>, dim3(njobs), dim3(64), 0, s, w, apod, A, B, nr, jobs, ny, unc_u, unc_v, stats, off);
+        hipLaunchKernelGGL(k_uq<32>, dim3(njobs), dim3(64), 0, s, w, apod, A, B, nr, jobs, ny, unc_u, unc_v, stats,
+                           (int)accumulate, off);
     else
-        hipLaunchKernelGGL(k_uq<64>, dim3(njobs), dim3(64), 0, s, w, apod, A, B, nr, jobs, ny, unc_u, unc_v, stats, off);
+        hipLaunchKernelGGL(k_uq<64>, dim3(njobs), dim3(64), 0, s, w, apod, A, B, nr, jobs, ny, unc_u, unc_v, stats,
+                           (int)accumulate, off);
+    HH_HIP(hipGetLastError());
+}
+
+void launch_uq_finalize(const double *stats, const int32_t *jobs, int njobs, int ny, float *unc_u, float *unc_v,
+                        hipStream_t s)
+{
+    if (njobs <= 0) return;
+    static const Offsets off = make_offsets();
+    hipLaunchKernelGGL(k_uq_finalize, dim3((njobs + 63) / 64), dim3(64), 0, s, stats, jobs, njobs, ny, unc_u, unc_v,
+                       off);
     HH_HIP(hipGetLastError());
 }
 

@@ -126,3 +126,58 @@ def run_piv(imgA, imgB, passes, *, effort=None, backend: str = "mi355x", predict
             ws.close()
     return PIVResult(r["x"], r["y"], r["u"], r["v"], r["peak_ratio"], r["correlation_moment"], r["uncertainty_u"],
                      r["uncertainty_v"], r["outliers"], r["mask"], passes[-1], None, kwargs.get("scale"))
+
+
+def run_piv_ensemble(pairs: Sequence, params, *, effort=None, backend: str = "mi355x", predictor_smoothing: bool = True,
+                     mask=None, workspace: Optional[PIVWorkspace] = None, **kwargs) -> PIVResult:
+    """run_piv_ensemble(pairs, params; backend) -> PIVResult (src/ensemble.jl:47-84).
+
+    ``pairs`` is a sequence of (imgA, imgB) arrays of one size; ``params`` a
+    PIVParameters or a schedule. Every pass sums each window's correlation
+    planes over all pairs in FP64 on the GPU and locates the peak once on the
+    sum; with a schedule, the previous pass's ensemble field is the shared
+    deformation predictor of every pair (replacement is forced on all but the
+    last pass). With ``uncertainty`` on the last pass, the Wieneke statistics
+    are pooled over the pairs."""
+    if effort is not None:
+        raise ArgumentError("effort cannot be combined with explicit PIVParameters or pass schedules")
+    unknown = set(kwargs) - {"threaded", "mask_threshold", "scale", "progress", "preprocess", "image_type"}
+    if unknown:
+        raise ArgumentError(f"unsupported run_piv_ensemble keyword(s): {', '.join(sorted(unknown))}")
+    be = _resolve_backend(backend)
+    passes = [params] if isinstance(params, PIVParameters) else list(params)
+    if not passes:
+        raise ArgumentError("at least one pass is required")
+    pairs = list(pairs)
+    if not pairs:
+        raise ArgumentError("pairs must not be empty")
+    preprocess = kwargs.get("preprocess")
+    arrays = []
+    for A, B in pairs:
+        A, B = np.asarray(A), np.asarray(B)
+        if preprocess is not None:
+            A, B = preprocess(A), preprocess(B)
+        if A.shape != B.shape:
+            raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
+        if arrays and A.shape != arrays[0][0].shape:
+            raise DimensionMismatch(f"all pairs must share the image size {arrays[0][0].shape}, got {A.shape}")
+        arrays.append((A, B))
+    shape = arrays[0][0].shape
+    if mask is not None:
+        m = np.asarray(mask, dtype=bool)
+        if m.shape != shape:
+            raise DimensionMismatch(f"mask must have the same size as the images, got {m.shape}")
+        if m.any():
+            raise ArgumentError("backend :mi355x does not support masks yet")
+    ws = workspace or PIVWorkspace(be)
+    try:
+        ctx = ws.ctx
+        ctx.check_passes(passes)
+        for p in passes:
+            pass_grid_counts(p, *shape)
+        r = ctx.run_piv_ensemble(arrays, passes, predictor_smoothing)
+    finally:
+        if workspace is None:
+            ws.close()
+    return PIVResult(r["x"], r["y"], r["u"], r["v"], r["peak_ratio"], r["correlation_moment"], r["uncertainty_u"],
+                     r["uncertainty_v"], r["outliers"], r["mask"], passes[-1], None, kwargs.get("scale"))

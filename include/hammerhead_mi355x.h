@@ -214,8 +214,40 @@ int hh_smooth_field(hh_ctx *ctx, int ny, int nx, const float *f, float *out);
  * pass grid); with NULL the result stays device-resident (hh_fetch_result). */
 int hh_run_piv(hh_ctx *ctx, const hh_pass *passes, int npasses,
                int predictor_smoothing, hh_field *result);
-/* Copy the device-resident result of the last hh_run_piv to host. */
+/* Copy the device-resident result of the last hh_run_piv or
+ * hh_run_piv_ensemble to host. */
 int hh_fetch_result(hh_ctx *ctx, hh_field *result);
+
+/* ---- ensemble (sum-of-correlation) PIV (A16) --------------------------- */
+
+/* _plane_accumulator / _uncertainty_accumulator of ensemble_pass
+ * (src/ensemble.jl:165-190; device form ext/HammerheadAMDGPUExt.jl:550-560):
+ * start FP64 per-window plane sums for jobs (int32 4 x njobs, 1-based) of
+ * one pass; with uncertainty the Wieneke statistics are pooled too. The sums
+ * live on the device until hh_ensemble_analyze. */
+int hh_ensemble_begin(hh_ctx *ctx, const hh_pass *pass, const int32_t *jobs, int njobs, int uncertainty);
+/* accumulate_planes! (src/ensemble.jl:317-343; device form
+ * ext/HammerheadAMDGPUExt.jl:561-621): add the correlation planes of the
+ * current pair (src = HH_SRC_RAW, or HH_SRC_WARP after hh_deform). */
+int hh_ensemble_add(hh_ctx *ctx, int src);
+/* ensemble_analyze! (src/ensemble.jl:284-313; device form
+ * ext/HammerheadAMDGPUExt.jl:622-670): peaks of the summed planes. u, v
+ * (ny x nx) hold the shared predictor at the nodes on entry and the total
+ * displacement on exit; alt_u/alt_v (nullable, ny x nx x (n_peaks-1)) get
+ * predictor-relative alternatives (NaN where absent); unc_u/unc_v receive
+ * the pooled uncertainty when the ensemble was begun with it. */
+int hh_ensemble_analyze(hh_ctx *ctx, int ny, int nx, float *u, float *v, float *peak_ratio,
+                        float *correlation_moment, float *alt_u, float *alt_v, float *unc_u, float *unc_v);
+/* The current FP64 plane sums, M x M x njobs (fftshifted, as
+ * correlation_plane!): the keep_correlation_planes view of an ensemble. */
+int hh_ensemble_planes(hh_ctx *ctx, double *planes);
+/* run_piv_ensemble(pairs, passes; predictor_smoothing)
+ * (src/ensemble.jl:47-84) on the device: A[q], B[q] are host column-major
+ * float32 nr x nc frames of pair q, uploaded once and kept resident in HBM
+ * for every pass. The final result is copied to `result` when non-NULL, and
+ * stays available to hh_fetch_result. */
+int hh_run_piv_ensemble(hh_ctx *ctx, const float *const *A, const float *const *B, int npairs, int nr, int nc,
+                        const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result);
 
 /* ---- stream and timing ------------------------------------------------- */
 

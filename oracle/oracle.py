@@ -365,5 +365,38 @@ def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64"):
     return res
 
 
+def run_piv_ensemble(pairs, passes, predictor_smoothing=True, nthreads=8, prec="f64", planes=False):
+    """run_piv_ensemble(pairs, passes): dict with the final PIVResult fields;
+    with ``planes`` also the last pass's FP64 plane sums (m1 x m2 x njobs)."""
+    dt = _dt(prec)
+    As = [_f(A, dt) for A, _ in pairs]
+    Bs = [_f(B, dt) for _, B in pairs]
+    nr, nc = As[0].shape
+    arr = (type(passes[0]) * len(passes))(*passes)
+    x, y, jobs = pass_grid(passes[-1], nr, nc, prec)
+    ny, nx = y.size, x.size
+    g = lambda: np.zeros((ny, nx), dtype=dt, order="F")
+    res = dict(x=np.zeros(nx, dtype=dt), y=np.zeros(ny, dtype=dt), u=g(), v=g(), peak_ratio=g(),
+               correlation_moment=g(), uncertainty_u=g(), uncertainty_v=g(),
+               outliers=np.zeros((ny, nx), dtype=np.uint8, order="F"))
+    last = passes[-1]
+    m1, m2 = (2 * s if last.padding else s for s in last.search_area_size)
+    pl = np.zeros((m1, m2, ny * nx), dtype=np.float64, order="F") if planes else None
+    pa = (C.c_void_p * len(As))(*[_p(A) for A in As])
+    pb = (C.c_void_p * len(Bs))(*[_p(B) for B in Bs])
+    rc = _fn("orc_run_piv_ensemble", prec)(pa, pb, C.c_int(len(As)), C.c_int(nr), C.c_int(nc), arr,
+                                           C.c_int(len(passes)), C.c_int(1 if predictor_smoothing else 0),
+                                           C.c_int(nthreads), _p(res["x"]), _p(res["y"]), _p(res["u"]),
+                                           _p(res["v"]), _p(res["peak_ratio"]), _p(res["correlation_moment"]),
+                                           _p(res["uncertainty_u"]), _p(res["uncertainty_v"]),
+                                           _p(res["outliers"]), _p(pl))
+    if rc != 0:
+        raise ValueError("invalid pass schedule or empty pair list")
+    res["outliers"] = res["outliers"].astype(bool)
+    if planes:
+        res["planes"] = pl
+    return res
+
+
 def default_threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))

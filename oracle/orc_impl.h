@@ -1435,6 +1435,209 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
     return rc;
 }
 
+/* ======================================================================
+ * Ensemble (sum-of-correlation) PIV (src/ensemble.jl:47-343), unmasked
+ * ==================================================================== */
+typedef struct {
+    const hh_pass *p;
+    const F(orc_corr) * c;
+    const REAL *A, *B;
+    int nr;
+    const int32_t *jobs;
+    double *acc;  /* np per job */
+    double *ust;  /* 2 x 44 per job, nullable */
+} F(orc_ens_add_args);
+
+/* accumulate_planes! (src/ensemble.jl:317-343): the pair's plane of each
+ * window is added to that window's sum; with uncertainty, its Wieneke
+ * statistics (window-sized B at the same origin) are pooled. */
+static void F(orc_ens_add_range)(void *arg, int j0, int j1)
+{
+    F(orc_ens_add_args) *a = (F(orc_ens_add_args) *)arg;
+    F(orc_scratch) s;
+    F(orc_scratch_init)(&s, a->c);
+    int wr = a->c->wr, wc = a->c->wc;
+    int mw = wr > wc ? wr : wc;
+    REAL *wA = (REAL *)malloc(sizeof(REAL) * wr * wc), *wB = (REAL *)malloc(sizeof(REAL) * wr * wc);
+    REAL *dC = (REAL *)malloc(sizeof(REAL) * mw * mw), *dCs = (REAL *)malloc(sizeof(REAL) * mw * mw);
+    size_t np = (size_t)a->c->m1 * a->c->m2;
+    for (int jb = j0; jb < j1; jb++) {
+        int rs = a->jobs[4 * jb + 2], cs = a->jobs[4 * jb + 3];
+        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs);
+        double *dst = a->acc + np * jb;
+        for (size_t e = 0; e < np; e++) dst[e] += (double)s.R[e];
+        if (a->ust)
+            F(orc_uq_accumulate)(a->ust + (size_t)2 * HH_UQ_NSTATS * jb, a->A, a->B, a->nr, rs, cs, wr, wc,
+                                 a->c->apod, wA, wB, dC, dCs);
+    }
+    free(wA); free(wB); free(dC); free(dCs);
+    F(orc_scratch_free)(&s);
+}
+
+typedef struct {
+    const hh_pass *p;
+    const F(orc_corr) * c;
+    const int32_t *jobs;
+    int ny, nx;
+    const double *acc, *ust;
+    REAL *u, *v, *pr, *cm, *alt_u, *alt_v, *unc_u, *unc_v;
+} F(orc_ens_an_args);
+
+/* ensemble_analyze! (src/ensemble.jl:284-313): u, v hold the shared
+ * predictor on entry and the total displacement on exit; alternatives are
+ * predictor-relative; pooled statistics are finalized. */
+static void F(orc_ens_an_range)(void *arg, int j0, int j1)
+{
+    F(orc_ens_an_args) *a = (F(orc_ens_an_args) *)arg;
+    const hh_pass *p = a->p;
+    size_t np = (size_t)a->c->m1 * a->c->m2;
+    size_t ngrid = (size_t)a->ny * a->nx;
+    REAL *R = (REAL *)malloc(sizeof(REAL) * np);
+    REAL vals[64], altu[64], altv[64], out[4];
+    int32_t locs[128];
+    for (int jb = j0; jb < j1; jb++) {
+        int gi = a->jobs[4 * jb], gj = a->jobs[4 * jb + 1];
+        size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * a->ny;
+        const double *src = a->acc + np * jb;
+        for (size_t e = 0; e < np; e++) R[e] = (REAL)src[e];
+        int found = F(orc_analyze_plane)(R, a->c->m1, a->c->m2, p, vals, locs, out, altu, altv);
+        if (a->alt_u) {
+            int mlim = found < p->n_peaks ? found : p->n_peaks;
+            for (int m = 2; m <= mlim; m++) {
+                a->alt_u[g + (size_t)(m - 2) * ngrid] = a->u[g] + altu[m - 2];
+                a->alt_v[g + (size_t)(m - 2) * ngrid] = a->v[g] + altv[m - 2];
+            }
+        }
+        a->u[g] += out[0];
+        a->v[g] += out[1];
+        a->pr[g] = out[2];
+        a->cm[g] = out[3];
+        if (a->ust) {
+            a->unc_u[g] = F(orc_uq_finalize)(a->ust + (size_t)2 * HH_UQ_NSTATS * jb);
+            a->unc_v[g] = F(orc_uq_finalize)(a->ust + (size_t)2 * HH_UQ_NSTATS * jb + HH_UQ_NSTATS);
+        }
+    }
+    free(R);
+}
+
+/* run_piv_ensemble(pairs, passes) (src/ensemble.jl:47-84) with
+ * ensemble_pass (src/ensemble.jl:133-265), unmasked. Each pass deforms
+ * every pair by the shared predictor built from the previous pass, sums
+ * each window's correlation planes over the pairs, analyzes the sums once
+ * and validates (replacement forced on all but the last pass). The plane
+ * sums are FP64 for either REAL (the reference sums in T); each sum is
+ * rounded to REAL before its analysis. planes_out (nullable) receives the
+ * last pass's sums, m1 x m2 x njobs. Returns 0, or -1 for an invalid
+ * schedule or an empty pair list. */
+int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, int npairs, int nr, int nc,
+                            const hh_pass *passes, int npasses, int predictor_smoothing, int nthreads, REAL *ox,
+                            REAL *oy, REAL *ou, REAL *ov, REAL *opr, REAL *ocm, REAL *ouu, REAL *ouv, uint8_t *oout,
+                            double *planes_out)
+{
+    if (npasses < 1 || npairs < 1) return -1;
+    size_t npx = (size_t)nr * nc;
+    REAL *cA = NULL, *cB = NULL, *wA = NULL, *wB = NULL;
+    if (npasses > 1) {
+        cA = (REAL *)malloc(sizeof(REAL) * (size_t)(nr + 2) * (nc + 2));
+        cB = (REAL *)malloc(sizeof(REAL) * (size_t)(nr + 2) * (nc + 2));
+        wA = (REAL *)malloc(sizeof(REAL) * npx);
+        wB = (REAL *)malloc(sizeof(REAL) * npx);
+    }
+    F(orc_pred) pred = {0, 0, NULL, NULL, NULL, NULL};
+    int pny = 0, pnx = 0;
+    REAL *px = NULL, *py = NULL, *pu = NULL, *pv = NULL;
+    int rc = 0;
+    for (int k = 0; k < npasses; k++) {
+        const hh_pass *p = &passes[k];
+        int force_replace = k < npasses - 1;
+        int have_pred = px != NULL;
+        if (have_pred) {
+            /* build_predictor (src/pipeline.jl:454-464), unmasked */
+            free(pred.x); free(pred.y); free(pred.u); free(pred.v);
+            pred.ny = pny;
+            pred.nx = pnx;
+            pred.x = px;
+            pred.y = py;
+            pred.u = (REAL *)malloc(sizeof(REAL) * pny * pnx);
+            pred.v = (REAL *)malloc(sizeof(REAL) * pny * pnx);
+            if (predictor_smoothing) {
+                F(orc_smooth)(pu, pny, pnx, pred.u);
+                F(orc_smooth)(pv, pny, pnx, pred.v);
+            } else {
+                memcpy(pred.u, pu, sizeof(REAL) * pny * pnx);
+                memcpy(pred.v, pv, sizeof(REAL) * pny * pnx);
+            }
+            free(pu); free(pv);
+            px = py = pu = pv = NULL;
+        }
+        int ny, nx;
+        if (F(orc_pass_grid)(p, nr, nc, &ny, &nx, NULL, NULL, NULL) != 0) {
+            rc = -1;
+            break;
+        }
+        size_t ng = (size_t)ny * nx;
+        REAL *x = (REAL *)malloc(sizeof(REAL) * nx), *y = (REAL *)malloc(sizeof(REAL) * ny);
+        int32_t *jobs = (int32_t *)malloc(sizeof(int32_t) * 4 * (ng > 0 ? ng : 1));
+        F(orc_pass_grid)(p, nr, nc, &ny, &nx, x, y, jobs);
+        F(orc_corr) c;
+        F(orc_corr_init)(&c, p);
+        size_t np = (size_t)c.m1 * c.m2;
+        double *acc = (double *)calloc(ng * np > 0 ? ng * np : 1, sizeof(double));
+        int unc = p->uncertainty && !force_replace;
+        double *ust = unc ? (double *)calloc(ng * 2 * HH_UQ_NSTATS > 0 ? ng * 2 * HH_UQ_NSTATS : 1, sizeof(double))
+                          : NULL;
+        REAL *u = (REAL *)calloc(ng > 0 ? ng : 1, sizeof(REAL)), *v = (REAL *)calloc(ng > 0 ? ng : 1, sizeof(REAL));
+        for (int q = 0; q < npairs; q++) {
+            const REAL *srcA = imgA[q], *srcB = imgB[q];
+            if (have_pred) {
+                F(orc_prefilter)(imgA[q], nr, nc, cA);
+                F(orc_prefilter)(imgB[q], nr, nc, cB);
+                F(orc_deform)(cA, cB, nr, nc, pred.y, pred.ny, pred.x, pred.nx, pred.u, pred.v, wA, wB, nthreads);
+                srcA = wA;
+                srcB = wB;
+            }
+            F(orc_ens_add_args) aa = {p, &c, srcA, srcB, nr, jobs, acc, ust};
+            orc_parallel_for((int)ng, nthreads, F(orc_ens_add_range), &aa);
+        }
+        /* u, v = the shared predictor at the nodes (zero without one) */
+        if (have_pred) F(orc_node_values)(pred.y, pred.ny, pred.x, pred.nx, pred.u, pred.v, y, ny, x, nx, u, v);
+        REAL *pr = (REAL *)calloc(ng > 0 ? ng : 1, sizeof(REAL)), *cm = (REAL *)calloc(ng > 0 ? ng : 1, sizeof(REAL));
+        REAL *uu = (REAL *)malloc(sizeof(REAL) * (ng > 0 ? ng : 1)), *uv = (REAL *)malloc(sizeof(REAL) * (ng > 0 ? ng : 1));
+        for (size_t q = 0; q < ng; q++) uu[q] = uv[q] = (REAL)NAN;
+        int nalt = p->n_peaks - 1;
+        REAL *au = nalt > 0 ? (REAL *)malloc(sizeof(REAL) * ng * nalt) : NULL;
+        REAL *av = nalt > 0 ? (REAL *)malloc(sizeof(REAL) * ng * nalt) : NULL;
+        if (au)
+            for (size_t q = 0; q < ng * nalt; q++) au[q] = av[q] = (REAL)NAN;
+        F(orc_ens_an_args) an = {p, &c, jobs, ny, nx, acc, ust, u, v, pr, cm, au, av, uu, uv};
+        orc_parallel_for((int)ng, nthreads, F(orc_ens_an_range), &an);
+        uint8_t *outl = (uint8_t *)calloc(ng > 0 ? ng : 1, 1);
+        F(orc_validate_and_replace)(p, ny, nx, force_replace, u, v, pr, cm, au, av, NULL, outl);
+        if (k == npasses - 1) {
+            if (ox) memcpy(ox, x, sizeof(REAL) * nx);
+            if (oy) memcpy(oy, y, sizeof(REAL) * ny);
+            if (ou) memcpy(ou, u, sizeof(REAL) * ng);
+            if (ov) memcpy(ov, v, sizeof(REAL) * ng);
+            if (opr) memcpy(opr, pr, sizeof(REAL) * ng);
+            if (ocm) memcpy(ocm, cm, sizeof(REAL) * ng);
+            if (ouu) memcpy(ouu, uu, sizeof(REAL) * ng);
+            if (ouv) memcpy(ouv, uv, sizeof(REAL) * ng);
+            if (oout) memcpy(oout, outl, ng);
+            if (planes_out) memcpy(planes_out, acc, sizeof(double) * ng * np);
+            free(x); free(y); free(u); free(v);
+        } else {
+            px = x; py = y; pu = u; pv = v;
+            pny = ny; pnx = nx;
+        }
+        free(jobs); free(acc); free(ust); free(pr); free(cm); free(uu); free(uv); free(au); free(av); free(outl);
+        F(orc_corr_free)(&c);
+    }
+    free(pred.x); free(pred.y); free(pred.u); free(pred.v);
+    free(px); free(py); free(pu); free(pv);
+    free(cA); free(cB); free(wA); free(wB);
+    return rc;
+}
+
 #undef F
 #undef REAL_EPS
 #undef RLOG

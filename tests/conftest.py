@@ -59,3 +59,20 @@ def particle_pair(shape, positions, dv, du, diameter=3.0):
         add_particle(A, p, diameter)
         add_particle(B, (p[0] + dv, p[1] + du), diameter)
     return np.asfortranarray(A), np.asfortranarray(B)
+
+
+def low_snr_pairs(n=96, npairs=24, du=1.5, dv=-2.0, nparticles=120, noise=1.5, seed=31):
+    """The ensemble scene of test/test_ensemble.jl:7-17 regenerated with
+    numpy's RNG: 120 particles over [-10, n+10]^2, a uniform shift, and heavy
+    uniform noise 1.5 U[0, 1] added to each frame."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for _ in range(npairs):
+        pos = [(rng.random() * (n + 20) - 10, rng.random() * (n + 20) - 10) for _ in range(nparticles)]
+        A, B = particle_pair((n, n), pos, dv, du)
+        A = A + noise * rng.random((n, n))
+        B = B + noise * rng.random((n, n))
+        pairs.append((np.asfortranarray(A), np.asfortranarray(B)))
+    return pairs

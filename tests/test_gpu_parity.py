@@ -246,3 +246,94 @@ def test_run_piv_multipass_vortex(hh, orc, ctx):
     assert rms < 5e-3
     assert dflag <= 0.002 * g["outliers"].size + 2
     assert srel < 0.01
+
+
+# ---------------------------------------------------------------- ensemble
+def _ens_compare(g, o, tol=1e-3, quality=True):
+    valid = ~o["outliers"]
+    dflag = int((g["outliers"] != o["outliers"]).sum())
+    du = np.abs(g["u"] - o["u"])[valid]
+    dv = np.abs(g["v"] - o["v"])[valid]
+    dpr = np.abs(g["peak_ratio"] - o["peak_ratio"])[valid]
+    dcm = np.abs(g["correlation_moment"] - o["correlation_moment"])[valid]
+    say("ensemble flag diffs", dflag, "max du dv dpr dcm", du.max(), dv.max(), dpr.max(), dcm.max())
+    assert dflag == 0
+    assert du.max() < tol and dv.max() < tol
+    if quality:
+        assert dpr.max() < tol and dcm.max() < tol
+
+
+@pytest.mark.parametrize("method", ["cross", "phase"])
+def test_ensemble_low_snr_vs_oracle(hh, orc, ctx, method):
+    # test/test_ka.jl:197-224 (device vs CPU ensemble within 1e-3) on the
+    # low-SNR scene of test/test_ensemble.jl: unpadded full-plane sums
+    from conftest import low_snr_pairs
+
+    pairs = [(A.astype(np.float32), B.astype(np.float32)) for A, B in low_snr_pairs()]
+    p = P(hh, window_size=32, overlap=16, correlation_method=method)
+    say("ensemble low-snr", method)
+    g = ctx.run_piv_ensemble(pairs, [p])
+    o = orc.run_piv_ensemble(pairs, [p.to_struct()], prec="f64")
+    # phase normalization divides by |P| + eps, which magnifies float rounding
+    # at weak frequencies: the secondary peaks of two float32 computations
+    # (the oracle's own Float32 restatement included) differ by a few percent,
+    # so, like the reference's device check, phase compares u and v only
+    _ens_compare(g, o, quality=method == "cross")
+    if method == "cross":
+        assert abs(np.median(g["u"]) - 1.5) < 0.25 and abs(np.median(g["v"]) + 2.0) < 0.25
+
+
+def test_ensemble_multipass_uq_vs_oracle(hh, orc, ctx):
+    # padded + Gaussian passes (box sums), shared-predictor deformation, pooled
+    # uncertainty on the last pass (test/test_ka.jl:227-268)
+    flow = hh.synthetic.lamb_oseen_flow(160.5, 128.5, 60.0, 4.0)
+    pairs = [hh.synthetic.particle_pair(256, 320, flow, seed=104000 + i, ppp=0.005, noise=0.05) for i in range(12)]
+    pairs = [(q.A, q.B) for q in pairs]
+    passes = hh.multipass_parameters([64, 32], padding=True, apodization="gauss", final=dict(uncertainty=True))
+    say("ensemble multipass")
+    g = ctx.run_piv_ensemble(pairs, passes)
+    o = orc.run_piv_ensemble(pairs, [p.to_struct() for p in passes], prec="f64")
+    _ens_compare(g, o)
+    fin = np.isfinite(g["uncertainty_u"]) & np.isfinite(o["uncertainty_u"])
+    assert np.array_equal(np.isfinite(g["uncertainty_u"]), np.isfinite(o["uncertainty_u"]))
+    srel = np.abs(g["uncertainty_u"][fin] - o["uncertainty_u"][fin]) / np.maximum(np.abs(o["uncertainty_u"][fin]), 1e-6)
+    say("ensemble sigma max rel", srel.max())
+    assert srel.max() < 0.01
+
+
+def test_ensemble_planes_and_hooks(hh, orc, ctx):
+    # hook path: begin / add per pair / analyze; FP64 sums against the oracle's
+    from conftest import low_snr_pairs
+
+    pairs = [(A.astype(np.float32), B.astype(np.float32)) for A, B in low_snr_pairs(n=64, npairs=5, seed=3)]
+    for p in (P(hh, window_size=16, overlap=8, padding=True, apodization="gauss"),
+              P(hh, window_size=32, overlap=16)):
+        x, y, jobs = hh.pass_grid(p, 64, 64)
+        ctx.ensemble_begin(p, jobs)
+        for A, B in pairs:
+            ctx.set_pair(A, B)
+            ctx.ensemble_add(hh.HH_SRC_RAW)
+        gp = ctx.ensemble_planes()
+        r = ctx.ensemble_analyze(y.size, x.size)
+        o = orc.run_piv_ensemble(pairs, [p.to_struct()], prec="f32", planes=True)
+        scale = np.abs(o["planes"]).max(axis=(0, 1))
+        rel = (np.abs(gp - o["planes"]).max(axis=(0, 1)) / scale).max()
+        say("ensemble planes", p.window_size, p.padding, "max rel", rel)
+        assert rel < 1e-5
+        # the analysis of the sums, before validation, against the oracle's
+        # validated result on the cells it kept
+        keep = ~o["outliers"]
+        assert np.abs(r["u"] - o["u"])[keep].max() < 1e-3 and np.abs(r["v"] - o["v"])[keep].max() < 1e-3
+
+
+def test_ensemble_one_pair_matches_run_piv(hh, ctx):
+    pair = hh.synthetic.config_vortex_1024(n=256)
+    passes = hh.multipass_parameters([32, 16], padding=True, apodization="gauss", final=dict(uncertainty=True))
+    e = ctx.run_piv_ensemble([(pair.A, pair.B)], passes)
+    ctx.set_pair(pair.A, pair.B)
+    r = ctx.run_piv(passes)
+    assert np.array_equal(e["outliers"], r["outliers"])
+    for k in ("u", "v", "peak_ratio", "correlation_moment", "uncertainty_u", "uncertainty_v"):
+        fin = np.isfinite(r[k])
+        assert np.array_equal(fin, np.isfinite(e[k])), k
+        assert np.abs(e[k][fin] - r[k][fin]).max() <= 1e-5 * max(1.0, np.abs(r[k][fin]).max()), k

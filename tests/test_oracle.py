@@ -13,7 +13,7 @@ import pytest
 from scipy.interpolate import CubicSpline
 from scipy.signal import correlate2d
 
-from conftest import add_particle, particle_pair
+from conftest import add_particle, low_snr_pairs, particle_pair
 
 PRECS = ["f64", "f32"]
 
@@ -540,3 +540,58 @@ def test_field_change_vs_numpy(orc):
     pu, pv = u + 0.1 * rng.normal(size=u.shape), v + 0.1 * rng.normal(size=v.shape)
     ref = np.quantile(np.maximum(np.abs(u - pu), np.abs(v - pv)).ravel(), 0.95, method="linear")
     assert orc.field_change(u, v, pu, pv) == pytest.approx(ref, rel=1e-14)
+
+
+# ---------------------------------------------------------------- ensemble
+@pytest.mark.parametrize("prec", PRECS)
+def test_ensemble_one_pair_is_run_piv(hh, orc, prec):
+    # one pair: every ensemble pass is exactly the single-pair pass (the
+    # plane sum of one plane is that plane), including the deforming passes,
+    # the forced replacement of earlier passes and uncertainty on the last
+    pair = hh.synthetic.particle_pair(128, 128, hh.synthetic.lamb_oseen_flow(64.5, 64.5, 20.0, 3.0), seed=7,
+                                      dtype=np.float64)
+    passes = [p.to_struct() for p in hh.multipass_parameters(
+        [32, 16], padding=True, apodization="gauss", final=dict(uncertainty=True))]
+    e = orc.run_piv_ensemble([(pair.A, pair.B)], passes, prec=prec)
+    r = orc.run_piv(pair.A, pair.B, passes, prec=prec)
+    for k in ("u", "v", "peak_ratio", "correlation_moment", "uncertainty_u", "uncertainty_v", "outliers"):
+        assert np.array_equal(e[k], r[k], equal_nan=True), k
+
+
+def test_ensemble_plane_sums(hh, orc):
+    # accumulate_planes! adds each pair's correlation_plane! (src/ensemble.jl:317-343):
+    # the sums equal the per-pair planes of process_windows! added in FP64
+    pairs = low_snr_pairs(n=64, npairs=3)
+    p = P(hh, window_size=32, overlap=16)
+    e = orc.run_piv_ensemble(pairs, [p], prec="f32", planes=True)
+    x, y, jobs = orc.pass_grid(p, 64, 64)
+    total = 0
+    for A, B in pairs:
+        r = orc.process_windows(p, A.astype(np.float32), B.astype(np.float32), jobs, y.size, x.size, prec="f32",
+                                planes=True)
+        total = total + r["planes"].astype(np.float64)
+    assert np.array_equal(e["planes"], total)
+    # identical pairs: the sum of N copies is exactly N times the plane
+    e4 = orc.run_piv_ensemble([pairs[0]] * 4, [p], prec="f32", planes=True)
+    e1 = orc.run_piv_ensemble([pairs[0]], [p], prec="f32", planes=True)
+    assert np.array_equal(e4["planes"], 4 * e1["planes"])
+    assert np.array_equal(e4["outliers"], e1["outliers"])
+    fin = np.isfinite(e1["u"])
+    assert np.abs(e4["u"][fin] - e1["u"][fin]).max() < 1e-5
+
+
+def test_ensemble_low_snr_reference_bands(hh, orc):
+    # behavioural bands of test/test_ensemble.jl:20-33 and :40-51 (unmasked)
+    du, dv = 1.5, -2.0
+    pairs = low_snr_pairs()
+    p = P(hh, window_size=32, overlap=16)
+    ens = orc.run_piv_ensemble(pairs, [p])
+    assert np.all(np.abs(ens["u"] - du) < 0.5) and np.all(np.abs(ens["v"] - dv) < 0.5)
+    assert abs(np.median(ens["u"]) - du) < 0.25 and abs(np.median(ens["v"]) - dv) < 0.25
+    assert ens["outliers"].sum() <= 2
+    single = orc.run_piv(*pairs[0], [p])
+    err = lambda r: np.median(np.abs(r["u"][np.isfinite(r["u"])] - du))
+    assert err(ens) <= err(single)
+    multi = orc.run_piv_ensemble(pairs, [q.to_struct() for q in hh.multipass_parameters([64, 32])])
+    assert abs(np.median(multi["u"]) - du) < 0.15 and abs(np.median(multi["v"]) - dv) < 0.15
+    assert np.all(np.abs(multi["u"] - du) < 0.5)
