@@ -26,8 +26,8 @@ EXPORTS = (
     "hh_set_pair", "hh_set_pair_device", "hh_prefilter", "hh_deform", "hh_get_images", "hh_get_coefs",
     "hh_process_windows", "hh_correlation_planes", "hh_uncertainty_sweep", "hh_validate_and_replace",
     "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
-    "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_stream", "hh_synchronize",
-    "hh_set_timing", "hh_kernel_time",
+    "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
+    "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_kernel_time",
 )
 
 
@@ -84,6 +84,9 @@ def load():
         "hh_ensemble_analyze": (i32, [vp, i32, i32] + [vp] * 8),
         "hh_ensemble_planes": (i32, [vp, vp]),
         "hh_run_piv_ensemble": (i32, [vp, vp, vp, i32, i32, i32, pp, i32, i32, C.POINTER(hh_field)]),
+        "hh_energy": (i32, [vp, pp, vp, i32, vp, i32, vp, vp, C.POINTER(C.c_double)]),
+        "hh_grad_energy": (i32, [vp, pp, vp, i32, vp, i32, vp, vp, C.POINTER(C.c_double), vp, vp]),
+        "hh_refine_nodes": (i32, [vp, pp, vp, i32, vp, i32, vp, vp, i32, vp]),
         "hh_stream": (vp, [vp]),
         "hh_synchronize": (i32, [vp]),
         "hh_set_timing": (i32, [vp, i32]),
@@ -364,6 +367,43 @@ class Context:
         res["outliers"] = res["outliers"].astype(bool)
         res["mask"] = res["mask"].astype(bool)
         return res
+
+    # ---- correlation energy and its adjoint ------------------------------------
+    @staticmethod
+    def _nodes(py, px, pu, pv):
+        py = np.ascontiguousarray(py, dtype=np.float32)
+        px = np.ascontiguousarray(px, dtype=np.float32)
+        pu, pv = _f32(pu).copy(order="F"), _f32(pv).copy(order="F")
+        if pu.shape != (py.size, px.size) or pv.shape != pu.shape:
+            raise DimensionMismatch(f"node field must be {(py.size, px.size)}, got {pu.shape} and {pv.shape}")
+        return py, px, pu, pv
+
+    def energy(self, p, py, px, pu, pv):
+        st = p.to_struct() if hasattr(p, "to_struct") else p
+        py, px, pu, pv = self._nodes(py, px, pu, pv)
+        E = C.c_double()
+        _check(self.h, load().hh_energy(self.h, C.byref(st), _ptr(py), py.size, _ptr(px), px.size, _ptr(pu), _ptr(pv),
+                                         C.byref(E)), "energy")
+        return E.value
+
+    def grad_energy(self, p, py, px, pu, pv):
+        """(E, dE/dpu, dE/dpv) by the hand-written adjoint."""
+        st = p.to_struct() if hasattr(p, "to_struct") else p
+        py, px, pu, pv = self._nodes(py, px, pu, pv)
+        E = C.c_double()
+        gu, gv = np.zeros_like(pu), np.zeros_like(pv)
+        _check(self.h, load().hh_grad_energy(self.h, C.byref(st), _ptr(py), py.size, _ptr(px), px.size, _ptr(pu),
+                                              _ptr(pv), C.byref(E), _ptr(gu), _ptr(gv)), "grad_energy")
+        return E.value, gu, gv
+
+    def refine_nodes(self, p, py, px, pu, pv, niter=20):
+        """niter gradient steps on the node field: (pu, pv, energies)."""
+        st = p.to_struct() if hasattr(p, "to_struct") else p
+        py, px, pu, pv = self._nodes(py, px, pu, pv)
+        en = np.zeros(niter + 1, dtype=np.float64)
+        _check(self.h, load().hh_refine_nodes(self.h, C.byref(st), _ptr(py), py.size, _ptr(px), px.size, _ptr(pu),
+                                               _ptr(pv), niter, _ptr(en)), "refine_nodes")
+        return pu, pv, en
 
     # ---- stream and timing -------------------------------------------------
     def stream(self) -> int:

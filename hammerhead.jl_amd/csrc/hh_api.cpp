@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -122,6 +123,11 @@ struct hh_ctx {
     DBuf<int32_t> ensjobs;
     DBuf<double> ensacc, ensstats;
     DBuf<float> arena;  // the pairs of hh_run_piv_ensemble, resident in HBM
+    // correlation energy and its adjoint
+    DBuf<float4> estats;
+    DBuf<float> ew, epart, egrad, enode;
+    DBuf<double> esum;
+    DBuf<int2> ecells;
     // final result of hh_run_piv
     PassGrid final_grid;
     bool have_result = false;
@@ -882,6 +888,169 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     c->have_result = true;
 }
 
+// ---- windowed correlation energy and its adjoint ------------------------------
+
+// lin_axis of the device (interp.h), in the same float arithmetic.
+int host_lin_axis(float q0, float qstep, int n, float q)
+{
+    if (n == 1) return 0;
+    float t = (q - q0) / qstep;
+    t = t < 0.f ? 0.f : t;
+    t = t > (float)(n - 1) ? (float)(n - 1) : t;
+    int i = (int)std::floor(t);
+    return i > n - 2 ? n - 2 : i;
+}
+
+struct EnergySetup {
+    EnergyArgs g{};
+    int ncy = 0, ncx = 0;
+    const int2 *rows = nullptr, *cols = nullptr;
+};
+
+// Node grid, windows and node-interval pixel ranges of one energy call; the
+// node field itself is uploaded by energy_nodes.
+EnergySetup energy_setup(hh_ctx *c, const hh_pass &p, const float *py, int gny, const float *px, int gnx)
+{
+    need_pair(c);
+    check_pass_valid(p);
+    if (p.search_area_size[0] != p.window_size[0] || p.search_area_size[1] != p.window_size[1] ||
+        p.window_size[0] != p.window_size[1])
+        throw Error(HH_EUNSUPPORTED, "the correlation-energy adjoint takes square windows without search margins");
+    const int w = p.window_size[0];
+    if (w != 8 && w != 16 && w != 32)
+        throw Error(HH_EUNSUPPORTED, "the correlation-energy adjoint supports 8, 16 and 32 px windows");
+    if (gny < 1 || gnx < 1 || !py || !px) throw Error(HH_EINVAL, "empty node grid");
+    if ((gny > 1 && !(py[1] > py[0])) || (gnx > 1 && !(px[1] > px[0])))
+        throw Error(HH_EINVAL, "node knots must increase");
+    if (!c->coefs_valid) do_prefilter(c);
+    PassGrid grid = make_grid(p, c->nr, c->nc);
+    if (grid.ny * grid.nx == 0) throw Error(HH_EINVAL, "the pass grid is empty");
+    EnergySetup s;
+    EnergyArgs &g = s.g;
+    g.cA = c->coefA.p;
+    g.cB = c->coefB.p;
+    g.nr = c->nr;
+    g.nc = c->nc;
+    g.gny = gny;
+    g.gnx = gnx;
+    g.y0 = py[0];
+    g.ysp = gny > 1 ? py[1] - py[0] : 1.f;
+    g.x0 = px[0];
+    g.xsp = gnx > 1 ? px[1] - px[0] : 1.f;
+    g.w = w;
+    g.ny = grid.ny;
+    g.nx = grid.nx;
+    g.r0 = grid.jobs[2];
+    g.c0 = grid.jobs[3];
+    g.st_r = w - p.overlap[0];
+    g.st_c = w - p.overlap[1];
+    g.apod = get_config(c, p).apod.p;
+    s.ncy = std::max(gny - 1, 1);
+    s.ncx = std::max(gnx - 1, 1);
+    std::vector<int2> cells(s.ncy + s.ncx, make_int2(INT32_MAX, INT32_MIN));
+    for (int r = 1; r <= c->nr; ++r) {
+        int2 &q = cells[host_lin_axis(g.y0, g.ysp, gny, (float)r)];
+        q.x = std::min(q.x, r);
+        q.y = std::max(q.y, r);
+    }
+    for (int cc = 1; cc <= c->nc; ++cc) {
+        int2 &q = cells[s.ncy + host_lin_axis(g.x0, g.xsp, gnx, (float)cc)];
+        q.x = std::min(q.x, cc);
+        q.y = std::max(q.y, cc);
+    }
+    // one pixel of slack each side (the kernel re-checks membership)
+    for (int k = 0; k < s.ncy + s.ncx; ++k) {
+        const int n = k < s.ncy ? c->nr : c->nc;
+        int2 &q = cells[k];
+        if (q.x > q.y) {
+            q = make_int2(1, 0);  // empty interval
+            continue;
+        }
+        q.x = std::max(1, q.x - 1);
+        q.y = std::min(n, q.y + 1);
+    }
+    int2 *dcells = c->ecells.ensure(cells.size());
+    HH_HIP(hipMemcpyAsync(dcells, cells.data(), sizeof(int2) * cells.size(), hipMemcpyHostToDevice, c->stream));
+    HH_HIP(hipStreamSynchronize(c->stream));
+    s.rows = dcells;
+    s.cols = dcells + s.ncy;
+    const size_t nw = (size_t)grid.ny * grid.nx;
+    c->estats.ensure(nw);
+    c->ew.ensure(nw);
+    c->esum.ensure(1);
+    c->epart.ensure((size_t)8 * s.ncy * s.ncx);
+    c->egrad.ensure((size_t)2 * gny * gnx);
+    c->enode.ensure((size_t)2 * gny * gnx);
+    return s;
+}
+
+void energy_nodes(hh_ctx *c, EnergySetup &s, const float *pu, const float *pv)
+{
+    if (!pu || !pv) throw Error(HH_EINVAL, "null node field");
+    const size_t n = (size_t)s.g.gny * s.g.gnx;
+    HH_HIP(hipMemcpyAsync(c->enode.p, pu, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HH_HIP(hipMemcpyAsync(c->enode.p + n, pv, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    s.g.pu = c->enode.p;
+    s.g.pv = c->enode.p + n;
+}
+
+double energy_eval(hh_ctx *c, EnergySetup &s, const float *pu, const float *pv, float *gu, float *gv)
+{
+    energy_nodes(c, s, pu, pv);
+    timed(c, "energy", s.g.w, [&] { launch_energy(s.g, c->estats.p, c->ew.p, c->esum.p, c->stream); });
+    const size_t n = (size_t)s.g.gny * s.g.gnx;
+    if (gu || gv) {
+        timed(c, "energy_adjoint", s.g.w, [&] {
+            launch_energy_adjoint(s.g, c->estats.p, s.rows, s.cols, s.ncy, s.ncx, c->epart.p, c->egrad.p,
+                                  c->egrad.p + n, c->stream);
+        });
+        if (gu) HH_HIP(hipMemcpyAsync(gu, c->egrad.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        if (gv) HH_HIP(hipMemcpyAsync(gv, c->egrad.p + n, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    double E = 0.0;
+    HH_HIP(hipMemcpyAsync(&E, c->esum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HH_HIP(hipStreamSynchronize(c->stream));
+    return E;
+}
+
+// Gradient descent on the node field with a backtracking step: the first
+// trial moves the largest node by tau px (0.25, then twice the last accepted
+// move, at most 1 px), halved until the energy decreases.
+void refine_nodes(hh_ctx *c, EnergySetup &s, float *pu, float *pv, int niter, double *energies)
+{
+    const size_t n = (size_t)s.g.gny * s.g.gnx;
+    std::vector<float> gu(n), gv(n), tu(n), tv(n);
+    double E = energy_eval(c, s, pu, pv, gu.data(), gv.data());
+    double tau = 0.25;
+    int it = 0;
+    for (; it < niter; ++it) {
+        if (energies) energies[it] = E;
+        double gmax = 0.0;
+        for (size_t k = 0; k < n; ++k) gmax = std::max(gmax, (double)std::max(std::fabs(gu[k]), std::fabs(gv[k])));
+        if (!(gmax > 0.0) || !std::isfinite(gmax) || !std::isfinite(E)) break;
+        double alpha = tau / gmax, Et = E;
+        bool ok = false;
+        for (int bt = 0; bt < 8 && !ok; ++bt) {
+            for (size_t k = 0; k < n; ++k) {
+                tu[k] = (float)(pu[k] - alpha * gu[k]);
+                tv[k] = (float)(pv[k] - alpha * gv[k]);
+            }
+            Et = energy_eval(c, s, tu.data(), tv.data(), nullptr, nullptr);
+            if (Et < E)
+                ok = true;
+            else
+                alpha /= 2;
+        }
+        if (!ok) break;
+        std::copy(tu.begin(), tu.end(), pu);
+        std::copy(tv.begin(), tv.end(), pv);
+        E = energy_eval(c, s, pu, pv, gu.data(), gv.data());
+        tau = std::min(1.0, 2.0 * alpha * gmax);
+    }
+    if (energies)
+        for (int k = it; k <= niter; ++k) energies[k] = E;
+}
+
 void fetch(hh_ctx *c, hh_field *r)
 {
     if (!c->have_result) throw Error(HH_ESTATE, "no result: call hh_run_piv first");
@@ -1400,6 +1569,38 @@ int hh_run_piv_ensemble(hh_ctx *c, const float *const *A, const float *const *B,
         if (!passes) throw Error(HH_EINVAL, "null pass schedule");
         run_ensemble(c, A, B, npairs, nr, nc, passes, npasses, predictor_smoothing);
         if (result) fetch(c, result);
+    });
+}
+
+int hh_energy(hh_ctx *c, const hh_pass *pass, const float *py, int gny, const float *px, int gnx, const float *pu,
+              const float *pv, double *energy)
+{
+    return guarded(c, [&] {
+        if (!pass || !energy) throw Error(HH_EINVAL, "null argument");
+        EnergySetup s = energy_setup(c, *pass, py, gny, px, gnx);
+        *energy = energy_eval(c, s, pu, pv, nullptr, nullptr);
+    });
+}
+
+int hh_grad_energy(hh_ctx *c, const hh_pass *pass, const float *py, int gny, const float *px, int gnx, const float *pu,
+                   const float *pv, double *energy, float *grad_u, float *grad_v)
+{
+    return guarded(c, [&] {
+        if (!pass || !grad_u || !grad_v) throw Error(HH_EINVAL, "null argument");
+        EnergySetup s = energy_setup(c, *pass, py, gny, px, gnx);
+        const double E = energy_eval(c, s, pu, pv, grad_u, grad_v);
+        if (energy) *energy = E;
+    });
+}
+
+int hh_refine_nodes(hh_ctx *c, const hh_pass *pass, const float *py, int gny, const float *px, int gnx, float *pu,
+                    float *pv, int niter, double *energies)
+{
+    return guarded(c, [&] {
+        if (!pass) throw Error(HH_EINVAL, "null argument");
+        if (niter < 0) throw Error(HH_EINVAL, "niter must be non-negative");
+        EnergySetup s = energy_setup(c, *pass, py, gny, px, gnx);
+        refine_nodes(c, s, pu, pv, niter, energies);
     });
 }
 

@@ -123,4 +123,21 @@ void launch_uq(int w, const float *apod, const float *A, const float *B, int nr,
 void launch_uq_finalize(const double *stats, const int32_t *jobs, int njobs, int ny, float *unc_u, float *unc_v,
                         hipStream_t s);
 
+// ---- correlation energy and its adjoint (k_adjoint.hip) ---------------------
+struct EnergyArgs {
+    const float *cA, *cB;  // prefiltered pair, (nr+2) x (nc+2) each
+    int nr, nc;
+    const float *pu, *pv;  // node field gny x gnx on uniform knots
+    int gny, gnx;
+    float y0, ysp, x0, xsp;
+    int w, ny, nx, r0, c0, st_r, st_c;  // windows of the pass
+    const float *apod;                   // w x w
+};
+// Per-window statistics (mA, mB, PA/N, PB/N) and energies, and E in *esum.
+void launch_energy(const EnergyArgs &g, float4 *stats, float *ew, double *esum, hipStream_t s);
+// dE/dpu, dE/dpv from the statistics of launch_energy; rows/cols: the
+// pixel range of each node interval (ncy, ncx of them), part: 8 per cell.
+void launch_energy_adjoint(const EnergyArgs &g, const float4 *stats, const int2 *rows, const int2 *cols, int ncy,
+                           int ncx, float *part, float *gu, float *gv, hipStream_t s);
+
 }  // namespace hh

@@ -1,0 +1,114 @@
+// interp.h — device interpolation helpers shared by the deformation and the
+// adjoint kernels: the bilinear Flat() predictor of the node grid and the
+// cubic B-spline sample of prefiltered coefficients.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hh {
+
+// _pk_lin_axis (src/ka_backend.jl:257-263), 0-based interval index.
+__device__ __forceinline__ void lin_axis(float q0, float qstep, int n, float q, int &i, float &l)
+{
+    if (n == 1) {
+        i = 0;
+        l = 0.f;
+        return;
+    }
+    float t = (q - q0) / qstep;
+    t = t < 0.f ? 0.f : t;
+    t = t > (float)(n - 1) ? (float)(n - 1) : t;
+    int ii = (int)floorf(t);
+    if (ii > n - 2) ii = n - 2;
+    i = ii;
+    l = t - (float)ii;
+}
+
+__device__ __forceinline__ float bilinear(const float *__restrict__ p, int gny, int gnx, float y0, float ysp,
+                                          float x0, float xsp, float y, float x)
+{
+    int iy, ix;
+    float ly, lx;
+    lin_axis(y0, ysp, gny, y, iy, ly);
+    lin_axis(x0, xsp, gnx, x, ix, lx);
+    const int iy2 = gny == 1 ? iy : iy + 1, ix2 = gnx == 1 ? ix : ix + 1;
+    const float a = p[iy + ix * gny] + lx * (p[iy + ix2 * gny] - p[iy + ix * gny]);
+    const float b = p[iy2 + ix * gny] + lx * (p[iy2 + ix2 * gny] - p[iy2 + ix * gny]);
+    return a + ly * (b - a);
+}
+
+__device__ __forceinline__ void cubw(float d, float w[4])
+{
+    const float d2 = d * d, d3 = d2 * d, e = 1.f - d;
+    w[0] = e * e * e / 6.f;
+    w[1] = (4.f - 6.f * d2 + 3.f * d3) / 6.f;
+    w[2] = (1.f + 3.f * d + 3.f * d2 - 3.f * d3) / 6.f;
+    w[3] = d3 / 6.f;
+}
+
+__device__ __forceinline__ float cubic(const float *__restrict__ C, int nr, int nc, float yy, float xx)
+{
+    if (yy < 1.f || yy > (float)nr || xx < 1.f || xx > (float)nc) return 0.f;
+    int iy = (int)floorf(yy);
+    if (iy > nr - 1) iy = nr - 1;
+    int ix = (int)floorf(xx);
+    if (ix > nc - 1) ix = nc - 1;
+    float wy[4], wx[4];
+    cubw(yy - (float)iy, wy);
+    cubw(xx - (float)ix, wx);
+    const int N1 = nr + 2;
+    float s = 0.f;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        s += wx[n] * (wy[0] * col[0] + wy[1] * col[1] + wy[2] * col[2] + wy[3] * col[3]);
+    }
+    return s;
+}
+
+// Derivative of the cubic B-spline weights of cubw with respect to d.
+__device__ __forceinline__ void cubw_d(float d, float g[4])
+{
+    const float e = 1.f - d;
+    g[0] = -e * e / 2.f;
+    g[1] = -2.f * d + 1.5f * d * d;
+    g[2] = 0.5f + d - 1.5f * d * d;
+    g[3] = d * d / 2.f;
+}
+
+// cubic() and its partial derivatives (sy along dim 1, sx along dim 2);
+// zero with zero slope outside [1, nr] x [1, nc].
+__device__ __forceinline__ float cubic_d(const float *__restrict__ C, int nr, int nc, float yy, float xx, float &sy,
+                                         float &sx)
+{
+    if (yy < 1.f || yy > (float)nr || xx < 1.f || xx > (float)nc) {
+        sy = sx = 0.f;
+        return 0.f;
+    }
+    int iy = (int)floorf(yy);
+    if (iy > nr - 1) iy = nr - 1;
+    int ix = (int)floorf(xx);
+    if (ix > nc - 1) ix = nc - 1;
+    float wy[4], wx[4], gy[4], gx[4];
+    cubw(yy - (float)iy, wy);
+    cubw(xx - (float)ix, wx);
+    cubw_d(yy - (float)iy, gy);
+    cubw_d(xx - (float)ix, gx);
+    const int N1 = nr + 2;
+    float s = 0.f, s_y = 0.f, s_x = 0.f;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        const float c0 = col[0], c1 = col[1], c2 = col[2], c3 = col[3];
+        const float cv = wy[0] * c0 + wy[1] * c1 + wy[2] * c2 + wy[3] * c3;
+        const float cd = gy[0] * c0 + gy[1] * c1 + gy[2] * c2 + gy[3] * c3;
+        s += wx[n] * cv;
+        s_y += wx[n] * cd;
+        s_x += gx[n] * cv;
+    }
+    sy = s_y;
+    sx = s_x;
+    return s;
+}
+
+}  // namespace hh

@@ -21,6 +21,7 @@
 // weights of _pk_cubic (src/ka_backend.jl:265-291), zero outside [1, n].
 
 #include "hh_internal.h"
+#include "interp.h"
 
 namespace hh {
 
@@ -98,65 +99,6 @@ __global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict_
 #pragma unroll
     for (int d = 1; d <= kTaps; ++d) s += taps.h[d] * (ext_value(get, k - d, nc) + ext_value(get, k + d, nc));
     coef[e] = s;
-}
-
-// _pk_lin_axis (src/ka_backend.jl:257-263), 0-based interval index.
-__device__ __forceinline__ void lin_axis(float q0, float qstep, int n, float q, int &i, float &l)
-{
-    if (n == 1) {
-        i = 0;
-        l = 0.f;
-        return;
-    }
-    float t = (q - q0) / qstep;
-    t = t < 0.f ? 0.f : t;
-    t = t > (float)(n - 1) ? (float)(n - 1) : t;
-    int ii = (int)floorf(t);
-    if (ii > n - 2) ii = n - 2;
-    i = ii;
-    l = t - (float)ii;
-}
-
-__device__ __forceinline__ float bilinear(const float *__restrict__ p, int gny, int gnx, float y0, float ysp,
-                                          float x0, float xsp, float y, float x)
-{
-    int iy, ix;
-    float ly, lx;
-    lin_axis(y0, ysp, gny, y, iy, ly);
-    lin_axis(x0, xsp, gnx, x, ix, lx);
-    const int iy2 = gny == 1 ? iy : iy + 1, ix2 = gnx == 1 ? ix : ix + 1;
-    const float a = p[iy + ix * gny] + lx * (p[iy + ix2 * gny] - p[iy + ix * gny]);
-    const float b = p[iy2 + ix * gny] + lx * (p[iy2 + ix2 * gny] - p[iy2 + ix * gny]);
-    return a + ly * (b - a);
-}
-
-__device__ __forceinline__ void cubw(float d, float w[4])
-{
-    const float d2 = d * d, d3 = d2 * d, e = 1.f - d;
-    w[0] = e * e * e / 6.f;
-    w[1] = (4.f - 6.f * d2 + 3.f * d3) / 6.f;
-    w[2] = (1.f + 3.f * d + 3.f * d2 - 3.f * d3) / 6.f;
-    w[3] = d3 / 6.f;
-}
-
-__device__ __forceinline__ float cubic(const float *__restrict__ C, int nr, int nc, float yy, float xx)
-{
-    if (yy < 1.f || yy > (float)nr || xx < 1.f || xx > (float)nc) return 0.f;
-    int iy = (int)floorf(yy);
-    if (iy > nr - 1) iy = nr - 1;
-    int ix = (int)floorf(xx);
-    if (ix > nc - 1) ix = nc - 1;
-    float wy[4], wx[4];
-    cubw(yy - (float)iy, wy);
-    cubw(xx - (float)ix, wx);
-    const int N1 = nr + 2;
-    float s = 0.f;
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
-        s += wx[n] * (wy[0] * col[0] + wy[1] * col[1] + wy[2] * col[2] + wy[3] * col[3]);
-    }
-    return s;
 }
 
 __global__ __launch_bounds__(256) void k_deform(const float *__restrict__ cA, const float *__restrict__ cB, int nr,

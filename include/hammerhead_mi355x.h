@@ -249,6 +249,30 @@ int hh_ensemble_planes(hh_ctx *ctx, double *planes);
 int hh_run_piv_ensemble(hh_ctx *ctx, const float *const *A, const float *const *B, int npairs, int nr, int nc,
                         const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result);
 
+/* ---- correlation energy and its adjoint (A17) -------------------------- */
+/* No counterpart in the reference, which has no gradient path (nearest:
+ * the non-gradient correlate_deformable, src/correlators.jl:620-638). For a
+ * node field pu, pv (gny x gnx, column-major) on uniform knots py, px, the
+ * current pair is warped symmetrically as in deform_images
+ * (src/pipeline.jl:936-965: bilinear Flat() node values, cubic B-spline
+ * samples A(r - v/2, c - u/2), B(r + v/2, c + u/2)), and over the windows of
+ * `pass` (square, 8/16/32 px; grid and apodization as pass_grid)
+ *     E = - sum_w sum_{x in w} p_x (A^_x - mean_w A^) (B^_x - mean_w B^).
+ * The prefiltered pair is computed on first use (hh_prefilter). */
+int hh_energy(hh_ctx *ctx, const hh_pass *pass, const float *py, int gny, const float *px, int gnx,
+              const float *pu, const float *pv, double *energy);
+/* E and its gradient dE/dpu, dE/dpv (gny x gnx) by the hand-written adjoint,
+ * which recomputes the warped windows instead of storing them. energy is
+ * nullable. */
+int hh_grad_energy(hh_ctx *ctx, const hh_pass *pass, const float *py, int gny, const float *px, int gnx,
+                   const float *pu, const float *pv, double *energy, float *grad_u, float *grad_v);
+/* Gradient-based refinement of a node field: niter descent steps with a
+ * backtracking step length (the largest node moves at most 1 px per step);
+ * pu, pv updated in place; energies (nullable, niter + 1) receives E before
+ * each step and at the end. */
+int hh_refine_nodes(hh_ctx *ctx, const hh_pass *pass, const float *py, int gny, const float *px, int gnx,
+                    float *pu, float *pv, int niter, double *energies);
+
 /* ---- stream and timing ------------------------------------------------- */
 
 /* The context's HIP stream (hipStream_t), for callers that time or order

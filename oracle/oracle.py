@@ -398,5 +398,23 @@ def run_piv_ensemble(pairs, passes, predictor_smoothing=True, nthreads=8, prec="
     return res
 
 
+def grad_energy(pass_struct, A, B, py, px, pu, pv, gradient=True, interp="cubic", prec="f64"):
+    """Windowed correlation energy of the node field (pu, pv on knots py, px)
+    and, with ``gradient``, its gradient: (E, gu, gv)."""
+    dt = _dt(prec)
+    A, B = _f(A, dt), _f(B, dt)
+    py = np.ascontiguousarray(py, dtype=dt)
+    px = np.ascontiguousarray(px, dtype=dt)
+    pu, pv = _f(pu, dt), _f(pv, dt)
+    gu = np.zeros_like(pu) if gradient else None
+    gv = np.zeros_like(pv) if gradient else None
+    fn = _fn("orc_grad_energy", prec)
+    fn.restype = C.c_double
+    E = fn(C.byref(pass_struct), _p(A), _p(B), C.c_int(A.shape[0]), C.c_int(A.shape[1]),
+           C.c_int(1 if interp == "cubic" else 0), _p(py), C.c_int(py.size),
+           _p(px), C.c_int(px.size), _p(pu), _p(pv), _p(gu), _p(gv))
+    return (E, gu, gv) if gradient else E
+
+
 def default_threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))

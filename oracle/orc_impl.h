@@ -1638,6 +1638,182 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
     return rc;
 }
 
+/* ======================================================================
+ * Windowed correlation energy and its adjoint (build-defined: the reference
+ * has no gradient path; its closest relative is correlate_deformable,
+ * src/correlators.jl:620-638). For a node field (pu, pv) on the uniform
+ * knots (py, px), pixel displacements are the bilinear Flat() values of
+ * the field (as apply_predictor, src/pipeline.jl:575-600); the pair is
+ * warped symmetrically, A^(r,c) = A(r - v/2, c - u/2) and
+ * B^(r,c) = B(r + v/2, c + u/2), with bilinear image sampling (zero outside
+ * [1, n]); over the windows of pass p (pass_grid, apodization p_x),
+ *     E = - sum_w sum_{x in w} p_x (A^_x - mean_w A^) (B^_x - mean_w B^).
+ * ==================================================================== */
+
+/* Bilinear sample of I (nr x nc, 1-based coordinates) and its partial
+ * derivatives; zero (and zero slope) outside [1, nr] x [1, nc]. */
+static REAL F(orc_bil_sample)(const REAL *I, int nr, int nc, REAL y, REAL x, REAL *sy, REAL *sx)
+{
+    if (y < 1 || y > (REAL)nr || x < 1 || x > (REAL)nc) {
+        *sy = *sx = 0;
+        return 0;
+    }
+    int i0 = (int)floor((double)y), j0 = (int)floor((double)x);
+    if (i0 > nr - 1) i0 = nr - 1;
+    if (j0 > nc - 1) j0 = nc - 1;
+    REAL fy = y - (REAL)i0, fx = x - (REAL)j0;
+    const REAL *q = I + (size_t)(i0 - 1) + (size_t)(j0 - 1) * nr;
+    REAL a = q[0], b = q[nr], c = q[1], d = q[nr + 1]; /* (i0,j0) (i0,j0+1) (i0+1,j0) (i0+1,j0+1) */
+    *sy = (1 - fx) * (c - a) + fx * (d - b);
+    *sx = (1 - fy) * (b - a) + fy * (d - c);
+    return (1 - fy) * ((1 - fx) * a + fx * b) + fy * ((1 - fx) * c + fx * d);
+}
+
+/* Cubic B-spline sample of the padded coefficients C ((nr+2) x (nc+2), as
+ * orc_cubic) and its partial derivatives; zero outside [1, nr] x [1, nc]. */
+static REAL F(orc_cub_sample)(const REAL *Cf, int nr, int nc, REAL y, REAL x, REAL *sy, REAL *sx)
+{
+    if (y < 1 || y > (REAL)nr || x < 1 || x > (REAL)nc) {
+        *sy = *sx = 0;
+        return 0;
+    }
+    int iy = (int)floor((double)y), ix = (int)floor((double)x);
+    if (iy > nr - 1) iy = nr - 1;
+    if (ix > nc - 1) ix = nc - 1;
+    REAL dy = y - (REAL)iy, dx = x - (REAL)ix;
+    REAL wy[4], wx[4], gy[4], gx[4];
+    REAL ey = 1 - dy, ex = 1 - dx;
+    wy[0] = ey * ey * ey / 6; wy[1] = (4 - 6 * dy * dy + 3 * dy * dy * dy) / 6;
+    wy[2] = (1 + 3 * dy + 3 * dy * dy - 3 * dy * dy * dy) / 6; wy[3] = dy * dy * dy / 6;
+    wx[0] = ex * ex * ex / 6; wx[1] = (4 - 6 * dx * dx + 3 * dx * dx * dx) / 6;
+    wx[2] = (1 + 3 * dx + 3 * dx * dx - 3 * dx * dx * dx) / 6; wx[3] = dx * dx * dx / 6;
+    gy[0] = -ey * ey / 2; gy[1] = -2 * dy + (REAL)1.5 * dy * dy; gy[2] = (REAL)0.5 + dy - (REAL)1.5 * dy * dy;
+    gy[3] = dy * dy / 2;
+    gx[0] = -ex * ex / 2; gx[1] = -2 * dx + (REAL)1.5 * dx * dx; gx[2] = (REAL)0.5 + dx - (REAL)1.5 * dx * dx;
+    gx[3] = dx * dx / 2;
+    int N1 = nr + 2;
+    REAL s = 0, s_y = 0, s_x = 0;
+    for (int n = 0; n < 4; n++) {
+        const REAL *col = Cf + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        REAL cv = wy[0] * col[0] + wy[1] * col[1] + wy[2] * col[2] + wy[3] * col[3];
+        REAL cd = gy[0] * col[0] + gy[1] * col[1] + gy[2] * col[2] + gy[3] * col[3];
+        s += wx[n] * cv;
+        s_y += wx[n] * cd;
+        s_x += gx[n] * cv;
+    }
+    *sy = s_y;
+    *sx = s_x;
+    return s;
+}
+
+/* E of the node field; with gu/gv (gny x gnx) non-NULL also dE/dpu, dE/dpv.
+ * interp: 0 bilinear image sampling, 1 the cubic B-spline interpolant of
+ * the deformation path (image_interpolant, src/pipeline.jl:909-915).
+ * Returns E, or NAN for an invalid pass. */
+double F(orc_grad_energy)(const hh_pass *p, const REAL *A, const REAL *B, int nr, int nc, int interp, const REAL *py,
+                          int gny, const REAL *px, int gnx, const REAL *pu, const REAL *pv, REAL *gu, REAL *gv)
+{
+    int ny, nx;
+    if (F(orc_pass_grid)(p, nr, nc, &ny, &nx, NULL, NULL, NULL) != 0) return NAN;
+    size_t ng = (size_t)ny * nx, npx = (size_t)nr * nc;
+    int32_t *jobs = (int32_t *)malloc(sizeof(int32_t) * 4 * (ng > 0 ? ng : 1));
+    F(orc_pass_grid)(p, nr, nc, &ny, &nx, NULL, NULL, jobs);
+    int wr = p->window_size[0], wc = p->window_size[1];
+    REAL *apod = (REAL *)malloc(sizeof(REAL) * wr * wc);
+    F(orc_apodization)(wr, wc, p->apodization, apod);
+    REAL y0 = py[0], ysp = gny > 1 ? py[1] - py[0] : 1, x0 = px[0], xsp = gnx > 1 ? px[1] - px[0] : 1;
+    /* per-pixel warp and slopes */
+    REAL *Ah = (REAL *)malloc(sizeof(REAL) * npx), *Bh = (REAL *)malloc(sizeof(REAL) * npx);
+    REAL *Ax = (REAL *)malloc(sizeof(REAL) * npx), *Ay = (REAL *)malloc(sizeof(REAL) * npx);
+    REAL *Bx = (REAL *)malloc(sizeof(REAL) * npx), *By = (REAL *)malloc(sizeof(REAL) * npx);
+    REAL *cA = NULL, *cB = NULL;
+    if (interp) {
+        cA = (REAL *)malloc(sizeof(REAL) * (size_t)(nr + 2) * (nc + 2));
+        cB = (REAL *)malloc(sizeof(REAL) * (size_t)(nr + 2) * (nc + 2));
+        F(orc_prefilter)(A, nr, nc, cA);
+        F(orc_prefilter)(B, nr, nc, cB);
+    }
+    for (int c = 1; c <= nc; c++)
+        for (int r = 1; r <= nr; r++) {
+            size_t e = (size_t)(r - 1) + (size_t)(c - 1) * nr;
+            REAL u = F(orc_bilinear)(pu, gny, gnx, y0, ysp, x0, xsp, (REAL)r, (REAL)c);
+            REAL v = F(orc_bilinear)(pv, gny, gnx, y0, ysp, x0, xsp, (REAL)r, (REAL)c);
+            if (interp) {
+                Ah[e] = F(orc_cub_sample)(cA, nr, nc, (REAL)r - v / 2, (REAL)c - u / 2, &Ay[e], &Ax[e]);
+                Bh[e] = F(orc_cub_sample)(cB, nr, nc, (REAL)r + v / 2, (REAL)c + u / 2, &By[e], &Bx[e]);
+            } else {
+                Ah[e] = F(orc_bil_sample)(A, nr, nc, (REAL)r - v / 2, (REAL)c - u / 2, &Ay[e], &Ax[e]);
+                Bh[e] = F(orc_bil_sample)(B, nr, nc, (REAL)r + v / 2, (REAL)c + u / 2, &By[e], &Bx[e]);
+            }
+        }
+    free(cA);
+    free(cB);
+    double *GA = gu ? (double *)calloc(npx, sizeof(double)) : NULL;
+    double *GB = gu ? (double *)calloc(npx, sizeof(double)) : NULL;
+    double E = 0, N = (double)wr * wc;
+    for (size_t k = 0; k < ng; k++) {
+        int rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
+        double sa = 0, sb = 0;
+        for (int j = 0; j < wc; j++)
+            for (int i = 0; i < wr; i++) {
+                size_t e = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
+                sa += Ah[e];
+                sb += Bh[e];
+            }
+        double mA = sa / N, mB = sb / N, PA = 0, PB = 0, Ew = 0;
+        for (int j = 0; j < wc; j++)
+            for (int i = 0; i < wr; i++) {
+                size_t e = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
+                double pw = apod[i + j * wr], a_ = Ah[e] - mA, b_ = Bh[e] - mB;
+                PA += pw * a_;
+                PB += pw * b_;
+                Ew -= pw * a_ * b_;
+            }
+        E += Ew;
+        if (!gu) continue;
+        for (int j = 0; j < wc; j++)
+            for (int i = 0; i < wr; i++) {
+                size_t e = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
+                double pw = apod[i + j * wr];
+                GA[e] += -pw * (Bh[e] - mB) + PB / N;
+                GB[e] += -pw * (Ah[e] - mA) + PA / N;
+            }
+    }
+    if (gu) {
+        for (size_t q = 0; q < (size_t)gny * gnx; q++) gu[q] = gv[q] = 0;
+        double *nu = (double *)calloc((size_t)gny * gnx, sizeof(double));
+        double *nv = (double *)calloc((size_t)gny * gnx, sizeof(double));
+        for (int c = 1; c <= nc; c++)
+            for (int r = 1; r <= nr; r++) {
+                size_t e = (size_t)(r - 1) + (size_t)(c - 1) * nr;
+                double gx = -0.5 * GA[e] * Ax[e] + 0.5 * GB[e] * Bx[e];
+                double gyv = -0.5 * GA[e] * Ay[e] + 0.5 * GB[e] * By[e];
+                int iy, ix;
+                REAL ly, lx;
+                F(orc_lin_axis)(y0, ysp, gny, (REAL)r, &iy, &ly);
+                F(orc_lin_axis)(x0, xsp, gnx, (REAL)c, &ix, &lx);
+                int iy2 = gny == 1 ? iy : iy + 1, ix2 = gnx == 1 ? ix : ix + 1;
+                double w00 = (1 - (double)ly) * (1 - (double)lx), w01 = (1 - (double)ly) * lx;
+                double w10 = (double)ly * (1 - (double)lx), w11 = (double)ly * lx;
+                nu[iy + ix * gny] += w00 * gx;
+                nu[iy + ix2 * gny] += w01 * gx;
+                nu[iy2 + ix * gny] += w10 * gx;
+                nu[iy2 + ix2 * gny] += w11 * gx;
+                nv[iy + ix * gny] += w00 * gyv;
+                nv[iy + ix2 * gny] += w01 * gyv;
+                nv[iy2 + ix * gny] += w10 * gyv;
+                nv[iy2 + ix2 * gny] += w11 * gyv;
+            }
+        for (size_t q = 0; q < (size_t)gny * gnx; q++) {
+            gu[q] = (REAL)nu[q];
+            gv[q] = (REAL)nv[q];
+        }
+        free(nu); free(nv);
+    }
+    free(jobs); free(apod); free(Ah); free(Bh); free(Ax); free(Ay); free(Bx); free(By); free(GA); free(GB);
+    return E;
+}
+
 #undef F
 #undef REAL_EPS
 #undef RLOG

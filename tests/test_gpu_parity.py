@@ -337,3 +337,78 @@ def test_ensemble_one_pair_matches_run_piv(hh, ctx):
         fin = np.isfinite(r[k])
         assert np.array_equal(fin, np.isfinite(e[k])), k
         assert np.abs(e[k][fin] - r[k][fin]).max() <= 1e-5 * max(1.0, np.abs(r[k][fin]).max()), k
+
+
+# ---------------------------------------------------------------- adjoint
+def _shear_nodes(hh, n):
+    pair = hh.synthetic.particle_pair(n, n, hh.synthetic.sine_shear_flow(3.0, 256.0), seed=105)
+    p32 = hh.PIVParameters(window_size=32, overlap=16)
+    x, y, _ = hh.pass_grid(p32, n, n)
+    return pair, y, x
+
+
+def test_energy_gradient_vs_oracle(hh, orc, ctx):
+    pair, py, px = _shear_nodes(hh, 256)
+    X, Y = np.meshgrid(px, py)
+    pu, pv = hh.synthetic.midpoint_truth(pair.flow, X, Y)
+    pu = np.asfortranarray(pu + 0.2 * np.sin(X / 23.0), dtype=np.float32)
+    pv = np.asfortranarray(pv + 0.2 * np.cos(Y / 31.0), dtype=np.float32)
+    ctx.set_pair(pair.A, pair.B)
+    p = P(hh, window_size=16, overlap=12, apodization="gauss")
+    say("energy gradient 256")
+    E, gu, gv = ctx.grad_energy(p, py, px, pu, pv)
+    Eo, ou, ov = orc.grad_energy(p.to_struct(), pair.A.astype(np.float64), pair.B.astype(np.float64), py, px,
+                                 pu.astype(np.float64), pv.astype(np.float64))
+    scale = max(np.abs(ou).max(), np.abs(ov).max())
+    eg = max(np.abs(gu - ou).max(), np.abs(gv - ov).max()) / scale
+    say("E", E, Eo, "rel", abs(E - Eo) / abs(Eo), "grad max rel", eg)
+    assert abs(E - Eo) <= 1e-5 * abs(Eo)
+    assert eg < 1e-3
+    assert ctx.energy(p, py, px, pu, pv) == E
+
+
+def test_energy_autograd_function(hh, ctx):
+    import torch
+
+    sys.path.insert(0, str(__import__("pathlib").Path(__file__).resolve().parent.parent))
+    from hammerhead_amd.adjoint import correlation_energy
+
+    pair, py, px = _shear_nodes(hh, 128)
+    ctx.set_pair(pair.A, pair.B)
+    p = P(hh, window_size=16, overlap=12, apodization="gauss")
+    pu = torch.zeros((py.size, px.size), dtype=torch.float32, requires_grad=True)
+    pv = torch.zeros((py.size, px.size), dtype=torch.float32, requires_grad=True)
+    E = correlation_energy(pu, pv, ctx, p, py, px)
+    E.backward()
+    E2, gu, gv = ctx.grad_energy(p, py, px, np.zeros((py.size, px.size)), np.zeros((py.size, px.size)))
+    assert float(E.detach()) == E2
+    assert np.array_equal(pu.grad.numpy(), gu) and np.array_equal(pv.grad.numpy(), gv)
+
+
+def test_refinement_reduces_error(hh, ctx):
+    # 1024^2 sine shear (3 px, 256 px wavelength): the 32/16 WIDIM field on its
+    # 63 x 63 nodes, refined by 20 gradient steps of the 16/12 Gaussian energy
+    from scipy.interpolate import RegularGridInterpolator
+
+    n = 1024
+    pair = hh.synthetic.config_sine_shear_1024()
+    ctx.set_pair(pair.A, pair.B)
+    r = ctx.run_piv(hh.multipass_parameters([64, 32], padding=True, apodization="gauss"))
+    py, px = r["y"], r["x"]
+    p = P(hh, window_size=16, overlap=12, apodization="gauss")
+    say("refine 1024")
+    u, v, en = ctx.refine_nodes(p, py, px, r["u"], r["v"], niter=20)
+    assert np.all(np.diff(en) <= 0) and en[-1] < en[0]
+    xs, ys, _ = hh.pass_grid(P(hh, window_size=16, overlap=12), n, n)
+    X, Y = np.meshgrid(xs, ys)
+    Tu, Tv = hh.synthetic.midpoint_truth(pair.flow, X, Y)
+    pts = np.stack([np.clip(Y, py[0], py[-1]).ravel(), np.clip(X, px[0], px[-1]).ravel()], 1)
+
+    def dense_rms(fu, fv):
+        iu = RegularGridInterpolator((py, px), fu)(pts).reshape(X.shape)
+        iv = RegularGridInterpolator((py, px), fv)(pts).reshape(X.shape)
+        return float(np.sqrt(np.mean((iu - Tu) ** 2 + (iv - Tv) ** 2)))
+
+    e0, e1 = dense_rms(r["u"], r["v"]), dense_rms(u, v)
+    say("dense rms before", e0, "after", e1, "energies", en[0], en[-1])
+    assert e1 < 0.9 * e0

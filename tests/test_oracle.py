@@ -595,3 +595,57 @@ def test_ensemble_low_snr_reference_bands(hh, orc):
     multi = orc.run_piv_ensemble(pairs, [q.to_struct() for q in hh.multipass_parameters([64, 32])])
     assert abs(np.median(multi["u"]) - du) < 0.15 and abs(np.median(multi["v"]) - dv) < 0.15
     assert np.all(np.abs(multi["u"] - du) < 0.5)
+
+
+# ---------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("interp", ["cubic", "bilinear"])
+def test_energy_at_rest_is_windowed_correlation(hh, orc, interp):
+    # zero node field: both interpolants reproduce the pixels, so E is the
+    # apodized zero-lag covariance of every window, computed here directly
+    rng = np.random.default_rng(2)
+    A, B = rng.random((40, 36)), rng.random((40, 36))
+    p = P(hh, window_size=8, overlap=4, apodization="gauss")
+    py, px = np.array([4.5, 20.5, 36.5]), np.array([4.5, 18.5, 32.5])
+    z = np.zeros((3, 3))
+    E = orc.grad_energy(p, A, B, py, px, z, z, gradient=False, interp=interp)
+    x, y, jobs = orc.pass_grid(p, 40, 36)
+    # apodization_window (src/correlators.jl:18-25): centre (w+1)/2, sigma w/4
+    i = np.arange(1, 9)
+    g1 = np.exp(-((i - 4.5) ** 2) / (2 * 2.0 ** 2))
+    apod = np.outer(g1, g1)
+    ref = 0.0
+    for k in range(jobs.shape[1]):
+        rs, cs = jobs[2, k] - 1, jobs[3, k] - 1
+        a = A[rs:rs + 8, cs:cs + 8]
+        b = B[rs:rs + 8, cs:cs + 8]
+        ref -= np.sum(apod * (a - a.mean()) * (b - b.mean()))
+    assert E == pytest.approx(ref, rel=1e-12, abs=1e-12)
+
+
+@pytest.mark.parametrize("interp", ["cubic", "bilinear"])
+def test_energy_gradient_central_differences(hh, orc, interp):
+    # the adjoint against central differences of the FP64 energy (h = 1e-3 px)
+    pair = hh.synthetic.particle_pair(96, 80, hh.synthetic.sine_shear_flow(2.0, 64.0), seed=5, dtype=np.float64)
+    p = P(hh, window_size=16, overlap=12, apodization="gauss")
+    py, px = np.arange(8.5, 96, 16.0), np.arange(8.5, 80, 16.0)
+    # samples are zero outside the image, so E jumps where a border pixel's
+    # displacement changes sign; positive fields keep every border sample on
+    # one side of the edge, which leaves E smooth in all node values
+    rng = np.random.default_rng(1)
+    pu = rng.uniform(0.2, 1, (py.size, px.size))
+    pv = rng.uniform(0.2, 1, (py.size, px.size))
+    E, gu, gv = orc.grad_energy(p, pair.A, pair.B, py, px, pu, pv, interp=interp)
+    h = 1e-3
+    scale = max(np.abs(gu).max(), np.abs(gv).max())
+    for (i, j) in [(0, 0), (2, 3), (py.size - 1, px.size - 1), (3, 0)]:
+        for field in ("u", "v"):
+            d = np.zeros_like(pu)
+            d[i, j] = h
+            args = lambda s: (pu + s * d, pv) if field == "u" else (pu, pv + s * d)
+            fd = (orc.grad_energy(p, pair.A, pair.B, py, px, *args(1), gradient=False, interp=interp) -
+                  orc.grad_energy(p, pair.A, pair.B, py, px, *args(-1), gradient=False, interp=interp)) / (2 * h)
+            g = (gu if field == "u" else gv)[i, j]
+            # bilinear sampling is piecewise linear: a sample crossing a pixel
+            # line inside +-h adds an O(h) kink term
+            tol = 1e-6 if interp == "cubic" else 2e-3
+            assert abs(fd - g) <= tol * scale, (field, i, j, fd, g)
