@@ -10,7 +10,10 @@ on every pass: 3969 + 16129 + 3 x 259081 = 797341 window correlations.
 The pair is uploaded to HBM before the timed region; results stay on the
 device. Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fft32|mfma16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mfma16|fft32]
+
+The reported form (default: the split-FP16 MFMA form) is timed first, then
+the other form on the same pair for the speedup and their deviation.
 """
 from __future__ import annotations
 
@@ -18,6 +21,8 @@ import argparse
 import json
 import math
 import os
+
+import numpy as np
 import pathlib
 import sys
 import time
@@ -71,15 +76,53 @@ def cpu_baseline(pair, passes, threads: int):
                       f"{dt:.1f} s; C restatement of the reference's Float32 run_piv"}
 
 
+def mfma_flops_per_window_m32() -> float:
+    """Issued matrix-core FLOPs of the 16-px MFMA kernel per window: 36
+    v_mfma_f32_32x32x16_f16 (stages 1-4 with the hi/lo split), 2 x 32 x 32 x 16
+    each."""
+    return 36 * 2 * 32 * 32 * 16
+
+
+def time_form(hh, pair, passes, form, steps, warmup, log):
+    """Steady-state vector fields/s of one correlation form, with per-kernel
+    HIP-event times on the context stream."""
+    ctx = hh.Context(0)
+    ctx.set_form(form)
+    ctx.set_pair(pair.A, pair.B)
+    for _ in range(warmup):
+        ctx.run_piv(passes, fetch=False)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run_piv(passes, fetch=False)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernels = {}
+    for fam in ("prefilter", "deform", "correlate", "correlate_uq", "validate", "uq"):
+        for w in (0, 16, 32, 64):
+            ms, n = ctx.kernel_time(fam, w)
+            if n:
+                kernels[f"{fam}_w{w}" if w else fam] = {"avg_ms": ms / n, "launches_per_step": n / steps}
+    ctx.set_timing(False)
+    res = ctx.run_piv(passes, fetch=True)
+    ctx.close()
+    log(f"form {form}: {steps} steps in {elapsed:.3f} s")
+    return elapsed, kernels, res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", choices=["fft32", "mfma16"], default="fft32")
+    ap.add_argument("--form", choices=["mfma16", "fft32"], default="mfma16",
+                    help="form whose throughput is the reported value (the other is timed beside it)")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the other form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "correlate_w16_traffic.txt"))
+    ap.add_argument("--traffic-file", default=None)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device copies of this size (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -106,57 +149,32 @@ def main(argv=None):
     pair, passes = workload(hh)
     log("workload ready")
     nr, nc = pair.A.shape
-    ctx = hh.Context(0)
-    if args.form == "mfma16":
-        ctx.set_form(hh.HH_FORM_MFMA16)
-    ctx.set_pair(pair.A, pair.B)
-
-    try:
-        import torch
-
-        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except ImportError:  # pragma: no cover
-        sync_torch = lambda: None
-
-    for _ in range(args.warmup):
-        ctx.run_piv(passes, fetch=False)
-    ctx.synchronize()
-    log("warmup done")
-
-    ctx.set_timing(True)
-    sync_torch()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.run_piv(passes, fetch=False)
-    ctx.synchronize()
-    sync_torch()
-    elapsed = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {elapsed:.3f} s")
-
-    kernels = {}
-    for fam in ("prefilter", "deform", "correlate", "correlate_uq", "validate", "uq"):
-        for w in (0, 16, 32, 64):
-            ms, n = ctx.kernel_time(fam, w)
-            if n:
-                kernels[f"{fam}_w{w}" if w else fam] = {"avg_ms": ms / n, "launches_per_step": n / args.steps}
-    ctx.set_timing(False)
-
-    res = ctx.run_piv(passes, fetch=True)
+    forms = {"mfma16": hh.HH_FORM_MFMA16, "fft32": hh.HH_FORM_FFT32}
+    other = "fft32" if args.form == "mfma16" else "mfma16"
+    elapsed, kernels, res = time_form(hh, pair, passes, forms[args.form], args.steps, args.warmup, log)
     rms = hh.synthetic.rms_error(res, pair.flow, midpoint=True)
 
-    # dominant kernel: the 16-px-window correlation (M = 32), 259081 windows
+    # dominant kernel: the 16-px-window correlation sweeps (M = 32), 259081 windows
     ny3, nx3 = hh.grid_size(passes[-1], nr, nc)
     nwin = ny3 * nx3
-    ms3, n3 = kernels["correlate_w16"]["avg_ms"], kernels["correlate_w16"]["launches_per_step"]
-    flops = nwin * fft_flops_per_window(32)
-    achieved = flops / (ms3 * 1e-3) / 1e12
-    traffic = read_traffic(pathlib.Path(args.traffic_file))
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3),
-                "peak": PEAK_FP32_TFLOPS if args.form == "fft32" else PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-                "frac": None, "traffic": traffic, "kernel": "correlate_w16",
-                "algorithmic_flops_per_launch": flops}
-    roofline["frac"] = round(achieved / roofline["peak"], 5)
+    ms3 = kernels["correlate_w16"]["avg_ms"]
+    fft_flops = nwin * fft_flops_per_window(32)
+    if args.form == "mfma16":
+        issued = nwin * mfma_flops_per_window_m32()
+        achieved = issued / (ms3 * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / PEAK_FP16_TFLOPS, 5), "traffic": None, "kernel": "correlate_w16 (mfma)",
+                    "algorithmic_flops_per_launch": issued, "flops_convention": "issued MFMA FLOPs (hi/lo split)",
+                    "fft_convention_tflops": round(fft_flops / (ms3 * 1e-3) / 1e12, 3)}
+        tfile = args.traffic_file or str(ROOT / "profiles" / "correlate_w16_mfma_traffic.txt")
+    else:
+        achieved = fft_flops / (ms3 * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None, "kernel": "correlate_w16 (fft32)",
+                    "algorithmic_flops_per_launch": fft_flops,
+                    "flops_convention": "FFT form: 3 x 5 M^2 log2 M^2 + 12 M^2 per window, vs the FP32 VALU peak"}
+        tfile = args.traffic_file or str(ROOT / "profiles" / "correlate_w16_traffic.txt")
+    roofline["traffic"] = read_traffic(pathlib.Path(tfile))
 
     ms_per_step = elapsed / args.steps * 1e3
     line = {
@@ -170,22 +188,30 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.form == "fft32" else "fp16x2-split/f32",
+        "dtype": "fp16x2-split/f32" if args.form == "mfma16" else "f32",
         "data": "synthetic (seeded particle images; device-resident pair)",
         "config": {"workload": "2048x2048 float32 pair, 64/32->32/16->16/12 x3 sweeps, pad+gauss, UQ",
                    "form": args.form, "window_correlations": 3969 + 16129 + 3 * nwin},
         "roofline": roofline,
         "kernels": kernels,
-        "accuracy": {"rms_px_vs_midpoint_truth": round(rms, 5),
-                     "outliers": int(res["outliers"].sum())},
+        "accuracy": {"rms_px_vs_midpoint_truth": round(rms, 5), "outliers": int(res["outliers"].sum())},
     }
+    if not args.no_compare:
+        e2, k2, r2 = time_form(hh, pair, passes, forms[other], args.steps, args.warmup, log)
+        v2 = args.steps / e2
+        same = (~res["outliers"]) & (~r2["outliers"])
+        dev = np.sqrt(np.mean(np.concatenate([(res["u"] - r2["u"])[same], (res["v"] - r2["v"])[same]]) ** 2))
+        line["compare"] = {"form": other, "value": round(v2, 4), "ms_per_step": round(e2 / args.steps * 1e3, 4),
+                           "speedup_of_reported_form": round((args.steps / elapsed) / v2, 4),
+                           "rms_px_between_forms": float(dev),
+                           "flag_diffs_between_forms": int((res["outliers"] != r2["outliers"]).sum()),
+                           "correlate_w16_avg_ms": k2["correlate_w16"]["avg_ms"]}
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(pair, passes, args.cpu_threads)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
-    ctx.close()
     return 0
 
 

@@ -1,0 +1,272 @@
+// box_peaks.h — peak search and peak analysis on the analysis box of a
+// correlation plane, shared by the register FFT and the MFMA DFT kernels.
+//
+// The box holds the overlap-gain support plus a zero margin; outside it the
+// plane is exactly zero. find_peaks_regionalmax! (src/correlators.jl:
+// 426-454, src/localmax.jl:3-19) over the full plane is reproduced by the
+// regional maxima of the box plus the plane corner (M, M), the one
+// zero-valued point that is always a local maximum; analyze_plane!
+// (src/correlators.jl:459-468) follows on lane 0 of the window.
+#pragma once
+
+#include <climits>
+
+#include "hh_internal.h"
+
+namespace hh {
+
+__device__ __forceinline__ bool better(float av, int al, float bv, int bl)
+{
+    if (al == INT_MAX) return false;
+    if (bl == INT_MAX) return true;
+    return av > bv || (av == bv && al < bl);
+}
+
+// Box plane accessor: plane (0-based, fftshifted) -> value; 0 outside box.
+template <int NB, int LO>
+struct BoxView {
+    const float *rb;
+    __device__ __forceinline__ float operator()(int ip, int jp) const
+    {
+        const int bi = ip - LO, bj = jp - LO;
+        return (bi >= 0 && bi < NB && bj >= 0 && bj < NB) ? rb[bi + bj * NB] : 0.f;
+    }
+};
+
+template <int M, typename V>
+__device__ float2 gauss3_b(const V &R, int i, int j)  // 1-based
+{
+    float di = 0.f, dj = 0.f;
+    const float I0 = R(i - 1, j - 1);
+    if (1 < i && i < M) {
+        const float Im = R(i - 2, j - 1), Ip = R(i, j - 1);
+        if (Im > 0.f && I0 > 0.f && Ip > 0.f) {
+            const float denom = logf(Im) - 2.f * logf(I0) + logf(Ip);
+            di = denom != 0.f ? (logf(Im) - logf(Ip)) / (2.f * denom) : 0.f;
+        }
+    }
+    if (1 < j && j < M) {
+        const float Im = R(i - 1, j - 2), Ip = R(i - 1, j);
+        if (Im > 0.f && I0 > 0.f && Ip > 0.f) {
+            const float denom = logf(Im) - 2.f * logf(I0) + logf(Ip);
+            dj = denom != 0.f ? (logf(Im) - logf(Ip)) / (2.f * denom) : 0.f;
+        }
+    }
+    return make_float2((float)i + di, (float)j + dj);
+}
+
+template <int M, typename V>
+__device__ float2 gauss9_b(const V &R, int pr, int pc)
+{
+    if (!(1 < pr && pr < M && 1 < pc && pc < M)) return gauss3_b<M>(R, pr, pc);
+    float sL = 0.f, sxL = 0.f, syL = 0.f, sxxL = 0.f, syyL = 0.f, sxyL = 0.f;
+    for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di) {
+            const float val = R(pr + di - 1, pc + dj - 1);
+            if (!(val > 0.f)) return gauss3_b<M>(R, pr, pc);
+            const float L = logf(val);
+            sL += L;
+            sxL += (float)dj * L;
+            syL += (float)di * L;
+            sxxL += (float)(dj * dj) * L;
+            syyL += (float)(di * di) * L;
+            sxyL += (float)(di * dj) * L;
+        }
+    const float a1 = sxL / 6.f, a2 = syL / 6.f, a3 = sxxL / 2.f - sL / 3.f, a4 = sxyL / 4.f,
+                a5 = syyL / 2.f - sL / 3.f;
+    const float D = 4.f * a3 * a5 - a4 * a4;
+    if (!(a3 < 0.f && D > 0.f)) return gauss3_b<M>(R, pr, pc);
+    const float dx = (a4 * a2 - 2.f * a5 * a1) / D;
+    const float dy = (a4 * a1 - 2.f * a3 * a2) / D;
+    if (!(fabsf(dx) <= 1.f && fabsf(dy) <= 1.f)) return gauss3_b<M>(R, pr, pc);
+    return make_float2((float)pr + dy, (float)pc + dx);
+}
+
+template <int M, typename V>
+__device__ float moment_b(const V &R, float pr, float pc)
+{
+    if (!(isfinite(pr) && isfinite(pc))) return NAN;
+    const int r0 = (int)rintf(pr), c0 = (int)rintf(pc);
+    const int rlo = max(1, r0 - 1), rhi = min(M, r0 + 1);
+    const int clo = max(1, c0 - 1), chi = min(M, c0 + 1);
+    if (rlo > rhi || clo > chi) return NAN;
+    double sumC = 0.0, sdx = 0.0, sdy = 0.0;
+    for (int c = clo; c <= chi; ++c)
+        for (int r = rlo; r <= rhi; ++r) {
+            double w = (double)R(r - 1, c - 1);
+            w = w > 0.0 ? w : 0.0;
+            const float dx = (float)c - pc, dy = (float)r - pr;
+            sumC += w;
+            sdx += (double)(dx * dx) * w;
+            sdy += (double)(dy * dy) * w;
+        }
+    if (sumC < 1e-9) return NAN;
+    return (float)sqrt((sdx + sdy) / sumC);
+}
+
+// Regional-max top-K over the box Rb (NB x NB column-major, plane rows and
+// columns LO .. LO + NB - 1, fftshifted) by the G lanes t = 0 .. G - 1 of
+// one window (NWAVE waves per window, exchanging through mv/ml), then the
+// per-window outputs of process_windows! (src/pipeline.jl:972-1027).
+template <int M, int K, int NB, int LO, int G, int NWAVE, bool UQ>
+__device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut &out, const float *Rb, int t, int tid,
+                                            float *mv, int *ml, bool active, int jb, int gi, int gj, float sig_u,
+                                            float sig_v)
+{
+#pragma clang fp contract(off)
+    constexpr int H = M / 2;
+    const int wave = tid >> 6;
+    (void)wave;
+    // ---- 5. regional-max top-K over the box ----
+    float tv[K];
+    int tl[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        tv[m] = 0.f;
+        tl[m] = INT_MAX;
+    }
+    for (int p = t; p < NB * NB; p += G) {
+        const int bi = p % NB, bj = p / NB;
+        const float I0 = Rb[p];
+        if (!(I0 > 0.f)) continue;  // zero-valued box points are never maxima
+        bool ok = true;
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+            for (int di = -1; di <= 1; ++di) {
+                if (di == 0 && dj == 0) continue;
+                const int ni = bi + di, nj = bj + dj;
+                const float In = (ni >= 0 && ni < NB && nj >= 0 && nj < NB) ? Rb[ni + nj * NB] : 0.f;
+                const bool earlier = dj < 0 || (dj == 0 && di < 0);
+                ok &= earlier ? !(In > I0) : !(In >= I0);
+            }
+        if (!ok) continue;
+        float cv = I0;
+        int cl = (LO + bi) + (LO + bj) * M;
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            if (better(cv, cl, tv[m], tl[m])) {
+                const float sv_ = tv[m];
+                const int sl_ = tl[m];
+                tv[m] = cv;
+                tl[m] = cl;
+                cv = sv_;
+                cl = sl_;
+            }
+        }
+    }
+    float sv[K];
+    int sl[K];
+    int nsel = 0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        float bv = tv[0];
+        int bl = tl[0];
+        constexpr int XMAX = G >= 64 ? 32 : G / 2;
+#pragma unroll
+        for (int o = XMAX; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(bv, o);
+            const int ol = __shfl_xor(bl, o);
+            if (better(ov, ol, bv, bl)) {
+                bv = ov;
+                bl = ol;
+            }
+        }
+        if constexpr (NWAVE > 1) {
+            if ((tid & 63) == 0) {
+                mv[wave] = bv;
+                ml[wave] = bl;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int w2 = 0; w2 < NWAVE; ++w2)
+                if (better(mv[w2], ml[w2], bv, bl)) {
+                    bv = mv[w2];
+                    bl = ml[w2];
+                }
+            __syncthreads();
+        }
+        sv[m] = bv;
+        sl[m] = bl;
+        nsel += bl != INT_MAX;
+        if (tl[0] == bl && bl != INT_MAX) {
+#pragma unroll
+            for (int q = 0; q + 1 < K; ++q) {
+                tv[q] = tv[q + 1];
+                tl[q] = tl[q + 1];
+            }
+            tv[K - 1] = 0.f;
+            tl[K - 1] = INT_MAX;
+        }
+    }
+    if (t != 0 || !active) return;
+    // the plane corner (M, M): zero-valued, always a regional maximum
+#pragma unroll
+    for (int m = 0; m < K; ++m)
+        if (m == nsel) {
+            sv[m] = 0.f;
+            sl[m] = M * M - 1;
+        }
+    nsel = nsel < K ? nsel + 1 : nsel;
+    int found = 1;
+#pragma unroll
+    for (int m = 1; m < K; ++m)
+        if (found == m && m < nsel && sv[m] > 0.f) found = m + 1;
+
+    const BoxView<NB, LO> R{Rb};
+    const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
+    const size_t ng = (size_t)out.ny * out.nx;
+    if (out.found) out.found[jb] = found;
+    if (out.locs) {
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            out.locs[2 * (K * jb + m)] = m < found ? sl[m] % M + 1 : 0;
+            out.locs[2 * (K * jb + m) + 1] = m < found ? sl[m] / M + 1 : 0;
+            out.vals[K * jb + m] = m < found ? sv[m] : NAN;
+        }
+    }
+    const int pr = sl[0] % M + 1, pc = sl[0] / M + 1;
+    float2 ref;
+    if (cfg.subpixel == HH_GAUSS9)
+        ref = gauss9_b<M>(R, pr, pc);
+    else if (cfg.subpixel == HH_SUBPIX_NONE)
+        ref = make_float2((float)pr, (float)pc);
+    else
+        ref = gauss3_b<M>(R, pr, pc);
+    const float cc = (float)(H + 1);
+    const float du = ref.y - cc, dv = ref.x - cc;
+    const float v1 = sv[0];
+    const float ratio = found >= 2 ? v1 / sv[1] : (v1 > 0.f ? INFINITY : NAN);
+    const float mom = moment_b<M>(R, ref.x, ref.y);
+    const float u0 = out.u0 ? out.u0[g] : 0.f, v0 = out.v0 ? out.v0[g] : 0.f;
+    out.u[g] = out.u0 ? u0 + du : du;
+    out.v[g] = out.v0 ? v0 + dv : dv;
+    out.pr[g] = ratio;
+    out.cm[g] = mom;
+    if constexpr (UQ) {
+        out.unc_u[g] = sig_u;
+        out.unc_v[g] = sig_v;
+    }
+    if (out.alt_u) {
+#pragma unroll
+        for (int m = 2; m <= K; ++m) {
+            if (m > cfg.n_peaks) continue;
+            float au = NAN, av = NAN;
+            if (m <= found) {
+                const float2 a = gauss3_b<M>(R, sl[m - 1] % M + 1, sl[m - 1] / M + 1);
+                au = a.y - cc;
+                av = a.x - cc;
+                if (out.u0) {
+                    au = u0 + au;
+                    av = v0 + av;
+                }
+            } else if (!out.u0) {
+                continue;
+            }
+            out.alt_u[g + (size_t)(m - 2) * ng] = au;
+            out.alt_v[g + (size_t)(m - 2) * ng] = av;
+        }
+    }
+}
+
+}  // namespace hh

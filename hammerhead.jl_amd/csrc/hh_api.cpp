@@ -387,6 +387,7 @@ CorrConfig make_corr(hh_ctx *c, const hh_pass &p)
     cfg.apod = cb.apod.p;
     cfg.gain = p.padding ? cb.gain.p : nullptr;
     cfg.wfilt = p.correlation_method == HH_PHASE ? cb.wfilt.p : nullptr;
+    cfg.form = c->form;
     return cfg;
 }
 
@@ -1148,7 +1149,6 @@ int hh_set_form(hh_ctx *c, int form)
 {
     return guarded(c, [&] {
         if (form != HH_FORM_FFT32 && form != HH_FORM_MFMA16) throw Error(HH_EINVAL, "unknown correlation form");
-        if (form == HH_FORM_MFMA16) throw Error(HH_EUNSUPPORTED, "the split-FP16 MFMA form is not built yet");
         c->form = form;
     });
 }

@@ -50,6 +50,7 @@ struct CorrConfig {
     const float *apod;  // w*w device
     const float *gain;  // M*M device (fftshifted), nullptr if not padded
     const float *wfilt; // M*M device phase filter (unshifted), or nullptr
+    int form;           // HH_FORM_FFT32 / HH_FORM_MFMA16
 };
 
 // Outputs of a correlation launch; any pointer may be null.
@@ -87,6 +88,10 @@ bool correlate_box_layout(const CorrConfig &cfg);
 // correlation launch (out.u0/v0 the shared predictor).
 void launch_plane_analyze(const CorrConfig &cfg, const double *acc, bool box, const int32_t *jobs, int njobs,
                           const CorrOut &out, hipStream_t s);
+
+// Split-FP16 MFMA DFT kernels (HH_FORM_MFMA16); false when not covered.
+bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
+                           int njobs, const CorrOut &out, hipStream_t s);
 
 // Specialized kernels; return false when the configuration is not covered.
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,

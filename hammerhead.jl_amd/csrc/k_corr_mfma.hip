@@ -1,0 +1,380 @@
+// k_corr_mfma.hip — correlation of 16-px windows padded to 32 x 32 as
+// dense DFT products on the gfx950 matrix cores (HH_FORM_MFMA16).
+//
+// Same per-window semantics as the FP32 kernels (correlation_plane! +
+// analyze_plane!, src/correlators.jl:326-341,459-468). The padded DFT of a
+// 16 x 16 window X is F32 X F32^T restricted to the 16 nonzero rows and
+// columns, and the correlation plane is only needed on the analysis box
+// (lags -7..7; outside it the overlap gain is exactly 0), so the whole
+// correlation is four small real GEMM stages per window on
+// v_mfma_f32_32x32x16_f16, each operand split into FP16 hi + lo parts
+// (hi*hi + hi*lo + lo*hi, FP32 accumulation, ~22-bit operands):
+//   1. [XA; XB] (32 x 16, rows = (image, r), K = c) x F1 (16 x 32): the
+//      column DFT of both windows, columns = Re k2 = 0..16, Im k2 = 1..15;
+//   2. per image, T1^T (rows = those 32 parts, K = r) x [cos | -sin](r, k1):
+//      the row DFT, all k1 = 0..31; Re/Im parts of one k2 land in the same
+//      lane and combine into Z(k1, k2), k2 = 0..16 (Hermitian half);
+//   3. the spectrum conj(ZA) ZB (or the filtered phase spectrum) per lane,
+//      then P (rows k1, K = Re/Im of k2) x table: the inverse along k2 for
+//      the 15 box columns l2, Hermitian weights folded into the table;
+//   4. Q^T (rows = (l2, Re/Im), K = k1) x [cos | sin](k1, l1): the inverse
+//      along k1 for the 15 box rows l1; Re parts minus Im parts give R.
+// Every stage feeds the next from registers: an MFMA result holds one
+// column per lane and 16 rows in a fixed pattern, which is exactly an
+// operand of the next product with K permuted the same way in the constant
+// table, so no data moves between lanes or through LDS until the box is
+// written for the peak search. Operands are scaled by powers of two to sit
+// in FP16 range (a per-window scale at the input, fixed ones after) and the
+// scale is removed exactly before the magnitude and gain.
+// One wave per window; persistent waves walk contiguous window ranges with
+// the constant tables resident in registers.
+
+#include "hh_internal.h"
+#include "box_peaks.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace hh {
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+struct Trig {
+    float c[32], s[32];  // cos, sin of 2 pi t / 32
+};
+
+struct Split {
+    h8 hi, lo;
+};
+
+// Row of the 32 x 32 MFMA result held in register slot v of lane half h,
+// and the K index of operand slot s in lane half h (same pattern).
+__device__ __forceinline__ int rmap(int h, int s) { return 8 * (s >> 2) + 4 * h + (s & 3); }
+
+__device__ __forceinline__ void put(Split &a, int s, float x)
+{
+    const _Float16 hi = (_Float16)x;
+    a.hi[s] = hi;
+    a.lo[s] = (_Float16)(x - (float)hi);
+}
+
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// Slots s, s + 1 (s even) from two values: one packed conversion per part.
+__device__ __forceinline__ void put2(Split &a, int s, float x0, float x1)
+{
+    const f2v x = {x0, x1};
+    const h2v hi = __builtin_convertvector(x, h2v);
+    const f2v r = x - __builtin_convertvector(hi, f2v);
+    const h2v lo = __builtin_convertvector(r, h2v);
+    a.hi[s] = hi[0];
+    a.hi[s + 1] = hi[1];
+    a.lo[s] = lo[0];
+    a.lo[s + 1] = lo[1];
+}
+
+__device__ __forceinline__ f16v mfma3(const Split &a, const Split &b, f16v acc)
+{
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.lo, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.hi, b.hi, acc, 0, 0, 0);
+    return acc;
+}
+
+__device__ __forceinline__ float max_img(float v)  // over the 32 lanes of one image
+{
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+
+__device__ __forceinline__ float sum_img(float v)
+{
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o);
+    return v + __shfl_xor(v, 32);
+}
+
+// LDS hand-off between the lanes of one wave (the waves of a block run
+// different numbers of windows, so no block barrier inside the loop).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kHW = 7;            // box half-width (Gaussian apodization)
+constexpr int kNB = 2 * kHW + 1;  // 15 box rows and columns
+constexpr int kS1 = -5;           // stage-2 operand scale
+constexpr int kS3 = -16;          // stage-3 operand scale (cross spectrum)
+constexpr int kS3P = 13;          // stage-3 operand scale (phase spectrum, |P| <= 1)
+constexpr int kS4 = -6;           // stage-4 operand scale
+
+template <int K, bool PHASE>
+__global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
+                                                     const float *__restrict__ B, int nr,
+                                                     const int32_t *__restrict__ jobs, int njobs, int chunk,
+                                                     CorrOut out)
+{
+#pragma clang fp contract(off)
+    constexpr int M = 32, H = 16, LO = H - kHW;
+    __shared__ float rbox[4][4][kNB * kNB + 1];  // [wave][window of the group]
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int h = l >> 5, n = l & 31;
+
+    // ---- constant operand tables (loop invariant) ----
+    Split F1, WrT, WiT, T3[3], C4[2], S4[2];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int c = 8 * h + s;  // stage 1: K = c, column n
+        float f;
+        if (n < 16)
+            f = tg.c[(c * n) & 31];
+        else if (n == 16)
+            f = (c & 1) ? -1.f : 1.f;
+        else
+            f = -tg.s[(c * (n - 16)) & 31];
+        put(F1, s, f);
+        const int r = rmap(h, s);  // stage 2: K = r, column k1 = n
+        put(WrT, s, tg.c[(n * r) & 31]);
+        put(WiT, s, -tg.s[(n * r) & 31]);
+        // stage 3: K = (Re | Im of k2 = rmap(h, s)) and (k2 = 16), column
+        // n = (b = n & 15, Re for n < 16 | Im), l2 = b - HW
+        const int b = n & 15, im = n >> 4;
+        const bool vb = b < kNB;
+        const int l2 = b - kHW;
+        const int k2 = rmap(h, s);
+        const float cw = k2 == 0 ? 1.f : 2.f;
+        const int t = (k2 * l2) & 31;
+        put(T3[0], s, vb ? (im ? cw * tg.s[t] : cw * tg.c[t]) : 0.f);
+        put(T3[1], s, vb ? (im ? cw * tg.c[t] : -cw * tg.s[t]) : 0.f);
+        const int t16 = (16 * l2) & 31;
+        float e = 0.f;
+        if (vb && h == 0 && s == 0) e = im ? tg.s[t16] : tg.c[t16];   // Re P(k1, 16)
+        if (vb && h == 0 && s == 1) e = im ? tg.c[t16] : -tg.s[t16];  // Im P(k1, 16)
+        put(T3[2], s, e);
+        // stage 4: K = k1 = 16 q + rmap(h, s), column a = n, l1 = a - HW
+        const bool va = n < kNB;
+        const int l1 = n - kHW;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int k1 = 16 * q + r;
+            const int t4 = (k1 * l1) & 31;
+            put(C4[q], s, va ? tg.c[t4] : 0.f);
+            put(S4[q], s, va ? tg.s[t4] : 0.f);
+        }
+    }
+    const f16v zero = {};
+    const int img = n >> 4, row = n & 15;
+    const float *src = img ? B : A;
+    float apod8[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) apod8[s] = cfg.apod[row + 16 * (8 * h + s)];
+
+    const int gw = blockIdx.x * 4 + wave;
+    const int w0 = gw * chunk;
+    const int w1 = min(njobs, w0 + chunk);
+    if (w0 >= w1) return;
+    // software pipeline: the pixels of window win + 1 and the job of win + 2
+    // are in flight while window win is processed
+    const int4 *jobs4 = reinterpret_cast<const int4 *>(jobs);
+    int4 jcur = jobs4[w0];
+    int4 jnext = jobs4[min(w0 + 1, w1 - 1)];
+    float pnext[8];
+    {
+        const float *p = src + (size_t)(jcur.w - 1 + 8 * h) * nr + (jcur.z - 1 + row);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) pnext[s] = p[(size_t)s * nr];
+    }
+    // windows in groups of four: the boxes of a group are analyzed together,
+    // 16 lanes per window
+    for (int g = w0; g < w1; g += 4) {
+        const int ng = min(4, w1 - g);
+        for (int j = 0; j < ng; ++j) {
+        const int win = g + j;
+        float *Rb = rbox[wave][j];
+        float px[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) px[s] = pnext[s];
+        {
+            const float *p = src + (size_t)(jnext.w - 1 + 8 * h) * nr + (jnext.z - 1 + row);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) pnext[s] = p[(size_t)s * nr];
+        }
+        jcur = jnext;
+        jnext = jobs4[min(win + 2, w1 - 1)];
+        // ---- 0. window: lane (image, row, 8 columns), mean, apodize ----
+        float sum = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) sum += px[s];
+        const float mean = sum_img(sum) / 256.f;
+        float x[8], amax = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            x[s] = apod8[s] * (px[s] - mean);
+            amax = fmaxf(amax, fabsf(x[s]));
+        }
+        amax = max_img(amax);
+        int ex = 0;
+        (void)frexpf(amax, &ex);
+        const int e0 = amax > 0.f ? 12 - ex : 0;  // |x| 2^e0 in [2^11, 2^12)
+        Split a1;
+#pragma unroll
+        for (int s = 0; s < 8; s += 2) put2(a1, s, ldexpf(x[s], e0), ldexpf(x[s + 1], e0));
+        const int eA = __shfl(e0, 0), eB = __shfl(e0, 16);
+
+        // ---- 1. column DFT of both windows ----
+        const f16v d1 = mfma3(a1, F1, zero);
+
+        // ---- 2. row DFT per image; Z(k1 = n, k2 = rmap(h, v)), Z(k1, 16) ----
+        float zr[2][9], zi[2][9];
+#pragma unroll
+        for (int I = 0; I < 2; ++I) {
+            Split a2;
+#pragma unroll
+            for (int s = 0; s < 8; s += 2) put2(a2, s, ldexpf(d1[8 * I + s], kS1), ldexpf(d1[8 * I + s + 1], kS1));
+            const f16v dr = mfma3(a2, WrT, zero);
+            const f16v di = mfma3(a2, WiT, zero);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                // k2 >= 1: Re = Wr Tr - Wi Ti, Im = Wr Ti + Wi Tr
+                zr[I][v] = dr[v] - di[v + 8];
+                zi[I][v] = dr[v + 8] + di[v];
+            }
+            // k2 = 0 (slot 0 of the first half) has no Im part; its Im slot
+            // carries the real column k2 = 16
+            if (h == 0) {
+                zr[I][0] = dr[0];
+                zi[I][0] = di[0];
+            }
+            zr[I][8] = dr[8];
+            zi[I][8] = di[8];
+        }
+
+        // ---- 3. spectrum and inverse along k2 ----
+        Split a3[3];
+        const int es = eA + eB + 2 * kS1;  // P scale: P_true 2^es
+        float sp_r[9], sp_i[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            float pr = zr[0][v] * zr[1][v] + zi[0][v] * zi[1][v];
+            float pi = zr[0][v] * zi[1][v] - zi[0][v] * zr[1][v];
+            if constexpr (PHASE) {
+                const int k2 = v < 8 ? rmap(h, v) : 16;
+                pr = ldexpf(pr, -es);
+                pi = ldexpf(pi, -es);
+                const float den = hypotf(pr, pi) + FLT_EPSILON;
+                const float wv = cfg.wfilt[n + k2 * M];
+                pr = ldexpf(wv * pr / den, kS3P);
+                pi = ldexpf(wv * pi / den, kS3P);
+            } else {
+                pr = ldexpf(pr, kS3);
+                pi = ldexpf(pi, kS3);
+            }
+            sp_r[v] = pr;
+            sp_i[v] = pi;
+        }
+#pragma unroll
+        for (int s = 0; s < 8; s += 2) {
+            put2(a3[0], s, sp_r[s], sp_r[s + 1]);
+            put2(a3[1], s, sp_i[s], sp_i[s + 1]);
+        }
+        {
+            const bool k16 = h == 0;
+            put2(a3[2], 0, k16 ? sp_r[8] : 0.f, k16 ? sp_i[8] : 0.f);
+#pragma unroll
+            for (int s = 2; s < 8; ++s) {
+                a3[2].hi[s] = (_Float16)0.f;
+                a3[2].lo[s] = (_Float16)0.f;
+            }
+        }
+        f16v d3 = mfma3(a3[0], T3[0], zero);
+        d3 = mfma3(a3[1], T3[1], d3);
+        d3 = mfma3(a3[2], T3[2], d3);
+
+        // ---- 4. inverse along k1 for the box rows ----
+        Split a4[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int s = 0; s < 8; s += 2)
+                put2(a4[q], s, ldexpf(d3[8 * q + s], kS4), ldexpf(d3[8 * q + s + 1], kS4));
+        f16v oc = mfma3(a4[0], C4[0], zero);
+        oc = mfma3(a4[1], C4[1], oc);
+        f16v os = mfma3(a4[0], S4[0], zero);
+        os = mfma3(a4[1], S4[1], os);
+
+        // R(l1 = n - HW, l2 = b - HW) = oc[Re row b] - os[Im row b]
+        const int etot = (PHASE ? kS3P : es + kS3) + kS4;
+        const float scale = ldexpf(1.f / (float)(M * M), -etot);
+        if (n < kNB) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int b = rmap(h, v);
+                if (b < kNB) {
+                    const float rv = (oc[v] - os[v + 8]) * scale;
+                    Rb[n + b * kNB] = fabsf(rv) * cfg.gain[(LO + n) + (LO + b) * M];
+                }
+            }
+        }
+        }
+        wave_sync();
+        const int slot = l >> 4;
+        const int jb = g + min(slot, ng - 1);
+        const int4 jj = jobs4[jb];
+        float sg = NAN;
+        analyze_box<M, K, kNB, LO, 16, 1, false>(cfg, out, rbox[wave][slot], l & 15, l, nullptr, nullptr, slot < ng,
+                                                  jb, jj.x, jj.y, sg, sg);
+        wave_sync();
+    }
+}
+
+}  // namespace
+
+bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
+                           int njobs, const CorrOut &out, hipStream_t s)
+{
+    // covered: 16-px windows padded to 32 with the Gaussian support box
+    if (cfg.M != 32 || cfg.w != 16 || !cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3)
+        return false;
+    if (cfg.box_lo != 16 - kHW || cfg.box_hi != 16 + kHW) return false;
+    if (out.planes || out.acc || out.unc_u) return false;
+    if (njobs <= 0) return true;
+    static Trig tg = [] {
+        Trig t;
+        for (int k = 0; k < 32; ++k) {
+            t.c[k] = (float)std::cos(2.0 * M_PI * k / 32.0);
+            t.s[k] = (float)std::sin(2.0 * M_PI * k / 32.0);
+        }
+        return t;
+    }();
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 8));  // 2 waves per SIMD
+    const int nblocks = (waves + 3) / 4;
+    const int chunk = (njobs + nblocks * 4 - 1) / (nblocks * 4);
+    if (cfg.method == HH_PHASE) {
+        if (cfg.K == 2)
+            hipLaunchKernelGGL((k_corr_mfma32<2, true>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
+                               njobs, chunk, out);
+        else
+            hipLaunchKernelGGL((k_corr_mfma32<3, true>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
+                               njobs, chunk, out);
+    } else {
+        if (cfg.K == 2)
+            hipLaunchKernelGGL((k_corr_mfma32<2, false>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
+                               njobs, chunk, out);
+        else
+            hipLaunchKernelGGL((k_corr_mfma32<3, false>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
+                               njobs, chunk, out);
+    }
+    HH_HIP(hipGetLastError());
+    return true;
+}
+
+}  // namespace hh

@@ -542,6 +542,7 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     (void)nc;
     if (njobs <= 0) return out.unc_u != nullptr;
     bool uq_done = false;
+    if (cfg.form == HH_FORM_MFMA16 && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
     if (launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
     const float2 *tw = twiddle_table(cfg.M, s);
     switch (cfg.M) {

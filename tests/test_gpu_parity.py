@@ -412,3 +412,51 @@ def test_refinement_reduces_error(hh, ctx):
     e0, e1 = dense_rms(r["u"], r["v"]), dense_rms(u, v)
     say("dense rms before", e0, "after", e1, "energies", en[0], en[-1])
     assert e1 < 0.9 * e0
+
+
+# ---------------------------------------------------------------- MFMA form
+@pytest.fixture(scope="module")
+def ctx_mfma(hh):
+    c = hh.Context(0)
+    c.set_form(hh.HH_FORM_MFMA16)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("method", ["cross", "phase"])
+def test_mfma_correlate_vs_oracle(hh, orc, ctx_mfma, c1, method):
+    # split-FP16 DFT products on the matrix cores, 16-px windows padded to 32
+    A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
+    ctx_mfma.set_pair(A, B)
+    for ov in (8, 12):
+        p = P(hh, window_size=16, overlap=ov, padding=True, apodization="gauss", correlation_method=method)
+        say("mfma correlate", method, ov)
+        s, g, o = compare_windows(orc, hh, ctx_mfma, A.astype(np.float64), B.astype(np.float64), p)
+        say(s)
+        assert s["peak_mismatch"] == s["near_ties"], s
+        assert s["n_over_tol"] == 0, s
+
+
+def test_mfma_multipass_vs_oracle(hh, orc, ctx_mfma):
+    # the metric's bar: the MFMA form within 0.005 px RMS of the FP64 oracle
+    pair = hh.synthetic.config_vortex_1024(n=512)
+    passes = hh.multipass_parameters([64, 32, 16], padding=True, apodization="gauss",
+                                     final=dict(overlap=12, max_iterations=3, convergence_tol=0.0,
+                                                uncertainty=True))
+    say("mfma run_piv vortex 512")
+    g, o = run_both(hh, orc, ctx_mfma, pair.A, pair.B, passes)
+    dflag = int((g["outliers"] != o["outliers"]).sum())
+    same = (~g["outliers"]) & (~o["outliers"])
+    du = np.abs(g["u"] - o["u"])[same]
+    dv = np.abs(g["v"] - o["v"])[same]
+    rms = math.sqrt(np.mean(np.concatenate([du, dv]) ** 2))
+    fin = same & np.isfinite(o["peak_ratio"]) & np.isfinite(g["peak_ratio"])
+    dpr = np.abs(g["peak_ratio"] - o["peak_ratio"])[fin] / np.abs(o["peak_ratio"][fin])
+    sfin = np.isfinite(g["uncertainty_u"]) & np.isfinite(o["uncertainty_u"])
+    srel = np.abs(g["uncertainty_u"][sfin] - o["uncertainty_u"][sfin]) / np.maximum(np.abs(o["uncertainty_u"][sfin]),
+                                                                                    1e-6)
+    say("mfma flag diffs", dflag, "rms", rms, "max", du.max(), dv.max(), "ratio rel max", dpr.max(),
+        "sigma rel median", np.median(srel))
+    assert rms < 0.005
+    assert dflag <= 0.002 * g["outliers"].size + 2
+    assert np.median(srel) < 0.01
