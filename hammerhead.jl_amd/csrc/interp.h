@@ -24,6 +24,16 @@ __device__ __forceinline__ void lin_axis(float q0, float qstep, int n, float q, 
     l = t - (float)ii;
 }
 
+// Bilinear value of grid p at interval (iy, ix) with fractions (ly, lx).
+__device__ __forceinline__ float bilinear_at(const float *__restrict__ p, int gny, int gnx, int iy, int ix, float ly,
+                                             float lx)
+{
+    const int iy2 = gny == 1 ? iy : iy + 1, ix2 = gnx == 1 ? ix : ix + 1;
+    const float a = p[iy + ix * gny] + lx * (p[iy + ix2 * gny] - p[iy + ix * gny]);
+    const float b = p[iy2 + ix * gny] + lx * (p[iy2 + ix2 * gny] - p[iy2 + ix * gny]);
+    return a + ly * (b - a);
+}
+
 __device__ __forceinline__ float bilinear(const float *__restrict__ p, int gny, int gnx, float y0, float ysp,
                                           float x0, float xsp, float y, float x)
 {
@@ -31,10 +41,7 @@ __device__ __forceinline__ float bilinear(const float *__restrict__ p, int gny, 
     float ly, lx;
     lin_axis(y0, ysp, gny, y, iy, ly);
     lin_axis(x0, xsp, gnx, x, ix, lx);
-    const int iy2 = gny == 1 ? iy : iy + 1, ix2 = gnx == 1 ? ix : ix + 1;
-    const float a = p[iy + ix * gny] + lx * (p[iy + ix2 * gny] - p[iy + ix * gny]);
-    const float b = p[iy2 + ix * gny] + lx * (p[iy2 + ix2 * gny] - p[iy2 + ix * gny]);
-    return a + ly * (b - a);
+    return bilinear_at(p, gny, gnx, iy, ix, ly, lx);
 }
 
 __device__ __forceinline__ void cubw(float d, float w[4])

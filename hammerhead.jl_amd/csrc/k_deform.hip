@@ -69,36 +69,51 @@ __device__ __forceinline__ float ext_value(const Get &get, int j, int n)
 }
 
 // Pass 1: along dim 1. out is (nr+2) x nc, logical row k at out[k + j*(nr+2)].
+// Block (x, j): logical rows [256 x, 256 x + 256) of column j; the extended
+// segment with its 16-sample halos is staged once in LDS.
 __global__ __launch_bounds__(256) void k_prefilter_rows(const float *__restrict__ img, int nr, int nc,
                                                          float *__restrict__ out, Taps taps)
 {
+    __shared__ float seg[256 + 2 * kTaps];
     const int N1 = nr + 2;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)N1 * nc) return;
-    const int k = (int)(e % N1);
-    const int j = (int)(e / N1);
+    const int j = blockIdx.y, k0 = blockIdx.x * 256;
     const float *col = img + (size_t)j * nr;
     auto get = [&](int q) { return col[q - 1]; };
-    float s = taps.h[0] * ext_value(get, k, nr);
+    for (int t = threadIdx.x; t < 256 + 2 * kTaps; t += 256) seg[t] = ext_value(get, k0 - kTaps + t, nr);
+    __syncthreads();
+    const int k = k0 + threadIdx.x;
+    if (k >= N1) return;
+    const float *x = seg + threadIdx.x + kTaps;
+    float s = taps.h[0] * x[0];
 #pragma unroll
-    for (int d = 1; d <= kTaps; ++d) s += taps.h[d] * (ext_value(get, k - d, nr) + ext_value(get, k + d, nr));
-    out[e] = s;
+    for (int d = 1; d <= kTaps; ++d) s += taps.h[d] * (x[-d] + x[d]);
+    out[k + (size_t)j * N1] = s;
 }
 
 // Pass 2: along dim 2 over the (nr+2) rows of tmp; coef is (nr+2) x (nc+2).
+// Block (x, y): rows [64 x, 64 x + 64), logical columns [64 y, 64 y + 64),
+// staged with the column halos in LDS (coalesced along the rows).
 __global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict__ tmp, int nr, int nc,
                                                          float *__restrict__ coef, Taps taps)
 {
+    __shared__ float tile[64 + 2 * kTaps][65];
     const int N1 = nr + 2, N2 = nc + 2;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)N1 * N2) return;
-    const int i = (int)(e % N1);
-    const int k = (int)(e / N1);
-    auto get = [&](int q) { return tmp[i + (size_t)(q - 1) * N1]; };
-    float s = taps.h[0] * ext_value(get, k, nc);
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int ti = threadIdx.x & 63, tg = threadIdx.x >> 6;
+    const int i = i0 + ti;
+    const bool row_ok = i < N1;
+    auto get = [&](int q) { return tmp[(size_t)(row_ok ? i : 0) + (size_t)(q - 1) * N1]; };
+    for (int t = tg; t < 64 + 2 * kTaps; t += 4) tile[t][ti] = row_ok ? ext_value(get, k0 - kTaps + t, nc) : 0.f;
+    __syncthreads();
+    if (!row_ok) return;
+    for (int t = tg; t < 64; t += 4) {
+        const int k = k0 + t;
+        if (k >= N2) break;
+        float s = taps.h[0] * tile[t + kTaps][ti];
 #pragma unroll
-    for (int d = 1; d <= kTaps; ++d) s += taps.h[d] * (ext_value(get, k - d, nc) + ext_value(get, k + d, nc));
-    coef[e] = s;
+        for (int d = 1; d <= kTaps; ++d) s += taps.h[d] * (tile[t + kTaps - d][ti] + tile[t + kTaps + d][ti]);
+        coef[i + (size_t)k * N1] = s;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_deform(const float *__restrict__ cA, const float *__restrict__ cB, int nr,
@@ -110,8 +125,12 @@ __global__ __launch_bounds__(256) void k_deform(const float *__restrict__ cA, co
     if (e >= (size_t)nr * nc) return;
     const int r = (int)(e % nr) + 1, c = (int)(e / nr) + 1;
     const float R = (float)r, Cc = (float)c;
-    const float du2 = bilinear(pu, gny, gnx, y0, ysp, x0, xsp, R, Cc) / 2.f;
-    const float dv2 = bilinear(pv, gny, gnx, y0, ysp, x0, xsp, R, Cc) / 2.f;
+    int iy, ix;
+    float ly, lx;
+    lin_axis(y0, ysp, gny, R, iy, ly);
+    lin_axis(x0, xsp, gnx, Cc, ix, lx);
+    const float du2 = bilinear_at(pu, gny, gnx, iy, ix, ly, lx) / 2.f;
+    const float dv2 = bilinear_at(pv, gny, gnx, iy, ix, ly, lx) / 2.f;
     wA[e] = cubic(cA, nr, nc, R - dv2, Cc - du2);
     wB[e] = cubic(cB, nr, nc, R + dv2, Cc + du2);
 }
@@ -135,9 +154,10 @@ void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp,
                       hipStream_t s)
 {
     static const Taps taps = make_taps();
-    const size_t n1 = (size_t)(nr + 2) * nc, n2 = (size_t)(nr + 2) * (nc + 2);
-    hipLaunchKernelGGL(k_prefilter_rows, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s, img, nr, nc, tmp, taps);
-    hipLaunchKernelGGL(k_prefilter_cols, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, tmp, nr, nc, coef, taps);
+    hipLaunchKernelGGL(k_prefilter_rows, dim3((unsigned)((nr + 2 + 255) / 256), (unsigned)nc), dim3(256), 0, s, img,
+                       nr, nc, tmp, taps);
+    hipLaunchKernelGGL(k_prefilter_cols, dim3((unsigned)((nr + 2 + 63) / 64), (unsigned)((nc + 2 + 63) / 64)), dim3(256),
+                       0, s, tmp, nr, nc, coef, taps);
     HH_HIP(hipGetLastError());
 }
 

@@ -249,20 +249,17 @@ __global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__
     if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
 }
 
-__global__ void k_zero_i32(int *p)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
-}
-
+// flags_out (nullable) receives a copy of invalid.
 __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref, const float *__restrict__ vref,
                                                  const uint8_t *__restrict__ invalid, int ny, int nx,
                                                  const int *__restrict__ nvalid, float *__restrict__ u_out,
-                                                 float *__restrict__ v_out)
+                                                 float *__restrict__ v_out, uint8_t *__restrict__ flags_out)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ny * nx) return;
     u_out[e] = uref[e];
     v_out[e] = vref[e];
+    if (flags_out) flags_out[e] = invalid[e];
     if (!invalid[e] || *nvalid < 3) return;
     const int r = e % ny, c = e / ny;
     {
@@ -331,6 +328,26 @@ __global__ void k_fill_u8(uint8_t *p, int n, uint8_t v)
     if (e < n) p[e] = v;
 }
 
+// Clear the flags and the valid-cell counter of one validation.
+__global__ void k_val_init(uint8_t *flags, int n, int *count)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) flags[e] = 0;
+    if (e == 0) *count = 0;
+}
+
+// Move a result computed into scratch back to the chain buffers.
+__global__ void k_copy_back(const float *__restrict__ su, const float *__restrict__ sv,
+                            const uint8_t *__restrict__ sf, float *__restrict__ u, float *__restrict__ v,
+                            uint8_t *__restrict__ f, int n)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    u[e] = su[e];
+    v[e] = sv[e];
+    if (sf) f[e] = sf[e];
+}
+
 template <int NB>
 void uod_launch(const ValidateArgs &a, double thr, double eps, hipStream_t s)
 {
@@ -357,7 +374,7 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
     const int n = a.ny * a.nx;
     if (n <= 0) return;
     const dim3 g((n + 255) / 256), b(256);
-    hipLaunchKernelGGL(k_fill_u8, g, b, 0, s, a.outliers, n, (uint8_t)0);
+    hipLaunchKernelGGL(k_val_init, g, b, 0, s, a.outliers, n, a.count);
     if (p.uod_enable) uod(a, p.uod_neighborhood, p.uod_threshold, 0.1, s);
     if (p.min_peak_ratio > 1)
         hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, n, (int)HH_V_PEAK_RATIO,
@@ -370,8 +387,12 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
             hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, n, (int)vd.kind, vd.a, vd.b,
                                a.outliers);
     }
-    // no mask on the device path: outliers &= !mask is the identity
-    if (a.alt_u && p.n_peaks > 1) {
+    // no mask on the device path: outliers &= !mask is the identity.
+    // Stages alternate between the chain buffers and the scratch buffers; a
+    // result left in scratch is copied back once at the end.
+    const bool sub = a.alt_u && p.n_peaks > 1;
+    const bool rep = a.force_replace || p.replace_outliers;
+    if (sub) {
         switch (p.uod_neighborhood) {
         case 1:
             hipLaunchKernelGGL(k_substitute<1>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
@@ -386,17 +407,20 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
                                a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
             break;
         }
-        HH_HIP(hipMemcpyAsync(a.u, a.su, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-        HH_HIP(hipMemcpyAsync(a.v, a.sv, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-        HH_HIP(hipMemcpyAsync(a.outliers, a.sflag, n, hipMemcpyDeviceToDevice, s));
     }
-    if (a.force_replace || p.replace_outliers) {
-        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, a.count);
-        hipLaunchKernelGGL(k_count_valid, g, b, 0, s, a.outliers, n, a.count);
-        hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, (const int *)a.count, a.su, a.sv);
-        HH_HIP(hipMemcpyAsync(a.u, a.su, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-        HH_HIP(hipMemcpyAsync(a.v, a.sv, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    if (rep) {
+        const uint8_t *flags = sub ? a.sflag : a.outliers;
+        hipLaunchKernelGGL(k_count_valid, g, b, 0, s, flags, n, a.count);
+        if (sub)  // substituted field in scratch: replace back into the chain
+            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.ny, a.nx, (const int *)a.count, a.u,
+                               a.v, a.outliers);
+        else
+            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, (const int *)a.count, a.su,
+                               a.sv, (uint8_t *)nullptr);
     }
+    if (sub != rep)  // exactly one stage ran: its result is in scratch
+        hipLaunchKernelGGL(k_copy_back, g, b, 0, s, a.su, a.sv, sub ? a.sflag : (const uint8_t *)nullptr, a.u, a.v,
+                           a.outliers, n);
     HH_HIP(hipGetLastError());
 }
 
