@@ -9,7 +9,10 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclan
 pids=()
 for f in k_correlate k_corr_fast k_corr_mfma k_deform k_validate k_uq k_adjoint hh_api; do
   src="$HERE/$f.hip"; [ -f "$src" ] || src="$HERE/$f.cpp"
-  hipcc "${FLAGS[@]}" -x hip -c "$src" -o "$OBJ/$f.o" 2> "$OBJ/$f.log" &
+  extra=()
+  # matrix-core accumulators in VGPRs: no AGPR round trips, higher occupancy
+  [ "$f" = k_corr_mfma ] && extra=(-mllvm -amdgpu-mfma-vgpr-form=1)
+  hipcc "${FLAGS[@]}" "${extra[@]}" -x hip -c "$src" -o "$OBJ/$f.o" 2> "$OBJ/$f.log" &
   pids+=($!)
 done
 rc=0

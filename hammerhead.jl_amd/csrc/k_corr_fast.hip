@@ -352,7 +352,8 @@ __device__ __forceinline__ void uq_add(UQAcc &a, int dr, int dc, double s)
         a.S00 = s;
         return;
     }
-    const int ring = (dr < 0 ? -dr : dr) > dc ? (dr < 0 ? -dr : dr) : dc;
+    const int adr = dr < 0 ? -dr : dr, adc = dc < 0 ? -dc : dc;
+    const int ring = adr > adc ? adr : adc;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         if (ring == q + 1) {
@@ -386,6 +387,46 @@ __device__ __forceinline__ void uq_init(UQAcc &a)
         a.rmax[q] = -INFINITY;
         a.rsum[q] = 0.0;
     }
+}
+
+// Covariance sums of one component whose centered, smoothed products sit one
+// row per lane (e[c], c < NC): lane shift s (row r with row r + s of the same
+// 16-lane row, by DPP, zero past the window) times in-lane offset o, with
+// s = 0, o = 0..4 (o = 0 is S00) and s = 1..4, o = -4..4. Up to the point
+// reflection S(-a, -b) = S(a, b) this is the half-plane offset set of the
+// ring sums (the ring of an offset is max(|s|, |o|)).
+template <int NC>
+__device__ __forceinline__ void uq_lags(const double (&e)[NC], UQAcc &acc)
+{
+    {
+        double p[5];
+#pragma unroll
+        for (int o = 0; o < 5; ++o) {
+            p[o] = 0.0;
+#pragma unroll
+            for (int c = 0; c + o < NC; ++c) p[o] += e[c] * e[c + o];
+        }
+        acc.S00 = row_sum_d(p[0]);
+#pragma unroll
+        for (int o = 1; o < 5; ++o) uq_add(acc, 0, o, row_sum_d(p[o]));
+    }
+    sfor<4>([&](auto ss) {
+        constexpr int s = decltype(ss)::value + 1;
+        double sh[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sh[c] = rowsh_d<s>(e[c]);
+        double p[9];
+#pragma unroll
+        for (int oi = 0; oi < 9; ++oi) {
+            const int o = oi - 4;
+            p[oi] = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (c + o >= 0 && c + o < NC) p[oi] += e[c] * sh[c + o];
+        }
+#pragma unroll
+        for (int oi = 0; oi < 9; ++oi) uq_add(acc, s, oi - 4, row_sum_d(p[oi]));
+    });
 }
 
 // uq_component! (src/uncertainty.jl:104-151) of both components of a 16-px
@@ -426,25 +467,7 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
         mu = row_sum_d(mu) / 240.0;
 #pragma unroll
         for (int c = 0; c < 15; ++c) d[c] -= mu;
-        // lag dr across lanes: a runtime loop keeps one shifted row live
-#pragma unroll 1
-        for (int dr = -4; dr <= 4; ++dr) {
-            const bool ok = r + dr >= 0 && r + dr < 16;
-            double sh[15];
-#pragma unroll
-            for (int c = 0; c < 15; ++c) {
-                const double v = __shfl(d[c], r + dr, 16);
-                sh[c] = ok ? v : 0.0;
-            }
-#pragma unroll
-            for (int dc = 0; dc < 5; ++dc) {
-                double s = 0.0;
-#pragma unroll
-                for (int c = 0; c + dc < 15; ++c) s += d[c] * sh[c + dc];
-                s = row_sum_d(s);
-                if (dc > 0 || dr >= 0) uq_add(acc, dr, dc, s);
-            }
-        }
+        uq_lags<15>(d, acc);
         sig_u = uq_sigma(acc);
     }
     // ---- v: transposed windows: d(j; i), i = row = lane 0..14, j = 0..15 ----
@@ -483,25 +506,7 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
         mu = row_sum_d(mu) / 240.0;
 #pragma unroll
         for (int j = 0; j < 16; ++j) d[j] = live ? d[j] - mu : 0.0;
-#pragma unroll 1
-        for (int dc = 0; dc < 5; ++dc) {
-            const bool ok = r + dc < 16;
-            double sh[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const double v = __shfl(d[j], r + dc, 16);
-                sh[j] = ok ? v : 0.0;
-            }
-#pragma unroll
-            for (int dr = -4; dr <= 4; ++dr) {
-                double s = 0.0;
-#pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (j + dr >= 0 && j + dr < 16) s += d[j] * sh[j + dr];
-                s = row_sum_d(s);
-                if (dc > 0 || dr >= 0) uq_add(acc, dr, dc, s);
-            }
-        }
+        uq_lags<16>(d, acc);
         sig_v = uq_sigma(acc);
     }
 }

@@ -85,18 +85,31 @@ __device__ __forceinline__ f16v mfma3(const Split &a, const Split &b, f16v acc)
     return acc;
 }
 
-__device__ __forceinline__ float max_img(float v)  // over the 32 lanes of one image
+// The 32 lanes of one image are DPP rows 0 and 2 (image A) or 1 and 3
+// (image B): row rotations reduce within a row, one exchange across halves.
+template <int R>
+__device__ __forceinline__ float rowror_f(float v)
 {
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return fmaxf(v, __shfl_xor(v, 32));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xF, 0xF, false));
 }
 
-__device__ __forceinline__ float sum_img(float v)
+__device__ __forceinline__ void img_reduce(float &sum, float &mx, float &mn)
 {
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o);
-    return v + __shfl_xor(v, 32);
+    sum += rowror_f<8>(sum);
+    mx = fmaxf(mx, rowror_f<8>(mx));
+    mn = fminf(mn, rowror_f<8>(mn));
+    sum += rowror_f<4>(sum);
+    mx = fmaxf(mx, rowror_f<4>(mx));
+    mn = fminf(mn, rowror_f<4>(mn));
+    sum += rowror_f<2>(sum);
+    mx = fmaxf(mx, rowror_f<2>(mx));
+    mn = fminf(mn, rowror_f<2>(mn));
+    sum += rowror_f<1>(sum);
+    mx = fmaxf(mx, rowror_f<1>(mx));
+    mn = fminf(mn, rowror_f<1>(mn));
+    sum += __shfl_xor(sum, 32);
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    mn = fminf(mn, __shfl_xor(mn, 32));
 }
 
 // LDS hand-off between the lanes of one wave (the waves of a block run
@@ -110,10 +123,12 @@ __device__ __forceinline__ void wave_sync()
 
 constexpr int kHW = 7;            // box half-width (Gaussian apodization)
 constexpr int kNB = 2 * kHW + 1;  // 15 box rows and columns
-constexpr int kS1 = -5;           // stage-2 operand scale
-constexpr int kS3 = -16;          // stage-3 operand scale (cross spectrum)
-constexpr int kS3P = 13;          // stage-3 operand scale (phase spectrum, |P| <= 1)
-constexpr int kS4 = -6;           // stage-4 operand scale
+// Operand scales. The input is scaled to |x| < 2^-1, so the stage-2 operand
+// (16-term sums) stays below 2^3, Z below 2^7 and the cross spectrum below
+// 2^14: stages 2 and 3 need no rescaling. The phase spectrum (|P| <= 1) is
+// raised to 2^12; the stage-3 result (34 weighted terms) is lowered by 2^-6.
+constexpr int kS3P = 12;  // stage-3 operand scale (phase spectrum)
+constexpr int kS4 = -6;   // stage-4 operand scale
 
 template <int K, bool PHASE>
 __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
@@ -127,48 +142,63 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int h = l >> 5, n = l & 31;
 
-    // ---- constant operand tables (loop invariant) ----
-    Split F1, WrT, WiT, T3[3], C4[2], S4[2];
+    // ---- constant operand tables: one copy per block in LDS, read per use
+    // (registers are left to the accumulators) ----
+    __shared__ Split tabs[10][64];
+    if (wave == 0) {
+        Split F1, WrT, WiT, T3[3], C4[2], S4[2];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const int c = 8 * h + s;  // stage 1: K = c, column n
-        float f;
-        if (n < 16)
-            f = tg.c[(c * n) & 31];
-        else if (n == 16)
-            f = (c & 1) ? -1.f : 1.f;
-        else
-            f = -tg.s[(c * (n - 16)) & 31];
-        put(F1, s, f);
-        const int r = rmap(h, s);  // stage 2: K = r, column k1 = n
-        put(WrT, s, tg.c[(n * r) & 31]);
-        put(WiT, s, -tg.s[(n * r) & 31]);
-        // stage 3: K = (Re | Im of k2 = rmap(h, s)) and (k2 = 16), column
-        // n = (b = n & 15, Re for n < 16 | Im), l2 = b - HW
-        const int b = n & 15, im = n >> 4;
-        const bool vb = b < kNB;
-        const int l2 = b - kHW;
-        const int k2 = rmap(h, s);
-        const float cw = k2 == 0 ? 1.f : 2.f;
-        const int t = (k2 * l2) & 31;
-        put(T3[0], s, vb ? (im ? cw * tg.s[t] : cw * tg.c[t]) : 0.f);
-        put(T3[1], s, vb ? (im ? cw * tg.c[t] : -cw * tg.s[t]) : 0.f);
-        const int t16 = (16 * l2) & 31;
-        float e = 0.f;
-        if (vb && h == 0 && s == 0) e = im ? tg.s[t16] : tg.c[t16];   // Re P(k1, 16)
-        if (vb && h == 0 && s == 1) e = im ? tg.c[t16] : -tg.s[t16];  // Im P(k1, 16)
-        put(T3[2], s, e);
-        // stage 4: K = k1 = 16 q + rmap(h, s), column a = n, l1 = a - HW
-        const bool va = n < kNB;
-        const int l1 = n - kHW;
+        for (int s = 0; s < 8; ++s) {
+            const int c = 8 * h + s;  // stage 1: K = c, column n
+            float f;
+            if (n < 16)
+                f = tg.c[(c * n) & 31];
+            else if (n == 16)
+                f = (c & 1) ? -1.f : 1.f;
+            else
+                f = -tg.s[(c * (n - 16)) & 31];
+            put(F1, s, f);
+            const int r = rmap(h, s);  // stage 2: K = r, column k1 = n
+            put(WrT, s, tg.c[(n * r) & 31]);
+            put(WiT, s, -tg.s[(n * r) & 31]);
+            // stage 3: K = (Re | Im of k2 = rmap(h, s)) and (k2 = 16), column
+            // n = (b = n & 15, Re for n < 16 | Im), l2 = b - HW
+            const int b = n & 15, im = n >> 4;
+            const bool vb = b < kNB;
+            const int l2 = b - kHW;
+            const int k2 = rmap(h, s);
+            const float cw = k2 == 0 ? 1.f : 2.f;
+            const int t = (k2 * l2) & 31;
+            put(T3[0], s, vb ? (im ? cw * tg.s[t] : cw * tg.c[t]) : 0.f);
+            put(T3[1], s, vb ? (im ? cw * tg.c[t] : -cw * tg.s[t]) : 0.f);
+            const int t16 = (16 * l2) & 31;
+            float e = 0.f;
+            if (vb && h == 0 && s == 0) e = im ? tg.s[t16] : tg.c[t16];   // Re P(k1, 16)
+            if (vb && h == 0 && s == 1) e = im ? tg.c[t16] : -tg.s[t16];  // Im P(k1, 16)
+            put(T3[2], s, e);
+            // stage 4: K = k1 = 16 q + rmap(h, s), column a = n, l1 = a - HW
+            const bool va = n < kNB;
+            const int l1 = n - kHW;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int k1 = 16 * q + r;
-            const int t4 = (k1 * l1) & 31;
-            put(C4[q], s, va ? tg.c[t4] : 0.f);
-            put(S4[q], s, va ? tg.s[t4] : 0.f);
+            for (int q = 0; q < 2; ++q) {
+                const int k1 = 16 * q + r;
+                const int t4 = (k1 * l1) & 31;
+                put(C4[q], s, va ? tg.c[t4] : 0.f);
+                put(S4[q], s, va ? tg.s[t4] : 0.f);
+            }
         }
+        tabs[0][l] = F1;
+        tabs[1][l] = WrT;
+        tabs[2][l] = WiT;
+        tabs[3][l] = T3[0];
+        tabs[4][l] = T3[1];
+        tabs[5][l] = T3[2];
+        tabs[6][l] = C4[0];
+        tabs[7][l] = C4[1];
+        tabs[8][l] = S4[0];
+        tabs[9][l] = S4[1];
     }
+    __syncthreads();
     const f16v zero = {};
     const int img = n >> 4, row = n & 15;
     const float *src = img ? B : A;
@@ -198,6 +228,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         for (int j = 0; j < ng; ++j) {
         const int win = g + j;
         float *Rb = rbox[wave][j];
+        // an opaque zero keeps the table reads inside the loop
+        int z;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+        const int tl = l + z;
         float px[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) px[s] = pnext[s];
@@ -209,27 +243,30 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         jcur = jnext;
         jnext = jobs4[min(win + 2, w1 - 1)];
         // ---- 0. window: lane (image, row, 8 columns), mean, apodize ----
-        float sum = 0.f;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) sum += px[s];
-        const float mean = sum_img(sum) / 256.f;
-        float x[8], amax = 0.f;
+        float sum = 0.f, pmx = px[0], pmn = px[0];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            x[s] = apod8[s] * (px[s] - mean);
-            amax = fmaxf(amax, fabsf(x[s]));
+            sum += px[s];
+            pmx = fmaxf(pmx, px[s]);
+            pmn = fminf(pmn, px[s]);
         }
-        amax = max_img(amax);
+        img_reduce(sum, pmx, pmn);
+        const float mean = sum / 256.f;
+        float x[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) x[s] = apod8[s] * (px[s] - mean);
+        // |x| <= max - min (apodization <= 1): scale that bound into [2^-2, 2^-1)
+        const float bound = pmx - pmn;
         int ex = 0;
-        (void)frexpf(amax, &ex);
-        const int e0 = amax > 0.f ? 12 - ex : 0;  // |x| 2^e0 in [2^11, 2^12)
+        (void)frexpf(bound, &ex);
+        const int e0 = bound > 0.f ? -1 - ex : 0;
         Split a1;
 #pragma unroll
         for (int s = 0; s < 8; s += 2) put2(a1, s, ldexpf(x[s], e0), ldexpf(x[s + 1], e0));
         const int eA = __shfl(e0, 0), eB = __shfl(e0, 16);
 
         // ---- 1. column DFT of both windows ----
-        const f16v d1 = mfma3(a1, F1, zero);
+        const f16v d1 = mfma3(a1, tabs[0][tl], zero);
 
         // ---- 2. row DFT per image; Z(k1 = n, k2 = rmap(h, v)), Z(k1, 16) ----
         float zr[2][9], zi[2][9];
@@ -237,9 +274,9 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         for (int I = 0; I < 2; ++I) {
             Split a2;
 #pragma unroll
-            for (int s = 0; s < 8; s += 2) put2(a2, s, ldexpf(d1[8 * I + s], kS1), ldexpf(d1[8 * I + s + 1], kS1));
-            const f16v dr = mfma3(a2, WrT, zero);
-            const f16v di = mfma3(a2, WiT, zero);
+            for (int s = 0; s < 8; s += 2) put2(a2, s, d1[8 * I + s], d1[8 * I + s + 1]);
+            const f16v dr = mfma3(a2, tabs[1][tl], zero);
+            const f16v di = mfma3(a2, tabs[2][tl], zero);
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 // k2 >= 1: Re = Wr Tr - Wi Ti, Im = Wr Ti + Wi Tr
@@ -258,7 +295,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 
         // ---- 3. spectrum and inverse along k2 ----
         Split a3[3];
-        const int es = eA + eB + 2 * kS1;  // P scale: P_true 2^es
+        const int es = eA + eB;  // P scale: P_true 2^es
         float sp_r[9], sp_i[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
@@ -272,9 +309,6 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
                 const float wv = cfg.wfilt[n + k2 * M];
                 pr = ldexpf(wv * pr / den, kS3P);
                 pi = ldexpf(wv * pi / den, kS3P);
-            } else {
-                pr = ldexpf(pr, kS3);
-                pi = ldexpf(pi, kS3);
             }
             sp_r[v] = pr;
             sp_i[v] = pi;
@@ -293,9 +327,9 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
                 a3[2].lo[s] = (_Float16)0.f;
             }
         }
-        f16v d3 = mfma3(a3[0], T3[0], zero);
-        d3 = mfma3(a3[1], T3[1], d3);
-        d3 = mfma3(a3[2], T3[2], d3);
+        f16v d3 = mfma3(a3[0], tabs[3][tl], zero);
+        d3 = mfma3(a3[1], tabs[4][tl], d3);
+        d3 = mfma3(a3[2], tabs[5][tl], d3);
 
         // ---- 4. inverse along k1 for the box rows ----
         Split a4[2];
@@ -304,13 +338,13 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 #pragma unroll
             for (int s = 0; s < 8; s += 2)
                 put2(a4[q], s, ldexpf(d3[8 * q + s], kS4), ldexpf(d3[8 * q + s + 1], kS4));
-        f16v oc = mfma3(a4[0], C4[0], zero);
-        oc = mfma3(a4[1], C4[1], oc);
-        f16v os = mfma3(a4[0], S4[0], zero);
-        os = mfma3(a4[1], S4[1], os);
+        f16v oc = mfma3(a4[0], tabs[6][tl], zero);
+        oc = mfma3(a4[1], tabs[7][tl], oc);
+        f16v os = mfma3(a4[0], tabs[8][tl], zero);
+        os = mfma3(a4[1], tabs[9][tl], os);
 
         // R(l1 = n - HW, l2 = b - HW) = oc[Re row b] - os[Im row b]
-        const int etot = (PHASE ? kS3P : es + kS3) + kS4;
+        const int etot = (PHASE ? kS3P : es) + kS4;
         const float scale = ldexpf(1.f / (float)(M * M), -etot);
         if (n < kNB) {
 #pragma unroll
@@ -355,7 +389,7 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
     }();
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 8));  // 2 waves per SIMD
+    const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 12));  // 3 waves per SIMD
     const int nblocks = (waves + 3) / 4;
     const int chunk = (njobs + nblocks * 4 - 1) / (nblocks * 4);
     if (cfg.method == HH_PHASE) {
