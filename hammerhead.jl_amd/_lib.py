@@ -28,6 +28,7 @@ EXPORTS = (
     "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_kernel_time",
+    "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
 )
 
 
@@ -91,6 +92,12 @@ def load():
         "hh_synchronize": (i32, [vp]),
         "hh_set_timing": (i32, [vp, i32]),
         "hh_kernel_time": (i32, [vp, C.c_char_p, i32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "hh_set_pairs": (i32, [vp, vp, vp, vp, vp, i32]),
+        "hh_run_piv_batch": (i32, [vp, pp, i32, i32, vp]),
+        "hh_fetch_pair": (i32, [vp, i32, C.POINTER(hh_field)]),
+        "hh_pair_grid": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "hh_set_graphs": (i32, [vp, i32]),
+        "hh_graph_replays": (i32, [vp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -304,6 +311,55 @@ class Context:
         res["outliers"] = res["outliers"].astype(bool)
         res["mask"] = res["mask"].astype(bool)
         return res
+
+    # ---- batches of pairs of uneven sizes --------------------------------------
+    def set_pairs(self, pairs):
+        """The frames of run_piv_sequence: a list of (A, B) of any sizes,
+        copied into one device-resident atlas."""
+        As = [_f32(A) for A, _ in pairs]
+        Bs = [_f32(B) for _, B in pairs]
+        if not As:
+            raise ArgumentError("pairs must not be empty")
+        for A, B in zip(As, Bs):
+            if A.shape != B.shape:
+                raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
+        nr = np.array([A.shape[0] for A in As], dtype=np.int32)
+        nc = np.array([A.shape[1] for A in As], dtype=np.int32)
+        pa = (C.c_void_p * len(As))(*[A.ctypes.data for A in As])
+        pb = (C.c_void_p * len(Bs))(*[B.ctypes.data for B in Bs])
+        _check(self.h, load().hh_set_pairs(self.h, pa, pb, _ptr(nr), _ptr(nc), len(As)), "set_pairs")
+        self.batch_shapes = list(zip(nr.tolist(), nc.tolist()))
+        self.nr = self.nc = 0
+
+    def run_piv_batch(self, passes, predictor_smoothing=True, fetch=True):
+        """run_piv of every pair of the batch; a list of result dicts."""
+        arr, n = pass_array(passes)
+        _check(self.h, load().hh_run_piv_batch(self.h, arr, n, 1 if predictor_smoothing else 0, None),
+               "run_piv_batch")
+        if not fetch:
+            return None
+        return [self.fetch_pair(q) for q in range(len(self.batch_shapes))]
+
+    def pair_grid(self, q):
+        ny, nx = C.c_int(), C.c_int()
+        _check(self.h, load().hh_pair_grid(self.h, q, C.byref(ny), C.byref(nx)), "pair_grid")
+        return ny.value, nx.value
+
+    def fetch_pair(self, q):
+        ny, nx = self.pair_grid(q)
+        res, f = self._result_arrays(ny, nx)
+        _check(self.h, load().hh_fetch_pair(self.h, q, C.byref(f)), "fetch_pair")
+        res["outliers"] = res["outliers"].astype(bool)
+        res["mask"] = res["mask"].astype(bool)
+        return res
+
+    def set_graphs(self, enable: bool):
+        _check(self.h, load().hh_set_graphs(self.h, 1 if enable else 0), "set_graphs")
+
+    def graph_replays(self) -> int:
+        n = C.c_int64()
+        _check(self.h, load().hh_graph_replays(self.h, C.byref(n)), "graph_replays")
+        return n.value
 
     # ---- ensemble ------------------------------------------------------------
     def ensemble_begin(self, p, jobs, uncertainty=False):

@@ -22,125 +22,14 @@
 #include <tuple>
 #include <vector>
 
-#include "hh_internal.h"
+#include "hh_ctx.h"
 
 using namespace hh;
 
-namespace {
+using namespace hh::host;
 
-template <typename T>
-struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    T *ensure(size_t m)
-    {
-        if (m > n) {
-            release();
-            if (m > 0) {
-                hipError_t e = hipMalloc(&p, sizeof(T) * m);
-                if (e != hipSuccess) {
-                    p = nullptr;
-                    throw Error(HH_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
-                }
-            }
-            n = m;
-        }
-        return p;
-    }
-};
-
-// Per window-configuration device constants (apodization, overlap gain,
-// phase filter), the engine pool of piv_correlation_engines
-// (src/pipeline.jl:104-137).
-struct ConfigBufs {
-    DBuf<float> apod, gain, wfilt;
-    int box_lo = 0, box_hi = 0;
-};
-
-struct TimedLaunch {
-    std::string family;
-    int w;
-    hipEvent_t a, b;
-};
-
-struct PassGrid {
-    int ny = 0, nx = 0;
-    std::vector<float> x, y;
-    std::vector<int32_t> jobs;
-};
-
-}  // namespace
-
-struct hh_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    int form = HH_FORM_FFT32;
-    int nr = 0, nc = 0;
-    const float *A = nullptr, *B = nullptr;  // current pair (owned or borrowed)
-    DBuf<float> ownA, ownB;
-    DBuf<float> coefA, coefB, ptmp;
-    bool coefs_valid = false;
-    DBuf<float> warpA, warpB;
-    bool warp_valid = false;
-    // predictor grid (device) + knots
-    DBuf<float> pu, pv;
-    int gny = 0, gnx = 0;
-    float y0 = 0, ysp = 1, x0 = 0, xsp = 1;
-    std::map<std::tuple<int, int, int, int>, std::unique_ptr<ConfigBufs>> configs;
-    // job lists per (w, overlap, nr, nc)
-    std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<DBuf<int32_t>>> jobcache;
-    // chain buffers
-    DBuf<float> gu, gv, gpr, gcm, galtu, galtv, guu, guv, gu0, gv0, gmu, gmv, gsu, gsv, gpx, gpy, gprevu, gprevv;
-    DBuf<uint8_t> gout, gsflag;
-    DBuf<int> gcount;
-    DBuf<int32_t> hookjobs;
-    DBuf<float> hk[12];
-    DBuf<double> hkd;
-    DBuf<int32_t> hki[2];
-    // ensemble accumulation (run_piv_ensemble): FP64 plane sums per window
-    // in the layout of correlate_box_layout, pooled uncertainty statistics
-    struct {
-        bool active = false;
-        hh_pass pass{};
-        CorrConfig cfg{};
-        bool box = false, unc = false;
-        size_t per = 0;
-        int njobs = 0, npairs = 0;
-        const int32_t *jobs = nullptr;       // device
-        std::vector<int32_t> hjobs;          // host copy for bounds checks
-    } ens;
-    DBuf<int32_t> ensjobs;
-    DBuf<double> ensacc, ensstats;
-    DBuf<float> arena;  // the pairs of hh_run_piv_ensemble, resident in HBM
-    // correlation energy and its adjoint
-    DBuf<float4> estats;
-    DBuf<float> ew, epart, egrad, enode;
-    DBuf<double> esum;
-    DBuf<int2> ecells;
-    // final result of hh_run_piv
-    PassGrid final_grid;
-    bool have_result = false;
-    // timing
-    bool timing = false;
-    std::vector<TimedLaunch> pending;
-    std::vector<hipEvent_t> event_pool;
-    std::map<std::pair<std::string, int>, std::pair<double, int64_t>> ktime;
-};
-
-namespace {
-
-thread_local std::string g_create_err;
+namespace hh {
+namespace host {
 
 hipEvent_t get_event(hh_ctx *c)
 {
@@ -154,20 +43,12 @@ hipEvent_t get_event(hh_ctx *c)
     return e;
 }
 
-// Run a launch, bracketed by events when timing is on.
-template <typename F>
-void timed(hh_ctx *c, const char *family, int w, F &&f)
-{
-    if (!c->timing) {
-        f();
-        return;
-    }
-    TimedLaunch t{family, w, get_event(c), get_event(c)};
-    HH_HIP(hipEventRecord(t.a, c->stream));
-    f();
-    HH_HIP(hipEventRecord(t.b, c->stream));
-    c->pending.push_back(t);
-}
+}  // namespace host
+}  // namespace hh
+
+namespace {
+
+thread_local std::string g_create_err;
 
 void drain_timing(hh_ctx *c)
 {
@@ -208,6 +89,11 @@ int guarded(hh_ctx *c, F &&f)
         return HH_ENOMEM;
     }
 }
+
+}  // namespace
+
+namespace hh {
+namespace host {
 
 // ---- parameter checks ---------------------------------------------------------
 
@@ -429,8 +315,8 @@ void do_prefilter(hh_ctx *c)
     c->coefB.ensure(n);
     c->ptmp.ensure((size_t)(c->nr + 2) * c->nc);
     timed(c, "prefilter", 0, [&] {
-        launch_prefilter(c->A, c->nr, c->nc, c->coefA.p, c->ptmp.p, nullptr, 0, c->stream);
-        launch_prefilter(c->B, c->nr, c->nc, c->coefB.p, c->ptmp.p, nullptr, 0, c->stream);
+        launch_prefilter(c->A, c->nr, c->nr, c->nc, c->coefA.p, c->ptmp.p, c->stream);
+        launch_prefilter(c->B, c->nr, c->nr, c->nc, c->coefB.p, c->ptmp.p, c->stream);
     });
     c->coefs_valid = true;
 }
@@ -514,8 +400,8 @@ void set_predictor_from_grid(hh_ctx *c, const PassGrid &g, const float *u, const
     c->pu.ensure(n);
     c->pv.ensure(n);
     if (smoothing) {
-        launch_smooth(u, g.ny, g.nx, c->pu.p, c->stream);
-        launch_smooth(v, g.ny, g.nx, c->pv.p, c->stream);
+        launch_smooth(one_grid(g.ny, g.nx), u, c->pu.p, c->stream);
+        launch_smooth(one_grid(g.ny, g.nx), v, c->pv.p, c->stream);
     } else {
         HH_HIP(hipMemcpyAsync(c->pu.p, u, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
         HH_HIP(hipMemcpyAsync(c->pv.p, v, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -528,172 +414,10 @@ void set_predictor_from_grid(hh_ctx *c, const PassGrid &g, const float *u, const
     c->xsp = g.nx > 1 ? g.x[1] - g.x[0] : 1.f;
 }
 
-void run_schedule(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
-{
-    need_pair(c);
-    if (np < 1) throw Error(HH_EINVAL, "at least one pass is required");
-    for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
-    bool deforms = np > 1;
-    for (int k = 0; k < np; ++k) deforms |= passes[k].max_iterations > 1;
-    // size every chain buffer for the largest pass up front: no buffer is
-    // reallocated while queued work may still read it
-    size_t maxng = 0, maxalt = 0;
-    int maxny = 0, maxnx = 0;
-    for (int k = 0; k < np; ++k) {
-        int ny = 0, nx = 0;
-        hh_grid_size(&passes[k], c->nr, c->nc, &ny, &nx);
-        maxng = std::max(maxng, (size_t)ny * nx);
-        maxalt = std::max(maxalt, (size_t)ny * nx * std::max(passes[k].n_peaks - 1, 0));
-        maxny = std::max(maxny, ny);
-        maxnx = std::max(maxnx, nx);
-    }
-    for (DBuf<float> *b : {&c->gu, &c->gv, &c->gpr, &c->gcm, &c->guu, &c->guv, &c->gu0, &c->gv0, &c->gsu, &c->gsv,
-                           &c->gmu, &c->gmv, &c->gprevu, &c->gprevv, &c->pu, &c->pv})
-        b->ensure(std::max<size_t>(maxng, 1));
-    c->galtu.ensure(std::max<size_t>(maxalt, 1));
-    c->galtv.ensure(std::max<size_t>(maxalt, 1));
-    c->gout.ensure(std::max<size_t>(maxng, 1));
-    c->gsflag.ensure(std::max<size_t>(maxng, 1));
-    c->gpx.ensure(std::max(maxnx, 1));
-    c->gpy.ensure(std::max(maxny, 1));
-    if (deforms) do_prefilter(c);
-    bool have_pred = false;
-    PassGrid prev;
-    c->have_result = false;
-    for (int k = 0; k < np; ++k) {
-        const hh_pass &p = passes[k];
-        const bool force_replace = k < np - 1;
-        PassGrid g = make_grid(p, c->nr, c->nc);
-        const int ny = g.ny, nx = g.nx;
-        const size_t ng = (size_t)ny * nx;
-        if (ng == 0) throw Error(HH_EINVAL, "the pass grid is empty");
-        if (k > 0) {
-            // build_predictor from the previous pass's result (unmasked)
-            set_predictor_from_grid(c, prev, c->gu.p, c->gv.p, smoothing);
-            have_pred = true;
-        }
-        const int32_t *djobs = device_jobs(c, p, g);
-        CorrConfig cfg = make_corr(c, p);
-        const int nalt = p.n_peaks - 1;
-        float *u = c->gu.ensure(ng), *v = c->gv.ensure(ng);
-        float *pr = c->gpr.ensure(ng), *cm = c->gcm.ensure(ng);
-        float *au = nalt > 0 ? c->galtu.ensure(ng * nalt) : nullptr;
-        float *av = nalt > 0 ? c->galtv.ensure(ng * nalt) : nullptr;
-        float *uu = c->guu.ensure(ng), *uv = c->guv.ensure(ng);
-        float *u0 = c->gu0.ensure(ng), *v0 = c->gv0.ensure(ng);
-        uint8_t *outl = c->gout.ensure(ng);
-        float *su = c->gsu.ensure(ng), *sv = c->gsv.ensure(ng);
-        uint8_t *sflag = c->gsflag.ensure(ng);
-        const int maxiter = p.max_iterations;
-        const bool unc = p.uncertainty && !force_replace;
-        const bool keep_measured = maxiter > 1 && !(force_replace || p.replace_outliers);
-        float *mu = keep_measured ? c->gmu.ensure(ng) : nullptr;
-        float *mv = keep_measured ? c->gmv.ensure(ng) : nullptr;
-        bool have_prev = false;
-        bool uq_fused = false;
-        fill(c, uu, ng, NAN);
-        fill(c, uv, ng, NAN);
-        const float *srcA = c->A, *srcB = c->B;
-        for (int it = 1; it <= maxiter; ++it) {
-            if (have_pred) {
-                do_deform(c);
-                srcA = c->warpA.p;
-                srcB = c->warpB.p;
-                float *gx = c->gpx.ensure(nx), *gy = c->gpy.ensure(ny);
-                HH_HIP(hipMemcpyAsync(gx, g.x.data(), sizeof(float) * nx, hipMemcpyHostToDevice, c->stream));
-                HH_HIP(hipMemcpyAsync(gy, g.y.data(), sizeof(float) * ny, hipMemcpyHostToDevice, c->stream));
-                launch_node_values(c->pu.p, c->pv.p, c->gny, c->gnx, c->y0, c->ysp, c->x0, c->xsp, gy, ny, gx, nx,
-                                   u0, v0, c->stream);
-            } else {
-                fill(c, u0, ng, 0.f);
-                fill(c, v0, ng, 0.f);
-                srcA = c->A;
-                srcB = c->B;
-            }
-            CorrOut out{};
-            out.u = u;
-            out.v = v;
-            out.pr = pr;
-            out.cm = cm;
-            out.alt_u = au;
-            out.alt_v = av;
-            out.u0 = u0;
-            out.v0 = v0;
-            out.ny = ny;
-            out.nx = nx;
-            // the uncertainty of an iterating pass is taken once on the final
-            // sweep's warp; with a fixed sweep count that sweep is known in
-            // advance and the statistics are fused into its correlation
-            if (unc && it == maxiter && (maxiter == 1 || p.convergence_tol == 0)) {
-                out.unc_u = uu;
-                out.unc_v = uv;
-            }
-            timed(c, out.unc_u ? "correlate_uq" : "correlate", cfg.w, [&] {
-                uq_fused = launch_correlate(cfg, srcA, srcB, c->nr, c->nc, djobs, (int)ng, out, c->stream) &&
-                           out.unc_u != nullptr;
-            });
-            if (keep_measured) {
-                HH_HIP(hipMemcpyAsync(mu, u, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
-                HH_HIP(hipMemcpyAsync(mv, v, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
-            }
-            ValidateArgs va{};
-            va.ny = ny;
-            va.nx = nx;
-            va.pass = &p;
-            va.force_replace = force_replace || maxiter > 1;
-            va.u = u;
-            va.v = v;
-            va.pr = pr;
-            va.cm = cm;
-            va.alt_u = au;
-            va.alt_v = av;
-            va.outliers = outl;
-            va.su = su;
-            va.sv = sv;
-            va.sflag = sflag;
-            va.count = c->gcount.ensure(1);
-            timed(c, "validate", cfg.w, [&] { launch_validate(va, c->stream); });
-            if (it == maxiter) break;
-            if (p.convergence_tol > 0) {
-                float *pu_ = c->gprevu.ensure(ng), *pv_ = c->gprevv.ensure(ng);
-                if (!have_prev) {
-                    have_prev = true;
-                } else {
-                    float *ch = c->gsu.p;  // scratch reuse (validation is done)
-                    hipLaunchKernelGGL(k_change, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream, u, v,
-                                       pu_, pv_, (int)ng, ch);
-                    std::vector<float> h(ng);
-                    HH_HIP(hipMemcpyAsync(h.data(), ch, sizeof(float) * ng, hipMemcpyDeviceToHost, c->stream));
-                    HH_HIP(hipStreamSynchronize(c->stream));
-                    std::vector<float> buf;
-                    buf.reserve(ng);
-                    for (float x : h)
-                        if (!std::isnan(x)) buf.push_back(x);
-                    const float change = buf.empty() ? 0.f : quantile95(buf);
-                    if (change < (float)p.convergence_tol) break;
-                }
-                HH_HIP(hipMemcpyAsync(pu_, u, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
-                HH_HIP(hipMemcpyAsync(pv_, v, sizeof(float) * ng, hipMemcpyDeviceToDevice, c->stream));
-            }
-            set_predictor_from_grid(c, g, u, v, smoothing);
-            have_pred = true;
-        }
-        if (unc && !uq_fused) {
-            const ConfigBufs &cb = get_config(c, p);
-            timed(c, "uq", cfg.w, [&] {
-                launch_uq(cfg.w, cb.apod.p, srcA, srcB, c->nr, djobs, (int)ng, ny, uu, uv, nullptr, false,
-                          c->stream);
-            });
-        }
-        if (keep_measured)
-            hipLaunchKernelGGL(k_restore, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream, u, v, mu, mv,
-                               outl, (int)ng);
-        prev = std::move(g);
-    }
-    HH_HIP(hipGetLastError());
-    c->final_grid = prev;
-    c->have_result = true;
-}
+}  // namespace host
+}  // namespace hh
+
+namespace {
 
 // ---- ensemble (sum-of-correlation) PIV: src/ensemble.jl:133-343 -----------
 
@@ -802,6 +526,7 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     c->A = arena;
     c->B = arena + npx;
     c->coefs_valid = c->warp_valid = c->have_result = false;
+    c->eng.have_result = false;
     size_t maxng = 0, maxalt = 0;
     int maxny = 0, maxnx = 0;
     for (int k = 0; k < np; ++k) {
@@ -866,8 +591,7 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
         fill(c, c->guv.p, ng, NAN);
         ens_analyze(c, ny, nx, u, v, c->gpr.p, c->gcm.p, au, av, c->guu.p, c->guv.p);
         ValidateArgs va{};
-        va.ny = ny;
-        va.nx = nx;
+        va.grids = one_grid(ny, nx);
         va.pass = &p;
         va.force_replace = force_replace;
         va.u = u;
@@ -1055,7 +779,7 @@ void refine_nodes(hh_ctx *c, EnergySetup &s, float *pu, float *pv, int niter, do
 void fetch(hh_ctx *c, hh_field *r)
 {
     if (!c->have_result) throw Error(HH_ESTATE, "no result: call hh_run_piv first");
-    const PassGrid &g = c->final_grid;
+    const PassGrid &g = c->final_grid;  // the ensemble driver's grid
     if (r->ny != g.ny || r->nx != g.nx)
         throw Error(HH_EDIM, "result grid is " + std::to_string(g.ny) + "x" + std::to_string(g.nx) + ", got " +
                                  std::to_string(r->ny) + "x" + std::to_string(r->nx));
@@ -1119,6 +843,7 @@ int hh_create(hh_ctx **out, int device)
     c->device = device;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         g_create_err = std::string("stream creation failed: ") + hipGetErrorString(e);
         delete c;
@@ -1138,6 +863,7 @@ int hh_destroy(hh_ctx *c)
         (void)hipEventDestroy(t.b);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    engine_release(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HH_OK;
@@ -1192,6 +918,7 @@ int hh_set_pair(hh_ctx *c, const float *A, const float *B, int nr, int nc)
         c->nr = nr;
         c->nc = nc;
         c->coefs_valid = c->warp_valid = c->have_result = false;
+        c->eng.have_result = false;
     });
 }
 
@@ -1206,6 +933,7 @@ int hh_set_pair_device(hh_ctx *c, const float *dA, const float *dB, int nr, int 
         c->nr = nr;
         c->nc = nc;
         c->coefs_valid = c->warp_valid = c->have_result = false;
+        c->eng.have_result = false;
     });
 }
 
@@ -1409,8 +1137,7 @@ int hh_validate_and_replace(hh_ctx *c, const hh_pass *pass, int ny, int nx, int 
                 if (mask[k]) throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks yet");
         const int nalt = pass->n_peaks - 1;
         ValidateArgs va{};
-        va.ny = ny;
-        va.nx = nx;
+        va.grids = one_grid(ny, nx);
         va.pass = pass;
         va.force_replace = force_replace;
         va.u = upload(c, c->hk[0], u, ng);
@@ -1443,7 +1170,7 @@ int hh_smooth_field(hh_ctx *c, int ny, int nx, const float *f, float *out)
         const size_t n = (size_t)ny * nx;
         float *df = upload(c, c->hk[10], f, n);
         float *dout = c->hk[11].ensure(n);
-        launch_smooth(df, ny, nx, dout, c->stream);
+        launch_smooth(one_grid(ny, nx), df, dout, c->stream);
         download(c, out, (const float *)dout, n);
         HH_HIP(hipStreamSynchronize(c->stream));
     });
@@ -1453,8 +1180,11 @@ int hh_run_piv(hh_ctx *c, const hh_pass *passes, int npasses, int predictor_smoo
 {
     return guarded(c, [&] {
         if (!passes) throw Error(HH_EINVAL, "null pass schedule");
-        run_schedule(c, passes, npasses, predictor_smoothing);
-        if (result) fetch(c, result);
+        need_pair(c);
+        engine_single_layout(c);
+        c->have_result = false;
+        engine_run(c, passes, npasses, predictor_smoothing);
+        if (result) engine_fetch(c, 0, result);
     });
 }
 
@@ -1462,7 +1192,65 @@ int hh_fetch_result(hh_ctx *c, hh_field *result)
 {
     return guarded(c, [&] {
         if (!result) throw Error(HH_EINVAL, "null result");
-        fetch(c, result);
+        if (c->eng.have_result)
+            engine_fetch(c, 0, result);
+        else
+            fetch(c, result);
+    });
+}
+
+int hh_set_pairs(hh_ctx *c, const float *const *A, const float *const *B, const int32_t *nr, const int32_t *nc,
+                 int npairs)
+{
+    return guarded(c, [&] { engine_set_pairs(c, A, B, nr, nc, npairs); });
+}
+
+int hh_run_piv_batch(hh_ctx *c, const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *results)
+{
+    return guarded(c, [&] {
+        if (!passes) throw Error(HH_EINVAL, "null pass schedule");
+        if (c->eng.lay.npairs < 1 || !c->eng.lay.A) throw Error(HH_ESTATE, "no batch: call hh_set_pairs first");
+        c->have_result = false;
+        engine_run(c, passes, npasses, predictor_smoothing);
+        if (results)
+            for (int q = 0; q < c->eng.lay.npairs; ++q) engine_fetch(c, q, &results[q]);
+    });
+}
+
+int hh_fetch_pair(hh_ctx *c, int q, hh_field *result)
+{
+    return guarded(c, [&] {
+        if (!result) throw Error(HH_EINVAL, "null result");
+        engine_fetch(c, q, result);
+    });
+}
+
+int hh_pair_grid(hh_ctx *c, int q, int *ny, int *nx)
+{
+    return guarded(c, [&] {
+        if (!ny || !nx) throw Error(HH_EINVAL, "null argument");
+        if (c->eng.plan.empty()) throw Error(HH_ESTATE, "no schedule has been planned");
+        if (q < 0 || q >= c->eng.lay.npairs) throw Error(HH_EINVAL, "pair index out of range");
+        *ny = c->eng.plan.back()->ny[q];
+        *nx = c->eng.plan.back()->nx[q];
+    });
+}
+
+int hh_set_graphs(hh_ctx *c, int enable)
+{
+    return guarded(c, [&] {
+        c->eng.use_graphs = enable != 0;
+        if (!enable && c->eng.graph) {
+            (void)hipGraphExecDestroy(c->eng.graph);
+            c->eng.graph = nullptr;
+        }
+    });
+}
+
+int hh_graph_replays(hh_ctx *c, int64_t *replays)
+{
+    return guarded(c, [&] {
+        if (replays) *replays = c->eng.graph_replays;
     });
 }
 

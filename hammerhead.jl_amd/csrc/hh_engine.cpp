@@ -1,0 +1,633 @@
+// hh_engine.cpp — run_piv on the device for one pair or for a batch of pairs
+// of uneven sizes.
+//
+// The orchestration follows run_piv / piv_pass (src/pipeline.jl:361-449,
+// 655-832) per pair: each pass builds its predictor from the previous
+// result (build_predictor, src/pipeline.jl:454-464), every sweep deforms the
+// prefiltered pair by it, correlates all windows, adds the predictor back,
+// validates and replaces, iterating passes rebuild their own predictor
+// between sweeps, and the Wieneke uncertainty runs once on the final warp. A
+// batch is run_piv_sequence (src/io.jl:229-247) without the serial loop: the
+// pairs are packed into one column-major image atlas (shelves of equal
+// width, stacked along the rows), so one correlation launch per sweep covers
+// the windows of every pair — window origins are atlas coordinates and each
+// job's grid index is its cell in the concatenated vector grids. Deformation,
+// predictor node values, validation and smoothing are one launch each over
+// all pairs through per-pair tables (no neighbourhood crosses a pair).
+//
+// With a fixed sweep count in every pass (max_iterations = 1 or
+// convergence_tol = 0) the whole schedule is a pure sequence of launches on
+// the context stream: it is captured once into a hipGraph and replayed.
+// Otherwise each pair stops iterating on its own q95 field change
+// (field_change, src/pipeline.jl:842-866, read back after every sweep) and
+// later sweeps run on the pairs that have not converged.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "hh_ctx.h"
+
+namespace hh {
+namespace host {
+
+namespace {
+
+// ---- elementwise kernels over the enumerated cells of a Grids ---------------
+
+__global__ void k_gfill(float *__restrict__ p, Grids g, float v)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < g.total) p[cell_of(g, e).b] = v;
+}
+
+__global__ void k_gcopy2(const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ x,
+                         float *__restrict__ y, Grids g)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.total) return;
+    const int k = cell_of(g, e).b;
+    x[k] = a[k];
+    y[k] = b[k];
+}
+
+// keep_measured (src/pipeline.jl:723,768-770,826-829): outliers get their
+// last measured value back.
+__global__ void k_grestore(float *__restrict__ u, float *__restrict__ v, const float *__restrict__ mu,
+                           const float *__restrict__ mv, const uint8_t *__restrict__ out, Grids g)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.total) return;
+    const int k = cell_of(g, e).b;
+    if (out[k]) {
+        u[k] = mu[k];
+        v[k] = mv[k];
+    }
+}
+
+// max(|du|, |dv|) per cell for field_change (src/pipeline.jl:849-866);
+// NaN/finite flips give +Inf, NaN in both sweeps gives NaN (dropped on host).
+__global__ void k_gchange(const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ pu,
+                          const float *__restrict__ pv, Grids g, float *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.total) return;
+    const int k = cell_of(g, e).b;
+    const float du = fabsf(u[k] - pu[k]), dv = fabsf(v[k] - pv[k]);
+    if (isnan(du) || isnan(dv)) {
+        const bool same = (isnan(u[k]) == isnan(pu[k])) && (isnan(v[k]) == isnan(pv[k]));
+        out[k] = same ? NAN : INFINITY;
+        return;
+    }
+    out[k] = du > dv ? du : dv;
+}
+
+dim3 cells_grid(const Grids &g) { return dim3((unsigned)((g.total + 255) / 256)); }
+
+void gfill(hh_ctx *c, float *p, const Grids &g, float v)
+{
+    if (g.total <= 0) return;
+    hipLaunchKernelGGL(k_gfill, cells_grid(g), dim3(256), 0, c->stream, p, g, v);
+    HH_HIP(hipGetLastError());
+}
+
+void gcopy2(hh_ctx *c, const float *a, const float *b, float *x, float *y, const Grids &g)
+{
+    if (g.total <= 0) return;
+    hipLaunchKernelGGL(k_gcopy2, cells_grid(g), dim3(256), 0, c->stream, a, b, x, y, g);
+    HH_HIP(hipGetLastError());
+}
+
+// ---- layout ---------------------------------------------------------------
+
+// Shelf packing: pairs sorted by width (then height), each shelf is a run of
+// atlas columns as wide as its pairs, pairs stacked along the rows up to the
+// tallest pair's height.
+void pack_layout(Layout &L, const int32_t *nr, const int32_t *nc, int npairs)
+{
+    L.npairs = npairs;
+    L.nr.assign(nr, nr + npairs);
+    L.nc.assign(nc, nc + npairs);
+    L.roff.assign(npairs, 0);
+    L.coff.assign(npairs, 0);
+    L.ld = *std::max_element(nr, nr + npairs);
+    std::vector<int> order(npairs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return nc[a] != nc[b] ? nc[a] > nc[b] : nr[a] > nr[b];
+    });
+    long long col = 0;
+    int shelf_w = -1, row = 0;
+    for (int q : order) {
+        if (nc[q] != shelf_w || row + nr[q] > L.ld) {
+            if (shelf_w > 0) col += shelf_w;
+            shelf_w = nc[q];
+            row = 0;
+        }
+        L.roff[q] = row;
+        L.coff[q] = (int)col;
+        row += nr[q];
+    }
+    col += shelf_w;
+    L.ncols = col;
+    L.coef_off.assign(npairs, 0);
+    size_t off = 0;
+    for (int q = 0; q < npairs; ++q) {
+        L.coef_off[q] = off;
+        off += (size_t)(nr[q] + 2) * (nc[q] + 2);
+    }
+    L.coef_total = off;
+}
+
+uint64_t g_layout_gen = 0;
+
+void invalidate_plan(Engine &E)
+{
+    if (E.graph) (void)hipGraphExecDestroy(E.graph);
+    E.graph = nullptr;
+    E.graph_failed = false;
+    E.graph_replays = 0;
+    E.key.clear();
+    E.plan.clear();
+}
+
+// Knots of a pass grid axis: first centre and spacing, in the float
+// arithmetic of the PassGrid coordinates (start + half, difference of the
+// first two).
+void knots(int r0, int step, float half, int n, float &q0, float &qsp)
+{
+    const float a = (float)r0 + half;
+    q0 = a;
+    qsp = n > 1 ? ((float)(r0 + step) + half) - a : 1.f;
+}
+
+template <typename T>
+T *upload_sync(hh_ctx *c, DBuf<T> &b, const std::vector<T> &h)
+{
+    T *p = b.ensure(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) HH_HIP(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+    return p;
+}
+
+// Deformation table of the pairs in `pairs` with the predictor held by the
+// grids of `pred` (per pair ny, nx, cell offset and knots).
+std::vector<DeformPair> deform_table(hh_ctx *c, const std::vector<int> &pairs, const PassPlan &pred, int &blocks)
+{
+    Engine &E = c->eng;
+    const Layout &L = E.lay;
+    const float *cA = E.coefs.p, *cB = E.coefs.p + L.coef_total;
+    std::vector<DeformPair> t;
+    blocks = 0;
+    for (int q : pairs) {
+        DeformPair d{};
+        d.cA = cA + L.coef_off[q];
+        d.cB = cB + L.coef_off[q];
+        d.pu = E.pu.p + pred.off[q];
+        d.pv = E.pv.p + pred.off[q];
+        d.nr = L.nr[q];
+        d.nc = L.nc[q];
+        d.gny = pred.ny[q];
+        d.gnx = pred.nx[q];
+        knots(pred.r0, pred.rstep, pred.half, d.gny, d.y0, d.ysp);
+        knots(pred.c0, pred.cstep, pred.half, d.gnx, d.x0, d.xsp);
+        d.out_off = (long long)L.roff[q] + (long long)L.coff[q] * L.ld;
+        d.blk0 = blocks;
+        blocks += (int)(((size_t)d.nr * d.nc + 255) / 256);
+        t.push_back(d);
+    }
+    return t;
+}
+
+// Tables of pass P over the pairs in `pairs` (jobs, segments, counters and
+// the deformation tables); prev is the previous pass (predictor source of
+// the first sweep) or null.
+void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vector<int> &pairs,
+                  const std::vector<int32_t> &hjobs_all, const std::vector<int> &job_off, PassTables &T)
+{
+    std::vector<int32_t> jobs;
+    std::vector<GridSeg> segs;
+    int start = 0;
+    for (size_t s = 0; s < pairs.size(); ++s) {
+        const int q = pairs[s];
+        const int n = P.ny[q] * P.nx[q];
+        jobs.insert(jobs.end(), hjobs_all.begin() + 4 * (size_t)job_off[q],
+                    hjobs_all.begin() + 4 * ((size_t)job_off[q] + n));
+        segs.push_back(GridSeg{start, P.off[q], P.ny[q], P.nx[q], (int)s, {0, 0, 0}});
+        start += n;
+    }
+    T.njobs = (int)(jobs.size() / 4);
+    upload_sync(c, T.jobs, jobs);
+    upload_sync(c, T.segs, segs);
+    T.grids = Grids{T.segs.p, (int)segs.size(), 0, 0, start, P.total};
+    T.count.ensure(std::max<size_t>(segs.size(), 1));
+    int b1 = 0, b2 = 0;
+    if (c->eng.deforms) {
+        if (prev) upload_sync(c, T.dprev, deform_table(c, pairs, *prev, b1));
+        upload_sync(c, T.dself, deform_table(c, pairs, P, b2));
+    }
+    T.blocks = std::max(b1, b2);  // every pair has the same pixel count in both tables
+    HH_HIP(hipStreamSynchronize(c->stream));
+}
+
+void serialize_pass(std::vector<uint8_t> &k, const hh_pass &p)
+{
+    auto put = [&](const void *v, size_t n) {
+        const uint8_t *b = (const uint8_t *)v;
+        k.insert(k.end(), b, b + n);
+    };
+    put(p.window_size, sizeof p.window_size);
+    put(p.search_area_size, sizeof p.search_area_size);
+    put(p.overlap, sizeof p.overlap);
+    put(&p.correlation_method, 4);
+    put(&p.padding, 4);
+    put(&p.apodization, 4);
+    put(&p.subpixel_method, 4);
+    put(&p.n_peaks, 4);
+    put(&p.peak_finder, 4);
+    put(&p.uncertainty, 4);
+    put(&p.uod_enable, 4);
+    put(&p.uod_threshold, 8);
+    put(&p.uod_neighborhood, 4);
+    put(&p.replace_outliers, 4);
+    put(&p.min_peak_ratio, 8);
+    put(&p.max_iterations, 4);
+    put(&p.convergence_tol, 8);
+    put(&p.n_validators, 4);
+    for (int q = 0; q < p.n_validators && q < HH_MAX_VALIDATORS; ++q) {
+        put(&p.validators[q].kind, 4);
+        put(&p.validators[q].neighborhood, 4);
+        put(&p.validators[q].a, 8);
+        put(&p.validators[q].b, 8);
+    }
+}
+
+void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
+{
+    Engine &E = c->eng;
+    const Layout &L = E.lay;
+    invalidate_plan(E);
+    E.smoothing = smoothing;
+    E.deforms = np > 1;
+    for (int k = 0; k < np; ++k) E.deforms |= passes[k].max_iterations > 1;
+    std::vector<int> all(L.npairs);
+    std::iota(all.begin(), all.end(), 0);
+    E.maxcells = 0;
+    E.maxalt = 0;
+    E.hjobs.assign(np, {});
+    for (int k = 0; k < np; ++k) {
+        auto P = std::make_unique<PassPlan>();
+        P->p = passes[k];
+        P->cfg = make_corr(c, passes[k]);
+        const hh_pass &p = passes[k];
+        const int w = p.window_size[0];
+        P->r0 = 1;
+        P->rstep = w - p.overlap[0];
+        P->c0 = 1;
+        P->cstep = p.window_size[1] - p.overlap[1];
+        P->half = (float)((w - 1) / 2.0);
+        P->fixed = p.max_iterations == 1 || p.convergence_tol == 0;
+        HostJobs &H = E.hjobs[k];
+        for (int q = 0; q < L.npairs; ++q) {
+            PassGrid g = make_grid(p, L.nr[q], L.nc[q]);
+            if ((size_t)g.ny * g.nx == 0) throw Error(HH_EINVAL, "the pass grid is empty");
+            if ((long long)P->total + (long long)g.ny * g.nx > (1LL << 30))
+                throw Error(HH_EINVAL, "too many vectors in one batch");
+            P->ny.push_back(g.ny);
+            P->nx.push_back(g.nx);
+            P->off.push_back(P->total);
+            H.off.push_back((int)(H.jobs.size() / 4));
+            for (size_t j = 0; j < (size_t)g.ny * g.nx; ++j) {
+                H.jobs.push_back(P->total + (int32_t)j + 1);  // cell of column-major job order
+                H.jobs.push_back(1);
+                H.jobs.push_back(g.jobs[4 * j + 2] + L.roff[q]);
+                H.jobs.push_back(g.jobs[4 * j + 3] + L.coff[q]);
+            }
+            P->total += g.ny * g.nx;
+        }
+        E.maxcells = std::max(E.maxcells, (size_t)P->total);
+        E.maxalt = std::max(E.maxalt, (size_t)P->total * std::max(p.n_peaks - 1, 0));
+        correlate_prepare(P->cfg, c->stream);
+        E.plan.push_back(std::move(P));
+    }
+    // buffers first (the tables hold pointers into them)
+    for (DBuf<float> *b : {&E.u, &E.v, &E.pr, &E.cm, &E.uu, &E.uv, &E.u0, &E.v0, &E.su, &E.sv, &E.mu, &E.mv,
+                           &E.prevu, &E.prevv, &E.pu, &E.pv, &E.ch})
+        b->ensure(std::max<size_t>(E.maxcells, 1));
+    E.altu.ensure(std::max<size_t>(E.maxalt, 1));
+    E.altv.ensure(std::max<size_t>(E.maxalt, 1));
+    E.out.ensure(std::max<size_t>(E.maxcells, 1));
+    E.sflag.ensure(std::max<size_t>(E.maxcells, 1));
+    if (E.deforms) {
+        E.coefs.ensure(2 * L.coef_total);
+        E.warp.ensure(2 * (size_t)L.ld * (size_t)L.ncols);
+        size_t t = 0;
+        for (int q = 0; q < L.npairs; ++q) t = std::max(t, (size_t)(L.nr[q] + 2) * L.nc[q]);
+        E.ptmp.ensure(t);
+    }
+    for (int k = 0; k < np; ++k)
+        build_tables(c, *E.plan[k], k > 0 ? E.plan[k - 1].get() : nullptr, all, E.hjobs[k].jobs, E.hjobs[k].off,
+                     E.plan[k]->all);
+}
+
+void predictor(hh_ctx *c, const Grids &g)
+{
+    Engine &E = c->eng;
+    if (E.smoothing) {
+        launch_smooth(g, E.u.p, E.pu.p, c->stream);
+        launch_smooth(g, E.v.p, E.pv.p, c->stream);
+    } else {
+        gcopy2(c, E.u.p, E.v.p, E.pu.p, E.pv.p, g);
+    }
+}
+
+// The launch sequence of the whole schedule. With `dynamic` false every
+// pass has a fixed sweep count and nothing here touches the host (the
+// sequence may be captured into a graph).
+void record(hh_ctx *c, bool dynamic)
+{
+    Engine &E = c->eng;
+    const Layout &L = E.lay;
+    const int np = (int)E.plan.size();
+    hipStream_t s = c->stream;
+    float *cA = E.coefs.p, *cB = E.coefs.p + L.coef_total;
+    float *wA = E.warp.p, *wB = E.warp.p + (size_t)L.ld * L.ncols;
+    if (E.deforms) {
+        timed(c, "prefilter", 0, [&] {
+            for (int q = 0; q < L.npairs; ++q) {
+                const size_t o = (size_t)L.roff[q] + (size_t)L.coff[q] * L.ld;
+                launch_prefilter(L.A + o, L.ld, L.nr[q], L.nc[q], cA + L.coef_off[q], E.ptmp.p, s);
+                launch_prefilter(L.B + o, L.ld, L.nr[q], L.nc[q], cB + L.coef_off[q], E.ptmp.p, s);
+            }
+        });
+    }
+    bool have_pred = false, pred_self = false;
+    for (int k = 0; k < np; ++k) {
+        PassPlan &P = *E.plan[k];
+        const hh_pass &p = P.p;
+        const bool force_replace = k < np - 1;
+        if (k > 0) {
+            predictor(c, E.plan[k - 1]->all.grids);
+            have_pred = true;
+            pred_self = false;
+        }
+        const int maxiter = p.max_iterations;
+        const bool unc = p.uncertainty && !force_replace;
+        const bool keep_measured = maxiter > 1 && !(force_replace || p.replace_outliers);
+        const int nalt = p.n_peaks - 1;
+        gfill(c, E.uu.p, P.all.grids, NAN);
+        gfill(c, E.uv.p, P.all.grids, NAN);
+        PassTables *T = &P.all;
+        std::vector<int> active(L.npairs);
+        std::iota(active.begin(), active.end(), 0);
+        bool uq_fused = false;
+        const float *srcA = L.A, *srcB = L.B;
+        for (int it = 1; it <= maxiter; ++it) {
+            const DeformPair *dt = pred_self ? T->dself.p : T->dprev.p;
+            if (have_pred) {
+                DeformTab tab{};
+                tab.pairs = dt;
+                tab.npairs = T->grids.nseg;
+                tab.nblocks = T->blocks;
+                timed(c, "deform", 0, [&] { launch_deform_batch(tab, L.ld, wA, wB, s); });
+                srcA = wA;
+                srcB = wB;
+                launch_node_values_batch(T->grids, dt, P.r0, P.rstep, P.c0, P.cstep, P.half, E.u0.p, E.v0.p, s);
+            } else {
+                gfill(c, E.u0.p, T->grids, 0.f);
+                gfill(c, E.v0.p, T->grids, 0.f);
+                srcA = L.A;
+                srcB = L.B;
+            }
+            CorrOut out{};
+            out.u = E.u.p;
+            out.v = E.v.p;
+            out.pr = E.pr.p;
+            out.cm = E.cm.p;
+            out.alt_u = nalt > 0 ? E.altu.p : nullptr;
+            out.alt_v = nalt > 0 ? E.altv.p : nullptr;
+            out.u0 = E.u0.p;
+            out.v0 = E.v0.p;
+            out.ny = P.total;  // jobs carry (cell + 1, 1)
+            out.nx = 1;
+            // the uncertainty of an iterating pass is taken once on the final
+            // sweep's warp; with a fixed sweep count that sweep is known in
+            // advance and the statistics are fused into its correlation
+            if (unc && it == maxiter && P.fixed) {
+                out.unc_u = E.uu.p;
+                out.unc_v = E.uv.p;
+            }
+            timed(c, out.unc_u ? "correlate_uq" : "correlate", P.cfg.w, [&] {
+                uq_fused = launch_correlate(P.cfg, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs, out, s) &&
+                           out.unc_u != nullptr;
+            });
+            if (keep_measured) gcopy2(c, E.u.p, E.v.p, E.mu.p, E.mv.p, T->grids);
+            ValidateArgs va{};
+            va.grids = T->grids;
+            va.pass = &p;
+            va.force_replace = force_replace || maxiter > 1;
+            va.u = E.u.p;
+            va.v = E.v.p;
+            va.pr = E.pr.p;
+            va.cm = E.cm.p;
+            va.alt_u = out.alt_u;
+            va.alt_v = out.alt_v;
+            va.outliers = E.out.p;
+            va.su = E.su.p;
+            va.sv = E.sv.p;
+            va.sflag = E.sflag.p;
+            va.count = T->count.p;
+            timed(c, "validate", P.cfg.w, [&] { launch_validate(va, s); });
+            if (it == maxiter) break;
+            if (!P.fixed) {
+                if (!dynamic) throw Error(HH_ESTATE, "internal: iterating pass without a fixed sweep count");
+                if (it > 1) {
+                    // field_change per pair; converged pairs stop here
+                    hipLaunchKernelGGL(k_gchange, cells_grid(T->grids), dim3(256), 0, s, E.u.p, E.v.p, E.prevu.p,
+                                       E.prevv.p, T->grids, E.ch.p);
+                    HH_HIP(hipGetLastError());
+                    std::vector<float> h(P.total);
+                    HH_HIP(hipMemcpyAsync(h.data(), E.ch.p, sizeof(float) * P.total, hipMemcpyDeviceToHost, s));
+                    HH_HIP(hipStreamSynchronize(s));
+                    std::vector<int> still;
+                    for (int q : active) {
+                        std::vector<float> buf;
+                        buf.reserve((size_t)P.ny[q] * P.nx[q]);
+                        for (int j = 0; j < P.ny[q] * P.nx[q]; ++j) {
+                            const float x = h[P.off[q] + j];
+                            if (!std::isnan(x)) buf.push_back(x);
+                        }
+                        const float change = buf.empty() ? 0.f : quantile95(buf);
+                        if (!(change < (float)p.convergence_tol)) still.push_back(q);
+                    }
+                    if (still.empty()) break;
+                    if (still.size() != active.size()) {
+                        active = still;
+                        build_tables(c, P, nullptr, active, E.hjobs[k].jobs, E.hjobs[k].off, E.sub);
+                        T = &E.sub;
+                    }
+                }
+                gcopy2(c, E.u.p, E.v.p, E.prevu.p, E.prevv.p, T->grids);
+            }
+            predictor(c, T->grids);
+            have_pred = true;
+            pred_self = true;
+        }
+        if (unc && !uq_fused) {
+            const ConfigBufs &cb = get_config(c, p);
+            timed(c, "uq", P.cfg.w, [&] {
+                launch_uq(P.cfg.w, cb.apod.p, srcA, srcB, L.ld, P.all.jobs.p, P.all.njobs, P.total, E.uu.p, E.uv.p,
+                          nullptr, false, s);
+            });
+        }
+        if (keep_measured)
+            hipLaunchKernelGGL(k_grestore, cells_grid(P.all.grids), dim3(256), 0, s, E.u.p, E.v.p, E.mu.p, E.mv.p,
+                               E.out.p, P.all.grids);
+    }
+    HH_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void engine_single_layout(hh_ctx *c)
+{
+    Layout &L = c->eng.lay;
+    if (L.npairs == 1 && L.A == c->A && L.B == c->B && L.nr[0] == c->nr && L.nc[0] == c->nc && L.ld == c->nr)
+        return;
+    const int32_t nr = c->nr, nc = c->nc;
+    pack_layout(L, &nr, &nc, 1);
+    L.A = c->A;
+    L.B = c->B;
+    L.gen = ++g_layout_gen;
+    invalidate_plan(c->eng);
+    c->eng.have_result = false;
+}
+
+void engine_set_pairs(hh_ctx *c, const float *const *A, const float *const *B, const int32_t *nr, const int32_t *nc,
+                      int npairs)
+{
+    if (npairs < 1) throw Error(HH_EINVAL, "pairs must not be empty");
+    if (!A || !B || !nr || !nc) throw Error(HH_EINVAL, "null pair list");
+    for (int q = 0; q < npairs; ++q) {
+        if (!A[q] || !B[q]) throw Error(HH_EINVAL, "null image in the pair list");
+        if (nr[q] < 2 || nc[q] < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
+    }
+    Engine &E = c->eng;
+    invalidate_plan(E);
+    E.have_result = false;
+    Layout &L = E.lay;
+    pack_layout(L, nr, nc, npairs);
+    if ((double)L.ld * (double)L.ncols > 4.0e9) throw Error(HH_EINVAL, "batch atlas exceeds 2^32 pixels");
+    const size_t per = (size_t)L.ld * (size_t)L.ncols;
+    float *at = E.atlas.ensure(2 * per);
+    for (int q = 0; q < npairs; ++q) {
+        float *dA = at + (size_t)L.roff[q] + (size_t)L.coff[q] * L.ld;
+        HH_HIP(hipMemcpy2DAsync(dA, sizeof(float) * L.ld, A[q], sizeof(float) * nr[q], sizeof(float) * nr[q], nc[q],
+                                hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipMemcpy2DAsync(dA + per, sizeof(float) * L.ld, B[q], sizeof(float) * nr[q], sizeof(float) * nr[q],
+                                nc[q], hipMemcpyHostToDevice, c->stream));
+    }
+    HH_HIP(hipStreamSynchronize(c->stream));
+    L.A = at;
+    L.B = at + per;
+    L.gen = ++g_layout_gen;
+    // the single-pair state no longer describes the context's images
+    c->A = c->B = nullptr;
+    c->coefs_valid = c->warp_valid = c->have_result = false;
+}
+
+void engine_run(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
+{
+    Engine &E = c->eng;
+    if (E.lay.npairs < 1 || !E.lay.A) throw Error(HH_ESTATE, "no image pair: call hh_set_pair or hh_set_pairs first");
+    if (np < 1) throw Error(HH_EINVAL, "at least one pass is required");
+    for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
+    std::vector<uint8_t> key;
+    auto put = [&](const void *v, size_t n) {
+        const uint8_t *b = (const uint8_t *)v;
+        key.insert(key.end(), b, b + n);
+    };
+    put(&E.lay.gen, sizeof E.lay.gen);
+    put(&c->form, sizeof c->form);
+    put(&smoothing, sizeof smoothing);
+    put(&np, sizeof np);
+    for (int k = 0; k < np; ++k) serialize_pass(key, passes[k]);
+    if (key != E.key) {
+        E.have_result = false;
+        plan(c, passes, np, smoothing);
+        E.key = key;
+    }
+    bool dynamic = false;
+    for (auto &P : E.plan) dynamic |= !P->fixed;
+    E.have_result = false;
+    c->have_result = false;
+    if (!dynamic && !c->timing && E.use_graphs && !E.graph_failed) {
+        if (!E.graph) {
+            hipGraph_t g = nullptr;
+            HH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            bool ok = true;
+            std::string why;
+            try {
+                record(c, false);
+            } catch (const Error &e) {
+                ok = false;
+                why = e.msg;
+            }
+            const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+            hipGraphExec_t ex = nullptr;
+            if (ok && ee == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+                E.graph = ex;
+            } else {
+                E.graph_failed = true;
+                (void)hipGetLastError();
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (!ok && why.find("internal") != std::string::npos) throw Error(HH_ESTATE, why);
+        }
+        if (E.graph) {
+            HH_HIP(hipGraphLaunch(E.graph, c->stream));
+            ++E.graph_replays;
+            E.have_result = true;
+            return;
+        }
+    }
+    record(c, dynamic);
+    E.have_result = true;
+}
+
+void engine_fetch(hh_ctx *c, int q, hh_field *r)
+{
+    Engine &E = c->eng;
+    if (!E.have_result || E.plan.empty()) throw Error(HH_ESTATE, "no result: call hh_run_piv first");
+    if (q < 0 || q >= E.lay.npairs) throw Error(HH_EINVAL, "pair index out of range");
+    const PassPlan &P = *E.plan.back();
+    const int ny = P.ny[q], nx = P.nx[q];
+    if (r->ny != ny || r->nx != nx)
+        throw Error(HH_EDIM, "result grid is " + std::to_string(ny) + "x" + std::to_string(nx) + ", got " +
+                                 std::to_string(r->ny) + "x" + std::to_string(r->nx));
+    const size_t ng = (size_t)ny * nx, o = (size_t)P.off[q];
+    auto d2h = [&](void *dst, const void *src, size_t bytes) {
+        if (dst) HH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    };
+    d2h(r->u, E.u.p + o, sizeof(float) * ng);
+    d2h(r->v, E.v.p + o, sizeof(float) * ng);
+    d2h(r->peak_ratio, E.pr.p + o, sizeof(float) * ng);
+    d2h(r->correlation_moment, E.cm.p + o, sizeof(float) * ng);
+    d2h(r->uncertainty_u, E.uu.p + o, sizeof(float) * ng);
+    d2h(r->uncertainty_v, E.uv.p + o, sizeof(float) * ng);
+    d2h(r->outliers, E.out.p + o, ng);
+    HH_HIP(hipStreamSynchronize(c->stream));
+    if (r->x)
+        for (int j = 0; j < nx; ++j) r->x[j] = (float)(P.c0 + j * P.cstep) + P.half;
+    if (r->y)
+        for (int i = 0; i < ny; ++i) r->y[i] = (float)(P.r0 + i * P.rstep) + P.half;
+    if (r->mask) std::memset(r->mask, 0, ng);
+}
+
+void engine_release(hh_ctx *c) { invalidate_plan(c->eng); }
+
+}  // namespace host
+}  // namespace hh

@@ -76,6 +76,10 @@ struct CorrOut {
 bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
                       int njobs, const CorrOut &out, hipStream_t s);
 
+// Builds the constant tables a correlation launch of cfg may need (device
+// allocations happen here, never inside a launch sequence).
+void correlate_prepare(const CorrConfig &cfg, hipStream_t s);
+
 // Plane-sum layout of the accumulate mode: true when the specialized kernels
 // take the configuration and add only the analysis box [box_lo, box_hi]^2
 // (box side nb = box_hi - box_lo + 1, column-major, nb^2 doubles per
@@ -89,6 +93,9 @@ bool correlate_box_layout(const CorrConfig &cfg);
 void launch_plane_analyze(const CorrConfig &cfg, const double *acc, bool box, const int32_t *jobs, int njobs,
                           const CorrOut &out, hipStream_t s);
 
+// Compute units of the current device (cached after the first call).
+int device_cu_count();
+
 // Split-FP16 MFMA DFT kernels (HH_FORM_MFMA16); false when not covered.
 bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s);
@@ -97,28 +104,110 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s, bool *uq_done);
 
-void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp, const float *cp, int ncp,
-                      hipStream_t s);
+// ---- vector grids of one or several pairs -----------------------------------
+// The cells of several vector grids stored back to back in one set of
+// buffers: segment s is one pair's ny x nx column-major grid at buffer cell
+// `off`. A launch enumerates the cells of its segments (all of them, or the
+// pairs still iterating) as 0..total-1, segment s from `start`. Kernels over
+// grids map an enumerated cell to (segment, local row, local column) and
+// its buffer cell off + local; neighbourhoods never cross a segment.
+struct GridSeg {
+    int start, off, ny, nx, pair;
+    int pad[3];
+};
+struct Grids {
+    const GridSeg *seg;  // device table of nseg segments, or nullptr: one grid
+    int nseg;
+    int ny, nx;          // the single grid when seg is null
+    int total;           // enumerated cells
+    int stride;          // buffer cells (the stride of the alternatives)
+};
+inline Grids one_grid(int ny, int nx) { return Grids{nullptr, 1, ny, nx, ny * nx, ny * nx}; }
+
+// Segment of enumerated cell e: the last s with seg[s].start <= e.
+__device__ __forceinline__ GridSeg grid_seg(const Grids &g, int e, int &s)
+{
+    if (!g.seg) {
+        s = 0;
+        return GridSeg{0, 0, g.ny, g.nx, 0, {0, 0, 0}};
+    }
+    int lo = 0, hi = g.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g.seg[mid].start <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    s = lo;
+    return g.seg[lo];
+}
+
+// Enumerated cell e: its segment, local row and column, and buffer cell.
+struct Cell {
+    GridSeg sg;
+    int si, r, c, b;
+};
+__device__ __forceinline__ Cell cell_of(const Grids &g, int e)
+{
+    Cell k;
+    k.sg = grid_seg(g, e, k.si);
+    const int el = e - k.sg.start;
+    k.r = el % k.sg.ny;
+    k.c = el / k.sg.ny;
+    k.b = k.sg.off + el;
+    return k;
+}
+
+// Deformation of one pair of a batch (deform_images, src/pipeline.jl:936-965):
+// its prefiltered coefficients, the predictor grid on uniform knots, and
+// where its warped pixels go in the (column-major, leading dimension ld)
+// warp atlas.
+struct DeformPair {
+    const float *cA, *cB;  // (nr+2) x (nc+2) each
+    const float *pu, *pv;  // gny x gnx
+    int nr, nc, gny, gnx;
+    float y0, ysp, x0, xsp;
+    long long out_off;     // atlas offset of pixel (1, 1)
+    int blk0;              // first block of the pair in a batched launch
+    int pad;
+};
+struct DeformTab {
+    const DeformPair *pairs;  // device table, or nullptr: `one`
+    int npairs;
+    int nblocks;
+    DeformPair one;
+};
+
+// img: nr x nc with leading dimension ld.
+void launch_prefilter(const float *img, int ld, int nr, int nc, float *coef, float *tmp, hipStream_t s);
 void launch_deform(const float *cA, const float *cB, int nr, int nc, const float *pu, const float *pv, int gny,
                    int gnx, float y0, float ysp, float x0, float xsp, float *wA, float *wB, hipStream_t s);
+// All pairs of a batch in one launch (ld: leading dimension of the atlas).
+void launch_deform_batch(const DeformTab &tab, int ld, float *wA, float *wB, hipStream_t s);
 void launch_node_values(const float *pu, const float *pv, int gny, int gnx, float y0, float ysp, float x0, float xsp,
                         const float *y, int ny, const float *x, int nx, float *u, float *v, hipStream_t s);
+// predictor_node_values of every segment of g (src/pipeline.jl:561-569):
+// the window centres of a pass are y = r0 + i rstep + half, x = c0 + j cstep
+// + half (0-based i, j); segment s reads the predictor of pairs[seg.pair].
+void launch_node_values_batch(const Grids &g, const DeformPair *pairs, int r0, int rstep, int c0, int cstep,
+                              float half, float *u, float *v, hipStream_t s);
 
 struct ValidateArgs {
-    int ny, nx;
+    Grids grids;
     const hh_pass *pass;     // host copy (scalars are passed by value)
     int force_replace;
     float *u, *v;            // in/out
     const float *pr, *cm;
-    const float *alt_u, *alt_v;  // nullable, ny*nx*(n_peaks-1)
+    const float *alt_u, *alt_v;  // nullable, grids.stride*(n_peaks-1)
     uint8_t *outliers;       // out
-    // scratch (device): ny*nx each
+    // scratch (device): grids.stride each
     float *su, *sv;
     uint8_t *sflag;
-    int *count;  // one device int
+    int *count;  // grids.nseg device ints
 };
 void launch_validate(const ValidateArgs &a, hipStream_t s);
-void launch_smooth(const float *f, int ny, int nx, float *out, hipStream_t s);
+void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s);
 
 // stats (nullable): 2 x 44 FP64 per job, overwritten, or added to when
 // accumulate (pooled statistics of the ensemble).

@@ -206,7 +206,13 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 #pragma unroll
     for (int s = 0; s < 8; ++s) apod8[s] = cfg.apod[row + 16 * (8 * h + s)];
 
-    const int gw = blockIdx.x * 4 + wave;
+    // XCD-aware order: block b runs on XCD b % 8; each XCD takes a contiguous
+    // range of window chunks, so the 75 %-overlapping windows of neighbouring
+    // chunks are fetched into one L2 instead of several
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int blk = xcd * q8 + min(xcd, r8) + (b >> 3);
+    const int gw = blk * 4 + wave;
     const int w0 = gw * chunk;
     const int w1 = min(njobs, w0 + chunk);
     if (w0 >= w1) return;
@@ -370,6 +376,18 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 
 }  // namespace
 
+int device_cu_count()
+{
+    // the first call happens while planning a schedule, outside any stream
+    // capture
+    static const int n = [] {
+        int dev = 0, v = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev);
+        return v > 0 ? v : 256;
+    }();
+    return n;
+}
+
 bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s)
 {
@@ -387,8 +405,7 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
         }
         return t;
     }();
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int ncu = device_cu_count();
     const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 12));  // 3 waves per SIMD
     const int nblocks = (waves + 3) / 4;
     const int chunk = (njobs + nblocks * 4 - 1) / (nblocks * 4);

@@ -20,6 +20,8 @@
 
 #include "hh_internal.h"
 
+#include <mutex>
+
 #include <cfloat>
 #include <climits>
 
@@ -522,7 +524,9 @@ __global__ __launch_bounds__(kThreads) void k_plane_analyze(CorrConfig cfg, cons
 static float2 *twiddle_table(int M, hipStream_t s)
 {
     // e^{-2 pi i k / M}, k < M, built once per size in double precision
+    static std::mutex mu;
     static float2 *tabs[kMaxM + 1] = {nullptr};
+    std::lock_guard<std::mutex> lock(mu);
     if (!tabs[M]) {
         std::vector<float2> h(M);
         for (int k = 0; k < M; ++k) {
@@ -534,6 +538,12 @@ static float2 *twiddle_table(int M, hipStream_t s)
         HH_HIP(hipStreamSynchronize(s));
     }
     return tabs[M];
+}
+
+void correlate_prepare(const CorrConfig &cfg, hipStream_t s)
+{
+    (void)device_cu_count();
+    if (cfg.M == 16 || cfg.M == 32 || cfg.M == 64 || cfg.M == 128) (void)twiddle_table(cfg.M, s);
 }
 
 bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,

@@ -71,13 +71,13 @@ __device__ __forceinline__ float ext_value(const Get &get, int j, int n)
 // Pass 1: along dim 1. out is (nr+2) x nc, logical row k at out[k + j*(nr+2)].
 // Block (x, j): logical rows [256 x, 256 x + 256) of column j; the extended
 // segment with its 16-sample halos is staged once in LDS.
-__global__ __launch_bounds__(256) void k_prefilter_rows(const float *__restrict__ img, int nr, int nc,
+__global__ __launch_bounds__(256) void k_prefilter_rows(const float *__restrict__ img, int ld, int nr, int nc,
                                                          float *__restrict__ out, Taps taps)
 {
     __shared__ float seg[256 + 2 * kTaps];
     const int N1 = nr + 2;
     const int j = blockIdx.y, k0 = blockIdx.x * 256;
-    const float *col = img + (size_t)j * nr;
+    const float *col = img + (size_t)j * ld;
     auto get = [&](int q) { return col[q - 1]; };
     for (int t = threadIdx.x; t < 256 + 2 * kTaps; t += 256) seg[t] = ext_value(get, k0 - kTaps + t, nr);
     __syncthreads();
@@ -116,23 +116,39 @@ __global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict_
     }
 }
 
-__global__ __launch_bounds__(256) void k_deform(const float *__restrict__ cA, const float *__restrict__ cB, int nr,
-                                                int nc, const float *__restrict__ pu, const float *__restrict__ pv,
-                                                int gny, int gnx, float y0, float ysp, float x0, float xsp,
-                                                float *__restrict__ wA, float *__restrict__ wB)
+// One warped pixel of each image per thread. Blocks [blk0, blk0 + n) of a
+// batched launch belong to one pair (found by bisection of the pair table);
+// within a pair, consecutive threads take consecutive pixels of a column.
+__global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__restrict__ wA,
+                                                float *__restrict__ wB)
 {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int q = 0;
+    if (tab.pairs) {
+        int lo = 0, hi = tab.npairs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tab.pairs[mid].blk0 <= (int)blockIdx.x)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        q = lo;
+    }
+    const DeformPair &P = tab.pairs ? tab.pairs[q] : tab.one;
+    const int nr = P.nr, nc = P.nc;
+    const size_t e = (size_t)(blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
     if (e >= (size_t)nr * nc) return;
     const int r = (int)(e % nr) + 1, c = (int)(e / nr) + 1;
     const float R = (float)r, Cc = (float)c;
     int iy, ix;
     float ly, lx;
-    lin_axis(y0, ysp, gny, R, iy, ly);
-    lin_axis(x0, xsp, gnx, Cc, ix, lx);
-    const float du2 = bilinear_at(pu, gny, gnx, iy, ix, ly, lx) / 2.f;
-    const float dv2 = bilinear_at(pv, gny, gnx, iy, ix, ly, lx) / 2.f;
-    wA[e] = cubic(cA, nr, nc, R - dv2, Cc - du2);
-    wB[e] = cubic(cB, nr, nc, R + dv2, Cc + du2);
+    lin_axis(P.y0, P.ysp, P.gny, R, iy, ly);
+    lin_axis(P.x0, P.xsp, P.gnx, Cc, ix, lx);
+    const float du2 = bilinear_at(P.pu, P.gny, P.gnx, iy, ix, ly, lx) / 2.f;
+    const float dv2 = bilinear_at(P.pv, P.gny, P.gnx, iy, ix, ly, lx) / 2.f;
+    const size_t o = (size_t)P.out_off + (size_t)(r - 1) + (size_t)(c - 1) * ld;
+    wA[o] = cubic(P.cA, nr, nc, R - dv2, Cc - du2);
+    wB[o] = cubic(P.cB, nr, nc, R + dv2, Cc + du2);
 }
 
 __global__ __launch_bounds__(256) void k_node_values(const float *__restrict__ pu, const float *__restrict__ pv,
@@ -148,14 +164,29 @@ __global__ __launch_bounds__(256) void k_node_values(const float *__restrict__ p
     v[e] = bilinear(pv, gny, gnx, y0, ysp, x0, xsp, y[i], x[j]);
 }
 
+// predictor_node_values for the cells of several grids: segment s holds
+// the window centres of pair seg.pair.
+__global__ __launch_bounds__(256) void k_node_values_batch(Grids g, const DeformPair *__restrict__ pairs, int r0,
+                                                           int rstep, int c0, int cstep, float half,
+                                                           float *__restrict__ u, float *__restrict__ v)
+{
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const Cell k = cell_of(g, ee);
+    const int e = k.b, i = k.r, j = k.c;
+    const DeformPair &P = pairs[k.sg.pair];
+    const float y = (float)(r0 + i * rstep) + half, x = (float)(c0 + j * cstep) + half;
+    u[e] = bilinear(P.pu, P.gny, P.gnx, P.y0, P.ysp, P.x0, P.xsp, y, x);
+    v[e] = bilinear(P.pv, P.gny, P.gnx, P.y0, P.ysp, P.x0, P.xsp, y, x);
+}
+
 }  // namespace
 
-void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp, const float *, int,
-                      hipStream_t s)
+void launch_prefilter(const float *img, int ld, int nr, int nc, float *coef, float *tmp, hipStream_t s)
 {
     static const Taps taps = make_taps();
     hipLaunchKernelGGL(k_prefilter_rows, dim3((unsigned)((nr + 2 + 255) / 256), (unsigned)nc), dim3(256), 0, s, img,
-                       nr, nc, tmp, taps);
+                       ld, nr, nc, tmp, taps);
     hipLaunchKernelGGL(k_prefilter_cols, dim3((unsigned)((nr + 2 + 63) / 64), (unsigned)((nc + 2 + 63) / 64)), dim3(256),
                        0, s, tmp, nr, nc, coef, taps);
     HH_HIP(hipGetLastError());
@@ -164,9 +195,27 @@ void launch_prefilter(const float *img, int nr, int nc, float *coef, float *tmp,
 void launch_deform(const float *cA, const float *cB, int nr, int nc, const float *pu, const float *pv, int gny,
                    int gnx, float y0, float ysp, float x0, float xsp, float *wA, float *wB, hipStream_t s)
 {
-    const size_t n = (size_t)nr * nc;
-    hipLaunchKernelGGL(k_deform, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cA, cB, nr, nc, pu, pv, gny,
-                       gnx, y0, ysp, x0, xsp, wA, wB);
+    DeformTab t{};
+    t.pairs = nullptr;
+    t.npairs = 1;
+    t.nblocks = (int)(((size_t)nr * nc + 255) / 256);
+    t.one = DeformPair{cA, cB, pu, pv, nr, nc, gny, gnx, y0, ysp, x0, xsp, 0, 0, 0};
+    launch_deform_batch(t, nr, wA, wB, s);
+}
+
+void launch_deform_batch(const DeformTab &tab, int ld, float *wA, float *wB, hipStream_t s)
+{
+    if (tab.nblocks <= 0) return;
+    hipLaunchKernelGGL(k_deform, dim3((unsigned)tab.nblocks), dim3(256), 0, s, tab, ld, wA, wB);
+    HH_HIP(hipGetLastError());
+}
+
+void launch_node_values_batch(const Grids &g, const DeformPair *pairs, int r0, int rstep, int c0, int cstep,
+                              float half, float *u, float *v, hipStream_t s)
+{
+    if (g.total <= 0) return;
+    hipLaunchKernelGGL(k_node_values_batch, dim3((g.total + 255) / 256), dim3(256), 0, s, g, pairs, r0, rstep, c0,
+                       cstep, half, u, v);
     HH_HIP(hipGetLastError());
 }
 

@@ -22,6 +22,7 @@
 
 #include "hh_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace hh {
@@ -62,16 +63,21 @@ __device__ __forceinline__ T median_sorted(const T (&x)[N], int n, bool hasnan)
 }
 
 template <int NB>
-__global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const float *__restrict__ v, int ny, int nx,
+__global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const float *__restrict__ v, Grids g,
                                              double thr, double eps, const uint8_t *__restrict__ exclude,
                                              uint8_t *__restrict__ flags)
 {
     constexpr int W = 2 * NB + 1;
     constexpr int N = W * W - 1;  // neighbours; the box centre is linear index N/2
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ny * nx) return;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const Cell k = cell_of(g, ee);
+    const int e = k.b, sgoff = k.sg.off, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
     if (exclude && exclude[e]) return;
-    const int r = e % ny, c = e / ny;
+    const GridSeg sg = k.sg;
+    const float *__restrict__ ug = u + sg.off;
+    const float *__restrict__ vg = v + sg.off;
+    const uint8_t *__restrict__ xg = exclude ? exclude + sgoff : nullptr;
     float nu[N], nv[N], su[N], sv[N];
     bool ok[N];
     int n = 0;
@@ -81,9 +87,9 @@ __global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const 
         const int lin = q < N / 2 ? q : q + 1;
         const int dc = lin / W - NB, dr = lin % W - NB;
         const int r2 = r + dr, c2 = c + dc;
-        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !(exclude && exclude[r2 + c2 * ny]);
-        nu[q] = ok[q] ? u[r2 + c2 * ny] : INFINITY;
-        nv[q] = ok[q] ? v[r2 + c2 * ny] : INFINITY;
+        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !(xg && xg[r2 + c2 * ny]);
+        nu[q] = ok[q] ? ug[r2 + c2 * ny] : INFINITY;
+        nv[q] = ok[q] ? vg[r2 + c2 * ny] : INFINITY;
         nanu |= ok[q] && isnan(nu[q]);
         nanv |= ok[q] && isnan(nv[q]);
         n += ok[q];
@@ -116,11 +122,12 @@ __global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const 
 // Local validators (src/quality.jl:479-496): OR into flags.
 __global__ __launch_bounds__(256) void k_local_validators(const float *__restrict__ u, const float *__restrict__ v,
                                                           const float *__restrict__ pr, const float *__restrict__ cm,
-                                                          int n, int kind, double a, double b,
+                                                          Grids g, int kind, double a, double b,
                                                           uint8_t *__restrict__ flags)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const int e = cell_of(g, ee).b;
     bool bad = false;
     if (kind == HH_V_PEAK_RATIO)
         bad = !((double)pr[e] >= a);
@@ -174,20 +181,25 @@ __device__ double median_box(const Get &get, int r0, int r1, int c0, int c1, int
 // by a network, so the Float64 median and MAD cost no memory round trips.
 template <int NB>
 __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ uref, const float *__restrict__ vref,
-                                                    const uint8_t *__restrict__ flags_in, int ny, int nx, double thr,
+                                                    const uint8_t *__restrict__ flags_in, Grids g, double thr,
                                                     const float *__restrict__ alt_u, const float *__restrict__ alt_v,
                                                     int nalt, float *__restrict__ u_out, float *__restrict__ v_out,
                                                     uint8_t *__restrict__ flags_out)
 {
     constexpr int W = 2 * NB + 1;
     constexpr int N = W * W - 1;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ny * nx) return;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const Cell k = cell_of(g, ee);
+    const int e = k.b, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
+    const GridSeg sg = k.sg;
     u_out[e] = uref[e];
     v_out[e] = vref[e];
     flags_out[e] = flags_in[e];
     if (!flags_in[e]) return;
-    const int r = e % ny, c = e / ny;
+    const float *__restrict__ ug = uref + sg.off;
+    const float *__restrict__ vg = vref + sg.off;
+    const uint8_t *__restrict__ fg = flags_in + sg.off;
     double nu[N], nv[N], su[N], sv[N];
     bool ok[N];
     int n = 0;
@@ -197,9 +209,9 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
         const int lin = q < N / 2 ? q : q + 1;
         const int dc = lin / W - NB, dr = lin % W - NB;
         const int r2 = r + dr, c2 = c + dc;
-        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !flags_in[r2 + c2 * ny];
-        nu[q] = ok[q] ? (double)uref[r2 + c2 * ny] : INFINITY;
-        nv[q] = ok[q] ? (double)vref[r2 + c2 * ny] : INFINITY;
+        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !fg[r2 + c2 * ny];
+        nu[q] = ok[q] ? (double)ug[r2 + c2 * ny] : INFINITY;
+        nv[q] = ok[q] ? (double)vg[r2 + c2 * ny] : INFINITY;
         nanu |= ok[q] && isnan(nu[q]);
         nanv |= ok[q] && isnan(nv[q]);
         n += ok[q];
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
     sort_net<double, N>(sv);
     const double madu = median_sorted<double, N>(su, n, nanru);
     const double madv = median_sorted<double, N>(sv, n, nanrv);
-    const size_t ng = (size_t)ny * nx;
+    const size_t ng = (size_t)g.stride;  // alternatives: stride x nalt
     for (int q = 0; q < nalt; ++q) {
         const float au = alt_u[e + q * ng], av = alt_v[e + q * ng];
         if (!(isfinite(au) && isfinite(av))) break;
@@ -236,32 +248,45 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
     }
 }
 
-// Number of valid cells (atomic per block): with fewer than 3 in the whole
-// grid no vector can be replaced and k_replace returns at once.
-__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, int n, int *__restrict__ count)
+// Number of valid cells per segment (one atomic per wave when the wave lies
+// in one segment): with fewer than 3 in a grid no vector of that grid can be
+// replaced and k_replace returns at once.
+__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, Grids g,
+                                                     int *__restrict__ count)
 {
-    __shared__ int part[4];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    int x = e < n && !invalid[e] ? 1 : 0;
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    const Cell k = cell_of(g, ee < g.total ? ee : g.total - 1);
+    const int si = k.si;
+    int x = ee < g.total && !invalid[k.b] ? 1 : 0;
+    const int s0 = __shfl(si, 0);
+    if (__all(si == s0)) {
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(count + s0, x);
+    } else if (x) {
+        atomicAdd(count + si, 1);
+    }
 }
 
 // flags_out (nullable) receives a copy of invalid.
-__global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref, const float *__restrict__ vref,
-                                                 const uint8_t *__restrict__ invalid, int ny, int nx,
+__global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_all, const float *__restrict__ vref_all,
+                                                 const uint8_t *__restrict__ invalid_all, Grids g,
                                                  const int *__restrict__ nvalid, float *__restrict__ u_out,
                                                  float *__restrict__ v_out, uint8_t *__restrict__ flags_out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ny * nx) return;
-    u_out[e] = uref[e];
-    v_out[e] = vref[e];
-    if (flags_out) flags_out[e] = invalid[e];
-    if (!invalid[e] || *nvalid < 3) return;
-    const int r = e % ny, c = e / ny;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const Cell k = cell_of(g, ee);
+    const int e = k.b;
+    const GridSeg sg = k.sg;
+    u_out[e] = uref_all[e];
+    v_out[e] = vref_all[e];
+    if (flags_out) flags_out[e] = invalid_all[e];
+    if (!invalid_all[e]) return;
+    if (nvalid[k.si] < 3) return;
+    const int ny = sg.ny, nx = sg.nx, r = k.r, c = k.c;
+    const float *__restrict__ uref = uref_all + sg.off;
+    const float *__restrict__ vref = vref_all + sg.off;
+    const uint8_t *__restrict__ invalid = invalid_all + sg.off;
     {
         // radius 1 (the common case) from registers
         double nu[9], nv[9];
@@ -308,11 +333,13 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref,
     }
 }
 
-__global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f, int ny, int nx, float *__restrict__ out)
+__global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f_all, Grids g, float *__restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ny * nx) return;
-    const int r = e % ny, c = e / ny;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const Cell k = cell_of(g, ee);
+    const int e = k.b, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
+    const float *__restrict__ f = f_all + k.sg.off;
     const int cl = c > 0 ? c - 1 : 0, cr = c < nx - 1 ? c + 1 : nx - 1;
     const int rl = r > 0 ? r - 1 : 0, rr = r < ny - 1 ? r + 1 : ny - 1;
     const float q = 0.25f;
@@ -328,21 +355,23 @@ __global__ void k_fill_u8(uint8_t *p, int n, uint8_t v)
     if (e < n) p[e] = v;
 }
 
-// Clear the flags and the valid-cell counter of one validation.
-__global__ void k_val_init(uint8_t *flags, int n, int *count)
+// Clear the flags and the valid-cell counters (one per segment) of one
+// validation.
+__global__ void k_val_init(uint8_t *flags, Grids g, int *count)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) flags[e] = 0;
-    if (e == 0) *count = 0;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee < g.total) flags[cell_of(g, ee).b] = 0;
+    if (ee < g.nseg) count[ee] = 0;
 }
 
 // Move a result computed into scratch back to the chain buffers.
 __global__ void k_copy_back(const float *__restrict__ su, const float *__restrict__ sv,
                             const uint8_t *__restrict__ sf, float *__restrict__ u, float *__restrict__ v,
-                            uint8_t *__restrict__ f, int n)
+                            uint8_t *__restrict__ f, Grids g)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const int e = cell_of(g, ee).b;
     u[e] = su[e];
     v[e] = sv[e];
     if (sf) f[e] = sf[e];
@@ -351,8 +380,8 @@ __global__ void k_copy_back(const float *__restrict__ su, const float *__restric
 template <int NB>
 void uod_launch(const ValidateArgs &a, double thr, double eps, hipStream_t s)
 {
-    const int n = a.ny * a.nx;
-    hipLaunchKernelGGL(k_uod<NB>, dim3((n + 255) / 256), dim3(256), 0, s, a.u, a.v, a.ny, a.nx, thr, eps,
+    const int n = a.grids.total;
+    hipLaunchKernelGGL(k_uod<NB>, dim3((n + 255) / 256), dim3(256), 0, s, a.u, a.v, a.grids, thr, eps,
                        (const uint8_t *)nullptr, a.outliers);
 }
 
@@ -371,20 +400,21 @@ void uod(const ValidateArgs &a, int nb, double thr, double eps, hipStream_t s)
 void launch_validate(const ValidateArgs &a, hipStream_t s)
 {
     const hh_pass &p = *a.pass;
-    const int n = a.ny * a.nx;
+    const int n = a.grids.total;
     if (n <= 0) return;
+    const int ntile = std::max(n, a.grids.nseg);
     const dim3 g((n + 255) / 256), b(256);
-    hipLaunchKernelGGL(k_val_init, g, b, 0, s, a.outliers, n, a.count);
+    hipLaunchKernelGGL(k_val_init, dim3((ntile + 255) / 256), b, 0, s, a.outliers, a.grids, a.count);
     if (p.uod_enable) uod(a, p.uod_neighborhood, p.uod_threshold, 0.1, s);
     if (p.min_peak_ratio > 1)
-        hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, n, (int)HH_V_PEAK_RATIO,
+        hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, a.grids, (int)HH_V_PEAK_RATIO,
                            p.min_peak_ratio, 0.0, a.outliers);
     for (int q = 0; q < p.n_validators; ++q) {
         const hh_validator &vd = p.validators[q];
         if (vd.kind == HH_V_UOD)
             uod(a, vd.neighborhood, vd.a, vd.b, s);
         else
-            hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, n, (int)vd.kind, vd.a, vd.b,
+            hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, a.grids, (int)vd.kind, vd.a, vd.b,
                                a.outliers);
     }
     // no mask on the device path: outliers &= !mask is the identity.
@@ -395,40 +425,40 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
     if (sub) {
         switch (p.uod_neighborhood) {
         case 1:
-            hipLaunchKernelGGL(k_substitute<1>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
+            hipLaunchKernelGGL(k_substitute<1>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
                                a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
             break;
         case 2:
-            hipLaunchKernelGGL(k_substitute<2>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
+            hipLaunchKernelGGL(k_substitute<2>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
                                a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
             break;
         default:
-            hipLaunchKernelGGL(k_substitute<3>, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, p.uod_threshold,
+            hipLaunchKernelGGL(k_substitute<3>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
                                a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
             break;
         }
     }
     if (rep) {
         const uint8_t *flags = sub ? a.sflag : a.outliers;
-        hipLaunchKernelGGL(k_count_valid, g, b, 0, s, flags, n, a.count);
+        hipLaunchKernelGGL(k_count_valid, g, b, 0, s, flags, a.grids, a.count);
         if (sub)  // substituted field in scratch: replace back into the chain
-            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.ny, a.nx, (const int *)a.count, a.u,
+            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.grids, (const int *)a.count, a.u,
                                a.v, a.outliers);
         else
-            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.ny, a.nx, (const int *)a.count, a.su,
+            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.grids, (const int *)a.count, a.su,
                                a.sv, (uint8_t *)nullptr);
     }
     if (sub != rep)  // exactly one stage ran: its result is in scratch
         hipLaunchKernelGGL(k_copy_back, g, b, 0, s, a.su, a.sv, sub ? a.sflag : (const uint8_t *)nullptr, a.u, a.v,
-                           a.outliers, n);
+                           a.outliers, a.grids);
     HH_HIP(hipGetLastError());
 }
 
-void launch_smooth(const float *f, int ny, int nx, float *out, hipStream_t s)
+void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s)
 {
-    const int n = ny * nx;
+    const int n = g.total;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_smooth, dim3((n + 255) / 256), dim3(256), 0, s, f, ny, nx, out);
+    hipLaunchKernelGGL(k_smooth, dim3((n + 255) / 256), dim3(256), 0, s, f, g, out);
     HH_HIP(hipGetLastError());
 }
 

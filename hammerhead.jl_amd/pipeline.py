@@ -10,7 +10,7 @@ runs on the GPU through ``hh_run_piv``. There is no CPU fallback: backend
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -181,3 +181,76 @@ def run_piv_ensemble(pairs: Sequence, params, *, effort=None, backend: str = "mi
             ws.close()
     return PIVResult(r["x"], r["y"], r["u"], r["v"], r["peak_ratio"], r["correlation_moment"], r["uncertainty_u"],
                      r["uncertainty_v"], r["outliers"], r["mask"], passes[-1], None, kwargs.get("scale"))
+
+
+def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi355x", preprocess=None,
+                     output=None, progress=False, on_result=None, image_type=np.float32, mask=None, scale=None,
+                     predictor_smoothing: bool = True, batch: Optional[int] = None,
+                     workspace: Optional[PIVWorkspace] = None, form: int = _lib.HH_FORM_FFT32,
+                     **kwargs) -> List[PIVResult]:
+    """run_piv_sequence(pairs, params; backend) -> Vector{PIVResult}
+    (src/io.jl:229-247, driver src/io.jl:307-358).
+
+    ``pairs`` holds (imgA, imgB) matrices of any sizes (pairs may differ).
+    Instead of one run_piv per pair, up to ``batch`` pairs (default: all) are
+    packed into one device-resident atlas and run together: every sweep of
+    every pass is one launch per stage over all pairs of the batch, and with
+    fixed sweep counts the schedule replays as a hipGraph. Each pair's result
+    equals run_piv on that pair alone. ``on_result(i, result)`` (1-based i)
+    and a callable ``progress(i, n)`` run in pair order after each batch.
+    Frames given as file paths and ``output`` (JLD2 persistence) belong to the
+    reference's I/O layer and are rejected."""
+    if effort is not None:
+        raise ArgumentError("effort cannot be combined with explicit PIVParameters or pass schedules")
+    unknown = set(kwargs) - {"threaded", "uncertainty_backend", "mask_threshold"}
+    if unknown:
+        raise ArgumentError(f"unsupported run_piv_sequence keyword(s): {', '.join(sorted(unknown))}")
+    if output is not None:
+        raise ArgumentError("backend :mi355x returns results in memory; output files are not supported")
+    be = _resolve_backend(backend)
+    passes = [params] if isinstance(params, PIVParameters) else list(params)
+    if not passes:
+        raise ArgumentError("at least one pass is required")
+    pairs = list(pairs)
+    if not pairs:
+        raise ArgumentError("pairs must not be empty")
+    if mask is not None and np.asarray(mask, dtype=bool).any():
+        raise ArgumentError("backend :mi355x does not support masks yet")
+    frames = []
+    for A, B in pairs:
+        if isinstance(A, (str, bytes)) or isinstance(B, (str, bytes)):
+            raise ArgumentError("backend :mi355x takes in-memory image matrices; load frames first")
+        A, B = np.asarray(A), np.asarray(B)
+        if preprocess is not None:
+            A, B = preprocess(A), preprocess(B)
+        if A.shape != B.shape:
+            raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
+        for p in passes:
+            pass_grid_counts(p, *A.shape)
+        frames.append((A, B))
+    n = len(frames)
+    step = n if batch is None else max(1, int(batch))
+    ws = workspace or PIVWorkspace(be)
+    results: List[PIVResult] = []
+    try:
+        ctx = ws.ctx
+        ctx.check_passes(passes)
+        if form != _lib.HH_FORM_FFT32:
+            ctx.set_form(form)
+        for q0 in range(0, n, step):
+            chunk = frames[q0:q0 + step]
+            ctx.set_pairs(chunk)
+            for q, r in enumerate(ctx.run_piv_batch(passes, predictor_smoothing)):
+                res = PIVResult(r["x"], r["y"], r["u"], r["v"], r["peak_ratio"], r["correlation_moment"],
+                                r["uncertainty_u"], r["uncertainty_v"], r["outliers"], r["mask"], passes[-1], None,
+                                scale)
+                results.append(res)
+                i = q0 + q + 1
+                if on_result is not None:
+                    on_result(i, res)
+                if callable(progress):
+                    progress(i, n)
+    finally:
+        if workspace is None:
+            ws.close()
+    return results

@@ -146,6 +146,32 @@ def config_sine_shear_1024(seed: int = 105, n: int = 1024) -> SyntheticPair:
     return particle_pair(n, n, sine_shear_flow(3.0, 256.0), seed, ppp=0.03, noise=0.02)
 
 
+BATCH_SIZES = ((512, 512), (768, 1024), (1024, 1280), (2048, 2048))  # (rows, cols)
+
+
+def config_batch(npairs: int = 64, seed0: int = 104000, sizes=BATCH_SIZES, ppp: float = 0.03):
+    """The batching workload: npairs Float32 pairs, pair i of size
+    sizes[i % len(sizes)] and seed seed0 + i, alternating a Lamb-Oseen vortex
+    (peak 8 px, core radius n/10) and a linear shear reaching 8 px at the
+    image edges, plus a per-pair uniform shift U[-1, 1] px (all <= 10 px)."""
+    out = []
+    for i in range(npairs):
+        nr, nc = sizes[i % len(sizes)]
+        rng = np.random.default_rng(seed0 + i + 7919)
+        u0, v0 = rng.uniform(-1.0, 1.0, 2)
+        if i % 2 == 0:
+            base = lamb_oseen_flow((nc + 1) / 2.0, (nr + 1) / 2.0, min(nr, nc) / 10.0, 8.0)
+        else:
+            base = linear_shear_flow(16.0 / nr, (nr + 1) / 2.0)
+
+        def flow(x, y, base=base, u0=u0, v0=v0):
+            u, v = base(x, y)
+            return u + u0, v + v0
+
+        out.append(particle_pair(nr, nc, flow, seed0 + i, ppp=ppp, noise=0.02))
+    return out
+
+
 def midpoint_truth(flow, x, y, iters: int = 8):
     """Displacement whose midpoint lies at (x, y): the truth a symmetric
     deformation measures (bench/run_benchmarks.jl:199-207)."""

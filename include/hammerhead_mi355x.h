@@ -218,6 +218,34 @@ int hh_run_piv(hh_ctx *ctx, const hh_pass *passes, int npasses,
  * hh_run_piv_ensemble to host. */
 int hh_fetch_result(hh_ctx *ctx, hh_field *result);
 
+/* ---- batches of pairs of uneven sizes ----------------------------------- */
+
+/* The frames of run_piv_sequence(pairs, params) (src/io.jl:229-247): npairs
+ * host column-major pairs, pair q of nr[q] x nc[q], copied into one
+ * device-resident image atlas. Replaces the context's current pair. */
+int hh_set_pairs(hh_ctx *ctx, const float *const *A, const float *const *B, const int32_t *nr,
+                 const int32_t *nc, int npairs);
+/* run_piv(A[q], B[q], passes; predictor_smoothing) for every pair of the
+ * batch (the per-pair loop of run_piv_sequence, src/io.jl:229-247, with the
+ * body of src/pipeline.jl:361-449): each sweep runs one launch per stage over
+ * all pairs. Results stay on the device; results may be NULL or an array of
+ * npairs fields, filled as hh_fetch_pair does. Each pair keeps its own
+ * convergence (field_change, src/pipeline.jl:842-866). */
+int hh_run_piv_batch(hh_ctx *ctx, const hh_pass *passes, int npasses, int predictor_smoothing,
+                     hh_field *results);
+/* The PIVResult of pair q of the last hh_run_piv_batch (or q = 0 after
+ * hh_run_piv). */
+int hh_fetch_pair(hh_ctx *ctx, int q, hh_field *result);
+/* Grid size of pair q's final pass for the last planned schedule. */
+int hh_pair_grid(hh_ctx *ctx, int q, int *ny, int *nx);
+
+/* Schedules whose passes all have a fixed sweep count (max_iterations = 1
+ * or convergence_tol = 0) are captured once into a hipGraph per context and
+ * replayed; 0 disables capture (every call launches eagerly). Returns the
+ * number of graph replays so far in *replays when non-NULL. */
+int hh_set_graphs(hh_ctx *ctx, int enable);
+int hh_graph_replays(hh_ctx *ctx, int64_t *replays);
+
 /* ---- ensemble (sum-of-correlation) PIV (A16) --------------------------- */
 
 /* _plane_accumulator / _uncertainty_accumulator of ensemble_pass
