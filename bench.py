@@ -84,21 +84,26 @@ def mfma_flops_per_window_m32() -> float:
 
 
 def time_form(hh, pair, passes, form, steps, warmup, log):
-    """Steady-state vector fields/s of one correlation form, with per-kernel
-    HIP-event times on the context stream."""
+    """Steady-state vector fields/s of one correlation form, two timed loops
+    of `steps` vector fields each on the same context:
+      * eager: every launch issued from the host, each kernel family
+        bracketed by HIP events on the context stream (the per-kernel times);
+      * graph: the whole schedule replayed from its captured hipGraph.
+    """
     ctx = hh.Context(0)
     ctx.set_form(form)
     ctx.set_pair(pair.A, pair.B)
-    for _ in range(warmup):
-        ctx.run_piv(passes, fetch=False)
+    for _ in range(max(warmup, 1)):
+        ctx.run_piv(passes, fetch=False)  # captures the graph on the first call
     ctx.synchronize()
     ctx.set_timing(True)
-    ctx.synchronize()
+    ctx.run_piv(passes, fetch=False)  # eager warm-up
+    ctx.set_timing(True)  # clears the warm-up times
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.run_piv(passes, fetch=False)
     ctx.synchronize()
-    elapsed = time.perf_counter() - t0
+    eager = time.perf_counter() - t0
     kernels = {}
     for fam in ("prefilter", "deform", "correlate", "correlate_uq", "validate", "uq"):
         for w in (0, 16, 32, 64):
@@ -106,10 +111,67 @@ def time_form(hh, pair, passes, form, steps, warmup, log):
             if n:
                 kernels[f"{fam}_w{w}" if w else fam] = {"avg_ms": ms / n, "launches_per_step": n / steps}
     ctx.set_timing(False)
+    r0 = ctx.graph_replays()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run_piv(passes, fetch=False)
+    ctx.synchronize()
+    graph = time.perf_counter() - t0
+    replays = ctx.graph_replays() - r0
     res = ctx.run_piv(passes, fetch=True)
+    res["tune"] = ctx.tune_report()
     ctx.close()
-    log(f"form {form}: {steps} steps in {elapsed:.3f} s")
-    return elapsed, kernels, res
+    log(f"form {form}: {steps} steps eager {eager:.3f} s, graph {graph:.3f} s ({replays} replays)")
+    return eager, graph, kernels, res, replays
+
+
+def time_end_to_end(hh, pair, passes, form, steps):
+    """Host pair in, host PIVResult out: upload, schedule and download per
+    vector field (PCIe-inclusive)."""
+    ctx = hh.Context(0)
+    ctx.set_form(form)
+    ctx.set_pair(pair.A, pair.B)
+    ctx.run_piv(passes, fetch=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.set_pair(pair.A, pair.B)
+        ctx.run_piv(passes, fetch=True)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return {"value": round(steps / dt, 4), "unit": "vector fields/s", "ms_per_step": round(dt / steps * 1e3, 4),
+            "includes": "H2D copy of the 2 x 16.8 MB pair, the schedule, D2H copy of the result grids"}
+
+
+def time_batch(hh, passes, form, npairs, steps, log):
+    """The batching workload: npairs pairs of sizes cycling 512^2, 768x1024,
+    1024x1280, 2048^2 (vortex / shear <= 10 px) in one device atlas, the
+    whole 64/32 -> 32/16 -> 16/12 x3 + UQ schedule per step over all pairs
+    (one launch per stage per sweep, replayed as a hipGraph)."""
+    pairs = hh.synthetic.config_batch(npairs=npairs)
+    ctx = hh.Context(0)
+    ctx.set_form(form)
+    ctx.set_pairs([(p.A, p.B) for p in pairs])
+    ctx.run_piv_batch(passes, fetch=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run_piv_batch(passes, fetch=False)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    res = [ctx.fetch_pair(q) for q in range(npairs)]
+    ctx.close()
+    rms = [hh.synthetic.rms_error(r, p.flow, midpoint=True) for r, p in zip(res, pairs)]
+    nwin = 0
+    for p in pairs:
+        for k, ps in enumerate(passes):
+            ny, nx = hh.grid_size(ps, *p.A.shape)
+            nwin += ny * nx * ps.max_iterations
+    mpx = sum(p.A.size for p in pairs) / 1e6
+    log(f"batch of {npairs}: {steps} steps in {dt:.3f} s")
+    return {"pairs": npairs, "sizes": "512x512, 768x1024, 1024x1280, 2048x2048 (rows x cols), cycled",
+            "pairs_per_s": round(npairs * steps / dt, 3), "ms_per_batch": round(dt / steps * 1e3, 3),
+            "megapixel_pairs_per_s": round(mpx * steps / dt, 2), "window_correlations_per_batch": nwin,
+            "rms_px_vs_midpoint_truth_median": round(float(np.median(rms)), 5)}
 
 
 def main(argv=None):
@@ -121,6 +183,9 @@ def main(argv=None):
                     help="form whose throughput is the reported value (the other is timed beside it)")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-end-to-end", action="store_true", help="skip the PCIe-inclusive rate")
+    ap.add_argument("--batch-pairs", type=int, default=64, help="pairs of the batching workload (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default=None)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -151,7 +216,9 @@ def main(argv=None):
     nr, nc = pair.A.shape
     forms = {"mfma16": hh.HH_FORM_MFMA16, "fft32": hh.HH_FORM_FFT32}
     other = "fft32" if args.form == "mfma16" else "mfma16"
-    elapsed, kernels, res = time_form(hh, pair, passes, forms[args.form], args.steps, args.warmup, log)
+    eager, graph, kernels, res, replays = time_form(hh, pair, passes, forms[args.form], args.steps, args.warmup,
+                                                    log)
+    elapsed = eager
     rms = hh.synthetic.rms_error(res, pair.flow, midpoint=True)
 
     # dominant kernel: the 16-px-window correlation sweeps (M = 32), 259081 windows
@@ -197,15 +264,24 @@ def main(argv=None):
         "accuracy": {"rms_px_vs_midpoint_truth": round(rms, 5), "outliers": int(res["outliers"].sum())},
     }
     if not args.no_compare:
-        e2, k2, r2 = time_form(hh, pair, passes, forms[other], args.steps, args.warmup, log)
+        e2, g2, k2, r2, _ = time_form(hh, pair, passes, forms[other], args.steps, args.warmup, log)
         v2 = args.steps / e2
         same = (~res["outliers"]) & (~r2["outliers"])
         dev = np.sqrt(np.mean(np.concatenate([(res["u"] - r2["u"])[same], (res["v"] - r2["v"])[same]]) ** 2))
         line["compare"] = {"form": other, "value": round(v2, 4), "ms_per_step": round(e2 / args.steps * 1e3, 4),
+                           "graph_value": round(args.steps / g2, 4),
+                           "graph_speedup_of_reported_form": round(g2 / graph, 4),
                            "speedup_of_reported_form": round((args.steps / elapsed) / v2, 4),
                            "rms_px_between_forms": float(dev),
                            "flag_diffs_between_forms": int((res["outliers"] != r2["outliers"]).sum()),
                            "correlate_w16_avg_ms": k2["correlate_w16"]["avg_ms"]}
+    line["autotune"] = res["tune"]
+    line["graph"] = {"value": round(args.steps / graph, 4), "ms_per_step": round(graph / args.steps * 1e3, 4),
+                     "replays": replays}
+    if not args.no_end_to_end:
+        line["pcie_inclusive"] = time_end_to_end(hh, pair, passes, forms[args.form], max(3, args.steps // 4))
+    if args.batch_pairs > 0:
+        line["batch"] = time_batch(hh, passes, forms[args.form], args.batch_pairs, args.batch_steps, log)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(pair, passes, args.cpu_threads)

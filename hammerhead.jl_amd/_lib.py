@@ -29,6 +29,7 @@ EXPORTS = (
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
+    "hh_set_autotune", "hh_tune_report",
 )
 
 
@@ -98,6 +99,8 @@ def load():
         "hh_pair_grid": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32)]),
         "hh_set_graphs": (i32, [vp, i32]),
         "hh_graph_replays": (i32, [vp, C.POINTER(C.c_int64)]),
+        "hh_set_autotune": (i32, [vp, i32]),
+        "hh_tune_report": (i32, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -355,6 +358,18 @@ class Context:
 
     def set_graphs(self, enable: bool):
         _check(self.h, load().hh_set_graphs(self.h, 1 if enable else 0), "set_graphs")
+
+    def set_autotune(self, enable: bool):
+        _check(self.h, load().hh_set_autotune(self.h, 1 if enable else 0), "set_autotune")
+
+    def tune_report(self) -> dict:
+        import json
+
+        need = C.c_size_t()
+        _check(self.h, load().hh_tune_report(self.h, None, 0, C.byref(need)), "tune_report")
+        buf = C.create_string_buffer(need.value)
+        _check(self.h, load().hh_tune_report(self.h, buf, need.value, None), "tune_report")
+        return json.loads(buf.value.decode())
 
     def graph_replays(self) -> int:
         n = C.c_int64()

@@ -125,24 +125,7 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
         tv[m] = 0.f;
         tl[m] = INT_MAX;
     }
-    for (int p = t; p < NB * NB; p += G) {
-        const int bi = p % NB, bj = p / NB;
-        const float I0 = Rb[p];
-        if (!(I0 > 0.f)) continue;  // zero-valued box points are never maxima
-        bool ok = true;
-#pragma unroll
-        for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll
-            for (int di = -1; di <= 1; ++di) {
-                if (di == 0 && dj == 0) continue;
-                const int ni = bi + di, nj = bj + dj;
-                const float In = (ni >= 0 && ni < NB && nj >= 0 && nj < NB) ? Rb[ni + nj * NB] : 0.f;
-                const bool earlier = dj < 0 || (dj == 0 && di < 0);
-                ok &= earlier ? !(In > I0) : !(In >= I0);
-            }
-        if (!ok) continue;
-        float cv = I0;
-        int cl = (LO + bi) + (LO + bj) * M;
+    auto insert = [&](float cv, int cl) {
 #pragma unroll
         for (int m = 0; m < K; ++m) {
             if (better(cv, cl, tv[m], tl[m])) {
@@ -153,6 +136,56 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
                 cv = sv_;
                 cl = sl_;
             }
+        }
+    };
+    if constexpr (G == 16 && NB <= 16) {
+        // the 16 lanes of a window are one DPP row: lane t walks box column
+        // t down the rows with its 3 x 3 neighbourhood in registers (left
+        // and right columns from lanes t -/+ 1, zero past the box), one LDS
+        // read per point; neighbours earlier in column order (left column,
+        // row above) disqualify when strictly greater, later ones when >=
+        const bool col = t < NB;
+        auto left = [](float v) {  // lane t gets lane t - 1 (0 at t = 0)
+            return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+        };
+        auto right = [](float v) {  // lane t gets lane t + 1 (0 at t = 15)
+            return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, false));
+        };
+        float cp = 0.f, lp = 0.f, rp = 0.f;
+        float cc = col ? Rb[t * NB] : 0.f;
+        float lc = left(cc), rc = right(cc);
+        for (int bi = 0; bi < NB; ++bi) {
+            const float cn = (col && bi + 1 < NB) ? Rb[bi + 1 + t * NB] : 0.f;
+            const float ln = left(cn), rn = right(cn);
+            const float I0 = cc;
+            const bool ok = I0 > 0.f && !(lp > I0) && !(lc > I0) && !(ln > I0) && !(cp > I0) && !(cn >= I0) &&
+                            !(rp >= I0) && !(rc >= I0) && !(rn >= I0);
+            if (ok) insert(I0, (LO + bi) + (LO + t) * M);
+            cp = cc;
+            cc = cn;
+            lp = lc;
+            lc = ln;
+            rp = rc;
+            rc = rn;
+        }
+    } else {
+        for (int p = t; p < NB * NB; p += G) {
+            const int bi = p % NB, bj = p / NB;
+            const float I0 = Rb[p];
+            if (!(I0 > 0.f)) continue;  // zero-valued box points are never maxima
+            bool ok = true;
+#pragma unroll
+            for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+                for (int di = -1; di <= 1; ++di) {
+                    if (di == 0 && dj == 0) continue;
+                    const int ni = bi + di, nj = bj + dj;
+                    const float In = (ni >= 0 && ni < NB && nj >= 0 && nj < NB) ? Rb[ni + nj * NB] : 0.f;
+                    const bool earlier = dj < 0 || (dj == 0 && di < 0);
+                    ok &= earlier ? !(In > I0) : !(In >= I0);
+                }
+            if (!ok) continue;
+            insert(I0, (LO + bi) + (LO + bj) * M);
         }
     }
     float sv[K];

@@ -862,8 +862,8 @@ int hh_destroy(hh_ctx *c)
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
     }
-    for (auto e : c->event_pool) (void)hipEventDestroy(e);
     engine_release(c);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HH_OK;
@@ -1239,10 +1239,27 @@ int hh_pair_grid(hh_ctx *c, int q, int *ny, int *nx)
 int hh_set_graphs(hh_ctx *c, int enable)
 {
     return guarded(c, [&] {
-        c->eng.use_graphs = enable != 0;
-        if (!enable && c->eng.graph) {
-            (void)hipGraphExecDestroy(c->eng.graph);
-            c->eng.graph = nullptr;
+        engine_set_graphs(c, enable != 0);
+    });
+}
+
+int hh_set_autotune(hh_ctx *c, int enable)
+{
+    return guarded(c, [&] {
+        c->tune.enabled = enable != 0;
+        if (!enable) c->tune.entries.clear();
+    });
+}
+
+int hh_tune_report(hh_ctx *c, char *buf, size_t len, size_t *needed)
+{
+    return guarded(c, [&] {
+        const std::string r = engine_tune_report(c);
+        if (needed) *needed = r.size() + 1;
+        if (buf && len) {
+            const size_t n = std::min(len - 1, r.size());
+            std::memcpy(buf, r.data(), n);
+            buf[n] = 0;
         }
     });
 }

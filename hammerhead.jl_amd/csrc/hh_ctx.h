@@ -92,13 +92,15 @@ struct PassTables {
     Grids grids{};
     DBuf<DeformPair> dprev;   // predictor: the previous pass's grids
     DBuf<DeformPair> dself;   // predictor: this pass's grids
+    DeformPair hprev{}, hself{};  // host copies when the tables hold one pair
     int blocks = 0;           // deformation blocks
     DBuf<int> count;          // one valid-cell counter per segment
 };
 
 struct PassPlan {
     hh_pass p{};
-    CorrConfig cfg{};
+    CorrConfig cfg{};       // plain sweeps
+    CorrConfig cfg_uq{};    // the sweep with fused uncertainty
     int r0 = 1, rstep = 1, c0 = 1, cstep = 1;
     float half = 0.f;                        // (w - 1) / 2
     std::vector<int> ny, nx, off;            // per pair grid and cell offset
@@ -114,11 +116,15 @@ struct HostJobs {
     std::vector<int> off;  // per pair: index of its first job
 };
 
-// Autotuned launch geometry, keyed by kernel family and problem shape.
+// Autotuned launch geometry, keyed by kernel family and problem shape: the
+// chosen candidate and the measured milliseconds of every candidate.
+struct TuneEntry {
+    int waves_per_simd = 0, linear_order = 0;
+    std::vector<std::pair<std::string, float>> ms;
+};
 struct Tuner {
     bool enabled = true;
-    std::map<std::string, int> choice;
-    std::map<std::string, std::vector<float>> timings;
+    std::map<std::string, TuneEntry> entries;
 };
 
 struct Engine {
@@ -248,6 +254,8 @@ void engine_run(hh_ctx *c, const hh_pass *passes, int np, int smoothing);
 // The final PIVResult of pair q.
 void engine_fetch(hh_ctx *c, int q, hh_field *r);
 void engine_release(hh_ctx *c);
+void engine_set_graphs(hh_ctx *c, bool enable);
+std::string engine_tune_report(hh_ctx *c);
 
 }  // namespace host
 }  // namespace hh

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 
@@ -145,10 +146,17 @@ void pack_layout(Layout &L, const int32_t *nr, const int32_t *nc, int npairs)
 
 uint64_t g_layout_gen = 0;
 
-void invalidate_plan(Engine &E)
+void drop_graph(hh_ctx *c)
 {
+    Engine &E = c->eng;
     if (E.graph) (void)hipGraphExecDestroy(E.graph);
     E.graph = nullptr;
+}
+
+void invalidate_plan(hh_ctx *c)
+{
+    Engine &E = c->eng;
+    drop_graph(c);
     E.graph_failed = false;
     E.graph_replays = 0;
     E.key.clear();
@@ -223,11 +231,19 @@ void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vecto
     upload_sync(c, T.jobs, jobs);
     upload_sync(c, T.segs, segs);
     T.grids = Grids{T.segs.p, (int)segs.size(), 0, 0, start, P.total};
+    // one grid starting at buffer cell 0: no table lookups in the kernels
+    if (segs.size() == 1 && segs[0].off == 0) T.grids = Grids{nullptr, 1, segs[0].ny, segs[0].nx, start, P.total};
     T.count.ensure(std::max<size_t>(segs.size(), 1));
     int b1 = 0, b2 = 0;
     if (c->eng.deforms) {
-        if (prev) upload_sync(c, T.dprev, deform_table(c, pairs, *prev, b1));
-        upload_sync(c, T.dself, deform_table(c, pairs, P, b2));
+        if (prev) {
+            const std::vector<DeformPair> t = deform_table(c, pairs, *prev, b1);
+            upload_sync(c, T.dprev, t);
+            T.hprev = t.size() == 1 ? t[0] : DeformPair{};
+        }
+        const std::vector<DeformPair> t = deform_table(c, pairs, P, b2);
+        upload_sync(c, T.dself, t);
+        T.hself = t.size() == 1 ? t[0] : DeformPair{};
     }
     T.blocks = std::max(b1, b2);  // every pair has the same pixel count in both tables
     HH_HIP(hipStreamSynchronize(c->stream));
@@ -265,11 +281,88 @@ void serialize_pass(std::vector<uint8_t> &k, const hh_pass &p)
     }
 }
 
+// ---- autotuner -----------------------------------------------------------
+// The launch geometry of a pass's correlation kernel: for the persistent
+// MFMA kernel the resident waves per SIMD (1-4, i.e. how many windows each
+// wave walks) and the window order (contiguous bands per XCD, so the
+// overlapping windows of neighbouring blocks share one L2, or plain launch
+// order); for the register FFT kernels the window order. Every candidate is
+// timed (best of 3, HIP events on the context stream) on the pass's own job
+// list over the pair as loaded, once per (family, configuration, window
+// count) per context. Each window's arithmetic does not depend on the
+// geometry, so the choice never changes a result.
+CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool uq)
+{
+    Engine &E = c->eng;
+    const char *fam = correlate_family(base, uq);
+    const bool mfma = std::string(fam) == "mfma32", reg = std::string(fam).rfind("reg", 0) == 0;
+    CorrConfig cfg = base;
+    if (!c->tune.enabled || !(mfma || reg) || P.all.njobs <= 0) return cfg;
+    const std::string key = std::string(fam) + (uq ? "+uq" : "") + " w" + std::to_string(base.w) + " M" +
+                            std::to_string(base.M) + " K" + std::to_string(base.K) + " m" +
+                            std::to_string(base.method) + " n" + std::to_string(P.all.njobs);
+    auto it = c->tune.entries.find(key);
+    if (it == c->tune.entries.end()) {
+        std::vector<std::pair<int, int>> cand;
+        for (int order = 0; order < 2; ++order) {
+            if (mfma)
+                for (int w = 1; w <= 4; ++w) cand.push_back({w, order});
+            else
+                cand.push_back({0, order});
+        }
+        CorrOut out{};
+        out.u = E.u.p;
+        out.v = E.v.p;
+        out.pr = E.pr.p;
+        out.cm = E.cm.p;
+        out.alt_u = base.n_peaks > 1 ? E.altu.p : nullptr;
+        out.alt_v = base.n_peaks > 1 ? E.altv.p : nullptr;
+        out.ny = P.total;
+        out.nx = 1;
+        if (uq) {
+            out.unc_u = E.uu.p;
+            out.unc_v = E.uv.p;
+        }
+        hipEvent_t a = get_event(c), b = get_event(c);
+        TuneEntry best;
+        float best_ms = 1e30f;
+        for (auto [w, order] : cand) {
+            CorrConfig t = base;
+            t.waves_per_simd = w;
+            t.linear_order = order;
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 4; ++rep) {
+                HH_HIP(hipEventRecord(a, c->stream));
+                launch_correlate(t, E.lay.A, E.lay.B, E.lay.ld, 0, P.all.jobs.p, P.all.njobs, out, c->stream);
+                HH_HIP(hipEventRecord(b, c->stream));
+                HH_HIP(hipEventSynchronize(b));
+                float ms = 0.f;
+                HH_HIP(hipEventElapsedTime(&ms, a, b));
+                if (rep > 0) ms_min = std::min(ms_min, ms);  // the first run warms caches
+            }
+            const std::string name = (mfma ? "waves_per_simd=" + std::to_string(w) + " " : std::string()) +
+                                     (order ? "order=linear" : "order=xcd_bands");
+            best.ms.push_back({name, ms_min});
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best.waves_per_simd = w;
+                best.linear_order = order;
+            }
+        }
+        c->event_pool.push_back(a);
+        c->event_pool.push_back(b);
+        it = c->tune.entries.emplace(key, best).first;
+    }
+    cfg.waves_per_simd = it->second.waves_per_simd;
+    cfg.linear_order = it->second.linear_order;
+    return cfg;
+}
+
 void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
 {
     Engine &E = c->eng;
     const Layout &L = E.lay;
-    invalidate_plan(E);
+    invalidate_plan(c);
     E.smoothing = smoothing;
     E.deforms = np > 1;
     for (int k = 0; k < np; ++k) E.deforms |= passes[k].max_iterations > 1;
@@ -282,6 +375,7 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         auto P = std::make_unique<PassPlan>();
         P->p = passes[k];
         P->cfg = make_corr(c, passes[k]);
+        P->cfg_uq = P->cfg;
         const hh_pass &p = passes[k];
         const int w = p.window_size[0];
         P->r0 = 1;
@@ -331,6 +425,13 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
     for (int k = 0; k < np; ++k)
         build_tables(c, *E.plan[k], k > 0 ? E.plan[k - 1].get() : nullptr, all, E.hjobs[k].jobs, E.hjobs[k].off,
                      E.plan[k]->all);
+    for (int k = 0; k < np; ++k) {
+        PassPlan &P = *E.plan[k];
+        const bool unc = P.p.uncertainty && k == np - 1 && P.fixed;
+        const bool plain = P.p.max_iterations > 1 || !unc;
+        if (plain) P.cfg = tune_corr(c, P, P.cfg, false);
+        P.cfg_uq = unc ? tune_corr(c, P, P.cfg_uq, true) : P.cfg;
+    }
 }
 
 void predictor(hh_ctx *c, const Grids &g)
@@ -389,9 +490,10 @@ void record(hh_ctx *c, bool dynamic)
             const DeformPair *dt = pred_self ? T->dself.p : T->dprev.p;
             if (have_pred) {
                 DeformTab tab{};
-                tab.pairs = dt;
+                tab.pairs = T->grids.nseg == 1 ? nullptr : dt;  // one pair: passed by value
                 tab.npairs = T->grids.nseg;
                 tab.nblocks = T->blocks;
+                tab.one = pred_self ? T->hself : T->hprev;
                 timed(c, "deform", 0, [&] { launch_deform_batch(tab, L.ld, wA, wB, s); });
                 srcA = wA;
                 srcB = wB;
@@ -421,7 +523,8 @@ void record(hh_ctx *c, bool dynamic)
                 out.unc_v = E.uv.p;
             }
             timed(c, out.unc_u ? "correlate_uq" : "correlate", P.cfg.w, [&] {
-                uq_fused = launch_correlate(P.cfg, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs, out, s) &&
+                uq_fused = launch_correlate(out.unc_u ? P.cfg_uq : P.cfg, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs,
+                                            out, s) &&
                            out.unc_u != nullptr;
             });
             if (keep_measured) gcopy2(c, E.u.p, E.v.p, E.mu.p, E.mv.p, T->grids);
@@ -502,7 +605,7 @@ void engine_single_layout(hh_ctx *c)
     L.A = c->A;
     L.B = c->B;
     L.gen = ++g_layout_gen;
-    invalidate_plan(c->eng);
+    invalidate_plan(c);
     c->eng.have_result = false;
 }
 
@@ -516,7 +619,7 @@ void engine_set_pairs(hh_ctx *c, const float *const *A, const float *const *B, c
         if (nr[q] < 2 || nc[q] < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
     }
     Engine &E = c->eng;
-    invalidate_plan(E);
+    invalidate_plan(c);
     E.have_result = false;
     Layout &L = E.lay;
     pack_layout(L, nr, nc, npairs);
@@ -564,6 +667,8 @@ void engine_run(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
     for (auto &P : E.plan) dynamic |= !P->fixed;
     E.have_result = false;
     c->have_result = false;
+    // per-kernel event timing (hh_set_timing) runs the sequence eagerly: event
+    // records captured into a graph cannot be read back
     if (!dynamic && !c->timing && E.use_graphs && !E.graph_failed) {
         if (!E.graph) {
             hipGraph_t g = nullptr;
@@ -627,7 +732,32 @@ void engine_fetch(hh_ctx *c, int q, hh_field *r)
     if (r->mask) std::memset(r->mask, 0, ng);
 }
 
-void engine_release(hh_ctx *c) { invalidate_plan(c->eng); }
+void engine_release(hh_ctx *c) { invalidate_plan(c); }
+
+std::string engine_tune_report(hh_ctx *c)
+{
+    std::string j = "{";
+    bool first = true;
+    for (auto &[key, e] : c->tune.entries) {
+        j += (first ? "\"" : ", \"") + key + "\": {\"choice\": {\"waves_per_simd\": " +
+             std::to_string(e.waves_per_simd) + ", \"linear_order\": " + std::to_string(e.linear_order) +
+             "}, \"ms\": {";
+        for (size_t q = 0; q < e.ms.size(); ++q) {
+            char buf[64];
+            std::snprintf(buf, sizeof buf, "%.4f", e.ms[q].second);
+            j += (q ? ", \"" : "\"") + e.ms[q].first + "\": " + buf;
+        }
+        j += "}}";
+        first = false;
+    }
+    return j + "}";
+}
+
+void engine_set_graphs(hh_ctx *c, bool enable)
+{
+    c->eng.use_graphs = enable;
+    if (!enable) drop_graph(c);
+}
 
 }  // namespace host
 }  // namespace hh

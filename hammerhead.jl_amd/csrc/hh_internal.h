@@ -51,7 +51,15 @@ struct CorrConfig {
     const float *gain;  // M*M device (fftshifted), nullptr if not padded
     const float *wfilt; // M*M device phase filter (unshifted), or nullptr
     int form;           // HH_FORM_FFT32 / HH_FORM_MFMA16
+    // launch geometry chosen by the autotuner (0: the kernel's default);
+    // results do not depend on it
+    int waves_per_simd; // persistent MFMA kernel: resident waves per SIMD
+    int linear_order;   // 1: windows in launch order instead of XCD bands
 };
+
+// Kernel family a correlation launch of cfg takes ("mfma32", "regL1",
+// "regL2", "regL4", "lds"), with or without fused uncertainty.
+const char *correlate_family(const CorrConfig &cfg, bool uq);
 
 // Outputs of a correlation launch; any pointer may be null.
 struct CorrOut {

@@ -550,7 +550,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
-    const int tile = xcd * q8 + min(xcd, r8) + slot;
+    const int tile = cfg.linear_order ? b : xcd * q8 + min(xcd, r8) + slot;
     const int win = tile * WPB + slot_w;
     const bool active = win < njobs;
     const int jb = active ? win : njobs - 1;
@@ -831,6 +831,12 @@ static int fast_halfwidth(const CorrConfig &cfg)
 }
 
 bool correlate_box_layout(const CorrConfig &cfg) { return fast_halfwidth(cfg) >= 0; }
+
+const char *fast_family(const CorrConfig &cfg)
+{
+    if (fast_halfwidth(cfg) < 0) return nullptr;
+    return cfg.M == 32 ? "regL1" : (cfg.M == 64 ? "regL2" : "regL4");
+}
 
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s, bool *uq_done)

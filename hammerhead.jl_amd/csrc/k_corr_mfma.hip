@@ -26,8 +26,8 @@
 // written for the peak search. Operands are scaled by powers of two to sit
 // in FP16 range (a per-window scale at the input, fixed ones after) and the
 // scale is removed exactly before the magnitude and gain.
-// One wave per window; persistent waves walk contiguous window ranges with
-// the constant tables resident in registers.
+// One wave per window; persistent waves walk contiguous window ranges, the
+// constant operand tables staged once per block in LDS.
 
 #include "hh_internal.h"
 #include "box_peaks.h"
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     // chunks are fetched into one L2 instead of several
     const int nb = gridDim.x, b = blockIdx.x;
     const int xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
-    const int blk = xcd * q8 + min(xcd, r8) + (b >> 3);
+    const int blk = cfg.linear_order ? b : xcd * q8 + min(xcd, r8) + (b >> 3);
     const int gw = blk * 4 + wave;
     const int w0 = gw * chunk;
     const int w1 = min(njobs, w0 + chunk);
@@ -376,6 +376,13 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 
 }  // namespace
 
+// covered: 16-px windows padded to 32 with the Gaussian support box
+bool mfma_covers(const CorrConfig &cfg)
+{
+    return cfg.M == 32 && cfg.w == 16 && cfg.padded && cfg.gain && cfg.finder == HH_REGIONALMAX && cfg.K <= 3 &&
+           cfg.box_lo == 16 - kHW && cfg.box_hi == 16 + kHW;
+}
+
 int device_cu_count()
 {
     // the first call happens while planning a schedule, outside any stream
@@ -391,10 +398,7 @@ int device_cu_count()
 bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s)
 {
-    // covered: 16-px windows padded to 32 with the Gaussian support box
-    if (cfg.M != 32 || cfg.w != 16 || !cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3)
-        return false;
-    if (cfg.box_lo != 16 - kHW || cfg.box_hi != 16 + kHW) return false;
+    if (!mfma_covers(cfg)) return false;
     if (out.planes || out.acc || out.unc_u) return false;
     if (njobs <= 0) return true;
     static Trig tg = [] {
@@ -406,7 +410,8 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
         return t;
     }();
     const int ncu = device_cu_count();
-    const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 12));  // 3 waves per SIMD
+    const int wps = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 3;
+    const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 4 * wps));
     const int nblocks = (waves + 3) / 4;
     const int chunk = (njobs + nblocks * 4 - 1) / (nblocks * 4);
     if (cfg.method == HH_PHASE) {

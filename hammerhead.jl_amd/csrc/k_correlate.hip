@@ -540,6 +540,16 @@ static float2 *twiddle_table(int M, hipStream_t s)
     return tabs[M];
 }
 
+const char *fast_family(const CorrConfig &cfg);
+bool mfma_covers(const CorrConfig &cfg);
+
+const char *correlate_family(const CorrConfig &cfg, bool uq)
+{
+    if (cfg.form == HH_FORM_MFMA16 && !uq && mfma_covers(cfg)) return "mfma32";
+    const char *f = fast_family(cfg);
+    return f ? f : "lds";
+}
+
 void correlate_prepare(const CorrConfig &cfg, hipStream_t s)
 {
     (void)device_cu_count();

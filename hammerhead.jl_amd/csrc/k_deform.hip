@@ -134,7 +134,13 @@ __global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__
         }
         q = lo;
     }
-    const DeformPair &P = tab.pairs ? tab.pairs[q] : tab.one;
+    // a copy, selected with uniform control flow: the fields stay in scalar
+    // registers (a reference to either source would force per-lane loads)
+    DeformPair P;
+    if (tab.pairs)
+        P = tab.pairs[q];
+    else
+        P = tab.one;
     const int nr = P.nr, nc = P.nc;
     const size_t e = (size_t)(blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
     if (e >= (size_t)nr * nc) return;
@@ -144,11 +150,16 @@ __global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__
     float ly, lx;
     lin_axis(P.y0, P.ysp, P.gny, R, iy, ly);
     lin_axis(P.x0, P.xsp, P.gnx, Cc, ix, lx);
+    const float *__restrict__ cA = P.cA;
+    const float *__restrict__ cB = P.cB;
     const float du2 = bilinear_at(P.pu, P.gny, P.gnx, iy, ix, ly, lx) / 2.f;
     const float dv2 = bilinear_at(P.pv, P.gny, P.gnx, iy, ix, ly, lx) / 2.f;
     const size_t o = (size_t)P.out_off + (size_t)(r - 1) + (size_t)(c - 1) * ld;
-    wA[o] = cubic(P.cA, nr, nc, R - dv2, Cc - du2);
-    wB[o] = cubic(P.cB, nr, nc, R + dv2, Cc + du2);
+    // both gathers before either store: all 32 coefficient loads in flight
+    const float va = cubic(cA, nr, nc, R - dv2, Cc - du2);
+    const float vb = cubic(cB, nr, nc, R + dv2, Cc + du2);
+    wA[o] = va;
+    wB[o] = vb;
 }
 
 __global__ __launch_bounds__(256) void k_node_values(const float *__restrict__ pu, const float *__restrict__ pv,

@@ -248,20 +248,38 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
     }
 }
 
-// Number of valid cells per segment (one atomic per wave when the wave lies
-// in one segment): with fewer than 3 in a grid no vector of that grid can be
-// replaced and k_replace returns at once.
+// Number of valid cells per segment: one atomic per block when the block
+// lies in one segment (always, for a single grid), else one per wave or per
+// lane. With fewer than 3 in a grid no vector of that grid can be replaced
+// and k_replace returns at once.
 __global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, Grids g,
                                                      int *__restrict__ count)
 {
+    __shared__ int part[4], segw[4];
     const int ee = blockIdx.x * blockDim.x + threadIdx.x;
     const Cell k = cell_of(g, ee < g.total ? ee : g.total - 1);
-    const int si = k.si;
+    const int si = k.si, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = ee < g.total && !invalid[k.b] ? 1 : 0;
     const int s0 = __shfl(si, 0);
-    if (__all(si == s0)) {
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(count + s0, x);
+    const bool uni = __all(si == s0);
+    int wsum = x;
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
+    if (lane == 0) {
+        part[wave] = wsum;
+        segw[wave] = uni ? s0 : -1;
+    }
+    __syncthreads();
+    const int nw = (int)(blockDim.x >> 6);
+    bool block_uni = true;
+    for (int w = 0; w < nw; ++w) block_uni &= segw[w] == segw[0] && segw[w] >= 0;
+    if (block_uni) {
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int w = 0; w < nw; ++w) t += part[w];
+            if (t) atomicAdd(count + segw[0], t);
+        }
+    } else if (uni) {
+        if (lane == 0 && wsum) atomicAdd(count + s0, wsum);
     } else if (x) {
         atomicAdd(count + si, 1);
     }

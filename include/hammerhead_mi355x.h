@@ -246,6 +246,17 @@ int hh_pair_grid(hh_ctx *ctx, int q, int *ny, int *nx);
 int hh_set_graphs(hh_ctx *ctx, int enable);
 int hh_graph_replays(hh_ctx *ctx, int64_t *replays);
 
+/* Autotuner of the correlation launch geometry (on by default): while a
+ * schedule is planned, each pass's correlation kernel is timed over its own
+ * job list for every candidate geometry (persistent MFMA waves per SIMD,
+ * window-to-XCD order) and the fastest is kept per context. Results do not
+ * depend on the choice. hh_tune_report writes the choices and timings as a
+ * JSON object (NUL-terminated, truncated to len; *needed = full size). No
+ * counterpart in the reference (its device batch size is a fixed keyword,
+ * src/ka_backend.jl). */
+int hh_set_autotune(hh_ctx *ctx, int enable);
+int hh_tune_report(hh_ctx *ctx, char *buf, size_t len, size_t *needed);
+
 /* ---- ensemble (sum-of-correlation) PIV (A16) --------------------------- */
 
 /* _plane_accumulator / _uncertainty_accumulator of ensemble_pass

@@ -156,3 +156,25 @@ def test_run_piv_sequence_api(hh, small_pairs):
         single = hh.run_piv(p.A, p.B, passes)
         assert np.array_equal(r.u, single.u, equal_nan=True)
         assert np.array_equal(r.outliers, single.outliers)
+
+
+@pytest.mark.parametrize("form", ["fft32", "mfma16"])
+def test_autotune_never_changes_results(hh, small_pairs, form):
+    passes = schedule(hh)
+    fm = hh.HH_FORM_MFMA16 if form == "mfma16" else hh.HH_FORM_FFT32
+    pairs = [(p.A, p.B) for p in small_pairs[:3]]
+    with hh.Context(0) as ct, hh.Context(0) as cn:
+        cn.set_autotune(False)
+        for c in (ct, cn):
+            c.set_form(fm)
+            c.set_pairs(pairs)
+        rt, rn = ct.run_piv_batch(passes), cn.run_piv_batch(passes)
+        rep = ct.tune_report()
+        say("tuned", rep)
+        assert rep and cn.tune_report() == {}
+        fams = " ".join(rep)
+        assert ("mfma32" in fams) == (form == "mfma16")
+        for key, e in rep.items():
+            assert len(e["ms"]) >= 2 and all(v > 0 for v in e["ms"].values())
+        for q in range(len(pairs)):
+            assert same(rt[q], rn[q]), f"pair {q}: {diff_report(rt[q], rn[q])}"
