@@ -306,7 +306,7 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
         std::vector<std::pair<int, int>> cand;
         for (int order = 0; order < 2; ++order) {
             if (mfma)
-                for (int w = 1; w <= 4; ++w) cand.push_back({w, order});
+                for (int w : {3, 2, 4, 1}) cand.push_back({w, order});
             else
                 cand.push_back({0, order});
         }
@@ -343,7 +343,9 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
             const std::string name = (mfma ? "waves_per_simd=" + std::to_string(w) + " " : std::string()) +
                                      (order ? "order=linear" : "order=xcd_bands");
             best.ms.push_back({name, ms_min});
-            if (ms_min < best_ms) {
+            // a later candidate must win by more than timing noise (2 %):
+            // the first listed (XCD bands) also keeps the L2 traffic lowest
+            if (ms_min < 0.98f * best_ms) {
                 best_ms = ms_min;
                 best.waves_per_simd = w;
                 best.linear_order = order;
