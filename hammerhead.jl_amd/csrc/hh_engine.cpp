@@ -295,10 +295,10 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
 {
     Engine &E = c->eng;
     const char *fam = correlate_family(base, uq);
-    const bool mfma = std::string(fam) == "mfma32", reg = std::string(fam).rfind("reg", 0) == 0;
+    const bool mfma = std::string(fam).rfind("mfma32", 0) == 0, reg = std::string(fam).rfind("reg", 0) == 0;
     CorrConfig cfg = base;
     if (!c->tune.enabled || !(mfma || reg) || P.all.njobs <= 0) return cfg;
-    const std::string key = std::string(fam) + (uq ? "+uq" : "") + " w" + std::to_string(base.w) + " M" +
+    const std::string key = std::string(fam) + (uq && !mfma ? "+uq" : "") + " w" + std::to_string(base.w) + " M" +
                             std::to_string(base.M) + " K" + std::to_string(base.K) + " m" +
                             std::to_string(base.method) + " n" + std::to_string(P.all.njobs);
     auto it = c->tune.entries.find(key);

@@ -511,11 +511,70 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
     }
 }
 
+// Wieneke statistics alone for 16-px windows (the uncertainty of a sweep
+// whose correlation runs elsewhere, e.g. on the matrix cores): 16 lanes per
+// window, one row per lane, windows formed exactly as k_corrL<1> forms them
+// (same loads, means and apodization), then uq_fused16. Blocks of 4 windows
+// in XCD bands.
+__global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
+                                                 const float *__restrict__ B, int nr,
+                                                 const int32_t *__restrict__ jobs, int njobs, int nblocks,
+                                                 CorrOut out)
+{
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x, line = tid & 15, slot_w = tid >> 4;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, slot = b >> 3;
+    const int q8 = nblocks >> 3, r8 = nblocks & 7;
+    const int tile = cfg.linear_order ? b : xcd * q8 + min(xcd, r8) + slot;
+    const int win = tile * 4 + slot_w;
+    const bool active = win < njobs;
+    const int jb = active ? win : njobs - 1;
+    const int gi = jobs[4 * jb], gj = jobs[4 * jb + 1], rs = jobs[4 * jb + 2], cs = jobs[4 * jb + 3];
+    const float *pa = A + (size_t)(cs - 1) * nr + (rs - 1);
+    const float *pb = B + (size_t)(cs - 1) * nr + (rs - 1);
+    float ra[16], rb[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        ra[j] = pa[line + (size_t)j * nr];
+        rb[j] = pb[line + (size_t)j * nr];
+    }
+    float sa = 0.f, sb = 0.f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        sa += ra[j];
+        sb += rb[j];
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+        sa += __shfl_xor(sa, o);
+        sb += __shfl_xor(sb, o);
+    }
+    const float meanA = sa / 256.f, meanB = sb / 256.f;
+    float2 x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float a = cfg.apod[line + 16 * j];
+        x[j] = make_float2(a * (ra[j] - meanA), a * (rb[j] - meanB));
+    }
+    float sig_u, sig_v;
+    uq_fused16(x, line, sig_u, sig_v);
+    if (active && line == 0) {
+        const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
+        out.unc_u[g] = sig_u;
+        out.unc_v[g] = sig_v;
+    }
+}
+
 // Correlation of w = 16 L windows padded to M = 32 L. L lanes per line of the
 // 2D transform, G = 16 L^2 lanes per window. HW: half-width of the analysis
 // box [M/2 - HW, M/2 + HW]^2, which holds the gain support plus a zero margin.
 template <int L, int K, bool PHASE, int HW, bool UQ>
-__global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, const float *__restrict__ A,
+// 64-px windows: at most 256 VGPRs, so two waves per SIMD stay resident (the
+// few spills cost less than the latency one wave per SIMD leaves exposed);
+// the other widths keep their natural allocation (measured faster).
+__global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_eu(L == 4 ? 2 : 1))) void k_corrL(
+    CorrConfig cfg, const float *__restrict__ A,
                                                            const float *__restrict__ B, int nr,
                                                            const int32_t *__restrict__ jobs, int njobs, int nblocks,
                                                            CorrOut out)
@@ -831,6 +890,17 @@ static int fast_halfwidth(const CorrConfig &cfg)
 }
 
 bool correlate_box_layout(const CorrConfig &cfg) { return fast_halfwidth(cfg) >= 0; }
+
+bool launch_uq16_rows(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+                      const CorrOut &out, hipStream_t s)
+{
+    if (cfg.w != 16 || !out.unc_u || !out.unc_v) return false;
+    if (njobs <= 0) return true;
+    const int nblocks = (njobs + 3) / 4;
+    hipLaunchKernelGGL(k_uq16_rows, dim3(nblocks), dim3(64), 0, s, cfg, A, B, nr, jobs, njobs, nblocks, out);
+    HH_HIP(hipGetLastError());
+    return true;
+}
 
 const char *fast_family(const CorrConfig &cfg)
 {

@@ -542,10 +542,12 @@ static float2 *twiddle_table(int M, hipStream_t s)
 
 const char *fast_family(const CorrConfig &cfg);
 bool mfma_covers(const CorrConfig &cfg);
+bool launch_uq16_rows(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+                      const CorrOut &out, hipStream_t s);
 
 const char *correlate_family(const CorrConfig &cfg, bool uq)
 {
-    if (cfg.form == HH_FORM_MFMA16 && !uq && mfma_covers(cfg)) return "mfma32";
+    if (cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg)) return uq ? "mfma32+rows" : "mfma32";
     const char *f = fast_family(cfg);
     return f ? f : "lds";
 }
@@ -562,6 +564,15 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     (void)nc;
     if (njobs <= 0) return out.unc_u != nullptr;
     bool uq_done = false;
+    if (cfg.form == HH_FORM_MFMA16 && out.unc_u && out.unc_v && !out.planes && !out.acc && mfma_covers(cfg)) {
+        // the sweep with uncertainty: matrix-core correlation, then the
+        // statistics of the same windows by the row kernel
+        CorrOut o = out;
+        o.unc_u = o.unc_v = nullptr;
+        if (launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, o, s) &&
+            launch_uq16_rows(cfg, A, B, nr, jobs, njobs, out, s))
+            return true;
+    }
     if (cfg.form == HH_FORM_MFMA16 && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
     if (launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
     const float2 *tw = twiddle_table(cfg.M, s);
