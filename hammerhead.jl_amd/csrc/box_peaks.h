@@ -232,7 +232,6 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             tl[K - 1] = INT_MAX;
         }
     }
-    if (t != 0 || !active) return;
     // the plane corner (M, M): zero-valued, always a regional maximum
 #pragma unroll
     for (int m = 0; m < K; ++m)
@@ -247,6 +246,36 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
         if (found == m && m < nsel && sv[m] > 0.f) found = m + 1;
 
     const BoxView<NB, LO> R{Rb};
+    // subpixel fits of the K candidates side by side: lane t of the window
+    // fits candidate t (lane 0 with the pass's method, the others gauss3 as
+    // the alternatives use), and lane 0 gathers them
+    float2 fit = make_float2(NAN, NAN);
+    if constexpr (G >= K) {
+        int slt = sl[0];
+#pragma unroll
+        for (int m = 1; m < K; ++m)
+            if (t == m) slt = sl[m];
+        if (t < K && t < found) {
+            const int fr = slt % M + 1, fc = slt / M + 1;
+            if (t == 0 && cfg.subpixel == HH_GAUSS9)
+                fit = gauss9_b<M>(R, fr, fc);
+            else if (t == 0 && cfg.subpixel == HH_SUBPIX_NONE)
+                fit = make_float2((float)fr, (float)fc);
+            else
+                fit = gauss3_b<M>(R, fr, fc);
+        }
+    }
+    float2 fits[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        if constexpr (G >= K) {
+            const int src = (tid & 63) - t + m;  // lane of candidate m in this wave
+            fits[m] = make_float2(__shfl(fit.x, src), __shfl(fit.y, src));
+        } else {
+            fits[m] = make_float2(NAN, NAN);
+        }
+    }
+    if (t != 0 || !active) return;
     const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
     const size_t ng = (size_t)out.ny * out.nx;
     if (out.found) out.found[jb] = found;
@@ -258,14 +287,18 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             out.vals[K * jb + m] = m < found ? sv[m] : NAN;
         }
     }
-    const int pr = sl[0] % M + 1, pc = sl[0] / M + 1;
     float2 ref;
-    if (cfg.subpixel == HH_GAUSS9)
-        ref = gauss9_b<M>(R, pr, pc);
-    else if (cfg.subpixel == HH_SUBPIX_NONE)
-        ref = make_float2((float)pr, (float)pc);
-    else
-        ref = gauss3_b<M>(R, pr, pc);
+    if constexpr (G >= K) {
+        ref = fits[0];
+    } else {
+        const int pr = sl[0] % M + 1, pc = sl[0] / M + 1;
+        if (cfg.subpixel == HH_GAUSS9)
+            ref = gauss9_b<M>(R, pr, pc);
+        else if (cfg.subpixel == HH_SUBPIX_NONE)
+            ref = make_float2((float)pr, (float)pc);
+        else
+            ref = gauss3_b<M>(R, pr, pc);
+    }
     const float cc = (float)(H + 1);
     const float du = ref.y - cc, dv = ref.x - cc;
     const float v1 = sv[0];
@@ -286,7 +319,11 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             if (m > cfg.n_peaks) continue;
             float au = NAN, av = NAN;
             if (m <= found) {
-                const float2 a = gauss3_b<M>(R, sl[m - 1] % M + 1, sl[m - 1] / M + 1);
+                float2 a;
+                if constexpr (G >= K)
+                    a = fits[m - 1];
+                else
+                    a = gauss3_b<M>(R, sl[m - 1] % M + 1, sl[m - 1] / M + 1);
                 au = a.y - cc;
                 av = a.x - cc;
                 if (out.u0) {

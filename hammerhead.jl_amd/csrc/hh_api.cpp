@@ -313,10 +313,9 @@ void do_prefilter(hh_ctx *c)
     const size_t n = (size_t)(c->nr + 2) * (c->nc + 2);
     c->coefA.ensure(n);
     c->coefB.ensure(n);
-    c->ptmp.ensure((size_t)(c->nr + 2) * c->nc);
+    c->ptmp.ensure((size_t)2 * (c->nr + 2) * c->nc);
     timed(c, "prefilter", 0, [&] {
-        launch_prefilter(c->A, c->nr, c->nr, c->nc, c->coefA.p, c->ptmp.p, c->stream);
-        launch_prefilter(c->B, c->nr, c->nr, c->nc, c->coefB.p, c->ptmp.p, c->stream);
+        launch_prefilter_pair(c->A, c->B, c->nr, c->nr, c->nc, c->coefA.p, c->coefB.p, c->ptmp.p, c->stream);
     });
     c->coefs_valid = true;
 }

@@ -421,7 +421,7 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         E.coefs.ensure(2 * L.coef_total);
         E.warp.ensure(2 * (size_t)L.ld * (size_t)L.ncols);
         size_t t = 0;
-        for (int q = 0; q < L.npairs; ++q) t = std::max(t, (size_t)(L.nr[q] + 2) * L.nc[q]);
+        for (int q = 0; q < L.npairs; ++q) t = std::max(t, (size_t)2 * (L.nr[q] + 2) * L.nc[q]);
         E.ptmp.ensure(t);
     }
     for (int k = 0; k < np; ++k)
@@ -462,8 +462,8 @@ void record(hh_ctx *c, bool dynamic)
         timed(c, "prefilter", 0, [&] {
             for (int q = 0; q < L.npairs; ++q) {
                 const size_t o = (size_t)L.roff[q] + (size_t)L.coff[q] * L.ld;
-                launch_prefilter(L.A + o, L.ld, L.nr[q], L.nc[q], cA + L.coef_off[q], E.ptmp.p, s);
-                launch_prefilter(L.B + o, L.ld, L.nr[q], L.nc[q], cB + L.coef_off[q], E.ptmp.p, s);
+                launch_prefilter_pair(L.A + o, L.B + o, L.ld, L.nr[q], L.nc[q], cA + L.coef_off[q],
+                                      cB + L.coef_off[q], E.ptmp.p, s);
             }
         });
     }

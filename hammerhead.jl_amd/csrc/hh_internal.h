@@ -188,7 +188,10 @@ struct DeformTab {
 };
 
 // img: nr x nc with leading dimension ld.
-void launch_prefilter(const float *img, int ld, int nr, int nc, float *coef, float *tmp, hipStream_t s);
+// Both frames of a pair, two launches (one per axis) covering both frames;
+// tmp holds 2 x (nr+2) x nc floats.
+void launch_prefilter_pair(const float *imgA, const float *imgB, int ld, int nr, int nc, float *coefA, float *coefB,
+                           float *tmp, hipStream_t s);
 void launch_deform(const float *cA, const float *cB, int nr, int nc, const float *pu, const float *pv, int gny,
                    int gnx, float y0, float ysp, float x0, float xsp, float *wA, float *wB, hipStream_t s);
 // All pairs of a batch in one launch (ld: leading dimension of the atlas).

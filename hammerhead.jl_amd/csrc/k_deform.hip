@@ -71,10 +71,14 @@ __device__ __forceinline__ float ext_value(const Get &get, int j, int n)
 // Pass 1: along dim 1. out is (nr+2) x nc, logical row k at out[k + j*(nr+2)].
 // Block (x, j): logical rows [256 x, 256 x + 256) of column j; the extended
 // segment with its 16-sample halos is staged once in LDS.
-__global__ __launch_bounds__(256) void k_prefilter_rows(const float *__restrict__ img, int ld, int nr, int nc,
-                                                         float *__restrict__ out, Taps taps)
+__global__ __launch_bounds__(256) void k_prefilter_rows(const float *__restrict__ imgA,
+                                                         const float *__restrict__ imgB, int ld, int nr, int nc,
+                                                         float *__restrict__ outA, float *__restrict__ outB,
+                                                         Taps taps)
 {
     __shared__ float seg[256 + 2 * kTaps];
+    const float *__restrict__ img = blockIdx.z ? imgB : imgA;
+    float *__restrict__ out = blockIdx.z ? outB : outA;
     const int N1 = nr + 2;
     const int j = blockIdx.y, k0 = blockIdx.x * 256;
     const float *col = img + (size_t)j * ld;
@@ -93,10 +97,14 @@ __global__ __launch_bounds__(256) void k_prefilter_rows(const float *__restrict_
 // Pass 2: along dim 2 over the (nr+2) rows of tmp; coef is (nr+2) x (nc+2).
 // Block (x, y): rows [64 x, 64 x + 64), logical columns [64 y, 64 y + 64),
 // staged with the column halos in LDS (coalesced along the rows).
-__global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict__ tmp, int nr, int nc,
-                                                         float *__restrict__ coef, Taps taps)
+__global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict__ tmpA,
+                                                         const float *__restrict__ tmpB, int nr, int nc,
+                                                         float *__restrict__ coefA, float *__restrict__ coefB,
+                                                         Taps taps)
 {
     __shared__ float tile[64 + 2 * kTaps][65];
+    const float *__restrict__ tmp = blockIdx.z ? tmpB : tmpA;
+    float *__restrict__ coef = blockIdx.z ? coefB : coefA;
     const int N1 = nr + 2, N2 = nc + 2;
     const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
     const int ti = threadIdx.x & 63, tg = threadIdx.x >> 6;
@@ -193,13 +201,15 @@ __global__ __launch_bounds__(256) void k_node_values_batch(Grids g, const Deform
 
 }  // namespace
 
-void launch_prefilter(const float *img, int ld, int nr, int nc, float *coef, float *tmp, hipStream_t s)
+void launch_prefilter_pair(const float *imgA, const float *imgB, int ld, int nr, int nc, float *coefA, float *coefB,
+                           float *tmp, hipStream_t s)
 {
     static const Taps taps = make_taps();
-    hipLaunchKernelGGL(k_prefilter_rows, dim3((unsigned)((nr + 2 + 255) / 256), (unsigned)nc), dim3(256), 0, s, img,
-                       ld, nr, nc, tmp, taps);
-    hipLaunchKernelGGL(k_prefilter_cols, dim3((unsigned)((nr + 2 + 63) / 64), (unsigned)((nc + 2 + 63) / 64)), dim3(256),
-                       0, s, tmp, nr, nc, coef, taps);
+    float *tA = tmp, *tB = tmp + (size_t)(nr + 2) * nc;
+    hipLaunchKernelGGL(k_prefilter_rows, dim3((unsigned)((nr + 2 + 255) / 256), (unsigned)nc, 2), dim3(256), 0, s,
+                       imgA, imgB, ld, nr, nc, tA, tB, taps);
+    hipLaunchKernelGGL(k_prefilter_cols, dim3((unsigned)((nr + 2 + 63) / 64), (unsigned)((nc + 2 + 63) / 64), 2),
+                       dim3(256), 0, s, tA, tB, nr, nc, coefA, coefB, taps);
     HH_HIP(hipGetLastError());
 }
 

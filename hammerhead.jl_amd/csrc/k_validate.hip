@@ -63,16 +63,21 @@ __device__ __forceinline__ T median_sorted(const T (&x)[N], int n, bool hasnan)
 }
 
 template <int NB>
+// init: the first validator of a validation, which writes every cell's flag
+// (so no separate clear is needed) and zeroes the per-segment counters;
+// otherwise flagged cells are ORed in.
 __global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const float *__restrict__ v, Grids g,
                                              double thr, double eps, const uint8_t *__restrict__ exclude,
-                                             uint8_t *__restrict__ flags)
+                                             uint8_t *__restrict__ flags, int init, int *__restrict__ count)
 {
     constexpr int W = 2 * NB + 1;
     constexpr int N = W * W - 1;  // neighbours; the box centre is linear index N/2
     const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (init && ee < g.nseg) count[ee] = 0;
     if (ee >= g.total) return;
     const Cell k = cell_of(g, ee);
     const int e = k.b, sgoff = k.sg.off, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
+    if (init) flags[e] = 0;
     if (exclude && exclude[e]) return;
     const GridSeg sg = k.sg;
     const float *__restrict__ ug = u + sg.off;
@@ -179,24 +184,23 @@ __device__ double median_box(const Get &get, int r0, int r1, int c0, int c1, int
 // substitute_alternatives! (src/quality.jl:307-351). The valid neighbours
 // of a flagged cell are gathered into registers (+inf padding) and sorted
 // by a network, so the Float64 median and MAD cost no memory round trips.
+// One cell of substitute_alternatives!; returns the cell's final flag.
 template <int NB>
-__global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ uref, const float *__restrict__ vref,
-                                                    const uint8_t *__restrict__ flags_in, Grids g, double thr,
-                                                    const float *__restrict__ alt_u, const float *__restrict__ alt_v,
-                                                    int nalt, float *__restrict__ u_out, float *__restrict__ v_out,
-                                                    uint8_t *__restrict__ flags_out)
+__device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ uref, const float *__restrict__ vref,
+                                                   const uint8_t *__restrict__ flags_in, const Grids &g,
+                                                   const Cell &k, double thr, const float *__restrict__ alt_u,
+                                                   const float *__restrict__ alt_v, int nalt,
+                                                   float *__restrict__ u_out, float *__restrict__ v_out,
+                                                   uint8_t *__restrict__ flags_out)
 {
     constexpr int W = 2 * NB + 1;
     constexpr int N = W * W - 1;
-    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ee >= g.total) return;
-    const Cell k = cell_of(g, ee);
     const int e = k.b, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
     const GridSeg sg = k.sg;
     u_out[e] = uref[e];
     v_out[e] = vref[e];
     flags_out[e] = flags_in[e];
-    if (!flags_in[e]) return;
+    if (!flags_in[e]) return 0;
     const float *__restrict__ ug = uref + sg.off;
     const float *__restrict__ vg = vref + sg.off;
     const uint8_t *__restrict__ fg = flags_in + sg.off;
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
         su[q] = nu[q];
         sv[q] = nv[q];
     }
-    if (n < 3) return;
+    if (n < 3) return 1;
     sort_net<double, N>(su);
     sort_net<double, N>(sv);
     const double mu = median_sorted<double, N>(su, n, nanu);
@@ -243,23 +247,21 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
             u_out[e] = au;
             v_out[e] = av;
             flags_out[e] = 0;
-            break;
+            return 0;
         }
     }
+    return 1;
 }
 
-// Number of valid cells per segment: one atomic per block when the block
-// lies in one segment (always, for a single grid), else one per wave or per
-// lane. With fewer than 3 in a grid no vector of that grid can be replaced
-// and k_replace returns at once.
-__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, Grids g,
-                                                     int *__restrict__ count)
+// Adds the valid cells of a block to their segments' counters: one atomic
+// per block when the block lies in one segment (always, for a single grid),
+// else one per wave or per lane. Every thread of the block calls it.
+__device__ __forceinline__ void add_valid(const Grids &g, int ee, bool valid, int *__restrict__ count)
 {
     __shared__ int part[4], segw[4];
-    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
     const Cell k = cell_of(g, ee < g.total ? ee : g.total - 1);
     const int si = k.si, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = ee < g.total && !invalid[k.b] ? 1 : 0;
+    const int x = ee < g.total && valid ? 1 : 0;
     const int s0 = __shfl(si, 0);
     const bool uni = __all(si == s0);
     int wsum = x;
@@ -283,6 +285,33 @@ __global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__
     } else if (x) {
         atomicAdd(count + si, 1);
     }
+}
+
+// Number of valid cells per segment: with fewer than 3 in a grid no vector
+// of that grid can be replaced and k_replace returns at once.
+__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, Grids g,
+                                                     int *__restrict__ count)
+{
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = ee < g.total && !invalid[cell_of(g, ee).b];
+    add_valid(g, ee, valid, count);
+}
+
+// substitute_alternatives! over the enumerated cells; with count non-null the
+// valid cells of the result are also counted per segment (for k_replace).
+template <int NB>
+__global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ uref, const float *__restrict__ vref,
+                                                    const uint8_t *__restrict__ flags_in, Grids g, double thr,
+                                                    const float *__restrict__ alt_u, const float *__restrict__ alt_v,
+                                                    int nalt, float *__restrict__ u_out, float *__restrict__ v_out,
+                                                    uint8_t *__restrict__ flags_out, int *__restrict__ count)
+{
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    uint8_t f = 1;
+    if (ee < g.total)
+        f = substitute_cell<NB>(uref, vref, flags_in, g, cell_of(g, ee), thr, alt_u, alt_v, nalt, u_out, v_out,
+                                flags_out);
+    if (count) add_valid(g, ee, f == 0, count);
 }
 
 // flags_out (nullable) receives a copy of invalid.
@@ -396,25 +425,28 @@ __global__ void k_copy_back(const float *__restrict__ su, const float *__restric
 }
 
 template <int NB>
-void uod_launch(const ValidateArgs &a, double thr, double eps, hipStream_t s)
+void uod_launch(const ValidateArgs &a, double thr, double eps, bool init, hipStream_t s)
 {
-    const int n = a.grids.total;
+    const int n = std::max(a.grids.total, a.grids.nseg);
     hipLaunchKernelGGL(k_uod<NB>, dim3((n + 255) / 256), dim3(256), 0, s, a.u, a.v, a.grids, thr, eps,
-                       (const uint8_t *)nullptr, a.outliers);
+                       (const uint8_t *)nullptr, a.outliers, init ? 1 : 0, a.count);
 }
 
-void uod(const ValidateArgs &a, int nb, double thr, double eps, hipStream_t s)
+void uod(const ValidateArgs &a, int nb, double thr, double eps, bool init, hipStream_t s)
 {
     switch (nb) {
-    case 1: uod_launch<1>(a, thr, eps, s); break;
-    case 2: uod_launch<2>(a, thr, eps, s); break;
-    case 3: uod_launch<3>(a, thr, eps, s); break;
+    case 1: uod_launch<1>(a, thr, eps, init, s); break;
+    case 2: uod_launch<2>(a, thr, eps, init, s); break;
+    case 3: uod_launch<3>(a, thr, eps, init, s); break;
     default: throw Error(HH_EUNSUPPORTED, "UOD neighborhood sizes 1, 2 and 3 are supported on the device");
     }
 }
 
 }  // namespace
 
+// Launches: the first UOD writes every flag and clears the counters (else a
+// clear kernel does), the substitution also counts the valid cells the
+// replacement needs (else a count kernel does), then the replacement.
 void launch_validate(const ValidateArgs &a, hipStream_t s)
 {
     const hh_pass &p = *a.pass;
@@ -422,15 +454,17 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
     if (n <= 0) return;
     const int ntile = std::max(n, a.grids.nseg);
     const dim3 g((n + 255) / 256), b(256);
-    hipLaunchKernelGGL(k_val_init, dim3((ntile + 255) / 256), b, 0, s, a.outliers, a.grids, a.count);
-    if (p.uod_enable) uod(a, p.uod_neighborhood, p.uod_threshold, 0.1, s);
+    if (p.uod_enable)
+        uod(a, p.uod_neighborhood, p.uod_threshold, 0.1, true, s);
+    else
+        hipLaunchKernelGGL(k_val_init, dim3((ntile + 255) / 256), b, 0, s, a.outliers, a.grids, a.count);
     if (p.min_peak_ratio > 1)
         hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, a.grids, (int)HH_V_PEAK_RATIO,
                            p.min_peak_ratio, 0.0, a.outliers);
     for (int q = 0; q < p.n_validators; ++q) {
         const hh_validator &vd = p.validators[q];
         if (vd.kind == HH_V_UOD)
-            uod(a, vd.neighborhood, vd.a, vd.b, s);
+            uod(a, vd.neighborhood, vd.a, vd.b, false, s);
         else
             hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, a.grids, (int)vd.kind, vd.a, vd.b,
                                a.outliers);
@@ -440,31 +474,32 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
     // result left in scratch is copied back once at the end.
     const bool sub = a.alt_u && p.n_peaks > 1;
     const bool rep = a.force_replace || p.replace_outliers;
+    int *cnt = rep ? a.count : nullptr;  // counters zeroed above
     if (sub) {
         switch (p.uod_neighborhood) {
         case 1:
             hipLaunchKernelGGL(k_substitute<1>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
-                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt);
             break;
         case 2:
             hipLaunchKernelGGL(k_substitute<2>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
-                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt);
             break;
         default:
             hipLaunchKernelGGL(k_substitute<3>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
-                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag);
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt);
             break;
         }
     }
     if (rep) {
-        const uint8_t *flags = sub ? a.sflag : a.outliers;
-        hipLaunchKernelGGL(k_count_valid, g, b, 0, s, flags, a.grids, a.count);
-        if (sub)  // substituted field in scratch: replace back into the chain
+        if (sub) {  // substituted field in scratch: replace back into the chain
             hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.grids, (const int *)a.count, a.u,
                                a.v, a.outliers);
-        else
+        } else {
+            hipLaunchKernelGGL(k_count_valid, g, b, 0, s, a.outliers, a.grids, a.count);
             hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.grids, (const int *)a.count, a.su,
                                a.sv, (uint8_t *)nullptr);
+        }
     }
     if (sub != rep)  // exactly one stage ran: its result is in scratch
         hipLaunchKernelGGL(k_copy_back, g, b, 0, s, a.su, a.sv, sub ? a.sflag : (const uint8_t *)nullptr, a.u, a.v,

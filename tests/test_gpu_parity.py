@@ -130,8 +130,23 @@ def test_prefilter_and_deform_vs_oracle(hh, orc, ctx):
     cA, cB = ctx.coefs()
     oA = orc.prefilter(pair.A.astype(np.float64))
     err = np.abs(cA - oA).max() / np.abs(oA).max()
-    say("prefilter rel err", err)
-    assert err < 5e-6
+    oB0 = orc.prefilter(pair.B.astype(np.float64))
+    errB = np.abs(cB - oB0).max() / np.abs(oB0).max()
+    say("prefilter rel err", err, errB)
+    assert err < 5e-6 and errB < 5e-6
+    # the smallest sizes of the fused 2D kernel (34) and of the two-pass one
+    for nr, nc in ((34, 35), (33, 40), (97, 34)):
+        small = hh.synthetic.particle_pair(nr, nc, hh.synthetic.uniform_flow(0.5, 0.2), seed=nr)
+        ctx.set_pair(small.A, small.B)
+        ctx.prefilter()
+        sA, sB = ctx.coefs()
+        for got, img in ((sA, small.A), (sB, small.B)):
+            ref = orc.prefilter(img.astype(np.float64))
+            e2 = np.abs(got - ref).max() / np.abs(ref).max()
+            say("prefilter", nr, nc, "rel err", e2)
+            assert e2 < 5e-6
+    ctx.set_pair(pair.A, pair.B)
+    ctx.prefilter()
     py = np.linspace(16.5, 184.5, 11)
     px = np.linspace(16.5, 164.5, 10)
     rng = np.random.default_rng(0)
