@@ -146,10 +146,10 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
         // row above) disqualify when strictly greater, later ones when >=
         const bool col = t < NB;
         auto left = [](float v) {  // lane t gets lane t - 1 (0 at t = 0)
-            return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xF, 0xF, true));
         };
         auto right = [](float v) {  // lane t gets lane t + 1 (0 at t = 15)
-            return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, false));
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x101, 0xF, 0xF, true));
         };
         float cp = 0.f, lp = 0.f, rp = 0.f;
         float cc = col ? Rb[t * NB] : 0.f;

@@ -177,7 +177,7 @@ constexpr int kQNEG4 = 0xB4; // [0,1,3,2]
 template <int CTRL>
 __device__ __forceinline__ float qp(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ float2 qp2(float2 v)
@@ -308,8 +308,8 @@ template <int S>
 __device__ __forceinline__ int rowsh_i(int v)  // lane r gets lane r + S of its row, 0 outside
 {
     if constexpr (S == 0) return v;
-    else if constexpr (S > 0) return __builtin_amdgcn_update_dpp(0, v, 0x100 + S, 0xF, 0xF, false);
-    else return __builtin_amdgcn_update_dpp(0, v, 0x110 - S, 0xF, 0xF, false);
+    else if constexpr (S > 0) return __builtin_amdgcn_mov_dpp(v, 0x100 + S, 0xF, 0xF, true);
+    else return __builtin_amdgcn_mov_dpp(v, 0x110 - S, 0xF, 0xF, true);
 }
 template <int S>
 __device__ __forceinline__ float rowsh(float v)
@@ -325,8 +325,8 @@ __device__ __forceinline__ double rowsh_d(double v)
 template <int R>
 __device__ __forceinline__ double rowror_d(double v)  // rotate within the row by R
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x120 + R, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x120 + R, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x120 + R, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x120 + R, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double row_sum_d(double v)  // sum over the 16 lanes, in every lane
@@ -412,17 +412,19 @@ __device__ __forceinline__ void uq_lags(const double (&e)[NC], UQAcc &acc)
     }
     sfor<4>([&](auto ss) {
         constexpr int s = decltype(ss)::value + 1;
-        double sh[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) sh[c] = rowsh_d<s>(e[c]);
+        // p[o] = sum_c e[c] sh[c + o] in increasing c, with the shifted row
+        // fetched one element at a time (sh[c'] feeds p[o] through e[c' - o])
         double p[9];
 #pragma unroll
-        for (int oi = 0; oi < 9; ++oi) {
-            const int o = oi - 4;
-            p[oi] = 0.0;
+        for (int oi = 0; oi < 9; ++oi) p[oi] = 0.0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-                if (c + o >= 0 && c + o < NC) p[oi] += e[c] * sh[c + o];
+        for (int cs = 0; cs < NC; ++cs) {
+            const double sh = rowsh_d<s>(e[cs]);
+#pragma unroll
+            for (int oi = 8; oi >= 0; --oi) {
+                const int o = oi - 4, c = cs - o;
+                if (c >= 0 && c < NC) p[oi] += e[c] * sh;
+            }
         }
 #pragma unroll
         for (int oi = 0; oi < 9; ++oi) uq_add(acc, s, oi - 4, row_sum_d(p[oi]));

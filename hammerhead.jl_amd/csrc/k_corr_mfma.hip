@@ -90,7 +90,7 @@ __device__ __forceinline__ f16v mfma3(const Split &a, const Split &b, f16v acc)
 template <int R>
 __device__ __forceinline__ float rowror_f(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + R, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ void img_reduce(float &sum, float &mx, float &mn)
