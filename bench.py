@@ -83,6 +83,17 @@ def mfma_flops_per_window_m32() -> float:
     return 36 * 2 * 32 * 32 * 16
 
 
+TUNE = {}  # autotuner choices loaded with --tune-file (key -> entry)
+
+
+def new_context(hh, form):
+    ctx = hh.Context(0)
+    ctx.set_form(form)
+    if TUNE:
+        ctx.tune_load(TUNE)
+    return ctx
+
+
 def time_form(hh, pair, passes, form, steps, warmup, log):
     """Steady-state vector fields/s of one correlation form, two timed loops
     of `steps` vector fields each on the same context:
@@ -90,8 +101,7 @@ def time_form(hh, pair, passes, form, steps, warmup, log):
         bracketed by HIP events on the context stream (the per-kernel times);
       * graph: the whole schedule replayed from its captured hipGraph.
     """
-    ctx = hh.Context(0)
-    ctx.set_form(form)
+    ctx = new_context(hh, form)
     ctx.set_pair(pair.A, pair.B)
     for _ in range(max(warmup, 1)):
         ctx.run_piv(passes, fetch=False)  # captures the graph on the first call
@@ -128,8 +138,7 @@ def time_form(hh, pair, passes, form, steps, warmup, log):
 def time_end_to_end(hh, pair, passes, form, steps):
     """Host pair in, host PIVResult out: upload, schedule and download per
     vector field (PCIe-inclusive)."""
-    ctx = hh.Context(0)
-    ctx.set_form(form)
+    ctx = new_context(hh, form)
     ctx.set_pair(pair.A, pair.B)
     ctx.run_piv(passes, fetch=True)
     t0 = time.perf_counter()
@@ -148,8 +157,7 @@ def time_batch(hh, passes, form, npairs, steps, log):
     whole 64/32 -> 32/16 -> 16/12 x3 + UQ schedule per step over all pairs
     (one launch per stage per sweep, replayed as a hipGraph)."""
     pairs = hh.synthetic.config_batch(npairs=npairs)
-    ctx = hh.Context(0)
-    ctx.set_form(form)
+    ctx = new_context(hh, form)
     ctx.set_pairs([(p.A, p.B) for p in pairs])
     ctx.run_piv_batch(passes, fetch=False)
     ctx.synchronize()
@@ -188,6 +196,9 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default=None)
+    ap.add_argument("--tune-file", default=None,
+                    help="autotuner choices: loaded if the file exists (no candidate timing), else written after "
+                         "the run (profiling runs reuse the timed run's launch geometry)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device copies of this size (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -208,6 +219,10 @@ def main(argv=None):
         return 2
 
     import __graft_entry__ as ge
+
+    tune_path = pathlib.Path(args.tune_file) if args.tune_file else None
+    if tune_path and tune_path.exists():
+        TUNE.update(json.loads(tune_path.read_text()))
 
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     hh = ge.load_package()
@@ -287,6 +302,11 @@ def main(argv=None):
         line["cpu_baseline"] = cpu_baseline(pair, passes, args.cpu_threads)
     else:
         line["cpu_baseline"] = None
+    if tune_path and not tune_path.exists():
+        tuned = dict(res["tune"])
+        if not args.no_compare:
+            tuned.update(r2["tune"])
+        tune_path.write_text(json.dumps(tuned))
     print(json.dumps(line), flush=True)
     return 0
 

@@ -29,7 +29,7 @@ EXPORTS = (
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
-    "hh_set_autotune", "hh_tune_report",
+    "hh_set_autotune", "hh_tune_report", "hh_tune_set",
 )
 
 
@@ -101,6 +101,7 @@ def load():
         "hh_graph_replays": (i32, [vp, C.POINTER(C.c_int64)]),
         "hh_set_autotune": (i32, [vp, i32]),
         "hh_tune_report": (i32, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "hh_tune_set": (i32, [vp, C.c_char_p, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -370,6 +371,13 @@ class Context:
         buf = C.create_string_buffer(need.value)
         _check(self.h, load().hh_tune_report(self.h, buf, need.value, None), "tune_report")
         return json.loads(buf.value.decode())
+
+    def tune_load(self, report: dict):
+        """Preload the choices of a tune_report() (no candidate timing)."""
+        for key, e in report.items():
+            ch = e["choice"]
+            _check(self.h, load().hh_tune_set(self.h, key.encode(), int(ch["waves_per_simd"]),
+                                              int(ch["linear_order"])), "tune_set")
 
     def graph_replays(self) -> int:
         n = C.c_int64()

@@ -1263,6 +1263,19 @@ int hh_tune_report(hh_ctx *c, char *buf, size_t len, size_t *needed)
     });
 }
 
+int hh_tune_set(hh_ctx *c, const char *key, int waves_per_simd, int linear_order)
+{
+    return guarded(c, [&] {
+        if (!key) throw Error(HH_EINVAL, "null key");
+        if (waves_per_simd < 0 || waves_per_simd > 8) throw Error(HH_EINVAL, "waves_per_simd must be 0-8");
+        TuneEntry e;
+        e.waves_per_simd = waves_per_simd;
+        e.linear_order = linear_order != 0;
+        c->tune.entries[key] = e;
+        c->eng.key.clear();  // re-plan with the choice
+    });
+}
+
 int hh_graph_replays(hh_ctx *c, int64_t *replays)
 {
     return guarded(c, [&] {

@@ -572,11 +572,7 @@ __global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *_
 // 2D transform, G = 16 L^2 lanes per window. HW: half-width of the analysis
 // box [M/2 - HW, M/2 + HW]^2, which holds the gain support plus a zero margin.
 template <int L, int K, bool PHASE, int HW, bool UQ>
-// 64-px windows: at most 256 VGPRs, so two waves per SIMD stay resident (the
-// few spills cost less than the latency one wave per SIMD leaves exposed);
-// the other widths keep their natural allocation (measured faster).
-__global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_eu(L == 4 ? 2 : 1))) void k_corrL(
-    CorrConfig cfg, const float *__restrict__ A,
+__global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, const float *__restrict__ A,
                                                            const float *__restrict__ B, int nr,
                                                            const int32_t *__restrict__ jobs, int njobs, int nblocks,
                                                            CorrOut out)

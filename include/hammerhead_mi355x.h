@@ -256,6 +256,9 @@ int hh_graph_replays(hh_ctx *ctx, int64_t *replays);
  * src/ka_backend.jl). */
 int hh_set_autotune(hh_ctx *ctx, int enable);
 int hh_tune_report(hh_ctx *ctx, char *buf, size_t len, size_t *needed);
+/* Preload one autotuner choice (a key of hh_tune_report) so that plans use it
+ * without timing candidates, e.g. choices saved by an earlier process. */
+int hh_tune_set(hh_ctx *ctx, const char *key, int waves_per_simd, int linear_order);
 
 /* ---- ensemble (sum-of-correlation) PIV (A16) --------------------------- */
 
