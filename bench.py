@@ -182,6 +182,37 @@ def time_batch(hh, passes, form, npairs, steps, log):
             "rms_px_vs_midpoint_truth_median": round(float(np.median(rms)), 5)}
 
 
+def time_ensemble(hh, npairs, steps, log):
+    """The ensemble workload: run_piv_ensemble over npairs pairs of
+    1024 x 1280 (rows x cols) at 0.005 ppp, noise 0.05, one 32/16 padded
+    Gaussian pass with FP64 plane sums. Timed end to end (host frames
+    uploaded by the call) and with the pairs already resident."""
+    t0 = time.perf_counter()
+    pairs, flow = hh.synthetic.config_ensemble(npairs=npairs)
+    log(f"ensemble workload of {npairs} pairs generated in {time.perf_counter() - t0:.1f} s")
+    p = hh.PIVParameters(window_size=32, overlap=16, padding=True, apodization="gauss")
+    shape = pairs[0].A.shape
+    ctx = hh.Context(0)
+    t0 = time.perf_counter()
+    res = ctx.run_piv_ensemble([(q.A, q.B) for q in pairs], [p], fetch=True)
+    e2e = time.perf_counter() - t0
+    del pairs
+    ctx.run_piv_ensemble_resident(npairs, shape, [p], fetch=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run_piv_ensemble_resident(npairs, shape, [p], fetch=False)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.close()
+    rms = hh.synthetic.rms_error(res, flow, midpoint=False)
+    log(f"ensemble: end to end {e2e * 1e3:.1f} ms, resident {dt * 1e3:.2f} ms")
+    return {"pairs": npairs, "size": f"{shape[0]}x{shape[1]} (rows x cols)", "pass": "32/16 pad+gauss, FP64 sums",
+            "ms_resident": round(dt * 1e3, 3), "pairs_per_s_resident": round(npairs / dt, 1),
+            "ms_end_to_end": round(e2e * 1e3, 1), "pairs_per_s_end_to_end": round(npairs / e2e, 1),
+            "rms_px_vs_truth": round(rms, 4), "outliers": int(res["outliers"].sum())}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +225,7 @@ def main(argv=None):
     ap.add_argument("--no-end-to-end", action="store_true", help="skip the PCIe-inclusive rate")
     ap.add_argument("--batch-pairs", type=int, default=64, help="pairs of the batching workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--ensemble-pairs", type=int, default=500, help="pairs of the ensemble workload (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic-file", default=None)
     ap.add_argument("--tune-file", default=None,
@@ -297,6 +329,8 @@ def main(argv=None):
         line["pcie_inclusive"] = time_end_to_end(hh, pair, passes, forms[args.form], max(3, args.steps // 4))
     if args.batch_pairs > 0:
         line["batch"] = time_batch(hh, passes, forms[args.form], args.batch_pairs, args.batch_steps, log)
+    if args.ensemble_pairs > 0:
+        line["ensemble"] = time_ensemble(hh, args.ensemble_pairs, 3, log)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(pair, passes, args.cpu_threads)

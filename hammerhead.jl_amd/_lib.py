@@ -447,6 +447,23 @@ class Context:
         res["mask"] = res["mask"].astype(bool)
         return res
 
+    def run_piv_ensemble_resident(self, npairs, shape, passes, predictor_smoothing=True, fetch=True):
+        """run_piv_ensemble again on the pairs the previous call uploaded."""
+        arr, n = pass_array(passes)
+        nr, nc = shape
+        self.nr, self.nc = nr, nc
+        if not fetch:
+            _check(self.h, load().hh_run_piv_ensemble(self.h, None, None, npairs, nr, nc, arr, n,
+                                                       1 if predictor_smoothing else 0, None), "run_piv_ensemble")
+            return None
+        ny, nx = grid_size(passes[-1], nr, nc)
+        res, f = self._result_arrays(ny, nx)
+        _check(self.h, load().hh_run_piv_ensemble(self.h, None, None, npairs, nr, nc, arr, n,
+                                                   1 if predictor_smoothing else 0, C.byref(f)), "run_piv_ensemble")
+        res["outliers"] = res["outliers"].astype(bool)
+        res["mask"] = res["mask"].astype(bool)
+        return res
+
     # ---- correlation energy and its adjoint ------------------------------------
     @staticmethod
     def _nodes(py, px, pu, pv):

@@ -507,17 +507,30 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
 {
     if (npairs < 1) throw Error(HH_EINVAL, "pairs must not be empty");
     if (np < 1) throw Error(HH_EINVAL, "at least one pass is required");
-    if (!hA || !hB) throw Error(HH_EINVAL, "null pair list");
     if (nr < 2 || nc < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
     for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
-    for (int q = 0; q < npairs; ++q)
-        if (!hA[q] || !hB[q]) throw Error(HH_EINVAL, "null image in the pair list");
     const size_t npx = (size_t)nr * nc;
-    float *arena = c->arena.ensure(2 * npx * npairs);
-    for (int q = 0; q < npairs; ++q) {
-        HH_HIP(hipMemcpyAsync(arena + 2 * npx * q, hA[q], sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
-        HH_HIP(hipMemcpyAsync(arena + 2 * npx * q + npx, hB[q], sizeof(float) * npx, hipMemcpyHostToDevice,
-                              c->stream));
+    float *arena = nullptr;
+    if (!hA && !hB) {
+        // the pairs of the previous call, still resident
+        if (c->arena_pairs != npairs || c->arena_nr != nr || c->arena_nc != nc)
+            throw Error(HH_ESTATE, "no resident pairs of this count and size: pass the frames");
+        arena = c->arena.p;
+    } else {
+        if (!hA || !hB) throw Error(HH_EINVAL, "null pair list");
+        for (int q = 0; q < npairs; ++q)
+            if (!hA[q] || !hB[q]) throw Error(HH_EINVAL, "null image in the pair list");
+        c->arena_pairs = 0;
+        arena = c->arena.ensure(2 * npx * npairs);
+        for (int q = 0; q < npairs; ++q) {
+            HH_HIP(hipMemcpyAsync(arena + 2 * npx * q, hA[q], sizeof(float) * npx, hipMemcpyHostToDevice,
+                                  c->stream));
+            HH_HIP(hipMemcpyAsync(arena + 2 * npx * q + npx, hB[q], sizeof(float) * npx, hipMemcpyHostToDevice,
+                                  c->stream));
+        }
+        c->arena_pairs = npairs;
+        c->arena_nr = nr;
+        c->arena_nc = nc;
     }
     if (nr != c->nr || nc != c->nc) c->jobcache.clear();
     c->nr = nr;

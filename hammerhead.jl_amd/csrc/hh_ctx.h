@@ -197,6 +197,7 @@ struct hh_ctx {
     hh::host::DBuf<int32_t> ensjobs;
     hh::host::DBuf<double> ensacc, ensstats;
     hh::host::DBuf<float> arena;  // the pairs of hh_run_piv_ensemble, resident in HBM
+    int arena_pairs = 0, arena_nr = 0, arena_nc = 0;
     // correlation energy and its adjoint
     hh::host::DBuf<float4> estats;
     hh::host::DBuf<float> ew, epart, egrad, enode;

@@ -172,6 +172,21 @@ def config_batch(npairs: int = 64, seed0: int = 104000, sizes=BATCH_SIZES, ppp: 
     return out
 
 
+def config_ensemble(npairs: int = 500, nr: int = 1024, nc: int = 1280, seed0: int = 104500, ppp: float = 0.005,
+                    noise: float = 0.05):
+    """The ensemble workload: npairs Float32 pairs of nr x nc at low seeding
+    (0.005 ppp) and heavy noise (sigma 0.05), all of one steady Lamb-Oseen
+    vortex (peak 6 px, core radius nc / 10) plus a uniform (1.2, -0.8) px
+    shift; pair i is seeded seed0 + i. Returns (pairs, flow)."""
+    base = lamb_oseen_flow((nc + 1) / 2.0, (nr + 1) / 2.0, nc / 10.0, 6.0)
+
+    def flow(x, y):
+        u, v = base(x, y)
+        return u + 1.2, v - 0.8
+
+    return [particle_pair(nr, nc, flow, seed0 + i, ppp=ppp, noise=noise) for i in range(npairs)], flow
+
+
 def midpoint_truth(flow, x, y, iters: int = 8):
     """Displacement whose midpoint lies at (x, y): the truth a symmetric
     deformation measures (bench/run_benchmarks.jl:199-207)."""

@@ -286,8 +286,9 @@ int hh_ensemble_planes(hh_ctx *ctx, double *planes);
 /* run_piv_ensemble(pairs, passes; predictor_smoothing)
  * (src/ensemble.jl:47-84) on the device: A[q], B[q] are host column-major
  * float32 nr x nc frames of pair q, uploaded once and kept resident in HBM
- * for every pass. The final result is copied to `result` when non-NULL, and
- * stays available to hh_fetch_result. */
+ * for every pass. With A = B = NULL the pairs of the previous call (same
+ * npairs, nr, nc) are used again without an upload. The final result is
+ * copied to `result` when non-NULL, and stays available to hh_fetch_result. */
 int hh_run_piv_ensemble(hh_ctx *ctx, const float *const *A, const float *const *B, int npairs, int nr, int nc,
                         const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result);
 

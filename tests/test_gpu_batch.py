@@ -178,3 +178,17 @@ def test_autotune_never_changes_results(hh, small_pairs, form):
             assert len(e["ms"]) >= 2 and all(v > 0 for v in e["ms"].values())
         for q in range(len(pairs)):
             assert same(rt[q], rn[q]), f"pair {q}: {diff_report(rt[q], rn[q])}"
+
+
+def test_ensemble_resident_rerun_matches(hh):
+    """run_piv_ensemble on the pairs the previous call left resident gives
+    the same result as the call that uploaded them; a count or size that does
+    not match the resident pairs is a call-order error."""
+    pairs, _ = hh.synthetic.config_ensemble(npairs=6, nr=160, nc=192)
+    p = hh.PIVParameters(window_size=32, overlap=16, padding=True, apodization="gauss")
+    with hh.Context(0) as c:
+        first = c.run_piv_ensemble([(q.A, q.B) for q in pairs], [p])
+        again = c.run_piv_ensemble_resident(len(pairs), pairs[0].A.shape, [p])
+        assert same(first, again), diff_report(first, again)
+        with pytest.raises(hh.DeviceError):
+            c.run_piv_ensemble_resident(len(pairs) + 1, pairs[0].A.shape, [p])
