@@ -138,18 +138,22 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             }
         }
     };
-    if constexpr (G == 16 && NB <= 16) {
-        // the 16 lanes of a window are one DPP row: lane t walks box column
-        // t down the rows with its 3 x 3 neighbourhood in registers (left
-        // and right columns from lanes t -/+ 1, zero past the box), one LDS
-        // read per point; neighbours earlier in column order (left column,
-        // row above) disqualify when strictly greater, later ones when >=
+    if constexpr ((G == 16 && NB <= 16) || (G == 32 && NB < 32)) {
+        // lane t walks box column t down the rows with its 3 x 3
+        // neighbourhood in registers (left and right columns from lanes
+        // t -/+ 1, zero past the box), one LDS read per point; neighbours
+        // earlier in column order (left column, row above) disqualify when
+        // strictly greater, later ones when >=. 16 lanes: DPP row shifts
+        // within the window's row; 32 lanes: wave shifts, whose reads across
+        // a window boundary land on lanes past the box, which hold zero
         const bool col = t < NB;
         auto left = [](float v) {  // lane t gets lane t - 1 (0 at t = 0)
-            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xF, 0xF, true));
+            constexpr int ctrl = G == 16 ? 0x111 : 0x138;  // row_shr:1 / wave_shr:1
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, true));
         };
-        auto right = [](float v) {  // lane t gets lane t + 1 (0 at t = 15)
-            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x101, 0xF, 0xF, true));
+        auto right = [](float v) {  // lane t gets lane t + 1
+            constexpr int ctrl = G == 16 ? 0x101 : 0x130;  // row_shl:1 / wave_shl:1
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, true));
         };
         float cp = 0.f, lp = 0.f, rp = 0.f;
         float cc = col ? Rb[t * NB] : 0.f;

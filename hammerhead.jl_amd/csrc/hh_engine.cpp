@@ -39,10 +39,13 @@ namespace {
 
 // ---- elementwise kernels over the enumerated cells of a Grids ---------------
 
-__global__ void k_gfill(float *__restrict__ p, Grids g, float v)
+__global__ void k_gfill(float *__restrict__ p, float *__restrict__ p2, Grids g, float v)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < g.total) p[cell_of(g, e).b] = v;
+    if (e >= g.total) return;
+    const int k = cell_of(g, e).b;
+    p[k] = v;
+    if (p2) p2[k] = v;
 }
 
 __global__ void k_gcopy2(const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ x,
@@ -89,10 +92,11 @@ __global__ void k_gchange(const float *__restrict__ u, const float *__restrict__
 
 dim3 cells_grid(const Grids &g) { return dim3((unsigned)((g.total + 255) / 256)); }
 
-void gfill(hh_ctx *c, float *p, const Grids &g, float v)
+// Two fields set to v in one launch (p2 nullable).
+void gfill(hh_ctx *c, float *p, float *p2, const Grids &g, float v)
 {
     if (g.total <= 0) return;
-    hipLaunchKernelGGL(k_gfill, cells_grid(g), dim3(256), 0, c->stream, p, g, v);
+    hipLaunchKernelGGL(k_gfill, cells_grid(g), dim3(256), 0, c->stream, p, p2, g, v);
     HH_HIP(hipGetLastError());
 }
 
@@ -295,7 +299,8 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
 {
     Engine &E = c->eng;
     const char *fam = correlate_family(base, uq);
-    const bool mfma = std::string(fam).rfind("mfma32", 0) == 0, reg = std::string(fam).rfind("reg", 0) == 0;
+    const bool mfma = std::string(fam).rfind("mfma", 0) == 0, reg = std::string(fam).rfind("reg", 0) == 0;
+    const bool mfma64 = std::string(fam) == "mfma64";
     CorrConfig cfg = base;
     if (!c->tune.enabled || !(mfma || reg) || P.all.njobs <= 0) return cfg;
     const std::string key = std::string(fam) + (uq && !mfma ? "+uq" : "") + " w" + std::to_string(base.w) + " M" +
@@ -305,7 +310,9 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
     if (it == c->tune.entries.end()) {
         std::vector<std::pair<int, int>> cand;
         for (int order = 0; order < 2; ++order) {
-            if (mfma)
+            if (mfma64)
+                for (int w : {2, 1}) cand.push_back({w, order});
+            else if (mfma)
                 for (int w : {3, 2, 4, 1}) cand.push_back({w, order});
             else
                 cand.push_back({0, order});
@@ -440,8 +447,7 @@ void predictor(hh_ctx *c, const Grids &g)
 {
     Engine &E = c->eng;
     if (E.smoothing) {
-        launch_smooth(g, E.u.p, E.pu.p, c->stream);
-        launch_smooth(g, E.v.p, E.pv.p, c->stream);
+        launch_smooth2(g, E.u.p, E.v.p, E.pu.p, E.pv.p, c->stream);
     } else {
         gcopy2(c, E.u.p, E.v.p, E.pu.p, E.pv.p, g);
     }
@@ -481,8 +487,11 @@ void record(hh_ctx *c, bool dynamic)
         const bool unc = p.uncertainty && !force_replace;
         const bool keep_measured = maxiter > 1 && !(force_replace || p.replace_outliers);
         const int nalt = p.n_peaks - 1;
-        gfill(c, E.uu.p, P.all.grids, NAN);
-        gfill(c, E.uv.p, P.all.grids, NAN);
+        // uncertainty grids: NaN unless this pass computes them; a fused
+        // computation writes every cell, so only the final pass's grids need
+        // the fill when the statistics are not fused (or not requested)
+        const bool last = k == np - 1;
+        if (last && !(unc && P.fixed)) gfill(c, E.uu.p, E.uv.p, P.all.grids, NAN);
         PassTables *T = &P.all;
         std::vector<int> active(L.npairs);
         std::iota(active.begin(), active.end(), 0);
@@ -501,8 +510,7 @@ void record(hh_ctx *c, bool dynamic)
                 srcB = wB;
                 launch_node_values_batch(T->grids, dt, P.r0, P.rstep, P.c0, P.cstep, P.half, E.u0.p, E.v0.p, s);
             } else {
-                gfill(c, E.u0.p, T->grids, 0.f);
-                gfill(c, E.v0.p, T->grids, 0.f);
+                gfill(c, E.u0.p, E.v0.p, T->grids, 0.f);
                 srcA = L.A;
                 srcB = L.B;
             }

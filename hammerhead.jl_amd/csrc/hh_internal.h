@@ -219,6 +219,8 @@ struct ValidateArgs {
 };
 void launch_validate(const ValidateArgs &a, hipStream_t s);
 void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s);
+// Two fields (u and v of a predictor) in one launch.
+void launch_smooth2(const Grids &g, const float *f, const float *f2, float *out, float *out2, hipStream_t s);
 
 // stats (nullable): 2 x 44 FP64 per job, overwritten, or added to when
 // accumulate (pooled statistics of the ensemble).

@@ -374,13 +374,311 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     }
 }
 
+// ============================================================================
+// 32-px windows padded to 64: the same four real GEMM stages, on 2 x 2 tiles
+// of 32 x 32 per product. One wave per window:
+//   1. D1[I][J] = X_I (rows r, K = c in 2 steps) x F1[.][J]: image I's
+//      column DFT, parts of tile J = Re k2 = 16J..16J+15 (rows 0-15) and Im of
+//      the same k2 (rows 16-31; the Im slot of k2 = 0 carries Re k2 = 32);
+//   2. per (J, k1 tile nt): D2 = D1[I][J]^T (K = r) x [cos | -sin](r, k1),
+//      Z(k1, k2) for 8 k2 per lane, the cross (or phase) spectrum P, and
+//   3. at once its contribution to Q[nt][ct] += P (K = Re/Im of k2) x table,
+//      the inverse along k2 for the 27 box columns (2 column tiles ct of
+//      (l2, Re/Im)) with the Hermitian weights in the table, so the spectrum
+//      is never held whole;
+//   4. R = Q^T (K = k1 in 4 steps) x [cos | sin](k1, l1) per column tile.
+// Constant operand tables (30 split 32 x 16 operands, 61 KB) live in LDS,
+// one copy per block; boxes of 2 windows are analyzed together, 32 lanes
+// per window.
+struct Trig64 {
+    float c[64], s[64];  // cos, sin of 2 pi t / 64
+};
+
+constexpr int kHW64 = 13;                 // box half-width (Gaussian apodization)
+constexpr int kNB64 = 2 * kHW64 + 1;      // 27 box rows and columns
+constexpr int kS4_64 = -8;                // stage-4 operand scale (66 terms of |P| < 2^14)
+enum {
+    T64_F1 = 0,    // [kk][J]
+    T64_WC = 4,    // [kk][nt]
+    T64_WS = 8,    // [kk][nt]
+    T64_T3R = 12,  // [J][ct]
+    T64_T3I = 16,  // [J][ct]
+    T64_T3N = 20,  // [ct]
+    T64_C4 = 22,   // [nt][kk]
+    T64_S4 = 26,   // [nt][kk]
+    T64_N = 30
+};
+
+// Operand table q for lane l (slot s of half h holds K index k(h, s)).
+__device__ void build_table64(int q, int l, const Trig64 &tg, Split &out)
+{
+    const int h = l >> 5, n = l & 31;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        float f = 0.f;
+        if (q < T64_WC) {  // F1[kk][J]: K = c, column = part n of tile J
+            const int kk = (q - T64_F1) >> 1, J = (q - T64_F1) & 1;
+            const int c = 16 * kk + 8 * h + s;
+            if (n < 16) {
+                f = tg.c[((16 * J + n) * c) & 63];
+            } else {
+                const int k2 = 16 * J + n - 16;
+                f = k2 == 0 ? ((c & 1) ? -1.f : 1.f) : -tg.s[(k2 * c) & 63];
+            }
+        } else if (q < T64_T3R) {  // Wc / Ws[kk][nt]: K = r, column k1
+            const bool sn = q >= T64_WS;
+            const int i = q - (sn ? T64_WS : T64_WC);
+            const int kk = i >> 1, nt = i & 1;
+            const int t = ((32 * nt + n) * (16 * kk + rmap(h, s))) & 63;
+            f = sn ? -tg.s[t] : tg.c[t];
+        } else if (q < T64_T3N) {  // T3re / T3im[J][ct]: K = k2, column (b, Re | Im)
+            const bool imk = q >= T64_T3I;
+            const int i = q - (imk ? T64_T3I : T64_T3R);
+            const int J = i >> 1, ct = i & 1;
+            const int k2 = 16 * J + rmap(h, s);
+            const int b = 16 * ct + (n & 15), imc = n >> 4;
+            if (b < kNB64) {
+                const float cw = k2 == 0 ? 1.f : 2.f;
+                const int t = (k2 * (b - kHW64)) & 63;
+                f = imk ? (imc ? cw * tg.c[t] : -cw * tg.s[t]) : (imc ? cw * tg.s[t] : cw * tg.c[t]);
+            }
+        } else if (q < T64_C4) {  // T3n[ct]: Re / Im of P(k1, 32) in slots 0 / 1 of half 0
+            const int ct = q - T64_T3N;
+            const int b = 16 * ct + (n & 15), imc = n >> 4;
+            if (b < kNB64 && h == 0 && s < 2) {
+                const int t = (32 * (b - kHW64)) & 63;
+                f = s == 0 ? (imc ? tg.s[t] : tg.c[t]) : (imc ? tg.c[t] : -tg.s[t]);
+            }
+        } else {  // C4 / S4[nt][kk]: K = k1, column l1
+            const bool sn = q >= T64_S4;
+            const int i = q - (sn ? T64_S4 : T64_C4);
+            const int nt = i >> 1, kk = i & 1;
+            if (n < kNB64) {
+                const int t = ((32 * nt + 16 * kk + rmap(h, s)) * (n - kHW64)) & 63;
+                f = sn ? tg.s[t] : tg.c[t];
+            }
+        }
+        put(out, s, f);
+    }
+}
+
+// WV waves per block, one block per CU: WV = 8 keeps two waves per SIMD
+// within 256 registers (some spill), WV = 4 one wave per SIMD with the whole
+// register file (no spill); the autotuner picks.
+template <int K, bool PHASE, int WV>
+__global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4, WV / 4))) void k_corr_mfma64(
+    CorrConfig cfg, Trig64 tg, const float *__restrict__ A, const float *__restrict__ B, int nr,
+    const int32_t *__restrict__ jobs, int njobs, int chunk, CorrOut out)
+{
+#pragma clang fp contract(off)
+    constexpr int M = 64, H = 32, LO = H - kHW64;
+    __shared__ Split tabs[T64_N][64];
+    __shared__ float rbox[WV][2][kNB64 * kNB64 + 1];
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int h = l >> 5, n = l & 31;
+    for (int q = wave; q < T64_N; q += WV) {
+        Split t;
+        build_table64(q, l, tg, t);
+        tabs[q][l] = t;
+    }
+    __syncthreads();
+    const f16v zero = {};
+    // apodization of this lane's pixels (row n, columns 16 kk + 8 h + s), read
+    // per window from the cache rather than held in registers
+    const float *__restrict__ apodl = cfg.apod + n + 32 * 8 * h;
+
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int blk = cfg.linear_order ? b : xcd * q8 + min(xcd, r8) + (b >> 3);
+    const int gw = blk * WV + wave;
+    const int w0 = gw * chunk;
+    const int w1 = min(njobs, w0 + chunk);
+    if (w0 >= w1) return;
+    const int4 *jobs4 = reinterpret_cast<const int4 *>(jobs);
+    for (int g = w0; g < w1; g += 2) {
+        const int ng = min(2, w1 - g);
+        for (int j = 0; j < ng; ++j) {
+            const int win = g + j;
+            const int4 jw = jobs4[win];
+            float *Rb = rbox[wave][j];
+            int z;
+            asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+            const int tl = l + z;
+            // ---- 0. both windows: lane (h, n) holds row n, columns 16 kk + 8 h + s
+            float px[2][16];
+#pragma unroll
+            for (int I = 0; I < 2; ++I) {
+                const float *p = (I ? B : A) + (size_t)(jw.w - 1 + 8 * h) * nr + (jw.z - 1 + n);
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) px[I][8 * kk + s] = p[(size_t)(16 * kk + s) * nr];
+            }
+            Split a1[2][2];
+            int eI[2];
+#pragma unroll
+            for (int I = 0; I < 2; ++I) {
+                float sum = 0.f, mx = px[I][0], mn = px[I][0];
+#pragma unroll
+                for (int s = 0; s < 16; ++s) {
+                    sum += px[I][s];
+                    mx = fmaxf(mx, px[I][s]);
+                    mn = fminf(mn, px[I][s]);
+                }
+                img_reduce(sum, mx, mn);  // DPP row (16 lanes) and lane + 32
+                sum += __shfl_xor(sum, 16);   // and the other row of the half: all 64 lanes
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mn = fminf(mn, __shfl_xor(mn, 16));
+                const float mean = sum / 1024.f;
+                const float bound = mx - mn;
+                int ex = 0;
+                (void)frexpf(bound, &ex);
+                const int e0 = bound > 0.f ? -3 - ex : 0;  // |x| < 2^-3
+                eI[I] = e0;
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                    for (int s = 0; s < 8; s += 2)
+                        put2(a1[I][kk], s, ldexpf(apodl[32 * (16 * kk + s)] * (px[I][8 * kk + s] - mean), e0),
+                             ldexpf(apodl[32 * (16 * kk + s + 1)] * (px[I][8 * kk + s + 1] - mean), e0));
+            }
+            const int es = eI[0] + eI[1];
+            f16v q3[2][2];  // Q[nt][ct]
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) q3[nt][ct] = zero;
+#pragma unroll
+            for (int J = 0; J < 2; ++J) {
+                // ---- 1. column DFT of both windows, parts of tile J ----
+                Split a2[2][2];
+#pragma unroll
+                for (int I = 0; I < 2; ++I) {
+                    f16v d1 = mfma3(a1[I][0], tabs[T64_F1 + 0 * 2 + J][tl], zero);
+                    d1 = mfma3(a1[I][1], tabs[T64_F1 + 1 * 2 + J][tl], d1);
+#pragma unroll
+                    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                        for (int s = 0; s < 8; s += 2) put2(a2[I][kk], s, d1[8 * kk + s], d1[8 * kk + s + 1]);
+                }
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) {
+                    // ---- 2. row DFT: Z(k1 = 32 nt + n, k2 = 16 J + rmap(h, v)) ----
+                    float zr[2][9], zi[2][9];
+#pragma unroll
+                    for (int I = 0; I < 2; ++I) {
+                        // one frame at a time keeps two products live, not four
+                        f16v dc = mfma3(a2[I][0], tabs[T64_WC + 0 * 2 + nt][tl], zero);
+                        dc = mfma3(a2[I][1], tabs[T64_WC + 1 * 2 + nt][tl], dc);
+                        f16v ds = mfma3(a2[I][0], tabs[T64_WS + 0 * 2 + nt][tl], zero);
+                        ds = mfma3(a2[I][1], tabs[T64_WS + 1 * 2 + nt][tl], ds);
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) {
+                            zr[I][v] = dc[v] - ds[v + 8];
+                            zi[I][v] = dc[v + 8] + ds[v];
+                        }
+                        if (J == 0 && h == 0) {  // k2 = 0 (no Im part) and k2 = 32 in its slot
+                            zr[I][0] = dc[0];
+                            zi[I][0] = ds[0];
+                        }
+                        zr[I][8] = dc[8];
+                        zi[I][8] = ds[8];
+                    }
+                    // ---- 3. spectrum; inverse along k2 into Q[nt][.] ----
+                    float sp_r[9], sp_i[9];
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) {
+                        float pr = zr[0][v] * zr[1][v] + zi[0][v] * zi[1][v];
+                        float pi = zr[0][v] * zi[1][v] - zi[0][v] * zr[1][v];
+                        if constexpr (PHASE) {
+                            const int k2 = v < 8 ? 16 * J + rmap(h, v) : 32;
+                            pr = ldexpf(pr, -es);
+                            pi = ldexpf(pi, -es);
+                            const float den = hypotf(pr, pi) + FLT_EPSILON;
+                            const float wv = cfg.wfilt[(32 * nt + n) + k2 * M];
+                            pr = ldexpf(wv * pr / den, kS3P);
+                            pi = ldexpf(wv * pi / den, kS3P);
+                        }
+                        sp_r[v] = pr;
+                        sp_i[v] = pi;
+                    }
+                    Split a3r, a3i;
+#pragma unroll
+                    for (int s = 0; s < 8; s += 2) {
+                        put2(a3r, s, sp_r[s], sp_r[s + 1]);
+                        put2(a3i, s, sp_i[s], sp_i[s + 1]);
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        q3[nt][ct] = mfma3(a3r, tabs[T64_T3R + J * 2 + ct][tl], q3[nt][ct]);
+                        q3[nt][ct] = mfma3(a3i, tabs[T64_T3I + J * 2 + ct][tl], q3[nt][ct]);
+                    }
+                    if (J == 0) {  // the Nyquist column k2 = 32 (half 0, slots 0 / 1)
+                        Split a3n;
+                        const bool k32 = h == 0;
+                        put2(a3n, 0, k32 ? sp_r[8] : 0.f, k32 ? sp_i[8] : 0.f);
+#pragma unroll
+                        for (int s = 2; s < 8; ++s) {
+                            a3n.hi[s] = (_Float16)0.f;
+                            a3n.lo[s] = (_Float16)0.f;
+                        }
+#pragma unroll
+                        for (int ct = 0; ct < 2; ++ct)
+                            q3[nt][ct] = mfma3(a3n, tabs[T64_T3N + ct][tl], q3[nt][ct]);
+                    }
+                }
+            }
+            // ---- 4. inverse along k1 for the box rows, per column tile ----
+            const int etot = (PHASE ? kS3P : es) + kS4_64;
+            const float scale = ldexpf(1.f / (float)(M * M), -etot);
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                f16v oc = zero, os = zero;
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                    for (int kk = 0; kk < 2; ++kk) {
+                        Split a4;
+#pragma unroll
+                        for (int s = 0; s < 8; s += 2)
+                            put2(a4, s, ldexpf(q3[nt][ct][8 * kk + s], kS4_64),
+                                 ldexpf(q3[nt][ct][8 * kk + s + 1], kS4_64));
+                        oc = mfma3(a4, tabs[T64_C4 + nt * 2 + kk][tl], oc);
+                        os = mfma3(a4, tabs[T64_S4 + nt * 2 + kk][tl], os);
+                    }
+                // R(l1 = n - HW, l2 = b - HW) = oc[Re row] - os[Im row]
+                if (n < kNB64) {
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+                        const int bb = 16 * ct + rmap(h, v);
+                        if (bb < kNB64) {
+                            const float rv = (oc[v] - os[v + 8]) * scale;
+                            Rb[n + bb * kNB64] = fabsf(rv) * cfg.gain[(LO + n) + (LO + bb) * M];
+                        }
+                    }
+                }
+            }
+        }
+        wave_sync();
+        const int slot = l >> 5;
+        const int jb = g + min(slot, ng - 1);
+        const int4 jj = jobs4[jb];
+        float sg = NAN;
+        analyze_box<M, K, kNB64, LO, 32, 1, false>(cfg, out, rbox[wave][slot], l & 31, l, nullptr, nullptr,
+                                                    slot < ng, jb, jj.x, jj.y, sg, sg);
+        wave_sync();
+    }
+}
+
 }  // namespace
 
-// covered: 16-px windows padded to 32 with the Gaussian support box
+// covered: 16- and 32-px windows padded to 32 / 64 with the Gaussian support box
 bool mfma_covers(const CorrConfig &cfg)
 {
-    return cfg.M == 32 && cfg.w == 16 && cfg.padded && cfg.gain && cfg.finder == HH_REGIONALMAX && cfg.K <= 3 &&
-           cfg.box_lo == 16 - kHW && cfg.box_hi == 16 + kHW;
+    if (!cfg.padded || !cfg.gain || cfg.finder != HH_REGIONALMAX || cfg.K > 3) return false;
+    if (cfg.M == 32 && cfg.w == 16) return cfg.box_lo == 16 - kHW && cfg.box_hi == 16 + kHW;
+    if (cfg.M == 64 && cfg.w == 32) return cfg.box_lo == 32 - kHW64 && cfg.box_hi == 32 + kHW64;
+    return false;
 }
 
 int device_cu_count()
@@ -401,6 +699,42 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
     if (!mfma_covers(cfg)) return false;
     if (out.planes || out.acc || out.unc_u) return false;
     if (njobs <= 0) return true;
+    const int ncu = device_cu_count();
+    if (cfg.M == 64) {
+        static const Trig64 tg64 = [] {
+            Trig64 t;
+            for (int k = 0; k < 64; ++k) {
+                t.c[k] = (float)std::cos(2.0 * M_PI * k / 64.0);
+                t.s[k] = (float)std::sin(2.0 * M_PI * k / 64.0);
+            }
+            return t;
+        }();
+        // one block per CU (tables and boxes take 84-108 KB of LDS):
+        // waves_per_simd 1 -> 4-wave blocks, 2 (default) -> 8-wave blocks;
+        // persistent waves walk contiguous window ranges
+        const int wps = cfg.waves_per_simd == 1 ? 1 : 2;
+        const int wv = 4 * wps;
+        const int nblocks = std::max(1, std::min((njobs + wv - 1) / wv, ncu));
+        const int chunk = (njobs + nblocks * wv - 1) / (nblocks * wv);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * wv), 0, s, cfg, tg64, A, B, nr, jobs, njobs, chunk,
+                               out);
+        };
+        const bool ph = cfg.method == HH_PHASE, k2 = cfg.K == 2;
+        if (wps == 1) {
+            if (ph)
+                k2 ? go(k_corr_mfma64<2, true, 4>) : go(k_corr_mfma64<3, true, 4>);
+            else
+                k2 ? go(k_corr_mfma64<2, false, 4>) : go(k_corr_mfma64<3, false, 4>);
+        } else {
+            if (ph)
+                k2 ? go(k_corr_mfma64<2, true, 8>) : go(k_corr_mfma64<3, true, 8>);
+            else
+                k2 ? go(k_corr_mfma64<2, false, 8>) : go(k_corr_mfma64<3, false, 8>);
+        }
+        HH_HIP(hipGetLastError());
+        return true;
+    }
     static Trig tg = [] {
         Trig t;
         for (int k = 0; k < 32; ++k) {
@@ -409,7 +743,6 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
         }
         return t;
     }();
-    const int ncu = device_cu_count();
     const int wps = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 3;
     const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 4 * wps));
     const int nblocks = (waves + 3) / 4;

@@ -547,7 +547,8 @@ bool launch_uq16_rows(const CorrConfig &cfg, const float *A, const float *B, int
 
 const char *correlate_family(const CorrConfig &cfg, bool uq)
 {
-    if (cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg)) return uq ? "mfma32+rows" : "mfma32";
+    if (cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg) && !(uq && cfg.w != 16))
+        return cfg.M == 64 ? "mfma64" : (uq ? "mfma32+rows" : "mfma32");
     const char *f = fast_family(cfg);
     return f ? f : "lds";
 }
@@ -564,7 +565,8 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     (void)nc;
     if (njobs <= 0) return out.unc_u != nullptr;
     bool uq_done = false;
-    if (cfg.form == HH_FORM_MFMA16 && out.unc_u && out.unc_v && !out.planes && !out.acc && mfma_covers(cfg)) {
+    if (cfg.form == HH_FORM_MFMA16 && out.unc_u && out.unc_v && !out.planes && !out.acc && mfma_covers(cfg) &&
+        cfg.w == 16) {
         // the sweep with uncertainty: matrix-core correlation, then the
         // statistics of the same windows by the row kernel
         CorrOut o = out;

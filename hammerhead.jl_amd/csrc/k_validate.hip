@@ -380,20 +380,24 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
     }
 }
 
-__global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f_all, Grids g, float *__restrict__ out)
+// smooth_field of one or two fields (g2/out2 nullable: u and v of a
+// predictor in one launch).
+__global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f_all, const float *__restrict__ f2_all,
+                                                Grids g, float *__restrict__ out, float *__restrict__ out2)
 {
     const int ee = blockIdx.x * blockDim.x + threadIdx.x;
     if (ee >= g.total) return;
     const Cell k = cell_of(g, ee);
     const int e = k.b, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
-    const float *__restrict__ f = f_all + k.sg.off;
     const int cl = c > 0 ? c - 1 : 0, cr = c < nx - 1 ? c + 1 : nx - 1;
     const int rl = r > 0 ? r - 1 : 0, rr = r < ny - 1 ? r + 1 : ny - 1;
     const float q = 0.25f;
-    auto tmp = [&](int rw) {
-        return q * ((f[rw + cl * ny] + 2.f * f[rw + c * ny]) + f[rw + cr * ny]);
+    auto sm = [&](const float *__restrict__ f) {
+        auto tmp = [&](int rw) { return q * ((f[rw + cl * ny] + 2.f * f[rw + c * ny]) + f[rw + cr * ny]); };
+        return q * ((tmp(rl) + 2.f * tmp(r)) + tmp(rr));
     };
-    out[e] = q * ((tmp(rl) + 2.f * tmp(r)) + tmp(rr));
+    out[e] = sm(f_all + k.sg.off);
+    if (f2_all) out2[e] = sm(f2_all + k.sg.off);
 }
 
 __global__ void k_fill_u8(uint8_t *p, int n, uint8_t v)
@@ -511,7 +515,16 @@ void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s)
 {
     const int n = g.total;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_smooth, dim3((n + 255) / 256), dim3(256), 0, s, f, g, out);
+    hipLaunchKernelGGL(k_smooth, dim3((n + 255) / 256), dim3(256), 0, s, f, (const float *)nullptr, g, out,
+                       (float *)nullptr);
+    HH_HIP(hipGetLastError());
+}
+
+void launch_smooth2(const Grids &g, const float *f, const float *f2, float *out, float *out2, hipStream_t s)
+{
+    const int n = g.total;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_smooth, dim3((n + 255) / 256), dim3(256), 0, s, f, f2, g, out, out2);
     HH_HIP(hipGetLastError());
 }
 

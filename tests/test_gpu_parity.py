@@ -439,13 +439,15 @@ def ctx_mfma(hh):
 
 
 @pytest.mark.parametrize("method", ["cross", "phase"])
-def test_mfma_correlate_vs_oracle(hh, orc, ctx_mfma, c1, method):
-    # split-FP16 DFT products on the matrix cores, 16-px windows padded to 32
+@pytest.mark.parametrize("w", [16, 32])
+def test_mfma_correlate_vs_oracle(hh, orc, ctx_mfma, c1, method, w):
+    # split-FP16 DFT products on the matrix cores, 16- and 32-px windows
+    # padded to 32 / 64
     A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
     ctx_mfma.set_pair(A, B)
-    for ov in (8, 12):
-        p = P(hh, window_size=16, overlap=ov, padding=True, apodization="gauss", correlation_method=method)
-        say("mfma correlate", method, ov)
+    for ov in ((8, 12) if w == 16 else (16, 24)):
+        p = P(hh, window_size=w, overlap=ov, padding=True, apodization="gauss", correlation_method=method)
+        say("mfma correlate", method, w, ov)
         s, g, o = compare_windows(orc, hh, ctx_mfma, A.astype(np.float64), B.astype(np.float64), p)
         say(s)
         assert s["peak_mismatch"] == s["near_ties"], s
