@@ -22,6 +22,49 @@ __device__ __forceinline__ bool better(float av, int al, float bv, int bl)
     return av > bv || (av == bv && al < bl);
 }
 
+// A candidate (value > 0, location >= 0) as one integer whose order is that
+// of better(): value bits (monotonic for positive floats) above the
+// complement of the location; an empty slot (INT_MAX) is 0.
+__device__ __forceinline__ unsigned long long cand_key(float v, int l)
+{
+    return l == INT_MAX ? 0ull : ((unsigned long long)__float_as_uint(v) << 32) | (0xFFFFFFFFu - (unsigned)l);
+}
+
+template <int R>
+__device__ __forceinline__ unsigned long long key_ror(unsigned long long k)
+{
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)k, 0x120 + R, 0xF, 0xF, true);   // row_ror:R
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(k >> 32), 0x120 + R, 0xF, 0xF, true);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+__device__ __forceinline__ unsigned long long key_xor(unsigned long long k, int o)
+{
+    const int lo = __shfl_xor((int)(unsigned)k, o), hi = __shfl_xor((int)(unsigned)(k >> 32), o);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// Maximum key over the G lanes of a window (groups of G lanes aligned to G;
+// at most the 64 lanes of a wave): rotations within each 16-lane DPP row
+// reduce the row, exchanges across rows finish it.
+template <int G>
+__device__ __forceinline__ unsigned long long cand_key_max(unsigned long long k)
+{
+    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+    if constexpr (G >= 16) {
+        k = mx(k, key_ror<8>(k));
+        k = mx(k, key_ror<4>(k));
+        k = mx(k, key_ror<2>(k));
+        k = mx(k, key_ror<1>(k));
+        if constexpr (G >= 32) k = mx(k, key_xor(k, 16));
+        if constexpr (G >= 64) k = mx(k, key_xor(k, 32));
+    } else {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) k = mx(k, key_xor(k, o));
+    }
+    return k;
+}
+
 // Box plane accessor: plane (0-based, fftshifted) -> value; 0 outside box.
 template <int NB, int LO>
 struct BoxView {
@@ -125,17 +168,24 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
         tv[m] = 0.f;
         tl[m] = INT_MAX;
     }
+    // A lane meets its candidates in increasing location order, so a new
+    // candidate goes before the first entry of smaller value (an equal value
+    // arrived earlier, at the smaller location, and stays ahead) and every
+    // entry from there on moves down one slot; an empty slot (0, INT_MAX)
+    // loses to any candidate (> 0). A value of 0 inserts nothing, so the
+    // column walk inserts without branching.
     auto insert = [&](float cv, int cl) {
+        bool moved = false;
 #pragma unroll
         for (int m = 0; m < K; ++m) {
-            if (better(cv, cl, tv[m], tl[m])) {
-                const float sv_ = tv[m];
-                const int sl_ = tl[m];
-                tv[m] = cv;
-                tl[m] = cl;
-                cv = sv_;
-                cl = sl_;
-            }
+            const bool c = moved || cv > tv[m];
+            moved = c;
+            const float sv_ = tv[m];
+            const int sl_ = tl[m];
+            tv[m] = c ? cv : sv_;
+            tl[m] = c ? cl : sl_;
+            cv = c ? sv_ : cv;
+            cl = c ? sl_ : cl;
         }
     };
     if constexpr ((G == 16 && NB <= 16) || (G == 32 && NB < 32)) {
@@ -162,9 +212,12 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             const float cn = (col && bi + 1 < NB) ? Rb[bi + 1 + t * NB] : 0.f;
             const float ln = left(cn), rn = right(cn);
             const float I0 = cc;
-            const bool ok = I0 > 0.f && !(lp > I0) && !(lc > I0) && !(ln > I0) && !(cp > I0) && !(cn >= I0) &&
-                            !(rp >= I0) && !(rc >= I0) && !(rn >= I0);
-            if (ok) insert(I0, (LO + bi) + (LO + t) * M);
+            // fmaxf skips NaN neighbours exactly as !(NaN > I0) and
+            // !(NaN >= I0) pass them
+            const float me = fmaxf(fmaxf(lp, lc), fmaxf(ln, cp));  // earlier: disqualify when >
+            const float ml_ = fmaxf(fmaxf(cn, rp), fmaxf(rc, rn));  // later: disqualify when >=
+            const bool ok = I0 > 0.f && !(me > I0) && !(ml_ >= I0);
+            insert(ok ? I0 : 0.f, (LO + bi) + (LO + t) * M);
             cp = cc;
             cc = cn;
             lp = lc;
@@ -197,18 +250,9 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
     int nsel = 0;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-        float bv = tv[0];
-        int bl = tl[0];
-        constexpr int XMAX = G >= 64 ? 32 : G / 2;
-#pragma unroll
-        for (int o = XMAX; o > 0; o >>= 1) {
-            const float ov = __shfl_xor(bv, o);
-            const int ol = __shfl_xor(bl, o);
-            if (better(ov, ol, bv, bl)) {
-                bv = ov;
-                bl = ol;
-            }
-        }
+        const unsigned long long best = cand_key_max<G>(cand_key(tv[0], tl[0]));
+        float bv = best ? __uint_as_float((unsigned)(best >> 32)) : 0.f;
+        int bl = best ? (int)(0xFFFFFFFFu - (unsigned)best) : INT_MAX;
         if constexpr (NWAVE > 1) {
             if ((tid & 63) == 0) {
                 mv[wave] = bv;

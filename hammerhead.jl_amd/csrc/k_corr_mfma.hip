@@ -64,17 +64,30 @@ __device__ __forceinline__ void put(Split &a, int s, float x)
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 
-// Slots s, s + 1 (s even) from two values: one packed conversion per part.
-__device__ __forceinline__ void put2(Split &a, int s, float x0, float x1)
+// Slots s, s + 1 (s even) from two values: hi by one packed conversion,
+// the residuals x - f32(hi) (exact in FP32) by mixed-precision FMAs that
+// read hi as FP16, lo by one packed conversion. neg1 is -1 held in a
+// register the compiler cannot see through, so x + hi * neg1 stays an FMA
+// (a constant -1 would fold it into a subtraction of a converted hi).
+__device__ __forceinline__ void put2(Split &a, int s, float x0, float x1, float neg1)
 {
     const f2v x = {x0, x1};
     const h2v hi = __builtin_convertvector(x, h2v);
-    const f2v r = x - __builtin_convertvector(hi, f2v);
+    const f2v r = {__builtin_fmaf((float)hi[0], neg1, x0), __builtin_fmaf((float)hi[1], neg1, x1)};
     const h2v lo = __builtin_convertvector(r, h2v);
     a.hi[s] = hi[0];
     a.hi[s + 1] = hi[1];
     a.lo[s] = lo[0];
     a.lo[s + 1] = lo[1];
+}
+
+// -1 from a register move the compiler cannot fold (read by ordinary VALU
+// instructions only, which wait for it in hardware).
+__device__ __forceinline__ float opaque_neg1()
+{
+    float v;
+    asm volatile("v_mov_b32 %0, -1.0" : "=v"(v));
+    return v;
 }
 
 __device__ __forceinline__ f16v mfma3(const Split &a, const Split &b, f16v acc)
@@ -200,6 +213,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     }
     __syncthreads();
     const f16v zero = {};
+    const float neg1 = opaque_neg1();
     const int img = n >> 4, row = n & 15;
     const float *src = img ? B : A;
     float apod8[8];
@@ -268,7 +282,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         const int e0 = bound > 0.f ? -1 - ex : 0;
         Split a1;
 #pragma unroll
-        for (int s = 0; s < 8; s += 2) put2(a1, s, ldexpf(x[s], e0), ldexpf(x[s + 1], e0));
+        for (int s = 0; s < 8; s += 2) put2(a1, s, ldexpf(x[s], e0), ldexpf(x[s + 1], e0), neg1);
         const int eA = __shfl(e0, 0), eB = __shfl(e0, 16);
 
         // ---- 1. column DFT of both windows ----
@@ -280,7 +294,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         for (int I = 0; I < 2; ++I) {
             Split a2;
 #pragma unroll
-            for (int s = 0; s < 8; s += 2) put2(a2, s, d1[8 * I + s], d1[8 * I + s + 1]);
+            for (int s = 0; s < 8; s += 2) put2(a2, s, d1[8 * I + s], d1[8 * I + s + 1], neg1);
             const f16v dr = mfma3(a2, tabs[1][tl], zero);
             const f16v di = mfma3(a2, tabs[2][tl], zero);
 #pragma unroll
@@ -321,12 +335,12 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         }
 #pragma unroll
         for (int s = 0; s < 8; s += 2) {
-            put2(a3[0], s, sp_r[s], sp_r[s + 1]);
-            put2(a3[1], s, sp_i[s], sp_i[s + 1]);
+            put2(a3[0], s, sp_r[s], sp_r[s + 1], neg1);
+            put2(a3[1], s, sp_i[s], sp_i[s + 1], neg1);
         }
         {
             const bool k16 = h == 0;
-            put2(a3[2], 0, k16 ? sp_r[8] : 0.f, k16 ? sp_i[8] : 0.f);
+            put2(a3[2], 0, k16 ? sp_r[8] : 0.f, k16 ? sp_i[8] : 0.f, neg1);
 #pragma unroll
             for (int s = 2; s < 8; ++s) {
                 a3[2].hi[s] = (_Float16)0.f;
@@ -343,7 +357,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int s = 0; s < 8; s += 2)
-                put2(a4[q], s, ldexpf(d3[8 * q + s], kS4), ldexpf(d3[8 * q + s + 1], kS4));
+                put2(a4[q], s, ldexpf(d3[8 * q + s], kS4), ldexpf(d3[8 * q + s + 1], kS4), neg1);
         f16v oc = mfma3(a4[0], tabs[6][tl], zero);
         oc = mfma3(a4[1], tabs[7][tl], oc);
         f16v os = mfma3(a4[0], tabs[8][tl], zero);
@@ -483,6 +497,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
     }
     __syncthreads();
     const f16v zero = {};
+    const float neg1 = opaque_neg1();
     // apodization of this lane's pixels (row n, columns 16 kk + 8 h + s), read
     // per window from the cache rather than held in registers
     const float *__restrict__ apodl = cfg.apod + n + 32 * 8 * h;
@@ -540,7 +555,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
 #pragma unroll
                     for (int s = 0; s < 8; s += 2)
                         put2(a1[I][kk], s, ldexpf(apodl[32 * (16 * kk + s)] * (px[I][8 * kk + s] - mean), e0),
-                             ldexpf(apodl[32 * (16 * kk + s + 1)] * (px[I][8 * kk + s + 1] - mean), e0));
+                             ldexpf(apodl[32 * (16 * kk + s + 1)] * (px[I][8 * kk + s + 1] - mean), e0), neg1);
             }
             const int es = eI[0] + eI[1];
             f16v q3[2][2];  // Q[nt][ct]
@@ -559,7 +574,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
 #pragma unroll
                     for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-                        for (int s = 0; s < 8; s += 2) put2(a2[I][kk], s, d1[8 * kk + s], d1[8 * kk + s + 1]);
+                        for (int s = 0; s < 8; s += 2) put2(a2[I][kk], s, d1[8 * kk + s], d1[8 * kk + s + 1], neg1);
                 }
 #pragma unroll
                 for (int nt = 0; nt < 2; ++nt) {
@@ -605,8 +620,8 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                     Split a3r, a3i;
 #pragma unroll
                     for (int s = 0; s < 8; s += 2) {
-                        put2(a3r, s, sp_r[s], sp_r[s + 1]);
-                        put2(a3i, s, sp_i[s], sp_i[s + 1]);
+                        put2(a3r, s, sp_r[s], sp_r[s + 1], neg1);
+                        put2(a3i, s, sp_i[s], sp_i[s + 1], neg1);
                     }
 #pragma unroll
                     for (int ct = 0; ct < 2; ++ct) {
@@ -616,7 +631,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                     if (J == 0) {  // the Nyquist column k2 = 32 (half 0, slots 0 / 1)
                         Split a3n;
                         const bool k32 = h == 0;
-                        put2(a3n, 0, k32 ? sp_r[8] : 0.f, k32 ? sp_i[8] : 0.f);
+                        put2(a3n, 0, k32 ? sp_r[8] : 0.f, k32 ? sp_i[8] : 0.f, neg1);
 #pragma unroll
                         for (int s = 2; s < 8; ++s) {
                             a3n.hi[s] = (_Float16)0.f;
@@ -642,7 +657,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
 #pragma unroll
                         for (int s = 0; s < 8; s += 2)
                             put2(a4, s, ldexpf(q3[nt][ct][8 * kk + s], kS4_64),
-                                 ldexpf(q3[nt][ct][8 * kk + s + 1], kS4_64));
+                                 ldexpf(q3[nt][ct][8 * kk + s + 1], kS4_64), neg1);
                         oc = mfma3(a4, tabs[T64_C4 + nt * 2 + kk][tl], oc);
                         os = mfma3(a4, tabs[T64_S4 + nt * 2 + kk][tl], os);
                     }

@@ -31,30 +31,75 @@ namespace {
 
 #pragma clang fp contract(off)
 
-// Odd-even transposition sort of N values in registers (fully unrolled).
-template <typename T, int N>
-__device__ __forceinline__ void sort_net(T (&x)[N])
+// Sort of N values in registers by Batcher's merge exchange (Knuth, TAOCP
+// vol. 3, 5.2.2 algorithm M): ceil(log2 N) merge rounds of comparators
+// (i, i + d) on the indices with (i & p) == r, 127 comparators for N = 24
+// against 288 for transposition, each one min and one max. The loops have
+// constant bounds and unroll into straight-line code; outputs nobody reads
+// (the upper half, for a median) are pruned by the compiler. Values are
+// ordered as by <; callers detect NaN separately, so the NaN rules of
+// min / max never reach a result.
+template <int N>
+constexpr int ceil_log2()
+{
+    int t = 0;
+    while ((1 << t) < N) ++t;
+    return t;
+}
+
+__device__ __forceinline__ float vmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return fmax(a, b); }
+
+template <typename T, int N, int P, int D, int R>
+__device__ __forceinline__ void merge_pass(T (&x)[N])
 {
 #pragma unroll
-    for (int round = 0; round < N; ++round) {
-#pragma unroll
-        for (int i = round & 1; i + 1 < N; i += 2) {
-            const T a = x[i], b = x[i + 1];
-            x[i] = a < b ? a : b;
-            x[i + 1] = a < b ? b : a;
+    for (int i = 0; i + D < N; ++i)
+        if ((i & P) == R) {
+            const T a = x[i], b = x[i + D];
+            x[i] = vmin(a, b);
+            x[i + D] = vmax(a, b);
         }
+}
+
+// passes (d = q - p, r = p) for q = 2^QI, 2^(QI-1), ..., 2p
+template <typename T, int N, int PI, int QI>
+__device__ __forceinline__ void merge_tail(T (&x)[N])
+{
+    if constexpr (QI > PI) {
+        merge_pass<T, N, (1 << PI), (1 << QI) - (1 << PI), (1 << PI)>(x);
+        merge_tail<T, N, PI, QI - 1>(x);
     }
 }
 
+template <typename T, int N, int PI>
+__device__ __forceinline__ void merge_round(T (&x)[N])
+{
+    if constexpr (PI >= 0) {
+        constexpr int t = ceil_log2<N>();
+        merge_pass<T, N, (1 << PI), (1 << PI), 0>(x);
+        merge_tail<T, N, PI, t - 1>(x);
+        merge_round<T, N, PI - 1>(x);
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void sort_net(T (&x)[N])
+{
+    merge_round<T, N, ceil_log2<N>() - 1>(x);
+}
+
 // median! of the first n of N sorted-with-padding registers (padding = +inf
-// placed after the valid values). hasnan forces NaN.
+// placed after the valid values, n <= N). hasnan forces NaN.
 template <typename T, int N>
 __device__ __forceinline__ T median_sorted(const T (&x)[N], int n, bool hasnan)
 {
     if (hasnan) return (T)NAN;
     T lo = 0, hi = 0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
+    for (int i = 0; i <= N / 2; ++i) {
         if (i == (n - 1) / 2) lo = x[i];
         if (i == n / 2) hi = x[i];
     }
@@ -92,9 +137,12 @@ __global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const 
         const int lin = q < N / 2 ? q : q + 1;
         const int dc = lin / W - NB, dr = lin % W - NB;
         const int r2 = r + dr, c2 = c + dc;
-        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !(xg && xg[r2 + c2 * ny]);
-        nu[q] = ok[q] ? ug[r2 + c2 * ny] : INFINITY;
-        nv[q] = ok[q] ? vg[r2 + c2 * ny] : INFINITY;
+        // clamped addresses: every load is in bounds and unconditional
+        const int rc = min(max(r2, 0), ny - 1), cc = min(max(c2, 0), nx - 1);
+        const int at = rc + cc * ny;
+        ok[q] = r2 == rc && c2 == cc && !(xg && xg[at]);
+        nu[q] = ok[q] ? ug[at] : INFINITY;
+        nv[q] = ok[q] ? vg[at] : INFINITY;
         nanu |= ok[q] && isnan(nu[q]);
         nanv |= ok[q] && isnan(nv[q]);
         n += ok[q];
