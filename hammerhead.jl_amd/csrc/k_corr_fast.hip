@@ -346,22 +346,6 @@ struct UQAcc {
     double rmax[4], rsum[4];
 };
 
-__device__ __forceinline__ void uq_add(UQAcc &a, int dr, int dc, double s)
-{
-    if (dr == 0 && dc == 0) {
-        a.S00 = s;
-        return;
-    }
-    const int adr = dr < 0 ? -dr : dr, adc = dc < 0 ? -dc : dc;
-    const int ring = adr > adc ? adr : adc;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (ring == q + 1) {
-            a.rmax[q] = s > a.rmax[q] ? s : a.rmax[q];
-            a.rsum[q] += s;
-        }
-}
-
 __device__ __forceinline__ float uq_sigma(const UQAcc &a)
 {
     double s2 = a.S00;
@@ -394,26 +378,83 @@ __device__ __forceinline__ void uq_init(UQAcc &a)
 // 16-lane row, by DPP, zero past the window) times in-lane offset o, with
 // s = 0, o = 0..4 (o = 0 is S00) and s = 1..4, o = -4..4. Up to the point
 // reflection S(-a, -b) = S(a, b) this is the half-plane offset set of the
-// ring sums (the ring of an offset is max(|s|, |o|)).
-template <int NC>
-__device__ __forceinline__ void uq_lags(const double (&e)[NC], UQAcc &acc)
+// ring sums (the ring of an offset is max(|s|, |o|)). The cross-lane totals
+// go through LDS: each lane writes its row partials of a group of lags (16 rows x kUQP doubles
+// per window, the row pitch chosen so 16 lanes hit distinct banks), then lane
+// r adds the 16 row partials of lags r and r + 16 and folds its totals into
+// per-ring maximum and sum, which DPP reductions finish over the window:
+// about a third of the instructions of a 16-lane DPP reduction per lag. Lags
+// are numbered s = 0, o = 0..4 (k = o; k = 0 is S00), then s = 1..4,
+// o = -4..4 (k = 5 + 9 (s - 1) + o + 4), in two groups (s <= 2: 23 lags;
+// s >= 3: 18) so the scratch is 3200 bytes per window.
+constexpr int kUQP = 25;
+
+__device__ __forceinline__ void lds_wave_sync()
 {
-    {
-        double p[5];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int uq_ring(int k)
+{
+    if (k < 5) return k;
+    const int s = 1 + (k - 5) / 9, o = (k - 5) % 9 - 4;
+    const int ao = o < 0 ? -o : o;
+    return s > ao ? s : ao;
+}
+
+__device__ __forceinline__ double row_max_d(double v)  // a NaN never replaces the running maximum
+{
+    double o = rowror_d<8>(v);
+    v = o > v ? o : v;
+    o = rowror_d<4>(v);
+    v = o > v ? o : v;
+    o = rowror_d<2>(v);
+    v = o > v ? o : v;
+    o = rowror_d<1>(v);
+    v = o > v ? o : v;
+    return v;
+}
+
+// Totals of the N lags of one group (global numbers kbase + 0..N-1).
+template <int N>
+__device__ __forceinline__ void uq_collect(const double *red, int r, int kbase, double (&rmax)[4], double (&rsum)[4],
+                                           double &s00)
+{
 #pragma unroll
-        for (int o = 0; o < 5; ++o) {
-            p[o] = 0.0;
+    for (int j = 0; j < 2; ++j) {
+        const int k = r + 16 * j;
+        if (k < N) {
+            double tot = 0.0;
 #pragma unroll
-            for (int c = 0; c + o < NC; ++c) p[o] += e[c] * e[c + o];
+            for (int q = 0; q < 16; ++q) tot += red[q * kUQP + k];
+            const int kg = kbase + k;
+            const int ring = uq_ring(kg);
+            if (kg == 0) s00 = tot;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ring == q + 1) {
+                    rmax[q] = tot > rmax[q] ? tot : rmax[q];
+                    rsum[q] += tot;
+                }
         }
-        acc.S00 = row_sum_d(p[0]);
-#pragma unroll
-        for (int o = 1; o < 5; ++o) uq_add(acc, 0, o, row_sum_d(p[o]));
     }
-    sfor<4>([&](auto ss) {
-        constexpr int s = decltype(ss)::value + 1;
-        // p[o] = sum_c e[c] sh[c + o] in increasing c, with the shifted row
-        // fetched one element at a time (sh[c'] feeds p[o] through e[c' - o])
+}
+
+// red: this window's 16 x kUQP scratch; r: the lane's row (0..15).
+template <int NC>
+__device__ __forceinline__ void uq_lags_lds(const double (&e)[NC], UQAcc &acc, double *red, int r)
+{
+    double rmax[4], rsum[4], s00 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        rmax[q] = -INFINITY;
+        rsum[q] = 0.0;
+    }
+    double *row = red + r * kUQP;
+    auto shifted = [&](auto ss, int base) {
+        constexpr int s = decltype(ss)::value;
         double p[9];
 #pragma unroll
         for (int oi = 0; oi < 9; ++oi) p[oi] = 0.0;
@@ -427,15 +468,40 @@ __device__ __forceinline__ void uq_lags(const double (&e)[NC], UQAcc &acc)
             }
         }
 #pragma unroll
-        for (int oi = 0; oi < 9; ++oi) uq_add(acc, s, oi - 4, row_sum_d(p[oi]));
-    });
+        for (int oi = 0; oi < 9; ++oi) row[base + oi] = p[oi];
+    };
+    // group 1: s = 0, 1, 2
+#pragma unroll
+    for (int o = 0; o < 5; ++o) {
+        double p = 0.0;
+#pragma unroll
+        for (int c = 0; c + o < NC; ++c) p += e[c] * e[c + o];
+        row[o] = p;
+    }
+    shifted(std::integral_constant<int, 1>{}, 5);
+    shifted(std::integral_constant<int, 2>{}, 14);
+    lds_wave_sync();
+    uq_collect<23>(red, r, 0, rmax, rsum, s00);
+    lds_wave_sync();
+    // group 2: s = 3, 4
+    shifted(std::integral_constant<int, 3>{}, 0);
+    shifted(std::integral_constant<int, 4>{}, 9);
+    lds_wave_sync();
+    uq_collect<18>(red, r, 23, rmax, rsum, s00);
+    lds_wave_sync();
+    acc.S00 = row_sum_d(s00);  // held by lane 0 alone: the sum is a broadcast
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        acc.rmax[q] = row_max_d(rmax[q]);
+        acc.rsum[q] = row_sum_d(rsum[q]);
+    }
 }
 
 // uq_component! (src/uncertainty.jl:104-151) of both components of a 16-px
 // window whose apodized, mean-subtracted rows sit one per lane (x[j] =
 // (wA[r, j], wB[r, j]), r = lane of the row). u: shift along columns
 // (in-lane); v: the transposed windows, shift along rows (across lanes).
-__device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &sig_v)
+__device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &sig_v, double *red)
 {
 #pragma clang fp contract(off)
     const float q = 0.25f;
@@ -469,7 +535,7 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
         mu = row_sum_d(mu) / 240.0;
 #pragma unroll
         for (int c = 0; c < 15; ++c) d[c] -= mu;
-        uq_lags<15>(d, acc);
+        uq_lags_lds<15>(d, acc, red, r);
         sig_u = uq_sigma(acc);
     }
     // ---- v: transposed windows: d(j; i), i = row = lane 0..14, j = 0..15 ----
@@ -508,7 +574,7 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
         mu = row_sum_d(mu) / 240.0;
 #pragma unroll
         for (int j = 0; j < 16; ++j) d[j] = live ? d[j] - mu : 0.0;
-        uq_lags<16>(d, acc);
+        uq_lags_lds<16>(d, acc, red, r);
         sig_v = uq_sigma(acc);
     }
 }
@@ -524,6 +590,7 @@ __global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *_
                                                  CorrOut out)
 {
 #pragma clang fp contract(off)
+    __shared__ double ured[4][16 * kUQP];
     const int tid = threadIdx.x, line = tid & 15, slot_w = tid >> 4;
     const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
@@ -560,7 +627,7 @@ __global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *_
         x[j] = make_float2(a * (ra[j] - meanA), a * (rb[j] - meanB));
     }
     float sig_u, sig_v;
-    uq_fused16(x, line, sig_u, sig_v);
+    uq_fused16(x, line, sig_u, sig_v, ured[slot_w]);
     if (active && line == 0) {
         const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
         out.unc_u[g] = sig_u;
@@ -690,7 +757,8 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     // fused uncertainty: the final sweep's windows are the ones the
     // post-loop uncertainty_sweep! reads (src/pipeline.jl:810-824)
     float sig_u = NAN, sig_v = NAN;
-    if constexpr (UQ && L == 1) uq_fused16(x, line, sig_u, sig_v);
+    // (the window's LDS slice is free until the transpose below)
+    if constexpr (UQ && L == 1) uq_fused16(x, line, sig_u, sig_v, reinterpret_cast<double *>(T));
 
     // ---- 2. row DFT (dim 2) and Hermitian split, k2 = 0..H ----
     const int mblk = block_of<L>(l);

@@ -454,6 +454,28 @@ def test_mfma_correlate_vs_oracle(hh, orc, ctx_mfma, c1, method, w):
         assert s["n_over_tol"] == 0, s
 
 
+def test_mfma_uq_vs_oracle(hh, orc, ctx_mfma):
+    # 16-px windows on the matrix cores with the Wieneke statistics beside
+    # them (one row per lane, cross-lane totals through LDS): sigma within 1 %
+    pair = hh.synthetic.particle_pair(160, 160, hh.synthetic.uniform_flow(0.3, 0.2), seed=12, noise=0.05)
+    ctx_mfma.set_pair(pair.A, pair.B)
+    for ov in (8, 12):
+        p = P(hh, window_size=16, overlap=ov, padding=True, apodization="gauss", uncertainty=True)
+        x, y, jobs = hh.pass_grid(p, 160, 160)
+        say("mfma uq", ov)
+        r = ctx_mfma.process_windows(p, jobs, y.size, x.size, src=hh.HH_SRC_RAW, uncertainty=True)
+        ou, ov_, _ = orc.uncertainty_sweep(p.to_struct(), pair.A.astype(np.float64), pair.B.astype(np.float64),
+                                           jobs, y.size, x.size, stats=True)
+        fu, fv = r["unc_u"], r["unc_v"]
+        assert np.array_equal(np.isfinite(fu), np.isfinite(ou)) and np.array_equal(np.isfinite(fv), np.isfinite(ov_))
+        fin = np.isfinite(ou)
+        relu = np.abs(fu[fin] - ou[fin]) / np.maximum(np.abs(ou[fin]), 1e-6)
+        fin = np.isfinite(ov_)
+        relv = np.abs(fv[fin] - ov_[fin]) / np.maximum(np.abs(ov_[fin]), 1e-6)
+        say("mfma uq max rel", relu.max(), relv.max())
+        assert relu.max() < 0.01 and relv.max() < 0.01
+
+
 def test_mfma_multipass_vs_oracle(hh, orc, ctx_mfma):
     # the metric's bar: the MFMA form within 0.005 px RMS of the FP64 oracle
     pair = hh.synthetic.config_vortex_1024(n=512)
