@@ -217,87 +217,116 @@ __device__ __forceinline__ float2 rot4(float2 b)
     return INV ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
 }
 
+// Per-lane constants of the cross-lane steps: the butterfly signs of the
+// lane's two index bits and whether it is the quad's lane 3 (the one whose
+// partial result takes the +-i twiddle), plus its row of four-step twiddles.
+struct XLane {
+    float sg1, sg2;  // -1 where bit 0 / bit 1 of the sub-lane is set, else +1
+    bool r3;
+    const float2 *tw;  // W_M^{-q k}, k = 0..31, q = sub-lane (LDS)
+};
+
+// Four-step twiddles W_M^{-q k} = (cos, -sin)(2 pi q k / M) for q < 4, k < 32,
+// from the same constexpr series as the compile-time twiddles. For M = 64
+// (two lanes per line) the quarter turns are exact, as in twmul; for M = 128
+// they are the rounded series values, as the per-lane constant selects had.
+struct TwTab {
+    float c[4][32], s[4][32];
+};
+template <int M>
+constexpr TwTab make_twtab()
+{
+    TwTab t{};
+    for (int q = 0; q < 4; ++q)
+        for (int k = 0; k < 32; ++k) {
+            const int r = (q * k) % M;
+            if (M == 64 && r % (M / 4) == 0) {
+                const int h = r / (M / 4);  // 0: 1, 1: -i, 2: -1, 3: +i
+                t.c[q][k] = h == 0 ? 1.f : (h == 2 ? -1.f : 0.f);
+                t.s[q][k] = h == 1 ? -1.f : (h == 3 ? 1.f : 0.f);
+            } else {
+                t.c[q][k] = (float)c_cos(angle(q * k, M));
+                t.s[q][k] = -(float)c_sin(angle(q * k, M));
+            }
+        }
+    return t;
+}
+__constant__ TwTab kTw64 = make_twtab<64>();
+__constant__ TwTab kTw128 = make_twtab<128>();
+
 // Forward L-point DFT across the lanes of a line, decimation in frequency:
-// lane l holds input q = l, returns output block block_of(l).
+// lane l holds input q = l, returns output block block_of(l). The butterflies
+// o +- v are single FMAs with the lane's sign (exact: one rounding, as the
+// add or subtract).
 template <int L>
-__device__ __forceinline__ float2 xdft_fwd(float2 v, int l)
+__device__ __forceinline__ float2 xdft_fwd(float2 v, const XLane &xl)
 {
     if constexpr (L == 1) {
         return v;
     } else if constexpr (L == 2) {
         const float2 o = qp2<kQX1>(v);
-        return (l & 1) ? csub2(o, v) : cadd2(v, o);
+        return make_float2(__builtin_fmaf(xl.sg1, v.x, o.x), __builtin_fmaf(xl.sg1, v.y, o.y));
     } else {
         float2 o = qp2<kQX2>(v);
-        float2 s = (l & 2) ? csub2(o, v) : cadd2(v, o);
-        if ((l & 3) == 3) s = rot4<false>(s);
+        float2 s = make_float2(__builtin_fmaf(xl.sg2, v.x, o.x), __builtin_fmaf(xl.sg2, v.y, o.y));
+        if (xl.r3) s = rot4<false>(s);
         o = qp2<kQX1>(s);
-        return (l & 1) ? csub2(o, s) : cadd2(s, o);
+        return make_float2(__builtin_fmaf(xl.sg1, s.x, o.x), __builtin_fmaf(xl.sg1, s.y, o.y));
     }
 }
 // Inverse L-point DFT across lanes, decimation in time: lane l holds input
-// block block_of(l), returns output q = l.
+// block block_of(l), returns output q = l. Lane 3 applies the +i twiddle to
+// its span-1 result before the span-2 exchange, so both spans are plain
+// sign butterflies.
 template <int L>
-__device__ __forceinline__ float2 xdft_inv(float2 v, int l)
+__device__ __forceinline__ float2 xdft_inv(float2 v, const XLane &xl)
 {
     if constexpr (L == 1) {
         return v;
     } else if constexpr (L == 2) {
         const float2 o = qp2<kQX1>(v);
-        return (l & 1) ? csub2(o, v) : cadd2(v, o);
+        return make_float2(__builtin_fmaf(xl.sg1, v.x, o.x), __builtin_fmaf(xl.sg1, v.y, o.y));
     } else {
         float2 o = qp2<kQX1>(v);
-        const float2 s = (l & 1) ? csub2(o, v) : cadd2(v, o);
+        float2 s = make_float2(__builtin_fmaf(xl.sg1, v.x, o.x), __builtin_fmaf(xl.sg1, v.y, o.y));
+        if (xl.r3) s = rot4<true>(s);
         o = qp2<kQX2>(s);
-        // twiddle (+i)^(l & 1) on the span-2 partner
-        const float2 os = (l & 1) ? rot4<true>(o) : o;
-        const float2 ss = (l & 1) ? rot4<true>(s) : s;
-        return (l & 2) ? csub2(o, ss) : cadd2(s, os);
+        return make_float2(__builtin_fmaf(xl.sg2, s.x, o.x), __builtin_fmaf(xl.sg2, s.y, o.y));
     }
 }
 
-// four-step twiddle W_M^{+-q k} for runtime q in [0, L), compile-time k
+// four-step twiddle W_M^{-+q k} for the lane's q, compile-time k: one LDS read
 template <int L, int K, bool INV>
-__device__ __forceinline__ float2 twq(float2 v, int q)
+__device__ __forceinline__ float2 twq(float2 v, const XLane &xl)
 {
-    constexpr int M = 32 * L;
     if constexpr (L == 1 || K == 0) {
         return v;
-    } else if constexpr (L == 2) {
-        const float2 a = twmul<M, K, INV>(v);
-        return q ? a : v;
     } else {
-        constexpr float c1 = (float)c_cos(angle(K, M)), c2 = (float)c_cos(angle(2 * K, M)),
-                        c3 = (float)c_cos(angle(3 * K, M));
-        constexpr float s1 = (float)(INV ? 1 : -1) * (float)c_sin(angle(K, M)),
-                        s2 = (float)(INV ? 1 : -1) * (float)c_sin(angle(2 * K, M)),
-                        s3 = (float)(INV ? 1 : -1) * (float)c_sin(angle(3 * K, M));
-        const float c = q == 1 ? c1 : (q == 2 ? c2 : c3);
-        const float s = q == 1 ? s1 : (q == 2 ? s2 : s3);
-        const float2 a = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
-        return q == 0 ? v : a;
+        const float2 w = xl.tw[K];
+        const float sn = INV ? -w.y : w.y;
+        return make_float2(v.x * w.x - v.y * sn, v.x * sn + v.y * w.x);
     }
 }
 
 // Forward M-point DFT of a line whose lane q holds x[L n + q], n < 16
 // (x[L n + q] = 0 for n >= 16): returns X[k + 32 block_of(l)] in X[k].
 template <int L>
-__device__ __forceinline__ void line_fwd(const float2 (&x)[16], float2 (&X)[32], int l)
+__device__ __forceinline__ void line_fwd(const float2 (&x)[16], float2 (&X)[32], const XLane &xl)
 {
     dft32_half(x, X);
     sfor<32>([&](auto kk) {
         constexpr int k = decltype(kk)::value;
-        X[k] = xdft_fwd<L>(twq<L, k, false>(X[k], l), l);
+        X[k] = xdft_fwd<L>(twq<L, k, false>(X[k], xl), xl);
     });
 }
 // Inverse M-point DFT: lane l holds X[k + 32 block_of(l)] in X[k]; returns
 // x[L n + l] in X[n] (unnormalized).
 template <int L>
-__device__ __forceinline__ void line_inv(float2 (&X)[32], int l)
+__device__ __forceinline__ void line_inv(float2 (&X)[32], const XLane &xl)
 {
     sfor<32>([&](auto kk) {
         constexpr int k = decltype(kk)::value;
-        X[k] = twq<L, k, true>(xdft_inv<L>(X[k], l), l);
+        X[k] = twq<L, k, true>(xdft_inv<L>(X[k], xl), xl);
     });
     fft<32, true>(X);
 }
@@ -505,9 +534,10 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 {
 #pragma clang fp contract(off)
     const float q = 0.25f;
+    UQAcc accu, accv;
     // ---- u: d(r, c), c = 0..14 in registers; lags dr across lanes ----
     {
-        UQAcc acc;
+        UQAcc &acc = accu;
         uq_init(acc);
         float dC[15];
         double c0 = 0.0, cp = 0.0, cm = 0.0;
@@ -536,11 +566,10 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 #pragma unroll
         for (int c = 0; c < 15; ++c) d[c] -= mu;
         uq_lags_lds<15>(d, acc, red, r);
-        sig_u = uq_sigma(acc);
     }
     // ---- v: transposed windows: d(j; i), i = row = lane 0..14, j = 0..15 ----
     {
-        UQAcc acc;
+        UQAcc &acc = accv;
         uq_init(acc);
         const bool live = r < 15;
         float dC[16];
@@ -575,8 +604,23 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 #pragma unroll
         for (int j = 0; j < 16; ++j) d[j] = live ? d[j] - mu : 0.0;
         uq_lags_lds<16>(d, acc, red, r);
-        sig_v = uq_sigma(acc);
     }
+    // every lane of the window holds both sets of statistics: even lanes
+    // finalize u, odd lanes v, and lane r takes v from lane r + 1
+    const bool odd = r & 1;
+    UQAcc a;
+    a.C0 = odd ? accv.C0 : accu.C0;
+    a.Cp = odd ? accv.Cp : accu.Cp;
+    a.Cm = odd ? accv.Cm : accu.Cm;
+    a.S00 = odd ? accv.S00 : accu.S00;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        a.rmax[q] = odd ? accv.rmax[q] : accu.rmax[q];
+        a.rsum[q] = odd ? accv.rsum[q] : accu.rsum[q];
+    }
+    const float sg = uq_sigma(a);
+    sig_u = sg;             // valid on even lanes (lane 0 writes)
+    sig_v = rowsh<1>(sg);
 }
 
 // Wieneke statistics alone for 16-px windows (the uncertainty of a sweep
@@ -663,11 +707,18 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     __shared__ float red[2 * NWAVE];
     __shared__ float mv[NWAVE];
     __shared__ int ml[NWAVE];
+    __shared__ float2 twl[4][33];  // four-step twiddles, rows padded against bank conflicts
 
     const int tid = threadIdx.x;
     const int slot_w = tid / G;               // window slot in the block
     const int t = tid % G;                    // lane within the window
     const int line = t / L, l = t % L;        // line (row / unit) and sub-lane
+    if constexpr (L > 1) {
+        const TwTab &tt = L == 2 ? kTw64 : kTw128;
+        for (int e = tid; e < 4 * 32; e += NT) twl[e >> 5][e & 31] = make_float2(tt.c[e >> 5][e & 31], tt.s[e >> 5][e & 31]);
+        __syncthreads();
+    }
+    const XLane xl{(l & 1) ? -1.f : 1.f, (l & 2) ? -1.f : 1.f, (l & 3) == 3, &twl[l][0]};
     const int wave = tid >> 6;
     // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous
     // range of tiles so overlapping windows share its L2
@@ -764,7 +815,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
     const int mblk = block_of<L>(l);
     {
         float2 Y[32];
-        line_fwd<L>(x, Y, l);
+        line_fwd<L>(x, Y, xl);
         sfor<32>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
             const int k2 = k + 32 * mblk;
@@ -795,8 +846,8 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
             fb[n] = u != 0 ? make_float2(v.z, v.w) : make_float2(v.z, vh.z);
         }
         float2 FA[32], FB[32];
-        line_fwd<L>(fa, FA, l);
-        line_fwd<L>(fb, FB, l);
+        line_fwd<L>(fa, FA, xl);
+        line_fwd<L>(fb, FB, xl);
         const bool unit0 = u == 0;
         sfor<17>([&](auto kk) {
             constexpr int k = decltype(kk)::value;
@@ -829,7 +880,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
             P[k] = spec(FA[k], gam, FB[k], gbm, k + 32 * mblk);
             if constexpr (km != k) P[km] = spec(FA[km], gam2, FB[km], gbm2, km + 32 * mblk);
         });
-        line_inv<L>(P, l);  // P[n] = Q(L n + l, u)
+        line_inv<L>(P, xl);  // P[n] = Q(L n + l, u)
     }
     __syncthreads();  // the LDS slice is reused for the kept rows
 
@@ -869,7 +920,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, con
             }
             c[k] = cv;
         }
-        line_inv<L>(c, l);  // c[n] = r(qa) + i r(qb) at l2 = L n + l
+        line_inv<L>(c, xl);  // c[n] = r(qa) + i r(qb) at l2 = L n + l
         constexpr float scale = 1.0f / (float)(M * M);
         const int ipa = LO + qa, ipb = LO + qb;
         sfor<32>([&](auto nn) {
