@@ -172,8 +172,7 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
     // candidate goes before the first entry of smaller value (an equal value
     // arrived earlier, at the smaller location, and stays ahead) and every
     // entry from there on moves down one slot; an empty slot (0, INT_MAX)
-    // loses to any candidate (> 0). A value of 0 inserts nothing, so the
-    // column walk inserts without branching.
+    // loses to any candidate (> 0).
     auto insert = [&](float cv, int cl) {
         bool moved = false;
 #pragma unroll
@@ -205,25 +204,33 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             constexpr int ctrl = G == 16 ? 0x101 : 0x130;  // row_shl:1 / wave_shl:1
             return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, true));
         };
-        float cp = 0.f, lp = 0.f, rp = 0.f;
+        // V: the lane's own column over rows bi-1..bi+1; a neighbour lane's V
+        // is that lane's three rows, so left(V) / right(V) are the left and
+        // right columns of the 3 x 3 neighbourhood. Points that pass are
+        // marked in a bit mask and inserted afterwards, lowest row first
+        // (increasing location), so the insertion runs once per regional
+        // maximum instead of once per point.
+        static_assert(NB <= 32, "one mask bit per box row");
+        unsigned okm = 0u;
+        float cp = 0.f;
         float cc = col ? Rb[t * NB] : 0.f;
-        float lc = left(cc), rc = right(cc);
         for (int bi = 0; bi < NB; ++bi) {
             const float cn = (col && bi + 1 < NB) ? Rb[bi + 1 + t * NB] : 0.f;
-            const float ln = left(cn), rn = right(cn);
+            const float V = fmaxf(fmaxf(cp, cc), cn);
             const float I0 = cc;
             // fmaxf skips NaN neighbours exactly as !(NaN > I0) and
             // !(NaN >= I0) pass them
-            const float me = fmaxf(fmaxf(lp, lc), fmaxf(ln, cp));  // earlier: disqualify when >
-            const float ml_ = fmaxf(fmaxf(cn, rp), fmaxf(rc, rn));  // later: disqualify when >=
+            const float me = fmaxf(left(V), cp);    // earlier: disqualify when >
+            const float ml_ = fmaxf(right(V), cn);  // later: disqualify when >=
             const bool ok = I0 > 0.f && !(me > I0) && !(ml_ >= I0);
-            insert(ok ? I0 : 0.f, (LO + bi) + (LO + t) * M);
+            okm |= (ok ? 1u : 0u) << bi;
             cp = cc;
             cc = cn;
-            lp = lc;
-            lc = ln;
-            rp = rc;
-            rc = rn;
+        }
+        while (okm) {
+            const int bi = __builtin_ctz(okm);
+            okm &= okm - 1u;
+            insert(Rb[bi + t * NB], (LO + bi) + (LO + t) * M);
         }
     } else {
         for (int p = t; p < NB * NB; p += G) {

@@ -219,6 +219,13 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     float apod8[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) apod8[s] = cfg.apod[row + 16 * (8 * h + s)];
+    // overlap gain of this lane's box points (row n, columns rmap(h, v))
+    float gain8[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const int bc = rmap(h, v);
+        gain8[v] = (n < kNB && bc < kNB) ? cfg.gain[(LO + n) + (LO + bc) * M] : 0.f;
+    }
 
     // XCD-aware order: block b runs on XCD b % 8; each XCD takes a contiguous
     // range of window chunks, so the 75 %-overlapping windows of neighbouring
@@ -372,7 +379,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
                 const int b = rmap(h, v);
                 if (b < kNB) {
                     const float rv = (oc[v] - os[v + 8]) * scale;
-                    Rb[n + b * kNB] = fabsf(rv) * cfg.gain[(LO + n) + (LO + b) * M];
+                    Rb[n + b * kNB] = fabsf(rv) * gain8[v];
                 }
             }
         }
