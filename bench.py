@@ -95,10 +95,13 @@ def new_context(hh, form):
 
 
 def time_form(hh, pair, passes, form, steps, warmup, log):
-    """Steady-state vector fields/s of one correlation form, two timed loops
-    of `steps` vector fields each on the same context:
-      * eager: every launch issued from the host, each kernel family
-        bracketed by HIP events on the context stream (the per-kernel times);
+    """Steady-state vector fields/s of one correlation form on one context:
+      * timed (the reported rate): `steps` vector fields issued eagerly from
+        the host, the dominant kernel (the 16-px correlation sweep) alone
+        bracketed by HIP events on the context stream, every other launch
+        uninstrumented;
+      * breakdown: the same loop with every kernel family bracketed (per-
+        kernel times; the events cost a little, so its rate is not reported);
       * graph: the whole schedule replayed from its captured hipGraph.
     """
     ctx = new_context(hh, form)
@@ -106,21 +109,32 @@ def time_form(hh, pair, passes, form, steps, warmup, log):
     for _ in range(max(warmup, 1)):
         ctx.run_piv(passes, fetch=False)  # captures the graph on the first call
     ctx.synchronize()
-    ctx.set_timing(True)
+    ctx.set_timing(True, only=("correlate", 16))
     ctx.run_piv(passes, fetch=False)  # eager warm-up
-    ctx.set_timing(True)  # clears the warm-up times
+    ctx.synchronize()
+    ctx.set_timing(True, only=("correlate", 16))  # clears the warm-up times
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.run_piv(passes, fetch=False)
     ctx.synchronize()
     eager = time.perf_counter() - t0
+    ms, n = ctx.kernel_time("correlate", 16)
+    dominant = {"avg_ms": ms / n, "launches_per_step": n / steps} if n else None
+    ctx.set_timing(True)
+    kb = max(1, min(steps, 10))
+    for _ in range(kb):
+        ctx.run_piv(passes, fetch=False)
+    ctx.synchronize()
     kernels = {}
     for fam in ("prefilter", "deform", "correlate", "correlate_uq", "validate", "uq"):
         for w in (0, 16, 32, 64):
             ms, n = ctx.kernel_time(fam, w)
             if n:
-                kernels[f"{fam}_w{w}" if w else fam] = {"avg_ms": ms / n, "launches_per_step": n / steps}
+                kernels[f"{fam}_w{w}" if w else fam] = {"avg_ms": ms / n, "launches_per_step": n / kb}
+    if dominant:
+        kernels["correlate_w16"] = dict(dominant, source="timed loop")
     ctx.set_timing(False)
+    ctx.synchronize()
     r0 = ctx.graph_replays()
     t0 = time.perf_counter()
     for _ in range(steps):

@@ -27,7 +27,7 @@ EXPORTS = (
     "hh_process_windows", "hh_correlation_planes", "hh_uncertainty_sweep", "hh_validate_and_replace",
     "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
-    "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_kernel_time",
+    "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
     "hh_set_autotune", "hh_tune_report", "hh_tune_set",
 )
@@ -92,6 +92,7 @@ def load():
         "hh_stream": (vp, [vp]),
         "hh_synchronize": (i32, [vp]),
         "hh_set_timing": (i32, [vp, i32]),
+        "hh_set_timing_only": (i32, [vp, C.c_char_p, i32]),
         "hh_kernel_time": (i32, [vp, C.c_char_p, i32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         "hh_set_pairs": (i32, [vp, vp, vp, vp, vp, i32]),
         "hh_run_piv_batch": (i32, [vp, pp, i32, i32, vp]),
@@ -508,8 +509,13 @@ class Context:
     def synchronize(self):
         _check(self.h, load().hh_synchronize(self.h), "synchronize")
 
-    def set_timing(self, enable: bool):
-        _check(self.h, load().hh_set_timing(self.h, 1 if enable else 0), "set_timing")
+    def set_timing(self, enable: bool, only=None):
+        """Per-kernel HIP-event timing; only=(family, window) times that
+        kernel family alone."""
+        if enable and only is not None:
+            _check(self.h, load().hh_set_timing_only(self.h, only[0].encode(), int(only[1])), "set_timing")
+        else:
+            _check(self.h, load().hh_set_timing(self.h, 1 if enable else 0), "set_timing")
 
     def kernel_time(self, family: str, window: int):
         ms, n = C.c_double(), C.c_int64()

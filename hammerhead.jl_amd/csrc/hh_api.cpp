@@ -1446,6 +1446,18 @@ int hh_set_timing(hh_ctx *c, int enable)
     return guarded(c, [&] {
         drain_timing(c);
         c->timing = enable != 0;
+        c->timing_family.clear();
+        c->ktime.clear();
+    });
+}
+
+int hh_set_timing_only(hh_ctx *c, const char *family, int window)
+{
+    return guarded(c, [&] {
+        drain_timing(c);
+        c->timing = true;
+        c->timing_family = family ? family : "";
+        c->timing_w = window;
         c->ktime.clear();
     });
 }

@@ -94,7 +94,7 @@ struct PassTables {
     DBuf<DeformPair> dself;   // predictor: this pass's grids
     DeformPair hprev{}, hself{};  // host copies when the tables hold one pair
     int blocks = 0;           // deformation blocks
-    DBuf<int> count;          // one valid-cell counter per segment
+    DBuf<int> count;          // one valid-cell counter per segment (saturating at 3)
 };
 
 struct PassPlan {
@@ -211,6 +211,8 @@ struct hh_ctx {
     hh::host::Tuner tune;
     // timing
     bool timing = false;
+    std::string timing_family;  // non-empty: time this family and window only
+    int timing_w = 0;
     std::vector<hh::host::TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::pair<std::string, int>, std::pair<double, int64_t>> ktime;
@@ -225,7 +227,7 @@ hipEvent_t get_event(hh_ctx *c);
 template <typename F>
 void timed(hh_ctx *c, const char *family, int w, F &&f)
 {
-    if (!c->timing) {
+    if (!c->timing || (!c->timing_family.empty() && (c->timing_family != family || c->timing_w != w))) {
         f();
         return;
     }

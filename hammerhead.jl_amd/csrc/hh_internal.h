@@ -215,7 +215,7 @@ struct ValidateArgs {
     // scratch (device): grids.stride each
     float *su, *sv;
     uint8_t *sflag;
-    int *count;  // grids.nseg device ints
+    int *count;  // grids.nseg device ints: valid cells per segment, saturating at 3
 };
 void launch_validate(const ValidateArgs &a, hipStream_t s);
 void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s);

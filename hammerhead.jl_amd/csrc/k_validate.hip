@@ -301,6 +301,15 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
     return 1;
 }
 
+// A counter is read only as "fewer than 3 valid cells", so it saturates: an
+// add whose counter already reads >= 3 is skipped (the final value is >= 3
+// exactly when the segment has >= 3 valid cells, and exact below that),
+// which keeps a whole grid's blocks from queueing atomics on one address.
+__device__ __forceinline__ void add_count(int *__restrict__ count, int seg, int x)
+{
+    if (__hip_atomic_load(count + seg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 3) atomicAdd(count + seg, x);
+}
+
 // Adds the valid cells of a block to their segments' counters: one atomic
 // per block when the block lies in one segment (always, for a single grid),
 // else one per wave or per lane. Every thread of the block calls it.
@@ -326,12 +335,12 @@ __device__ __forceinline__ void add_valid(const Grids &g, int ee, bool valid, in
         if (threadIdx.x == 0) {
             int t = 0;
             for (int w = 0; w < nw; ++w) t += part[w];
-            if (t) atomicAdd(count + segw[0], t);
+            if (t) add_count(count, segw[0], t);
         }
     } else if (uni) {
-        if (lane == 0 && wsum) atomicAdd(count + s0, wsum);
+        if (lane == 0 && wsum) add_count(count, s0, wsum);
     } else if (x) {
-        atomicAdd(count + si, 1);
+        add_count(count, si, 1);
     }
 }
 

@@ -326,6 +326,11 @@ int hh_synchronize(hh_ctx *ctx);
  * total milliseconds and launch count of a kernel family by name
  * ("correlate", "deform", "prefilter", "validate", "uq"). */
 int hh_set_timing(hh_ctx *ctx, int enable);
+/* Timing of one kernel family and window only (family NULL: every family,
+ * as hh_set_timing(ctx, 1)); the other launches carry no events. No
+ * counterpart in the reference: for benchmarks that time one kernel inside
+ * an otherwise uninstrumented schedule. */
+int hh_set_timing_only(hh_ctx *ctx, const char *family, int window);
 int hh_kernel_time(hh_ctx *ctx, const char *family, int window, double *ms_total,
                    int64_t *launches);
 
