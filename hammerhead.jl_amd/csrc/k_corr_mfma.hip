@@ -106,23 +106,23 @@ __device__ __forceinline__ float rowror_f(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + R, 0xF, 0xF, true));
 }
 
-__device__ __forceinline__ void img_reduce(float &sum, float &mx, float &mn)
+// Sum and maximum over the 32 lanes of one image: row rotations reduce
+// within a DPP row, one exchange across halves.
+__device__ __forceinline__ float img_sum(float v)
 {
-    sum += rowror_f<8>(sum);
-    mx = fmaxf(mx, rowror_f<8>(mx));
-    mn = fminf(mn, rowror_f<8>(mn));
-    sum += rowror_f<4>(sum);
-    mx = fmaxf(mx, rowror_f<4>(mx));
-    mn = fminf(mn, rowror_f<4>(mn));
-    sum += rowror_f<2>(sum);
-    mx = fmaxf(mx, rowror_f<2>(mx));
-    mn = fminf(mn, rowror_f<2>(mn));
-    sum += rowror_f<1>(sum);
-    mx = fmaxf(mx, rowror_f<1>(mx));
-    mn = fminf(mn, rowror_f<1>(mn));
-    sum += __shfl_xor(sum, 32);
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
-    mn = fminf(mn, __shfl_xor(mn, 32));
+    v += rowror_f<8>(v);
+    v += rowror_f<4>(v);
+    v += rowror_f<2>(v);
+    v += rowror_f<1>(v);
+    return v + __shfl_xor(v, 32);
+}
+__device__ __forceinline__ float img_max(float v)
+{
+    v = fmaxf(v, rowror_f<8>(v));
+    v = fmaxf(v, rowror_f<4>(v));
+    v = fmaxf(v, rowror_f<2>(v));
+    v = fmaxf(v, rowror_f<1>(v));
+    return fmaxf(v, __shfl_xor(v, 32));
 }
 
 // LDS hand-off between the lanes of one wave (the waves of a block run
@@ -270,20 +270,20 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         jcur = jnext;
         jnext = jobs4[min(win + 2, w1 - 1)];
         // ---- 0. window: lane (image, row, 8 columns), mean, apodize ----
-        float sum = 0.f, pmx = px[0], pmn = px[0];
+        float sum = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) sum += px[s];
+        const float mean = img_sum(sum) / 256.f;
+        float x[8], amx = 0.f;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            sum += px[s];
-            pmx = fmaxf(pmx, px[s]);
-            pmn = fminf(pmn, px[s]);
+            const float d = px[s] - mean;
+            amx = fmaxf(amx, fabsf(d));
+            x[s] = apod8[s] * d;
         }
-        img_reduce(sum, pmx, pmn);
-        const float mean = sum / 256.f;
-        float x[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) x[s] = apod8[s] * (px[s] - mean);
-        // |x| <= max - min (apodization <= 1): scale that bound into [2^-2, 2^-1)
-        const float bound = pmx - pmn;
+        // |x| <= max |px - mean| (apodization <= 1): scale that bound into
+        // [2^-2, 2^-1)
+        const float bound = img_max(amx);
         int ex = 0;
         (void)frexpf(bound, &ex);
         const int e0 = bound > 0.f ? -1 - ex : 0;
@@ -326,8 +326,8 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         float sp_r[9], sp_i[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
-            float pr = zr[0][v] * zr[1][v] + zi[0][v] * zi[1][v];
-            float pi = zr[0][v] * zi[1][v] - zi[0][v] * zr[1][v];
+            float pr = __builtin_fmaf(zr[0][v], zr[1][v], zi[0][v] * zi[1][v]);
+            float pi = __builtin_fmaf(zr[0][v], zi[1][v], -(zi[0][v] * zr[1][v]));
             if constexpr (PHASE) {
                 const int k2 = v < 8 ? rmap(h, v) : 16;
                 pr = ldexpf(pr, -es);
@@ -540,19 +540,20 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
             int eI[2];
 #pragma unroll
             for (int I = 0; I < 2; ++I) {
-                float sum = 0.f, mx = px[I][0], mn = px[I][0];
+                float sum = 0.f;
+#pragma unroll
+                for (int s = 0; s < 16; ++s) sum += px[I][s];
+                sum = img_sum(sum);           // DPP row (16 lanes) and lane + 32
+                sum += __shfl_xor(sum, 16);   // and the other row of the half: all 64 lanes
+                const float mean = sum / 1024.f;
+                float amx = 0.f;
 #pragma unroll
                 for (int s = 0; s < 16; ++s) {
-                    sum += px[I][s];
-                    mx = fmaxf(mx, px[I][s]);
-                    mn = fminf(mn, px[I][s]);
+                    px[I][s] -= mean;
+                    amx = fmaxf(amx, fabsf(px[I][s]));
                 }
-                img_reduce(sum, mx, mn);  // DPP row (16 lanes) and lane + 32
-                sum += __shfl_xor(sum, 16);   // and the other row of the half: all 64 lanes
-                mx = fmaxf(mx, __shfl_xor(mx, 16));
-                mn = fminf(mn, __shfl_xor(mn, 16));
-                const float mean = sum / 1024.f;
-                const float bound = mx - mn;
+                amx = img_max(amx);
+                const float bound = fmaxf(amx, __shfl_xor(amx, 16));  // >= |x| (apodization <= 1)
                 int ex = 0;
                 (void)frexpf(bound, &ex);
                 const int e0 = bound > 0.f ? -3 - ex : 0;  // |x| < 2^-3
@@ -561,8 +562,8 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                 for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
                     for (int s = 0; s < 8; s += 2)
-                        put2(a1[I][kk], s, ldexpf(apodl[32 * (16 * kk + s)] * (px[I][8 * kk + s] - mean), e0),
-                             ldexpf(apodl[32 * (16 * kk + s + 1)] * (px[I][8 * kk + s + 1] - mean), e0), neg1);
+                        put2(a1[I][kk], s, ldexpf(apodl[32 * (16 * kk + s)] * px[I][8 * kk + s], e0),
+                             ldexpf(apodl[32 * (16 * kk + s + 1)] * px[I][8 * kk + s + 1], e0), neg1);
             }
             const int es = eI[0] + eI[1];
             f16v q3[2][2];  // Q[nt][ct]
@@ -610,8 +611,8 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                     float sp_r[9], sp_i[9];
 #pragma unroll
                     for (int v = 0; v < 9; ++v) {
-                        float pr = zr[0][v] * zr[1][v] + zi[0][v] * zi[1][v];
-                        float pi = zr[0][v] * zi[1][v] - zi[0][v] * zr[1][v];
+                        float pr = __builtin_fmaf(zr[0][v], zr[1][v], zi[0][v] * zi[1][v]);
+                        float pi = __builtin_fmaf(zr[0][v], zi[1][v], -(zi[0][v] * zr[1][v]));
                         if constexpr (PHASE) {
                             const int k2 = v < 8 ? 16 * J + rmap(h, v) : 32;
                             pr = ldexpf(pr, -es);
