@@ -511,6 +511,7 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
     const size_t npx = (size_t)nr * nc;
     float *arena = nullptr;
+    bool upload = false;
     if (!hA && !hB) {
         // the pairs of the previous call, still resident
         if (c->arena_pairs != npairs || c->arena_nr != nr || c->arena_nc != nc)
@@ -522,15 +523,16 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
             if (!hA[q] || !hB[q]) throw Error(HH_EINVAL, "null image in the pair list");
         c->arena_pairs = 0;
         arena = c->arena.ensure(2 * npx * npairs);
-        for (int q = 0; q < npairs; ++q) {
-            HH_HIP(hipMemcpyAsync(arena + 2 * npx * q, hA[q], sizeof(float) * npx, hipMemcpyHostToDevice,
-                                  c->stream));
-            HH_HIP(hipMemcpyAsync(arena + 2 * npx * q + npx, hB[q], sizeof(float) * npx, hipMemcpyHostToDevice,
-                                  c->stream));
+        // uploads go on a copy stream, pair by pair, and the first pass's
+        // accumulation of pair q waits only for pair q: the transfer of the
+        // next pairs overlaps the correlation of the earlier ones
+        if (!c->copy_stream) HH_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        while (c->upload_ev.size() < 4) {
+            hipEvent_t e;
+            HH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c->upload_ev.push_back(e);
         }
-        c->arena_pairs = npairs;
-        c->arena_nr = nr;
-        c->arena_nc = nc;
+        upload = true;
     }
     if (nr != c->nr || nc != c->nc) c->jobcache.clear();
     c->nr = nr;
@@ -576,6 +578,17 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
         for (int q = 0; q < npairs; ++q) {
             c->A = arena + 2 * npx * q;
             c->B = arena + 2 * npx * q + npx;
+            if (upload && k == 0) {
+                HH_HIP(hipMemcpyAsync(arena + 2 * npx * q, hA[q], sizeof(float) * npx, hipMemcpyHostToDevice,
+                                      c->copy_stream));
+                HH_HIP(hipMemcpyAsync(arena + 2 * npx * q + npx, hB[q], sizeof(float) * npx, hipMemcpyHostToDevice,
+                                      c->copy_stream));
+                // a ring event: the stream's wait takes the state recorded
+                // now, so re-recording it for a later pair is safe
+                hipEvent_t ev = c->upload_ev[q % c->upload_ev.size()];
+                HH_HIP(hipEventRecord(ev, c->copy_stream));
+                HH_HIP(hipStreamWaitEvent(c->stream, ev, 0));
+            }
             if (have_pred) {
                 do_prefilter(c);
                 do_deform(c);
@@ -583,6 +596,11 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
             } else {
                 ens_add(c, c->A, c->B);
             }
+        }
+        if (upload && k == 0) {
+            c->arena_pairs = npairs;
+            c->arena_nr = nr;
+            c->arena_nc = nc;
         }
         float *u = c->gu.p, *v = c->gv.p;
         if (have_pred) {
@@ -876,6 +894,11 @@ int hh_destroy(hh_ctx *c)
     }
     engine_release(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    for (auto e : c->upload_ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HH_OK;

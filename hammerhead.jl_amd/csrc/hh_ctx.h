@@ -198,6 +198,8 @@ struct hh_ctx {
     hh::host::DBuf<double> ensacc, ensstats;
     hh::host::DBuf<float> arena;  // the pairs of hh_run_piv_ensemble, resident in HBM
     int arena_pairs = 0, arena_nr = 0, arena_nc = 0;
+    hipStream_t copy_stream = nullptr;        // pair uploads overlapped with the first pass
+    std::vector<hipEvent_t> upload_ev;        // ring of upload-complete events
     // correlation energy and its adjoint
     hh::host::DBuf<float4> estats;
     hh::host::DBuf<float> ew, epart, egrad, enode;
