@@ -182,6 +182,22 @@ def time_batch(hh, passes, form, npairs, steps, log):
     dt = time.perf_counter() - t0
     res = [ctx.fetch_pair(q) for q in range(npairs)]
     ctx.close()
+    # run_piv_sequence from host frames, 16 pairs per batch on two alternating
+    # contexts (upload and schedule of batch i + 1 overlap batch i): pairs/s
+    # including the uploads and the result fetches
+    seq = None
+    if npairs >= 32:
+        ws = hh.piv_workspace()
+        frames = [(p.A, p.B) for p in pairs]
+        form_kw = dict(form=form)
+        hh.run_piv_sequence(frames, passes, batch=16, workspace=ws, **form_kw)  # plans, tunes, captures
+        t0 = time.perf_counter()
+        hh.run_piv_sequence(frames, passes, batch=16, workspace=ws, **form_kw)
+        dseq = time.perf_counter() - t0
+        ws.close()
+        seq = {"pairs": npairs, "batch": 16, "pairs_per_s": round(npairs / dseq, 3),
+               "includes": "host frames in, PIVResult out, two contexts alternating"}
+        log(f"sequence of {npairs} (batches of 16): {dseq:.3f} s")
     rms = [hh.synthetic.rms_error(r, p.flow, midpoint=True) for r, p in zip(res, pairs)]
     nwin = 0
     for p in pairs:
@@ -193,7 +209,7 @@ def time_batch(hh, passes, form, npairs, steps, log):
     return {"pairs": npairs, "sizes": "512x512, 768x1024, 1024x1280, 2048x2048 (rows x cols), cycled",
             "pairs_per_s": round(npairs * steps / dt, 3), "ms_per_batch": round(dt / steps * 1e3, 3),
             "megapixel_pairs_per_s": round(mpx * steps / dt, 2), "window_correlations_per_batch": nwin,
-            "rms_px_vs_midpoint_truth_median": round(float(np.median(rms)), 5)}
+            "rms_px_vs_midpoint_truth_median": round(float(np.median(rms)), 5), "sequence_end_to_end": seq}
 
 
 def time_ensemble(hh, npairs, steps, log):

@@ -130,6 +130,7 @@ struct Tuner {
 struct Engine {
     Layout lay;
     DBuf<float> atlas;       // owned batch atlas: frames A then B
+    DBuf<float> stage;       // upload staging of a frame narrower than the atlas
     DBuf<float> coefs;       // coefficient arrays: all A, then all B
     DBuf<float> warp;        // warp atlas: A then B
     DBuf<float> ptmp;        // prefilter scratch

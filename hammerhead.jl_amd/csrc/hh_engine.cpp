@@ -629,24 +629,47 @@ void engine_set_pairs(hh_ctx *c, const float *const *A, const float *const *B, c
         if (nr[q] < 2 || nc[q] < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
     }
     Engine &E = c->eng;
-    invalidate_plan(c);
     E.have_result = false;
     Layout &L = E.lay;
-    pack_layout(L, nr, nc, npairs);
-    if ((double)L.ld * (double)L.ncols > 4.0e9) throw Error(HH_EINVAL, "batch atlas exceeds 2^32 pixels");
+    // the same pair sizes in the same order as the resident batch (a stream
+    // of equally shaped batches): the frames go into the same atlas places
+    // and the plan and captured graph stay valid
+    const bool same = L.A && L.npairs == npairs && std::equal(nr, nr + npairs, L.nr.begin()) &&
+                      std::equal(nc, nc + npairs, L.nc.begin()) && L.A == E.atlas.p;
+    if (!same) {
+        invalidate_plan(c);
+        pack_layout(L, nr, nc, npairs);
+        if ((double)L.ld * (double)L.ncols > 4.0e9) throw Error(HH_EINVAL, "batch atlas exceeds 2^32 pixels");
+    }
     const size_t per = (size_t)L.ld * (size_t)L.ncols;
     float *at = E.atlas.ensure(2 * per);
+    // each frame crosses PCIe as one contiguous copy: straight into the
+    // atlas when its column length is the atlas's, else into a staging
+    // buffer and from there into its atlas place by a device-side 2D copy
+    // (a pitched copy from pageable host memory goes column by column)
+    size_t maxpx = 0;
+    for (int q = 0; q < npairs; ++q)
+        if (nr[q] != L.ld) maxpx = std::max(maxpx, (size_t)nr[q] * nc[q]);
+    float *stage = maxpx ? E.stage.ensure(maxpx) : nullptr;
     for (int q = 0; q < npairs; ++q) {
         float *dA = at + (size_t)L.roff[q] + (size_t)L.coff[q] * L.ld;
-        HH_HIP(hipMemcpy2DAsync(dA, sizeof(float) * L.ld, A[q], sizeof(float) * nr[q], sizeof(float) * nr[q], nc[q],
-                                hipMemcpyHostToDevice, c->stream));
-        HH_HIP(hipMemcpy2DAsync(dA + per, sizeof(float) * L.ld, B[q], sizeof(float) * nr[q], sizeof(float) * nr[q],
-                                nc[q], hipMemcpyHostToDevice, c->stream));
+        const size_t npx = (size_t)nr[q] * nc[q];
+        for (int f = 0; f < 2; ++f) {
+            const float *src = f ? B[q] : A[q];
+            float *dst = dA + (f ? per : 0);
+            if (nr[q] == L.ld) {
+                HH_HIP(hipMemcpyAsync(dst, src, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
+            } else {
+                HH_HIP(hipMemcpyAsync(stage, src, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
+                HH_HIP(hipMemcpy2DAsync(dst, sizeof(float) * L.ld, stage, sizeof(float) * nr[q], sizeof(float) * nr[q],
+                                        nc[q], hipMemcpyDeviceToDevice, c->stream));
+            }
+        }
     }
     HH_HIP(hipStreamSynchronize(c->stream));
     L.A = at;
     L.B = at + per;
-    L.gen = ++g_layout_gen;
+    if (!same) L.gen = ++g_layout_gen;
     // the single-pair state no longer describes the context's images
     c->A = c->B = nullptr;
     c->coefs_valid = c->warp_valid = c->have_result = false;

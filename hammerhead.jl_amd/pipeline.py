@@ -59,10 +59,20 @@ class PIVWorkspace:
 
     def __init__(self, backend: str = "mi355x", device: int = 0):
         self.backend = _resolve_backend(backend)
+        self.device = device
         self.ctx = _lib.Context(device)
+        self._ctx2 = None  # second context of a pipelined run_piv_sequence
+
+    def second_context(self):
+        if self._ctx2 is None:
+            self._ctx2 = _lib.Context(self.device)
+        return self._ctx2
 
     def close(self):
         self.ctx.close()
+        if self._ctx2 is not None:
+            self._ctx2.close()
+            self._ctx2 = None
 
 
 def piv_workspace(backend: str = "mi355x", device: int = 0) -> PIVWorkspace:
@@ -196,8 +206,11 @@ def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi
     packed into one device-resident atlas and run together: every sweep of
     every pass is one launch per stage over all pairs of the batch, and with
     fixed sweep counts the schedule replays as a hipGraph. Each pair's result
-    equals run_piv on that pair alone. ``on_result(i, result)`` (1-based i)
-    and a callable ``progress(i, n)`` run in pair order after each batch.
+    equals run_piv on that pair alone. With more than one batch, two device
+    contexts alternate: batch i + 1 is uploaded (and its schedule enqueued)
+    while batch i still runs, and batch i's results are fetched after that.
+    ``on_result(i, result)`` (1-based i) and a callable ``progress(i, n)`` run
+    in pair order after each batch.
     Frames given as file paths and ``output`` (JLD2 persistence) belong to the
     reference's I/O layer and are rejected."""
     if effort is not None:
@@ -233,14 +246,14 @@ def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi
     ws = workspace or PIVWorkspace(be)
     results: List[PIVResult] = []
     try:
-        ctx = ws.ctx
-        ctx.check_passes(passes)
-        if form != _lib.HH_FORM_FFT32:
+        ctxs = [ws.ctx] if step >= n else [ws.ctx, ws.second_context()]
+        for ctx in ctxs:
+            ctx.check_passes(passes)
             ctx.set_form(form)
-        for q0 in range(0, n, step):
-            chunk = frames[q0:q0 + step]
-            ctx.set_pairs(chunk)
-            for q, r in enumerate(ctx.run_piv_batch(passes, predictor_smoothing)):
+
+        def emit(ctx, q0):
+            for q in range(len(ctx.batch_shapes)):
+                r = ctx.fetch_pair(q)
                 res = PIVResult(r["x"], r["y"], r["u"], r["v"], r["peak_ratio"], r["correlation_moment"],
                                 r["uncertainty_u"], r["uncertainty_v"], r["outliers"], r["mask"], passes[-1], None,
                                 scale)
@@ -250,6 +263,16 @@ def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi
                     on_result(i, res)
                 if callable(progress):
                     progress(i, n)
+
+        pending = None
+        for b, q0 in enumerate(range(0, n, step)):
+            ctx = ctxs[b % len(ctxs)]
+            ctx.set_pairs(frames[q0:q0 + step])
+            ctx.run_piv_batch(passes, predictor_smoothing, fetch=False)
+            if pending is not None:
+                emit(*pending)
+            pending = (ctx, q0)
+        emit(*pending)
     finally:
         if workspace is None:
             ws.close()
