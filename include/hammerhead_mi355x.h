@@ -222,7 +222,9 @@ int hh_fetch_result(hh_ctx *ctx, hh_field *result);
 
 /* The frames of run_piv_sequence(pairs, params) (src/io.jl:229-247): npairs
  * host column-major pairs, pair q of nr[q] x nc[q], copied into one
- * device-resident image atlas. Replaces the context's current pair. */
+ * device-resident image atlas. Replaces the context's current pair. With
+ * the same pair sizes in the same order as the resident batch, only the
+ * frames change: the planned schedule (and its captured graph) is kept. */
 int hh_set_pairs(hh_ctx *ctx, const float *const *A, const float *const *B, const int32_t *nr,
                  const int32_t *nc, int npairs);
 /* run_piv(A[q], B[q], passes; predictor_smoothing) for every pair of the
