@@ -270,7 +270,7 @@ def pass_grid(pass_struct, nr, nc, prec="f64"):
 
 
 def process_windows(pass_struct, A, B, jobs, ny, nx, uncertainty=False, debug=False, planes=False, nthreads=8,
-                    prec="f64"):
+                    prec="f64", mask=None):
     """process_windows! over jobs (4 x n int32). Returns a dict of grids
     (residual u, v, peak_ratio, moment, alt_u, alt_v, unc_u, unc_v) and, with
     debug, per-job peak locs (2 x K x n), vals (K x n), found (n)."""
@@ -297,7 +297,7 @@ def process_windows(pass_struct, A, B, jobs, ny, nx, uncertainty=False, debug=Fa
                                      _p(res["peak_ratio"]), _p(res["correlation_moment"]),
                                      _p(res["alt_u"]) if pass_struct.n_peaks > 1 else None,
                                      _p(res["alt_v"]) if pass_struct.n_peaks > 1 else None, _p(uu), _p(uv),
-                                     _p(locs), _p(vals), _p(found), _p(pl), C.c_int(nthreads))
+                                     _p(locs), _p(vals), _p(found), _p(pl), _p(_mask(mask)), C.c_int(nthreads))
     if uncertainty:
         res["unc_u"], res["unc_v"] = uu, uv
     if debug:
@@ -307,7 +307,7 @@ def process_windows(pass_struct, A, B, jobs, ny, nx, uncertainty=False, debug=Fa
     return res
 
 
-def uncertainty_sweep(pass_struct, A, B, jobs, ny, nx, stats=False, nthreads=8, prec="f64"):
+def uncertainty_sweep(pass_struct, A, B, jobs, ny, nx, stats=False, nthreads=8, prec="f64", mask=None):
     dt = _dt(prec)
     A, B = _f(A, dt), _f(B, dt)
     jobs = _f(jobs, np.int32)
@@ -316,7 +316,8 @@ def uncertainty_sweep(pass_struct, A, B, jobs, ny, nx, stats=False, nthreads=8, 
     uv = np.full((ny, nx), np.nan, dtype=dt, order="F")
     st = np.zeros((44, 2, nj), dtype=np.float64, order="F") if stats else None
     _fn("orc_uncertainty_sweep", prec)(C.byref(pass_struct), _p(A), _p(B), C.c_int(A.shape[0]), _p(jobs),
-                                       C.c_int(nj), C.c_int(ny), _p(uu), _p(uv), _p(st), C.c_int(nthreads))
+                                       C.c_int(nj), C.c_int(ny), _p(uu), _p(uv), _p(st), _p(_mask(mask)),
+                                       C.c_int(nthreads))
     return (uu, uv, st) if stats else (uu, uv)
 
 
@@ -342,8 +343,22 @@ def field_change(u, v, pu, pv, prec="f64"):
     return float(fn(_p(u), _p(v), _p(pu), _p(pv), None, C.c_int(u.size)))
 
 
-def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64"):
-    """run_piv(imgA, imgB, passes): dict with the final PIVResult fields."""
+def _mask(mask):
+    return None if mask is None else np.asfortranarray(mask, dtype=np.uint8)
+
+
+def grid_mask(pass_struct, nr, nc, mask, threshold=0.5):
+    """pass_grid's grid-level mask (ny x nx bool) of an image-sized mask."""
+    x, y, _ = pass_grid(pass_struct, nr, nc)
+    gm = np.zeros((y.size, x.size), dtype=np.uint8, order="F")
+    lib().orc_grid_mask_f64(C.byref(pass_struct), C.c_int(nr), C.c_int(nc), _p(_mask(mask)), C.c_double(threshold),
+                            _p(gm))
+    return gm.astype(bool)
+
+
+def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask=None, mask_threshold=0.5):
+    """run_piv(imgA, imgB, passes; mask, mask_threshold): dict with the final
+    PIVResult fields (``mask``: the final grid mask)."""
     dt = _dt(prec)
     A, B = _f(A, dt), _f(B, dt)
     nr, nc = A.shape
@@ -353,15 +368,19 @@ def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64"):
     g = lambda: np.zeros((ny, nx), dtype=dt, order="F")
     res = dict(x=np.zeros(nx, dtype=dt), y=np.zeros(ny, dtype=dt), u=g(), v=g(), peak_ratio=g(),
                correlation_moment=g(), uncertainty_u=g(), uncertainty_v=g(),
-               outliers=np.zeros((ny, nx), dtype=np.uint8, order="F"))
+               outliers=np.zeros((ny, nx), dtype=np.uint8, order="F"),
+               mask=np.zeros((ny, nx), dtype=np.uint8, order="F"))
+    m = _mask(mask)
     rc = _fn("orc_run_piv", prec)(_p(A), _p(B), C.c_int(nr), C.c_int(nc), arr, C.c_int(len(passes)),
                                   C.c_int(1 if predictor_smoothing else 0), C.c_int(nthreads), _p(res["x"]),
                                   _p(res["y"]), _p(res["u"]), _p(res["v"]), _p(res["peak_ratio"]),
                                   _p(res["correlation_moment"]), _p(res["uncertainty_u"]),
-                                  _p(res["uncertainty_v"]), _p(res["outliers"]))
+                                  _p(res["uncertainty_v"]), _p(res["outliers"]), _p(m), C.c_double(mask_threshold),
+                                  _p(res["mask"]))
     if rc != 0:
         raise ValueError("invalid pass schedule")
     res["outliers"] = res["outliers"].astype(bool)
+    res["mask"] = res["mask"].astype(bool)
     return res
 
 

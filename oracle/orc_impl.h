@@ -255,19 +255,49 @@ static void F(orc_scratch_free)(F(orc_scratch) * s)
  * are apod .* (x - mean) at the top-left of the zeroed (padded) buffers;
  * forward FFT both, cross (or phase) spectrum, normalized inverse,
  * fftshift + magnitude, overlap gain. (rs, cs) are 1-based origins. */
-static void F(orc_plane)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *A, const REAL *B, int nr,
-                         int rs, int cs)
+/* Window means and apodized windows (load_windows!, src/correlators.jl:
+ * 199-227; load_uncertainty_windows!, src/uncertainty.jl:51-79): with a
+ * pixel mask (image-sized, nullable) and any masked pixel in the window, the
+ * means are over the valid pixels (summed in T, column-major) and masked
+ * pixels enter as zero; a clean window takes the unmasked path. */
+static int F(orc_window_means)(const REAL *A, const REAL *B, int nr, int rs, int cs, int wr, int wc,
+                               const uint8_t *mask, REAL *meanA, REAL *meanB)
 {
-    int wr = c->wr, wc = c->wc, m1 = c->m1;
-    size_t np = (size_t)c->m1 * c->m2;
+    int anym = 0;
+    if (mask)
+        for (int j = 0; j < wc && !anym; j++)
+            for (int i = 0; i < wr; i++)
+                if (mask[(size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr]) {
+                    anym = 1;
+                    break;
+                }
     REAL sA = 0, sB = 0;
+    int n = 0;
     for (int j = 0; j < wc; j++)
         for (int i = 0; i < wr; i++) {
             size_t g = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
+            if (anym && mask[g]) continue;
             sA += A[g];
             sB += B[g];
+            n++;
         }
-    REAL meanA = sA / (REAL)(wr * wc), meanB = sB / (REAL)(wr * wc);
+    if (anym) {
+        *meanA = n > 0 ? sA / (REAL)n : (REAL)0;
+        *meanB = n > 0 ? sB / (REAL)n : (REAL)0;
+    } else {
+        *meanA = sA / (REAL)(wr * wc);
+        *meanB = sB / (REAL)(wr * wc);
+    }
+    return anym;
+}
+
+static void F(orc_plane)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *A, const REAL *B, int nr,
+                         int rs, int cs, const uint8_t *mask)
+{
+    int wr = c->wr, wc = c->wc, m1 = c->m1;
+    size_t np = (size_t)c->m1 * c->m2;
+    REAL meanA, meanB;
+    const int anym = F(orc_window_means)(A, B, nr, rs, cs, wr, wc, mask, &meanA, &meanB);
     memset(s->c1r, 0, sizeof(REAL) * np);
     memset(s->c1i, 0, sizeof(REAL) * np);
     memset(s->c2r, 0, sizeof(REAL) * np);
@@ -275,6 +305,7 @@ static void F(orc_plane)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *
     for (int j = 0; j < wc; j++)
         for (int i = 0; i < wr; i++) {
             size_t g = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
+            if (anym && mask[g]) continue;  /* masked pixels stay zero */
             REAL a = c->apod[i + (size_t)j * wr];
             s->c1r[i + (size_t)j * m1] = a * (A[g] - meanA);
             s->c2r[i + (size_t)j * m1] = a * (B[g] - meanB);
@@ -574,24 +605,23 @@ static void F(orc_uq_component)(double *s, const REAL *A, const REAL *B, int nr,
     }
 }
 
-/* accumulate_uncertainty! with load_uncertainty_windows! (unmasked)
- * (src/uncertainty.jl:51-96). stats: 2 x 44, row-major per component. */
+/* accumulate_uncertainty! with load_uncertainty_windows!
+ * (src/uncertainty.jl:51-96); mask nullable (image-sized). stats: 2 x 44,
+ * row-major per component. */
 void F(orc_uq_accumulate)(double *stats, const REAL *A, const REAL *B, int nr, int rs, int cs, int wr, int wc,
-                          const REAL *apod, REAL *wA, REAL *wB, REAL *dC, REAL *dCs)
+                          const REAL *apod, REAL *wA, REAL *wB, REAL *dC, REAL *dCs, const uint8_t *mask)
 {
     int off[ORC_UQ_NOFF][2], rings[ORC_UQ_MAXOFF + 1];
     orc_uq_offsets(off, rings);
-    REAL sA = 0, sB = 0;
+    REAL meanA, meanB;
+    const int anym = F(orc_window_means)(A, B, nr, rs, cs, wr, wc, mask, &meanA, &meanB);
     for (int j = 0; j < wc; j++)
         for (int i = 0; i < wr; i++) {
             size_t g = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
-            sA += A[g];
-            sB += B[g];
-        }
-    REAL meanA = sA / (REAL)(wr * wc), meanB = sB / (REAL)(wr * wc);
-    for (int j = 0; j < wc; j++)
-        for (int i = 0; i < wr; i++) {
-            size_t g = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
+            if (anym && mask[g]) {
+                wA[i + j * wr] = wB[i + j * wr] = (REAL)0;
+                continue;
+            }
             REAL a = apod[i + j * wr];
             wA[i + j * wr] = a * (A[g] - meanA);
             wB[i + j * wr] = a * (B[g] - meanB);
@@ -1057,6 +1087,7 @@ typedef struct {
     REAL *vals_out;
     int32_t *found_out;
     REAL *planes;
+    const uint8_t *mask; /* image-sized pixel mask, nullable */
 } F(orc_pw_args);
 
 static void F(orc_pw_range)(void *arg, int j0, int j1)
@@ -1078,7 +1109,7 @@ static void F(orc_pw_range)(void *arg, int j0, int j1)
     for (int jb = j0; jb < j1; jb++) {
         int gi = a->jobs[4 * jb], gj = a->jobs[4 * jb + 1], rs = a->jobs[4 * jb + 2], cs = a->jobs[4 * jb + 3];
         size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * a->ny;
-        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs);
+        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs, a->mask);
         if (a->planes) memcpy(a->planes + np * jb, s.R, sizeof(REAL) * np);
         REAL out[4];
         int found = F(orc_analyze_plane)(s.R, a->c->m1, a->c->m2, p, vals, locs, out, altu, altv);
@@ -1103,7 +1134,7 @@ static void F(orc_pw_range)(void *arg, int j0, int j1)
         if (a->unc_u) {
             double st[2 * HH_UQ_NSTATS];
             memset(st, 0, sizeof st);
-            F(orc_uq_accumulate)(st, a->A, a->B, a->nr, rs, cs, wr, wc, a->c->apod, wA, wB, dC, dCs);
+            F(orc_uq_accumulate)(st, a->A, a->B, a->nr, rs, cs, wr, wc, a->c->apod, wA, wB, dC, dCs, a->mask);
             a->unc_u[g] = F(orc_uq_finalize)(st);
             a->unc_v[g] = F(orc_uq_finalize)(st + HH_UQ_NSTATS);
         }
@@ -1114,17 +1145,17 @@ static void F(orc_pw_range)(void *arg, int j0, int j1)
 
 /* process_windows! over a job list (jobs: int32 4 x njobs, 1-based).
  * Debug outputs (locs 2 x K x njobs, vals K x njobs, found, planes
- * m1 x m2 x njobs) nullable. */
+ * m1 x m2 x njobs) and the pixel mask nullable. */
 void F(orc_process_windows)(const hh_pass *p, const REAL *A, const REAL *B, int nr, int nc, const int32_t *jobs,
                             int njobs, int ny, int nx, REAL *u, REAL *v, REAL *pr, REAL *cm, REAL *alt_u,
                             REAL *alt_v, REAL *unc_u, REAL *unc_v, int32_t *locs, REAL *vals, int32_t *found,
-                            REAL *planes, int nthreads)
+                            REAL *planes, const uint8_t *mask, int nthreads)
 {
     (void)nc;
     F(orc_corr) c;
     F(orc_corr_init)(&c, p);
     F(orc_pw_args) a = {p, &c, A, B, nr, jobs, ny, nx, u, v, pr, cm, alt_u, alt_v, unc_u, unc_v,
-                        locs, vals, found, planes};
+                        locs, vals, found, planes, mask};
     orc_parallel_for(njobs, nthreads, F(orc_pw_range), &a);
     F(orc_corr_free)(&c);
 }
@@ -1136,7 +1167,7 @@ void F(orc_correlation_plane)(const hh_pass *p, const REAL *subA, const REAL *su
     F(orc_corr_init)(&c, p);
     F(orc_scratch) s;
     F(orc_scratch_init)(&s, &c);
-    F(orc_plane)(&c, &s, subA, subB, c.wr, 1, 1);
+    F(orc_plane)(&c, &s, subA, subB, c.wr, 1, 1, NULL);
     memcpy(R, s.R, sizeof(REAL) * c.m1 * c.m2);
     F(orc_scratch_free)(&s);
     F(orc_corr_free)(&c);
@@ -1158,6 +1189,7 @@ typedef struct {
     const int32_t *jobs;
     REAL *unc_u, *unc_v;
     double *stats;
+    const uint8_t *mask;
 } F(orc_uq_args);
 
 static void F(orc_uq_range)(void *arg, int j0, int j1)
@@ -1171,7 +1203,7 @@ static void F(orc_uq_range)(void *arg, int j0, int j1)
         int gi = a->jobs[4 * jb], gj = a->jobs[4 * jb + 1], rs = a->jobs[4 * jb + 2], cs = a->jobs[4 * jb + 3];
         double st[2 * HH_UQ_NSTATS];
         memset(st, 0, sizeof st);
-        F(orc_uq_accumulate)(st, a->A, a->B, a->nr, rs, cs, wr, wc, a->apod, wA, wB, dC, dCs);
+        F(orc_uq_accumulate)(st, a->A, a->B, a->nr, rs, cs, wr, wc, a->apod, wA, wB, dC, dCs, a->mask);
         size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * a->ny;
         if (a->unc_u) a->unc_u[g] = F(orc_uq_finalize)(st);
         if (a->unc_v) a->unc_v[g] = F(orc_uq_finalize)(st + HH_UQ_NSTATS);
@@ -1181,13 +1213,13 @@ static void F(orc_uq_range)(void *arg, int j0, int j1)
 }
 
 /* uncertainty_sweep! (src/pipeline.jl:873-899). stats nullable
- * (2 x 44 per job: component-major). */
+ * (2 x 44 per job: component-major); mask nullable (image-sized). */
 void F(orc_uncertainty_sweep)(const hh_pass *p, const REAL *A, const REAL *B, int nr, const int32_t *jobs, int njobs,
-                              int ny, REAL *unc_u, REAL *unc_v, double *stats, int nthreads)
+                              int ny, REAL *unc_u, REAL *unc_v, double *stats, const uint8_t *mask, int nthreads)
 {
     REAL *apod = (REAL *)malloc(sizeof(REAL) * p->window_size[0] * p->window_size[1]);
     F(orc_apodization)(p->window_size[0], p->window_size[1], p->apodization, apod);
-    F(orc_uq_args) a = {p, A, B, apod, nr, ny, jobs, unc_u, unc_v, stats};
+    F(orc_uq_args) a = {p, A, B, apod, nr, ny, jobs, unc_u, unc_v, stats, mask};
     orc_parallel_for(njobs, nthreads, F(orc_uq_range), &a);
     free(apod);
 }
@@ -1228,6 +1260,31 @@ int F(orc_pass_grid)(const hh_pass *p, int nr, int nc, int *ny, int *nx, REAL *x
     return 0;
 }
 
+/* The grid-level mask of pass_grid (src/pipeline.jl:542-549): a window whose
+ * masked-pixel fraction (of the window or of its search area) reaches
+ * thr produces no vector. gmask: ny x nx, column-major. */
+void F(orc_grid_mask)(const hh_pass *p, int nr, int nc, const uint8_t *mask, double thr, uint8_t *gmask)
+{
+    int ny, nx;
+    if (F(orc_pass_grid)(p, nr, nc, &ny, &nx, NULL, NULL, NULL) != 0) return;
+    int wr = p->window_size[0], wc = p->window_size[1];
+    int sr = p->search_area_size[0], sc = p->search_area_size[1];
+    int mr = (sr - wr) / 2, mc = (sc - wc) / 2;
+    int r0 = 1 + mr, rstep = wr - p->overlap[0], c0 = 1 + mc, cstep = wc - p->overlap[1];
+    for (int gj = 0; gj < nx; gj++)
+        for (int gi = 0; gi < ny; gi++) {
+            const int rs = r0 + gi * rstep, cs = c0 + gj * cstep;
+            long nw = 0, ns = 0;
+            for (int j = 0; j < wc; j++)
+                for (int i = 0; i < wr; i++) nw += mask[(size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr];
+            const int srs = rs - mr, scs = cs - mc;
+            for (int j = 0; j < sc; j++)
+                for (int i = 0; i < sr; i++) ns += mask[(size_t)(srs - 1 + i) + (size_t)(scs - 1 + j) * nr];
+            const double fw = (double)nw / (double)(wr * wc), fs = (double)ns / (double)(sr * sc);
+            gmask[gi + (size_t)gj * ny] = fw >= thr || fs >= thr;
+        }
+}
+
 /* field_change (src/pipeline.jl:849-866): q95 of max(|du|, |dv|) over
  * unmasked cells, NaN<->finite flips count as Inf. */
 REAL F(orc_field_change)(const REAL *u, const REAL *v, const REAL *pu, const REAL *pv, const uint8_t *mask, int n)
@@ -1254,13 +1311,39 @@ typedef struct {
     REAL *x, *y, *u, *v; /* predictor grid (smoothed) */
 } F(orc_pred);
 
-/* run_piv(imgA, imgB, passes) (src/pipeline.jl:361-449) with piv_pass
- * (src/pipeline.jl:655-832), unmasked. Outputs the final pass's PIVResult
- * into caller arrays sized by the final grid (any may be NULL except the
- * ones needed). Returns 0, or -1 for an invalid schedule. */
+/* build_predictor (src/pipeline.jl:454-464): masked cells (NaN) are filled
+ * from valid neighbours (replace_vectors!) on copies, then smoothed. */
+static void F(orc_build_predictor)(const REAL *u, const REAL *v, const uint8_t *gmask, int ny, int nx, int smoothing,
+                                   REAL *pu, REAL *pv)
+{
+    size_t ng = (size_t)ny * nx;
+    int anyg = 0;
+    if (gmask)
+        for (size_t q = 0; q < ng; q++) anyg |= gmask[q];
+    REAL *tu = (REAL *)malloc(sizeof(REAL) * ng), *tv = (REAL *)malloc(sizeof(REAL) * ng);
+    memcpy(tu, u, sizeof(REAL) * ng);
+    memcpy(tv, v, sizeof(REAL) * ng);
+    if (anyg) F(orc_replace)(tu, tv, gmask, ny, nx);
+    if (smoothing) {
+        F(orc_smooth)(tu, ny, nx, pu);
+        F(orc_smooth)(tv, ny, nx, pv);
+    } else {
+        memcpy(pu, tu, sizeof(REAL) * ng);
+        memcpy(pv, tv, sizeof(REAL) * ng);
+    }
+    free(tu);
+    free(tv);
+}
+
+/* run_piv(imgA, imgB, passes; mask, mask_threshold) (src/pipeline.jl:
+ * 361-449) with piv_pass (src/pipeline.jl:655-832). mask: image-sized pixel
+ * mask, nullable; omask (nullable) receives the final grid mask. Outputs the
+ * final pass's PIVResult into caller arrays sized by the final grid (any may
+ * be NULL except the ones needed). Returns 0, or -1 for an invalid schedule. */
 int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_pass *passes, int npasses,
                    int predictor_smoothing, int nthreads, REAL *ox, REAL *oy, REAL *ou, REAL *ov, REAL *opr,
-                   REAL *ocm, REAL *ouu, REAL *ouv, uint8_t *oout)
+                   REAL *ocm, REAL *ouu, REAL *ouv, uint8_t *oout, const uint8_t *mask, double mask_threshold,
+                   uint8_t *omask)
 {
     if (npasses < 1) return -1;
     int deforms = npasses > 1;
@@ -1280,6 +1363,7 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
     /* previous result */
     int pny = 0, pnx = 0;
     REAL *px = NULL, *py = NULL, *pu = NULL, *pv = NULL;
+    uint8_t *pmask = NULL; /* the previous pass's grid mask */
     int rc = 0;
     for (int k = 0; k < npasses; k++) {
         const hh_pass *p = &passes[k];
@@ -1292,15 +1376,10 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
             pred.x = px; pred.y = py;
             pred.u = (REAL *)malloc(sizeof(REAL) * pny * pnx);
             pred.v = (REAL *)malloc(sizeof(REAL) * pny * pnx);
-            if (predictor_smoothing) {
-                F(orc_smooth)(pu, pny, pnx, pred.u);
-                F(orc_smooth)(pv, pny, pnx, pred.v);
-            } else {
-                memcpy(pred.u, pu, sizeof(REAL) * pny * pnx);
-                memcpy(pred.v, pv, sizeof(REAL) * pny * pnx);
-            }
-            free(pu); free(pv);
+            F(orc_build_predictor)(pu, pv, pmask, pny, pnx, predictor_smoothing, pred.u, pred.v);
+            free(pu); free(pv); free(pmask);
             px = py = pu = pv = NULL;
+            pmask = NULL;
             have_pred = 1;
         }
         int ny, nx;
@@ -1312,6 +1391,19 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
         REAL *x = (REAL *)malloc(sizeof(REAL) * nx), *y = (REAL *)malloc(sizeof(REAL) * ny);
         int32_t *jobs = (int32_t *)malloc(sizeof(int32_t) * 4 * (ng > 0 ? ng : 1));
         F(orc_pass_grid)(p, nr, nc, &ny, &nx, x, y, jobs);
+        /* grid mask: masked windows produce no job */
+        uint8_t *gmask = (uint8_t *)calloc(ng > 0 ? ng : 1, 1);
+        int anyg = 0, njobs = (int)ng;
+        if (mask) {
+            F(orc_grid_mask)(p, nr, nc, mask, mask_threshold, gmask);
+            njobs = 0;
+            for (size_t q = 0; q < ng; q++) {
+                anyg |= gmask[q];
+                if (gmask[q]) continue;
+                memmove(jobs + 4 * (size_t)njobs, jobs + 4 * q, 4 * sizeof(int32_t));
+                njobs++;
+            }
+        }
         int maxiter = p->max_iterations;
         REAL *ru = (REAL *)calloc(ng, sizeof(REAL)), *rv = (REAL *)calloc(ng, sizeof(REAL));
         REAL *pr = (REAL *)calloc(ng, sizeof(REAL)), *cm = (REAL *)calloc(ng, sizeof(REAL));
@@ -1345,8 +1437,9 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
             memset(rv, 0, sizeof(REAL) * ng);
             if (au)
                 for (size_t q = 0; q < ng * nalt; q++) au[q] = av[q] = (REAL)NAN;
-            F(orc_process_windows)(p, srcA, srcB, nr, nc, jobs, (int)ng, ny, nx, ru, rv, pr, cm, au, av,
-                                   fused_unc ? uu : NULL, fused_unc ? uv : NULL, NULL, NULL, NULL, NULL, nthreads);
+            F(orc_process_windows)(p, srcA, srcB, nr, nc, jobs, njobs, ny, nx, ru, rv, pr, cm, au, av,
+                                   fused_unc ? uu : NULL, fused_unc ? uv : NULL, NULL, NULL, NULL, NULL, mask,
+                                   nthreads);
             if (au)
                 for (int m = 0; m < nalt; m++)
                     for (size_t q = 0; q < ng; q++) {
@@ -1357,11 +1450,15 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
                 u[q] += ru[q];
                 v[q] += rv[q];
             }
+            if (anyg)
+                for (size_t q = 0; q < ng; q++)
+                    if (gmask[q]) u[q] = v[q] = pr[q] = cm[q] = (REAL)NAN;
             if (keep_measured) {
                 memcpy(mu_, u, sizeof(REAL) * ng);
                 memcpy(mv_, v, sizeof(REAL) * ng);
             }
-            F(orc_validate_and_replace)(p, ny, nx, force_replace || maxiter > 1, u, v, pr, cm, au, av, NULL, outl);
+            F(orc_validate_and_replace)(p, ny, nx, force_replace || maxiter > 1, u, v, pr, cm, au, av,
+                                        anyg ? gmask : NULL, outl);
             if (it == maxiter) break;
             if (p->convergence_tol > 0) {
                 if (!prevu) {
@@ -1370,7 +1467,7 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
                     memcpy(prevu, u, sizeof(REAL) * ng);
                     memcpy(prevv, v, sizeof(REAL) * ng);
                 } else {
-                    REAL ch = F(orc_field_change)(u, v, prevu, prevv, NULL, (int)ng);
+                    REAL ch = F(orc_field_change)(u, v, prevu, prevv, anyg ? gmask : NULL, (int)ng);
                     if (ch < (REAL)p->convergence_tol) break;
                     memcpy(prevu, u, sizeof(REAL) * ng);
                     memcpy(prevv, v, sizeof(REAL) * ng);
@@ -1393,18 +1490,12 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
             pred.nx = nx;
             memcpy(pred.x, x, sizeof(REAL) * nx);
             memcpy(pred.y, y, sizeof(REAL) * ny);
-            if (predictor_smoothing) {
-                F(orc_smooth)(u, ny, nx, pred.u);
-                F(orc_smooth)(v, ny, nx, pred.v);
-            } else {
-                memcpy(pred.u, u, sizeof(REAL) * ng);
-                memcpy(pred.v, v, sizeof(REAL) * ng);
-            }
+            F(orc_build_predictor)(u, v, anyg ? gmask : NULL, ny, nx, predictor_smoothing, pred.u, pred.v);
             have_pred = 1;
             if (!deforms) break; /* unreachable: iterating passes deform */
         }
         if (unc && !fused_unc)
-            F(orc_uncertainty_sweep)(p, srcA, srcB, nr, jobs, (int)ng, ny, uu, uv, NULL, nthreads);
+            F(orc_uncertainty_sweep)(p, srcA, srcB, nr, jobs, njobs, ny, uu, uv, NULL, mask, nthreads);
         if (keep_measured)
             for (size_t q = 0; q < ng; q++)
                 if (outl[q]) {
@@ -1421,16 +1512,19 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
             if (ouu) memcpy(ouu, uu, sizeof(REAL) * ng);
             if (ouv) memcpy(ouv, uv, sizeof(REAL) * ng);
             if (oout) memcpy(oout, outl, ng);
+            if (omask) memcpy(omask, gmask, ng);
             free(x); free(y); free(u); free(v);
+            free(gmask);
         } else {
             px = x; py = y; pu = u; pv = v;
             pny = ny; pnx = nx;
+            pmask = gmask;
         }
         free(jobs); free(ru); free(rv); free(pr); free(cm); free(au); free(av);
         free(uu); free(uv); free(mu_); free(mv_); free(prevu); free(prevv); free(outl);
     }
     free(pred.x); free(pred.y); free(pred.u); free(pred.v);
-    free(px); free(py); free(pu); free(pv);
+    free(px); free(py); free(pu); free(pv); free(pmask);
     free(cA); free(cB); free(wA); free(wB);
     return rc;
 }
@@ -1463,12 +1557,12 @@ static void F(orc_ens_add_range)(void *arg, int j0, int j1)
     size_t np = (size_t)a->c->m1 * a->c->m2;
     for (int jb = j0; jb < j1; jb++) {
         int rs = a->jobs[4 * jb + 2], cs = a->jobs[4 * jb + 3];
-        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs);
+        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs, NULL);
         double *dst = a->acc + np * jb;
         for (size_t e = 0; e < np; e++) dst[e] += (double)s.R[e];
         if (a->ust)
             F(orc_uq_accumulate)(a->ust + (size_t)2 * HH_UQ_NSTATS * jb, a->A, a->B, a->nr, rs, cs, wr, wc,
-                                 a->c->apod, wA, wB, dC, dCs);
+                                 a->c->apod, wA, wB, dC, dCs, NULL);
     }
     free(wA); free(wB); free(dC); free(dCs);
     F(orc_scratch_free)(&s);
