@@ -29,7 +29,7 @@ EXPORTS = (
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
-    "hh_set_autotune", "hh_tune_report", "hh_tune_set",
+    "hh_set_autotune", "hh_tune_report", "hh_tune_set", "hh_set_mask",
 )
 
 
@@ -103,6 +103,7 @@ def load():
         "hh_set_autotune": (i32, [vp, i32]),
         "hh_tune_report": (i32, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "hh_tune_set": (i32, [vp, C.c_char_p, i32, i32]),
+        "hh_set_mask": (i32, [vp, vp, i32, i32, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -188,6 +189,16 @@ class Context:
         self.nr, self.nc = nr, nc
         _check(self.h, load().hh_set_pair_device(self.h, C.c_void_p(dA_ptr), C.c_void_p(dB_ptr), nr, nc),
                "set_pair_device")
+
+    def set_mask(self, mask=None, mask_threshold=0.5):
+        """The mask / mask_threshold of every following run_piv and
+        run_piv_batch (None removes it); an identical mask keeps the plan."""
+        if mask is None:
+            _check(self.h, load().hh_set_mask(self.h, None, 0, 0, float(mask_threshold)), "set_mask")
+            return
+        m = np.asfortranarray(np.asarray(mask, dtype=bool), dtype=np.uint8)
+        _check(self.h, load().hh_set_mask(self.h, _ptr(m), m.shape[0], m.shape[1], float(mask_threshold)),
+               "set_mask")
 
     def prefilter(self):
         _check(self.h, load().hh_prefilter(self.h), "prefilter")

@@ -1167,9 +1167,9 @@ int hh_validate_and_replace(hh_ctx *c, const hh_pass *pass, int ny, int nx, int 
         if (pass->uod_neighborhood > 3) throw Error(HH_EUNSUPPORTED, "uod_neighborhood up to 3 on the device");
         if (ny < 1 || nx < 1) throw Error(HH_EDIM, "empty grid");
         const size_t ng = (size_t)ny * nx;
+        bool anym = false;
         if (mask)
-            for (size_t k = 0; k < ng; ++k)
-                if (mask[k]) throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks yet");
+            for (size_t k = 0; k < ng && !anym; ++k) anym = mask[k] != 0;
         const int nalt = pass->n_peaks - 1;
         ValidateArgs va{};
         va.grids = one_grid(ny, nx);
@@ -1188,6 +1188,14 @@ int hh_validate_and_replace(hh_ctx *c, const hh_pass *pass, int ny, int nx, int 
         va.sv = c->gsv.ensure(ng);
         va.sflag = c->gsflag.ensure(ng);
         va.count = c->gcount.ensure(1);
+        if (anym) {
+            uint8_t *dm = c->gmask.ensure(ng);
+            std::vector<uint8_t> m(ng);
+            for (size_t k = 0; k < ng; ++k) m[k] = mask[k] != 0;
+            HH_HIP(hipMemcpyAsync(dm, m.data(), ng, hipMemcpyHostToDevice, c->stream));
+            HH_HIP(hipStreamSynchronize(c->stream));
+            va.mask = dm;
+        }
         c->have_result = false;
         timed(c, "validate", pass->window_size[0], [&] { launch_validate(va, c->stream); });
         download(c, u, va.u, ng);
@@ -1232,6 +1240,11 @@ int hh_fetch_result(hh_ctx *c, hh_field *result)
         else
             fetch(c, result);
     });
+}
+
+int hh_set_mask(hh_ctx *c, const uint8_t *mask, int nr, int nc, double mask_threshold)
+{
+    return guarded(c, [&] { engine_set_mask(c, mask, nr, nc, mask_threshold); });
 }
 
 int hh_set_pairs(hh_ctx *c, const float *const *A, const float *const *B, const int32_t *nr, const int32_t *nc,

@@ -95,6 +95,12 @@ struct PassTables {
     DeformPair hprev{}, hself{};  // host copies when the tables hold one pair
     int blocks = 0;           // deformation blocks
     DBuf<int> count;          // one valid-cell counter per segment (saturating at 3)
+    // windows with masked pixels below the grid-mask threshold: the tile
+    // builder's list (4 x npart, 0-based image row / column and mask row /
+    // column) and the correlation jobs on the tile atlas (cell + 1, 1, 1,
+    // t w + 1)
+    DBuf<int32_t> mjobs, tjobs;
+    int npart = 0;
 };
 
 struct PassPlan {
@@ -107,13 +113,23 @@ struct PassPlan {
     int total = 0;                           // cells over all pairs
     bool fixed = true;                       // sweep count known in advance
     PassTables all;
+    // grid mask of pass_grid (src/pipeline.jl:542-549): every pair has the
+    // mask's size, so one ny x nx grid (host) serves all; the device copy
+    // covers the buffer cells of every pair
+    std::vector<uint8_t> gm;
+    DBuf<uint8_t> gmask;
+    bool anyg = false;
 };
 
 // Host copies of one pass's job lists (rebuilt into subsets when pairs stop
 // iterating).
 struct HostJobs {
     std::vector<int32_t> jobs;
-    std::vector<int> off;  // per pair: index of its first job
+    std::vector<int> off, cnt;  // per pair: index of its first job, job count
+    // windows with masked pixels, 5 ints each: buffer cell, image row and
+    // column in the atlas, row and column in the mask (0-based)
+    std::vector<int32_t> part;
+    std::vector<int> poff, pcnt;
 };
 
 // Autotuned launch geometry, keyed by kernel family and problem shape: the
@@ -150,6 +166,17 @@ struct Engine {
     bool have_result = false;
     // tables of an active subset of a sweep (dynamic convergence)
     PassTables sub;
+    // pixel mask of run_piv(...; mask, mask_threshold), shared by every pair
+    // (host and device copies, column-major mnr x mnc), its threshold, and a
+    // generation that changes whenever either changes
+    std::vector<uint8_t> hmask;
+    int mnr = 0, mnc = 0;
+    double mthr = 0.5;
+    uint64_t mgen = 0;
+    DBuf<uint8_t> dmask;
+    DBuf<float> tiles;       // tile atlas of the masked windows: A then B
+    size_t maxtile = 0;      // floats per frame
+    DBuf<int> mcount;        // scratch counters of the predictor's fill
 };
 
 }  // namespace host
@@ -177,7 +204,7 @@ struct hh_ctx {
     std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<hh::host::DBuf<int32_t>>> jobcache;
     // chain buffers of the ensemble driver
     hh::host::DBuf<float> gu, gv, gpr, gcm, galtu, galtv, guu, guv, gsu, gsv, gpx, gpy;
-    hh::host::DBuf<uint8_t> gout, gsflag;
+    hh::host::DBuf<uint8_t> gout, gsflag, gmask;
     hh::host::DBuf<int> gcount;
     hh::host::DBuf<int32_t> hookjobs;
     hh::host::DBuf<float> hk[12];
@@ -259,6 +286,8 @@ void engine_set_pairs(hh_ctx *c, const float *const *A, const float *const *B, c
 void engine_run(hh_ctx *c, const hh_pass *passes, int np, int smoothing);
 // The final PIVResult of pair q.
 void engine_fetch(hh_ctx *c, int q, hh_field *r);
+// The pixel mask of every following run (mask null: none).
+void engine_set_mask(hh_ctx *c, const uint8_t *mask, int nr, int nc, double threshold);
 void engine_release(hh_ctx *c);
 void engine_set_graphs(hh_ctx *c, bool enable);
 std::string engine_tune_report(hh_ctx *c);

@@ -214,25 +214,35 @@ std::vector<DeformPair> deform_table(hh_ctx *c, const std::vector<int> &pairs, c
     return t;
 }
 
-// Tables of pass P over the pairs in `pairs` (jobs, segments, counters and
-// the deformation tables); prev is the previous pass (predictor source of
-// the first sweep) or null.
-void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vector<int> &pairs,
-                  const std::vector<int32_t> &hjobs_all, const std::vector<int> &job_off, PassTables &T)
+// Tables of pass P over the pairs in `pairs` (jobs, windows with masked
+// pixels, segments, counters and the deformation tables); prev is the
+// previous pass (predictor source of the first sweep) or null.
+void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vector<int> &pairs, const HostJobs &H,
+                  PassTables &T)
 {
-    std::vector<int32_t> jobs;
+    std::vector<int32_t> jobs, mjobs, tjobs;
     std::vector<GridSeg> segs;
+    const int w = P.p.window_size[0];
     int start = 0;
     for (size_t s = 0; s < pairs.size(); ++s) {
         const int q = pairs[s];
         const int n = P.ny[q] * P.nx[q];
-        jobs.insert(jobs.end(), hjobs_all.begin() + 4 * (size_t)job_off[q],
-                    hjobs_all.begin() + 4 * ((size_t)job_off[q] + n));
+        jobs.insert(jobs.end(), H.jobs.begin() + 4 * (size_t)H.off[q],
+                    H.jobs.begin() + 4 * ((size_t)H.off[q] + H.cnt[q]));
+        for (int t = 0; t < H.pcnt[q]; ++t) {
+            const int32_t *e = &H.part[5 * ((size_t)H.poff[q] + t)];
+            const int k = (int)(tjobs.size() / 4);
+            tjobs.insert(tjobs.end(), {e[0] + 1, 1, 1, k * w + 1});
+            mjobs.insert(mjobs.end(), {e[1], e[2], e[3], e[4]});
+        }
         segs.push_back(GridSeg{start, P.off[q], P.ny[q], P.nx[q], (int)s, {0, 0, 0}});
         start += n;
     }
     T.njobs = (int)(jobs.size() / 4);
+    T.npart = (int)(tjobs.size() / 4);
     upload_sync(c, T.jobs, jobs);
+    upload_sync(c, T.mjobs, mjobs);
+    upload_sync(c, T.tjobs, tjobs);
     upload_sync(c, T.segs, segs);
     T.grids = Grids{T.segs.p, (int)segs.size(), 0, 0, start, P.total};
     // one grid starting at buffer cell 0: no table lookups in the kernels
@@ -367,12 +377,46 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
     return cfg;
 }
 
+// Masked-pixel counts of rectangles from a summed-area table of the mask
+// (mod 2^32: every count asked for is below 2^32, so the wrapped prefix
+// sums still difference exactly).
+struct MaskCounts {
+    std::vector<uint32_t> S;
+    int h = 0;
+    void build(const std::vector<uint8_t> &m, int nr, int nc)
+    {
+        h = nr + 1;
+        S.assign((size_t)(nr + 1) * (nc + 1), 0u);
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nr; ++i)
+                S[(i + 1) + (size_t)(j + 1) * h] = (uint32_t)m[i + (size_t)j * nr] + S[i + (size_t)(j + 1) * h] +
+                                                   S[(i + 1) + (size_t)j * h] - S[i + (size_t)j * h];
+    }
+    // rows r0 .. r0 + nr - 1, columns c0 .. c0 + nc - 1 (0-based)
+    uint32_t rect(int r0, int c0, int nr, int nc) const
+    {
+        const int r1 = r0 + nr, c1 = c0 + nc;
+        return S[r1 + (size_t)c1 * h] - S[r0 + (size_t)c1 * h] - S[r1 + (size_t)c0 * h] + S[r0 + (size_t)c0 * h];
+    }
+};
+
 void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
 {
     Engine &E = c->eng;
     const Layout &L = E.lay;
     invalidate_plan(c);
     E.smoothing = smoothing;
+    const bool masked = !E.hmask.empty();
+    MaskCounts mc;
+    if (masked) {
+        for (int q = 0; q < L.npairs; ++q)
+            if (L.nr[q] != E.mnr || L.nc[q] != E.mnc)
+                throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(E.mnr) +
+                                         ", " + std::to_string(E.mnc) + ") for images of (" +
+                                         std::to_string(L.nr[q]) + ", " + std::to_string(L.nc[q]) + ")");
+        mc.build(E.hmask, E.mnr, E.mnc);
+    }
+    E.maxtile = 0;
     E.deforms = np > 1;
     for (int k = 0; k < np; ++k) E.deforms |= passes[k].max_iterations > 1;
     std::vector<int> all(L.npairs);
@@ -394,22 +438,53 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         P->half = (float)((w - 1) / 2.0);
         P->fixed = p.max_iterations == 1 || p.convergence_tol == 0;
         HostJobs &H = E.hjobs[k];
+        // per window: 0 clean, 1 with masked pixels, 2 grid-masked (no job)
+        std::vector<uint8_t> kind;
         for (int q = 0; q < L.npairs; ++q) {
             PassGrid g = make_grid(p, L.nr[q], L.nc[q]);
-            if ((size_t)g.ny * g.nx == 0) throw Error(HH_EINVAL, "the pass grid is empty");
-            if ((long long)P->total + (long long)g.ny * g.nx > (1LL << 30))
-                throw Error(HH_EINVAL, "too many vectors in one batch");
+            const size_t ng = (size_t)g.ny * g.nx;
+            if (ng == 0) throw Error(HH_EINVAL, "the pass grid is empty");
+            if ((long long)P->total + (long long)ng > (1LL << 30)) throw Error(HH_EINVAL, "too many vectors in one batch");
+            if (masked && kind.empty()) {
+                // pass_grid (src/pipeline.jl:542-549): the masked fraction of
+                // the window (the search area is the window on the device)
+                const int wr = p.window_size[0], wc = p.window_size[1];
+                kind.assign(ng, 0);
+                P->gm.assign(ng, 0);
+                for (size_t j = 0; j < ng; ++j) {
+                    const uint32_t nm = mc.rect(g.jobs[4 * j + 2] - 1, g.jobs[4 * j + 3] - 1, wr, wc);
+                    const bool gm = (double)nm / (double)(wr * wc) >= E.mthr;
+                    P->gm[j] = gm;
+                    P->anyg |= gm;
+                    kind[j] = gm ? 2 : (nm > 0 ? 1 : 0);
+                }
+            }
             P->ny.push_back(g.ny);
             P->nx.push_back(g.nx);
             P->off.push_back(P->total);
             H.off.push_back((int)(H.jobs.size() / 4));
-            for (size_t j = 0; j < (size_t)g.ny * g.nx; ++j) {
-                H.jobs.push_back(P->total + (int32_t)j + 1);  // cell of column-major job order
-                H.jobs.push_back(1);
-                H.jobs.push_back(g.jobs[4 * j + 2] + L.roff[q]);
-                H.jobs.push_back(g.jobs[4 * j + 3] + L.coff[q]);
+            H.poff.push_back((int)(H.part.size() / 5));
+            for (size_t j = 0; j < ng; ++j) {
+                const int kj = kind.empty() ? 0 : kind[j];
+                const int rs = g.jobs[4 * j + 2], cs = g.jobs[4 * j + 3];
+                if (kj == 0) {
+                    H.jobs.push_back(P->total + (int32_t)j + 1);  // cell of column-major job order
+                    H.jobs.push_back(1);
+                    H.jobs.push_back(rs + L.roff[q]);
+                    H.jobs.push_back(cs + L.coff[q]);
+                } else if (kj == 1) {
+                    H.part.insert(H.part.end(), {P->total + (int32_t)j, rs - 1 + L.roff[q], cs - 1 + L.coff[q], rs - 1,
+                                                 cs - 1});
+                }
             }
-            P->total += g.ny * g.nx;
+            H.cnt.push_back((int)(H.jobs.size() / 4) - H.off.back());
+            H.pcnt.push_back((int)(H.part.size() / 5) - H.poff.back());
+            P->total += (int)ng;
+        }
+        {
+            const size_t npart = H.part.size() / 5, w = (size_t)p.window_size[0];
+            if (npart * w >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows with masked pixels");
+            E.maxtile = std::max(E.maxtile, npart * w * w);
         }
         E.maxcells = std::max(E.maxcells, (size_t)P->total);
         E.maxalt = std::max(E.maxalt, (size_t)P->total * std::max(p.n_peaks - 1, 0));
@@ -424,6 +499,16 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
     E.altv.ensure(std::max<size_t>(E.maxalt, 1));
     E.out.ensure(std::max<size_t>(E.maxcells, 1));
     E.sflag.ensure(std::max<size_t>(E.maxcells, 1));
+    if (E.maxtile) E.tiles.ensure(2 * E.maxtile);
+    E.mcount.ensure((size_t)L.npairs);
+    for (int k = 0; k < np; ++k) {
+        PassPlan &P = *E.plan[k];
+        if (!P.anyg) continue;
+        std::vector<uint8_t> all((size_t)P.total);
+        for (int q = 0; q < L.npairs; ++q) std::copy(P.gm.begin(), P.gm.end(), all.begin() + P.off[q]);
+        upload_sync(c, P.gmask, all);
+        HH_HIP(hipStreamSynchronize(c->stream));  // `all` ends with this iteration
+    }
     if (E.deforms) {
         E.coefs.ensure(2 * L.coef_total);
         E.warp.ensure(2 * (size_t)L.ld * (size_t)L.ncols);
@@ -432,8 +517,7 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         E.ptmp.ensure(t);
     }
     for (int k = 0; k < np; ++k)
-        build_tables(c, *E.plan[k], k > 0 ? E.plan[k - 1].get() : nullptr, all, E.hjobs[k].jobs, E.hjobs[k].off,
-                     E.plan[k]->all);
+        build_tables(c, *E.plan[k], k > 0 ? E.plan[k - 1].get() : nullptr, all, E.hjobs[k], E.plan[k]->all);
     for (int k = 0; k < np; ++k) {
         PassPlan &P = *E.plan[k];
         const bool unc = P.p.uncertainty && k == np - 1 && P.fixed;
@@ -443,13 +527,22 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
     }
 }
 
-void predictor(hh_ctx *c, const Grids &g)
+// build_predictor (src/pipeline.jl:454-464) from the field of pass P: its
+// masked cells filled from the unmasked ones (into the validation scratch),
+// then smoothed.
+void predictor(hh_ctx *c, const Grids &g, const PassPlan &P)
 {
     Engine &E = c->eng;
+    const float *u = E.u.p, *v = E.v.p;
+    if (P.anyg) {
+        launch_fill_masked(g, E.u.p, E.v.p, P.gmask.p, E.mcount.p, E.su.p, E.sv.p, c->stream);
+        u = E.su.p;
+        v = E.sv.p;
+    }
     if (E.smoothing) {
-        launch_smooth2(g, E.u.p, E.v.p, E.pu.p, E.pv.p, c->stream);
+        launch_smooth2(g, u, v, E.pu.p, E.pv.p, c->stream);
     } else {
-        gcopy2(c, E.u.p, E.v.p, E.pu.p, E.pv.p, g);
+        gcopy2(c, u, v, E.pu.p, E.pv.p, g);
     }
 }
 
@@ -479,7 +572,7 @@ void record(hh_ctx *c, bool dynamic)
         const hh_pass &p = P.p;
         const bool force_replace = k < np - 1;
         if (k > 0) {
-            predictor(c, E.plan[k - 1]->all.grids);
+            predictor(c, E.plan[k - 1]->all.grids, *E.plan[k - 1]);
             have_pred = true;
             pred_self = false;
         }
@@ -488,10 +581,12 @@ void record(hh_ctx *c, bool dynamic)
         const bool keep_measured = maxiter > 1 && !(force_replace || p.replace_outliers);
         const int nalt = p.n_peaks - 1;
         // uncertainty grids: NaN unless this pass computes them; a fused
-        // computation writes every cell, so only the final pass's grids need
-        // the fill when the statistics are not fused (or not requested)
+        // computation writes every cell with a window, so only the final
+        // pass's grids need the fill when the statistics are not fused (or
+        // not requested) or cells are masked
         const bool last = k == np - 1;
-        if (last && !(unc && P.fixed)) gfill(c, E.uu.p, E.uv.p, P.all.grids, NAN);
+        if (last && (!(unc && P.fixed) || P.anyg)) gfill(c, E.uu.p, E.uv.p, P.all.grids, NAN);
+        float *tA = E.tiles.p, *tB = E.tiles.p + E.maxtile;
         PassTables *T = &P.all;
         std::vector<int> active(L.npairs);
         std::iota(active.begin(), active.end(), 0);
@@ -532,11 +627,23 @@ void record(hh_ctx *c, bool dynamic)
                 out.unc_u = E.uu.p;
                 out.unc_v = E.uv.p;
             }
+            const CorrConfig &cc = out.unc_u ? P.cfg_uq : P.cfg;
+            bool fused = true;
             timed(c, out.unc_u ? "correlate_uq" : "correlate", P.cfg.w, [&] {
-                uq_fused = launch_correlate(out.unc_u ? P.cfg_uq : P.cfg, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs,
-                                            out, s) &&
-                           out.unc_u != nullptr;
+                fused = launch_correlate(cc, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs, out, s);
             });
+            if (T->npart > 0) {
+                // windows with masked pixels, as centred tiles
+                timed(c, "mask_tiles", P.cfg.w, [&] {
+                    launch_mask_tiles(srcA, srcB, L.ld, E.dmask.p, E.mnr, T->mjobs.p, T->npart, P.cfg.w, tA, tB, s);
+                });
+                timed(c, "correlate_masked", P.cfg.w, [&] {
+                    const bool f = launch_correlate(cc, tA, tB, P.cfg.w, 0, T->tjobs.p, T->npart, out, s);
+                    fused = fused && f;
+                });
+            }
+            uq_fused = fused && out.unc_u != nullptr;
+            if (P.anyg) launch_mask_nan(T->grids, P.gmask.p, E.u.p, E.v.p, E.pr.p, E.cm.p, s);
             if (keep_measured) gcopy2(c, E.u.p, E.v.p, E.mu.p, E.mv.p, T->grids);
             ValidateArgs va{};
             va.grids = T->grids;
@@ -553,6 +660,7 @@ void record(hh_ctx *c, bool dynamic)
             va.sv = E.sv.p;
             va.sflag = E.sflag.p;
             va.count = T->count.p;
+            va.mask = P.anyg ? P.gmask.p : nullptr;
             timed(c, "validate", P.cfg.w, [&] { launch_validate(va, s); });
             if (it == maxiter) break;
             if (!P.fixed) {
@@ -579,13 +687,13 @@ void record(hh_ctx *c, bool dynamic)
                     if (still.empty()) break;
                     if (still.size() != active.size()) {
                         active = still;
-                        build_tables(c, P, nullptr, active, E.hjobs[k].jobs, E.hjobs[k].off, E.sub);
+                        build_tables(c, P, nullptr, active, E.hjobs[k], E.sub);
                         T = &E.sub;
                     }
                 }
                 gcopy2(c, E.u.p, E.v.p, E.prevu.p, E.prevv.p, T->grids);
             }
-            predictor(c, T->grids);
+            predictor(c, T->grids, P);
             have_pred = true;
             pred_self = true;
         }
@@ -594,6 +702,14 @@ void record(hh_ctx *c, bool dynamic)
             timed(c, "uq", P.cfg.w, [&] {
                 launch_uq(P.cfg.w, cb.apod.p, srcA, srcB, L.ld, P.all.jobs.p, P.all.njobs, P.total, E.uu.p, E.uv.p,
                           nullptr, false, s);
+                if (P.all.npart > 0) {
+                    // every pair's masked windows from its final warp (the
+                    // tiles of the last sweep may cover a subset of pairs)
+                    launch_mask_tiles(srcA, srcB, L.ld, E.dmask.p, E.mnr, P.all.mjobs.p, P.all.npart, P.cfg.w, tA,
+                                      tB, s);
+                    launch_uq(P.cfg.w, cb.apod.p, tA, tB, P.cfg.w, P.all.tjobs.p, P.all.npart, P.total, E.uu.p,
+                              E.uv.p, nullptr, false, s);
+                }
             });
         }
         if (keep_measured)
@@ -687,6 +803,7 @@ void engine_run(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         key.insert(key.end(), b, b + n);
     };
     put(&E.lay.gen, sizeof E.lay.gen);
+    put(&E.mgen, sizeof E.mgen);
     put(&c->form, sizeof c->form);
     put(&smoothing, sizeof smoothing);
     put(&np, sizeof np);
@@ -762,7 +879,46 @@ void engine_fetch(hh_ctx *c, int q, hh_field *r)
         for (int j = 0; j < nx; ++j) r->x[j] = (float)(P.c0 + j * P.cstep) + P.half;
     if (r->y)
         for (int i = 0; i < ny; ++i) r->y[i] = (float)(P.r0 + i * P.rstep) + P.half;
-    if (r->mask) std::memset(r->mask, 0, ng);
+    if (r->mask) {
+        if (P.anyg)
+            std::memcpy(r->mask, P.gm.data(), ng);
+        else
+            std::memset(r->mask, 0, ng);
+    }
+}
+
+void engine_set_mask(hh_ctx *c, const uint8_t *mask, int nr, int nc, double threshold)
+{
+    Engine &E = c->eng;
+    std::vector<uint8_t> m;
+    if (mask) {
+        if (nr < 1 || nc < 1) throw Error(HH_EDIM, "mask must not be empty");
+        if (!(threshold > 0 && threshold <= 1)) {
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "mask_threshold must be in (0, 1], got %g", threshold);
+            throw Error(HH_EINVAL, buf);
+        }
+        m.resize((size_t)nr * nc);
+        bool any = false;
+        for (size_t k = 0; k < m.size(); ++k) any |= (m[k] = mask[k] != 0);
+        if (!any) m.clear();  // nothing masked: the unmasked schedule gives the same result
+    }
+    if (m.empty()) {
+        if (E.hmask.empty()) return;
+        E.hmask.clear();
+        E.mnr = E.mnc = 0;
+    } else {
+        if (m == E.hmask && nr == E.mnr && nc == E.mnc && threshold == E.mthr) return;
+        uint8_t *d = E.dmask.ensure(m.size());
+        HH_HIP(hipMemcpyAsync(d, m.data(), m.size(), hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipStreamSynchronize(c->stream));
+        E.hmask = std::move(m);
+        E.mnr = nr;
+        E.mnc = nc;
+        E.mthr = threshold;
+    }
+    ++E.mgen;
+    E.have_result = false;
 }
 
 void engine_release(hh_ctx *c) { invalidate_plan(c); }

@@ -216,8 +216,26 @@ struct ValidateArgs {
     float *su, *sv;
     uint8_t *sflag;
     int *count;  // grids.nseg device ints: valid cells per segment, saturating at 3
+    // grid mask (nullable, grids.stride bytes): masked cells are excluded
+    // from the UOD, never flagged or substituted, and NaN after a replacement
+    const uint8_t *mask;
 };
 void launch_validate(const ValidateArgs &a, hipStream_t s);
+// u, v, peak ratio and correlation moment of the masked cells set to NaN.
+void launch_mask_nan(const Grids &g, const uint8_t *mask, float *u, float *v, float *pr, float *cm, hipStream_t s);
+// build_predictor's fill (src/pipeline.jl:454-459): the masked cells of
+// (u, v) replaced from the unmasked ones (replace_vectors!) into (ou, ov);
+// count: grids.nseg device ints of scratch.
+void launch_fill_masked(const Grids &g, const float *u, const float *v, const uint8_t *mask, int *count, float *ou,
+                        float *ov, hipStream_t s);
+// Windows with masked pixels (load_windows!, src/correlators.jl:214-227):
+// for each of the nw entries of mjobs (4 ints, 0-based: image row, image
+// column, mask row, mask column of the window's first pixel) a w x w tile
+// of both frames, at column t w of the tile atlases tA, tB (leading
+// dimension w): valid pixels minus the mean of the window's valid pixels,
+// masked pixels 0. mask: column-major, leading dimension mld.
+void launch_mask_tiles(const float *A, const float *B, int ld, const uint8_t *mask, int mld, const int32_t *mjobs,
+                       int nw, int w, float *tA, float *tB, hipStream_t s);
 void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s);
 // Two fields (u and v of a predictor) in one launch.
 void launch_smooth2(const Grids &g, const float *f, const float *f2, float *out, float *out2, hipStream_t s);

@@ -239,7 +239,7 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
                                                    const Cell &k, double thr, const float *__restrict__ alt_u,
                                                    const float *__restrict__ alt_v, int nalt,
                                                    float *__restrict__ u_out, float *__restrict__ v_out,
-                                                   uint8_t *__restrict__ flags_out)
+                                                   uint8_t *__restrict__ flags_out, const uint8_t *__restrict__ mask)
 {
     constexpr int W = 2 * NB + 1;
     constexpr int N = W * W - 1;
@@ -248,10 +248,11 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
     u_out[e] = uref[e];
     v_out[e] = vref[e];
     flags_out[e] = flags_in[e];
-    if (!flags_in[e]) return 0;
+    if (!flags_in[e]) return 0;  // masked cells carry no flag
     const float *__restrict__ ug = uref + sg.off;
     const float *__restrict__ vg = vref + sg.off;
     const uint8_t *__restrict__ fg = flags_in + sg.off;
+    const uint8_t *__restrict__ mg = mask ? mask + sg.off : nullptr;
     double nu[N], nv[N], su[N], sv[N];
     bool ok[N];
     int n = 0;
@@ -261,7 +262,7 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
         const int lin = q < N / 2 ? q : q + 1;
         const int dc = lin / W - NB, dr = lin % W - NB;
         const int r2 = r + dr, c2 = c + dc;
-        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !fg[r2 + c2 * ny];
+        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !fg[r2 + c2 * ny] && !(mg && mg[r2 + c2 * ny]);
         nu[q] = ok[q] ? (double)ug[r2 + c2 * ny] : INFINITY;
         nv[q] = ok[q] ? (double)vg[r2 + c2 * ny] : INFINITY;
         nanu |= ok[q] && isnan(nu[q]);
@@ -346,36 +347,52 @@ __device__ __forceinline__ void add_valid(const Grids &g, int ee, bool valid, in
 
 // Number of valid cells per segment: with fewer than 3 in a grid no vector
 // of that grid can be replaced and k_replace returns at once.
-__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid, Grids g,
+// mask (nullable) marks further invalid cells.
+__global__ __launch_bounds__(256) void k_count_valid(const uint8_t *__restrict__ invalid,
+                                                     const uint8_t *__restrict__ mask, Grids g,
                                                      int *__restrict__ count)
 {
     const int ee = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = ee < g.total && !invalid[cell_of(g, ee).b];
+    bool valid = false;
+    if (ee < g.total) {
+        const int b = cell_of(g, ee).b;
+        valid = !invalid[b] && !(mask && mask[b]);
+    }
     add_valid(g, ee, valid, count);
 }
 
 // substitute_alternatives! over the enumerated cells; with count non-null the
-// valid cells of the result are also counted per segment (for k_replace).
+// valid cells of the result (unflagged and unmasked) are also counted per
+// segment (for k_replace). mask (nullable): cells that are neither
+// neighbours nor substituted.
 template <int NB>
 __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ uref, const float *__restrict__ vref,
                                                     const uint8_t *__restrict__ flags_in, Grids g, double thr,
                                                     const float *__restrict__ alt_u, const float *__restrict__ alt_v,
                                                     int nalt, float *__restrict__ u_out, float *__restrict__ v_out,
-                                                    uint8_t *__restrict__ flags_out, int *__restrict__ count)
+                                                    uint8_t *__restrict__ flags_out, int *__restrict__ count,
+                                                    const uint8_t *__restrict__ mask)
 {
     const int ee = blockIdx.x * blockDim.x + threadIdx.x;
-    uint8_t f = 1;
-    if (ee < g.total)
-        f = substitute_cell<NB>(uref, vref, flags_in, g, cell_of(g, ee), thr, alt_u, alt_v, nalt, u_out, v_out,
-                                flags_out);
-    if (count) add_valid(g, ee, f == 0, count);
+    bool valid = false;
+    if (ee < g.total) {
+        const Cell k = cell_of(g, ee);
+        valid = substitute_cell<NB>(uref, vref, flags_in, g, k, thr, alt_u, alt_v, nalt, u_out, v_out, flags_out,
+                                    mask) == 0 &&
+                !(mask && mask[k.b]);
+    }
+    if (count) add_valid(g, ee, valid, count);
 }
 
-// flags_out (nullable) receives a copy of invalid.
+// flags_out (nullable) receives a copy of invalid. mask (nullable): cells
+// that are invalid as well and end as NaN (validate_and_replace!,
+// src/pipeline.jl:636-642: the invalid set is outliers .| mask, masked
+// vectors are NaN afterwards).
 __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_all, const float *__restrict__ vref_all,
                                                  const uint8_t *__restrict__ invalid_all, Grids g,
                                                  const int *__restrict__ nvalid, float *__restrict__ u_out,
-                                                 float *__restrict__ v_out, uint8_t *__restrict__ flags_out)
+                                                 float *__restrict__ v_out, uint8_t *__restrict__ flags_out,
+                                                 const uint8_t *__restrict__ mask_all)
 {
     const int ee = blockIdx.x * blockDim.x + threadIdx.x;
     if (ee >= g.total) return;
@@ -385,12 +402,18 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
     u_out[e] = uref_all[e];
     v_out[e] = vref_all[e];
     if (flags_out) flags_out[e] = invalid_all[e];
+    if (mask_all && mask_all[e]) {
+        u_out[e] = v_out[e] = NAN;
+        return;
+    }
     if (!invalid_all[e]) return;
     if (nvalid[k.si] < 3) return;
     const int ny = sg.ny, nx = sg.nx, r = k.r, c = k.c;
     const float *__restrict__ uref = uref_all + sg.off;
     const float *__restrict__ vref = vref_all + sg.off;
     const uint8_t *__restrict__ invalid = invalid_all + sg.off;
+    const uint8_t *__restrict__ mask = mask_all ? mask_all + sg.off : nullptr;
+    auto bad = [&](int at) { return invalid[at] || (mask && mask[at]); };
     {
         // radius 1 (the common case) from registers
         double nu[9], nv[9];
@@ -399,7 +422,7 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             const int r2 = r + q % 3 - 1, c2 = c + q / 3 - 1;
-            const bool ok = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !invalid[r2 + c2 * ny];
+            const bool ok = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !bad(r2 + c2 * ny);
             nu[q] = ok ? (double)uref[r2 + c2 * ny] : INFINITY;
             nv[q] = ok ? (double)vref[r2 + c2 * ny] : INFINITY;
             nanu |= ok && isnan(nu[q]);
@@ -419,15 +442,15 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
         const int r0 = max(0, r - rad), r1 = min(ny - 1, r + rad), c0 = max(0, c - rad), c1 = min(nx - 1, c + rad);
         int n = 0;
         for (int c2 = c0; c2 <= c1; ++c2)
-            for (int r2 = r0; r2 <= r1; ++r2) n += !invalid[r2 + c2 * ny];
+            for (int r2 = r0; r2 <= r1; ++r2) n += !bad(r2 + c2 * ny);
         if (n < 3) continue;
         auto gu = [&](int r2, int c2, double &x) {
-            if (invalid[r2 + c2 * ny]) return false;
+            if (bad(r2 + c2 * ny)) return false;
             x = (double)uref[r2 + c2 * ny];
             return true;
         };
         auto gv = [&](int r2, int c2, double &x) {
-            if (invalid[r2 + c2 * ny]) return false;
+            if (bad(r2 + c2 * ny)) return false;
             x = (double)vref[r2 + c2 * ny];
             return true;
         };
@@ -472,6 +495,33 @@ __global__ void k_val_init(uint8_t *flags, Grids g, int *count)
     if (ee < g.nseg) count[ee] = 0;
 }
 
+// Masked cells leave the outlier set (outliers .&= .!mask,
+// src/pipeline.jl:628).
+__global__ void k_unflag_masked(uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask, Grids g)
+{
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const int e = cell_of(g, ee).b;
+    if (mask[e]) flags[e] = 0;
+}
+
+// Masked cells of a sweep hold NaN in u, v, peak ratio and correlation moment
+// (src/pipeline.jl:763-766).
+__global__ void k_mask_nan(const uint8_t *__restrict__ mask, Grids g, float *__restrict__ u, float *__restrict__ v,
+                           float *__restrict__ pr, float *__restrict__ cm)
+{
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const int e = cell_of(g, ee).b;
+    if (mask[e]) u[e] = v[e] = pr[e] = cm[e] = NAN;
+}
+
+__global__ void k_zero_count(int *count, int n)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) count[e] = 0;
+}
+
 // Move a result computed into scratch back to the chain buffers.
 __global__ void k_copy_back(const float *__restrict__ su, const float *__restrict__ sv,
                             const uint8_t *__restrict__ sf, float *__restrict__ u, float *__restrict__ v,
@@ -489,8 +539,8 @@ template <int NB>
 void uod_launch(const ValidateArgs &a, double thr, double eps, bool init, hipStream_t s)
 {
     const int n = std::max(a.grids.total, a.grids.nseg);
-    hipLaunchKernelGGL(k_uod<NB>, dim3((n + 255) / 256), dim3(256), 0, s, a.u, a.v, a.grids, thr, eps,
-                       (const uint8_t *)nullptr, a.outliers, init ? 1 : 0, a.count);
+    hipLaunchKernelGGL(k_uod<NB>, dim3((n + 255) / 256), dim3(256), 0, s, a.u, a.v, a.grids, thr, eps, a.mask,
+                       a.outliers, init ? 1 : 0, a.count);
 }
 
 void uod(const ValidateArgs &a, int nb, double thr, double eps, bool init, hipStream_t s)
@@ -530,7 +580,7 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
             hipLaunchKernelGGL(k_local_validators, g, b, 0, s, a.u, a.v, a.pr, a.cm, a.grids, (int)vd.kind, vd.a, vd.b,
                                a.outliers);
     }
-    // no mask on the device path: outliers &= !mask is the identity.
+    if (a.mask) hipLaunchKernelGGL(k_unflag_masked, g, b, 0, s, a.outliers, a.mask, a.grids);
     // Stages alternate between the chain buffers and the scratch buffers; a
     // result left in scratch is copied back once at the end.
     const bool sub = a.alt_u && p.n_peaks > 1;
@@ -540,31 +590,51 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
         switch (p.uod_neighborhood) {
         case 1:
             hipLaunchKernelGGL(k_substitute<1>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
-                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt);
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt, a.mask);
             break;
         case 2:
             hipLaunchKernelGGL(k_substitute<2>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
-                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt);
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt, a.mask);
             break;
         default:
             hipLaunchKernelGGL(k_substitute<3>, g, b, 0, s, a.u, a.v, a.outliers, a.grids, p.uod_threshold,
-                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt);
+                               a.alt_u, a.alt_v, p.n_peaks - 1, a.su, a.sv, a.sflag, cnt, a.mask);
             break;
         }
     }
     if (rep) {
         if (sub) {  // substituted field in scratch: replace back into the chain
             hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.grids, (const int *)a.count, a.u,
-                               a.v, a.outliers);
+                               a.v, a.outliers, a.mask);
         } else {
-            hipLaunchKernelGGL(k_count_valid, g, b, 0, s, a.outliers, a.grids, a.count);
+            hipLaunchKernelGGL(k_count_valid, g, b, 0, s, a.outliers, a.mask, a.grids, a.count);
             hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.grids, (const int *)a.count, a.su,
-                               a.sv, (uint8_t *)nullptr);
+                               a.sv, (uint8_t *)nullptr, a.mask);
         }
     }
     if (sub != rep)  // exactly one stage ran: its result is in scratch
         hipLaunchKernelGGL(k_copy_back, g, b, 0, s, a.su, a.sv, sub ? a.sflag : (const uint8_t *)nullptr, a.u, a.v,
                            a.outliers, a.grids);
+    HH_HIP(hipGetLastError());
+}
+
+void launch_mask_nan(const Grids &g, const uint8_t *mask, float *u, float *v, float *pr, float *cm, hipStream_t s)
+{
+    if (g.total <= 0) return;
+    hipLaunchKernelGGL(k_mask_nan, dim3((g.total + 255) / 256), dim3(256), 0, s, mask, g, u, v, pr, cm);
+    HH_HIP(hipGetLastError());
+}
+
+void launch_fill_masked(const Grids &g, const float *u, const float *v, const uint8_t *mask, int *count, float *ou,
+                        float *ov, hipStream_t s)
+{
+    const int n = g.total;
+    if (n <= 0) return;
+    const dim3 gr((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_zero_count, dim3((g.nseg + 255) / 256), b, 0, s, count, g.nseg);
+    hipLaunchKernelGGL(k_count_valid, gr, b, 0, s, mask, (const uint8_t *)nullptr, g, count);
+    hipLaunchKernelGGL(k_replace, gr, b, 0, s, u, v, mask, g, (const int *)count, ou, ov, (uint8_t *)nullptr,
+                       (const uint8_t *)nullptr);
     HH_HIP(hipGetLastError());
 }
 

@@ -90,12 +90,29 @@ def pass_grid(p: PIVParameters, nr: int, nc: int):
     return x, y, np.asfortranarray(jobs)
 
 
+def _check_mask(mask, mask_threshold, shape):
+    """The mask keyword checks of run_piv (src/pipeline.jl:393-396): the
+    mask's size first, then the threshold's range."""
+    m = None
+    if mask is not None:
+        m = np.asarray(mask, dtype=bool)
+        if m.shape != tuple(shape):
+            raise DimensionMismatch(f"mask must have the same size as the images, got {m.shape}")
+    thr = float(mask_threshold)
+    if not (0 < thr <= 1):
+        raise ArgumentError(f"mask_threshold must be in (0, 1], got {mask_threshold}")
+    return m, thr
+
+
 def run_piv(imgA, imgB, passes, *, effort=None, backend: str = "mi355x", predictor_smoothing: bool = True,
             mask=None, workspace: Optional[PIVWorkspace] = None, form: int = _lib.HH_FORM_FFT32, **kwargs):
     """run_piv(imgA, imgB, passes; backend) -> PIVResult (src/pipeline.jl:361-449).
 
     ``passes`` is a PIVParameters or a list of them. The computation runs in
-    float32 on the GPU (the reference's Float32 pipeline)."""
+    float32 on the GPU (the reference's Float32 pipeline). ``mask`` (an
+    image-sized boolean matrix) and ``mask_threshold`` act as in the
+    reference: windows whose masked fraction reaches the threshold give NaN
+    vectors flagged in ``result.mask``; other windows use their valid pixels."""
     if effort is not None:
         raise ArgumentError("effort cannot be combined with explicit PIVParameters or pass schedules")
     unknown = set(kwargs) - {"threaded", "uncertainty_backend", "mask_threshold", "scale", "roi"}
@@ -115,12 +132,7 @@ def run_piv(imgA, imgB, passes, *, effort=None, backend: str = "mi355x", predict
     B = np.asarray(imgB)
     if A.shape != B.shape:
         raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
-    if mask is not None:
-        m = np.asarray(mask, dtype=bool)
-        if m.shape != A.shape:
-            raise DimensionMismatch(f"mask must have the same size as the images, got {m.shape}")
-        if m.any():
-            raise ArgumentError("backend :mi355x does not support masks yet")
+    m, thr = _check_mask(mask, kwargs.get("mask_threshold", 0.5), A.shape)
     ws = workspace or PIVWorkspace(be)
     try:
         ctx = ws.ctx
@@ -129,6 +141,7 @@ def run_piv(imgA, imgB, passes, *, effort=None, backend: str = "mi355x", predict
             pass_grid_counts(p, *A.shape)
         if form != _lib.HH_FORM_FFT32:
             ctx.set_form(form)
+        ctx.set_mask(m, thr)
         ctx.set_pair(A, B)
         r = ctx.run_piv(passes, predictor_smoothing)
     finally:
@@ -227,8 +240,8 @@ def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi
     pairs = list(pairs)
     if not pairs:
         raise ArgumentError("pairs must not be empty")
-    if mask is not None and np.asarray(mask, dtype=bool).any():
-        raise ArgumentError("backend :mi355x does not support masks yet")
+    m = None
+    thr = kwargs.get("mask_threshold", 0.5)
     frames = []
     for A, B in pairs:
         if isinstance(A, (str, bytes)) or isinstance(B, (str, bytes)):
@@ -240,6 +253,7 @@ def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi
             raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
         for p in passes:
             pass_grid_counts(p, *A.shape)
+        m, thr = _check_mask(mask, thr, A.shape)
         frames.append((A, B))
     n = len(frames)
     step = n if batch is None else max(1, int(batch))
@@ -250,6 +264,7 @@ def run_piv_sequence(pairs: Sequence, params, *, effort=None, backend: str = "mi
         for ctx in ctxs:
             ctx.check_passes(passes)
             ctx.set_form(form)
+            ctx.set_mask(m, thr)
 
         def emit(ctx, q0):
             for q in range(len(ctx.batch_shapes)):

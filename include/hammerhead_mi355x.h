@@ -145,7 +145,7 @@ int hh_set_pair(hh_ctx *ctx, const float *A, const float *B, int nr, int nc);
  * nr x nc). The pointers must stay valid while ctx uses them. */
 int hh_set_pair_device(hh_ctx *ctx, const float *dA, const float *dB, int nr, int nc);
 
-/* ---- deformation (A10) ------------------------------------------------- */
+/* ---- deformation ------------------------------------------------------- */
 
 /* Cubic B-spline prefilter of both images on the device: the coefficients of
  * Interpolations' BSpline(Cubic(Line(OnGrid()))) (image_interpolant,
@@ -162,7 +162,7 @@ int hh_get_images(hh_ctx *ctx, int src, float *A, float *B);
 /* Copy the prefiltered coefficients ((nr+2) x (nc+2) each) to host. */
 int hh_get_coefs(hh_ctx *ctx, float *cA, float *cB);
 
-/* ---- per-window correlation (A4-A8) ------------------------------------ */
+/* ---- per-window correlation ------------------------------------------- */
 
 /* process_windows! (src/pipeline.jl:972-1027; device engine
  * ext/HammerheadAMDGPUExt.jl:398-493). jobs: int32 4 x njobs column-major
@@ -191,7 +191,7 @@ int hh_uncertainty_sweep(hh_ctx *ctx, const hh_pass *pass, int src,
                          const int32_t *jobs, int njobs, int ny, int nx,
                          float *unc_u, float *unc_v, double *stats);
 
-/* ---- validation and predictor (A9, A11) -------------------------------- */
+/* ---- validation and predictor ------------------------------------------ */
 
 /* validate_and_replace! (src/pipeline.jl:617-645): UOD, peak-ratio and
  * extra validators, mask exclusion, peak substitution (alt_u/alt_v nullable:
@@ -217,6 +217,22 @@ int hh_run_piv(hh_ctx *ctx, const hh_pass *passes, int npasses,
 /* Copy the device-resident result of the last hh_run_piv or
  * hh_run_piv_ensemble to host. */
 int hh_fetch_result(hh_ctx *ctx, hh_field *result);
+
+/* The mask and mask_threshold keywords of run_piv (src/pipeline.jl:361-449,
+ * checks at 393-396) and run_piv_sequence (src/io.jl:229-247): an nr x nc
+ * column-major pixel mask (nonzero = masked) applied to every following
+ * hh_run_piv / hh_run_piv_batch, whose pairs must all be nr x nc (HH_EDIM at
+ * the run otherwise). Windows whose masked-pixel fraction reaches
+ * mask_threshold produce no vector (pass_grid, src/pipeline.jl:542-549):
+ * u, v, peak_ratio, correlation_moment and the uncertainties hold NaN there,
+ * they are never outliers and result->mask flags them; windows with fewer
+ * masked pixels are correlated from the mean of their valid pixels with the
+ * masked ones as zero (load_windows!, src/correlators.jl:214-227). mask NULL
+ * (or one with no pixel set) removes the mask; the threshold must be in
+ * (0, 1] (HH_EINVAL). The mask stays set across hh_set_pair* and
+ * hh_set_pairs; setting an identical mask again keeps the planned schedule.
+ * The ensemble (hh_run_piv_ensemble) does not use it. */
+int hh_set_mask(hh_ctx *ctx, const uint8_t *mask, int nr, int nc, double mask_threshold);
 
 /* ---- batches of pairs of uneven sizes ----------------------------------- */
 
@@ -262,7 +278,7 @@ int hh_tune_report(hh_ctx *ctx, char *buf, size_t len, size_t *needed);
  * without timing candidates, e.g. choices saved by an earlier process. */
 int hh_tune_set(hh_ctx *ctx, const char *key, int waves_per_simd, int linear_order);
 
-/* ---- ensemble (sum-of-correlation) PIV (A16) --------------------------- */
+/* ---- ensemble (sum-of-correlation) PIV --------------------------------- */
 
 /* _plane_accumulator / _uncertainty_accumulator of ensemble_pass
  * (src/ensemble.jl:165-190; device form ext/HammerheadAMDGPUExt.jl:550-560):
@@ -294,7 +310,7 @@ int hh_ensemble_planes(hh_ctx *ctx, double *planes);
 int hh_run_piv_ensemble(hh_ctx *ctx, const float *const *A, const float *const *B, int npairs, int nr, int nc,
                         const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result);
 
-/* ---- correlation energy and its adjoint (A17) -------------------------- */
+/* ---- correlation energy and its adjoint -------------------------------- */
 /* No counterpart in the reference, which has no gradient path (nearest:
  * the non-gradient correlate_deformable, src/correlators.jl:620-638). For a
  * node field pu, pv (gny x gnx, column-major) on uniform knots py, px, the

@@ -5,6 +5,7 @@ import ctypes
 import pathlib
 import re
 
+import numpy as np
 import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parent.parent
@@ -95,3 +96,19 @@ def test_unsupported_options_rejected_up_front(hh):
         hh.run_piv(None, None, hh.PIVParameters(), backend="cuda")
     with pytest.raises(hh.ArgumentError):
         hh.run_piv(None, None, hh.PIVParameters(), effort="low")
+
+
+def test_mask_keyword_checks(hh):
+    # run_piv's mask checks (src/pipeline.jl:393-396) come before any device
+    # work: the mask's size, then the threshold's range
+    A = np.zeros((64, 64), np.float32)
+    p = hh.PIVParameters(window_size=32, overlap=16)
+    with pytest.raises(hh.DimensionMismatch):
+        hh.run_piv(A, A, p, mask=np.zeros((64, 63), bool))
+    for thr in (0.0, -0.5, 1.5):
+        with pytest.raises(hh.ArgumentError):
+            hh.run_piv(A, A, p, mask=np.zeros((64, 64), bool), mask_threshold=thr)
+        with pytest.raises(hh.ArgumentError):
+            hh.run_piv(A, A, p, mask_threshold=thr)
+    with pytest.raises(hh.DimensionMismatch):
+        hh.run_piv_sequence([(A, A)], p, mask=np.zeros((32, 64), bool))

@@ -1,0 +1,193 @@
+"""Pixel masks on the device (run_piv(...; mask, mask_threshold),
+src/pipeline.jl:361-449 with 527-555, 617-645, 655-832) against the oracle,
+through the C ABI.
+
+Bars: the grid mask bit-exact; masked cells NaN in u, v, peak ratio, moment
+and both uncertainties, never outliers; outlier flags within the near-tie
+allowance of the run_piv bars; u, v RMS < 5e-3 px over the cells valid in
+both (1e-3 px max for a single pass); sigma median within 1 %; validation
+with a mask bit-exact; a batch of masked pairs bitwise equal to run_piv on
+each pair alone.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "peak_ratio", "correlation_moment")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def disk_scene(hh, n=256, seed=61, speckle=0.0):
+    """A uniform-shift particle pair with a disk of uncorrelated garbage in
+    both frames (the masked region), optionally with scattered masked pixels
+    that leave most windows partly masked."""
+    pair = hh.synthetic.particle_pair(n, n, hh.synthetic.uniform_flow(2.3, -1.7), seed, ppp=0.03, noise=0.02)
+    rng = np.random.default_rng(seed + 1)
+    i = np.arange(n)[:, None]
+    j = np.arange(n)[None, :]
+    m = (i - 110.3) ** 2 + (j - 140.7) ** 2 <= 37.5 ** 2
+    A, B = pair.A.copy(), pair.B.copy()
+    A[m] = 3 * rng.random(m.sum())
+    B[m] = 3 * rng.random(m.sum())
+    if speckle:
+        m = m | (rng.random((n, n)) < speckle)
+    return np.asfortranarray(A, dtype=np.float32), np.asfortranarray(B, dtype=np.float32), m
+
+
+def run_masked(hh, orc, ctx, A, B, m, passes, thr=0.5):
+    ctx.set_mask(m, thr)
+    ctx.set_pair(A, B)
+    g = ctx.run_piv(passes)
+    ctx.set_mask(None)
+    o = orc.run_piv(A.astype(np.float64), B.astype(np.float64), [p.to_struct() for p in passes], mask=m,
+                    mask_threshold=thr)
+    return g, o
+
+
+def check_masked(g, o, rms_tol=5e-3, max_tol=None, uq=False):
+    gm = g["mask"]
+    assert np.array_equal(gm, o["mask"])
+    assert gm.any() and not gm.all()
+    keys = FIELDS + (("uncertainty_u", "uncertainty_v") if uq else ())
+    for k in keys:
+        assert np.all(np.isnan(g[k][gm])), k
+    assert not np.any(g["outliers"] & gm)
+    dflag = int((g["outliers"] != o["outliers"]).sum())
+    same = ~g["outliers"] & ~o["outliers"] & ~gm
+    du = np.abs(g["u"] - o["u"])[same]
+    dv = np.abs(g["v"] - o["v"])[same]
+    rms = math.sqrt(np.mean(np.concatenate([du, dv]) ** 2))
+    say("masked cells", int(gm.sum()), "of", gm.size, "flag diffs", dflag, "rms", rms, "max", du.max(), dv.max())
+    assert np.all(np.isfinite(g["u"][~gm]))
+    assert dflag <= 0.002 * gm.size + 2
+    assert rms < rms_tol
+    if max_tol is not None:
+        assert du.max() < max_tol and dv.max() < max_tol
+    if uq:
+        fin = np.isfinite(g["uncertainty_u"]) & np.isfinite(o["uncertainty_u"])
+        assert np.array_equal(np.isfinite(g["uncertainty_u"]), np.isfinite(o["uncertainty_u"]))
+        srel = np.abs(g["uncertainty_u"][fin] - o["uncertainty_u"][fin]) / np.maximum(
+            np.abs(o["uncertainty_u"][fin]), 1e-6)
+        say("sigma rel median", np.median(srel))
+        assert np.median(srel) < 0.01
+
+
+@pytest.mark.parametrize("form", ["fft32", "mfma16"])
+@pytest.mark.parametrize("speckle", [0.0, 0.01])
+def test_run_piv_mask_multipass_vs_oracle(hh, orc, form, speckle):
+    A, B, m = disk_scene(hh, speckle=speckle)
+    passes = hh.multipass_parameters([32, 16], padding=True, apodization="gauss",
+                                     final=dict(overlap=8, max_iterations=2, convergence_tol=0.0, uncertainty=True))
+    say("mask multipass", form, "speckle", speckle)
+    with hh.Context(0) as c:
+        c.set_form(hh.HH_FORM_MFMA16 if form == "mfma16" else hh.HH_FORM_FFT32)
+        g, o = run_masked(hh, orc, c, A, B, m, passes)
+    check_masked(g, o, uq=True)
+    good = ~g["mask"]
+    assert abs(np.median(g["u"][good]) - 2.3) < 0.05 and abs(np.median(g["v"][good]) + 1.7) < 0.05
+
+
+@pytest.mark.parametrize("thr", [0.5, 1.0])
+def test_run_piv_mask_single_pass_vs_oracle(hh, orc, ctx, thr):
+    # alternatives substituted around the mask (n_peaks = 3), replacement on
+    A, B, m = disk_scene(hh, seed=67, speckle=0.004)
+    p = hh.PIVParameters(window_size=32, overlap=16, padding=True, apodization="gauss", n_peaks=3,
+                         replace_outliers=True)
+    say("mask single pass thr", thr)
+    g, o = run_masked(hh, orc, ctx, A, B, m, [p], thr=thr)
+    assert np.array_equal(g["mask"], orc.grid_mask(p.to_struct(), *A.shape, m, thr))
+    check_masked(g, o, max_tol=1e-3)
+
+
+def test_validate_with_mask_bit_exact(hh, orc, ctx):
+    rng = np.random.default_rng(23)
+    ny, nx = 37, 41
+    for trial, nb in enumerate((1, 2, 3)):
+        u = (2.0 + 0.1 * rng.standard_normal((ny, nx))).astype(np.float32)
+        v = (-1.0 + 0.1 * rng.standard_normal((ny, nx))).astype(np.float32)
+        bad = rng.random((ny, nx)) < 0.08
+        u[bad] += rng.uniform(-5, 5, bad.sum()).astype(np.float32)
+        m = np.zeros((ny, nx), bool)
+        m[10:19, 12:25] = True
+        m |= rng.random((ny, nx)) < 0.03
+        u[m] = np.nan
+        v[m] = np.nan
+        pr = rng.uniform(1.0, 3.0, (ny, nx)).astype(np.float32)
+        pr[m] = np.nan
+        cm = rng.uniform(0.1, 1.0, (ny, nx)).astype(np.float32)
+        au = np.full((ny, nx, 2), np.nan, np.float32)
+        av = np.full((ny, nx, 2), np.nan, np.float32)
+        au[..., 0] = (2.0 + 0.05 * rng.standard_normal((ny, nx))).astype(np.float32)
+        av[..., 0] = (-1.0 + 0.05 * rng.standard_normal((ny, nx))).astype(np.float32)
+        p = hh.PIVParameters(uod_neighborhood=nb, min_peak_ratio=1.2, n_peaks=2)
+        for force in (True, False):
+            say("validate mask", trial, nb, force)
+            gu, gv, gf = ctx.validate_and_replace(p, u, v, pr, cm, au, av, force_replace=force, mask=m)
+            ou, ov, of = orc.validate_and_replace(p.to_struct(), u, v, pr, cm, au, av, force_replace=force,
+                                                  mask=m.astype(np.uint8), prec="f32")
+            assert not gf[m].any()
+            assert np.array_equal(gf, of)
+            assert np.array_equal(gu, ou, equal_nan=True) and np.array_equal(gv, ov, equal_nan=True)
+
+
+def _same(a, b):
+    keys = FIELDS + ("uncertainty_u", "uncertainty_v", "outliers", "mask")
+    return all(np.array_equal(a[k], b[k], equal_nan=(k not in ("outliers", "mask"))) for k in keys)
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_masked_batch_equals_single_runs(hh, dynamic):
+    """A batch of equally sized pairs under one mask (run_piv_sequence with a
+    mask) gives exactly what run_piv gives on each pair; with convergence_tol
+    > 0 the pairs stop at different sweeps, so later sweeps correlate the
+    masked windows of a subset of the pairs."""
+    pairs = hh.synthetic.config_batch(npairs=4, seed0=205000, sizes=((192, 224),))
+    m = np.zeros((192, 224), bool)
+    m[60:130, 90:150] = True
+    m[5:9, :] = True
+    final = dict(overlap=8, max_iterations=4 if dynamic else 2, convergence_tol=0.02 if dynamic else 0.0,
+                 uncertainty=True)
+    passes = hh.multipass_parameters([32, 16], padding=True, apodization="gauss", final=final)
+    say("masked batch, dynamic", dynamic)
+    with hh.Context(0) as cb, hh.Context(0) as cs:
+        cb.set_mask(m, 0.5)
+        cs.set_mask(m, 0.5)
+        cb.set_pairs([(p.A, p.B) for p in pairs])
+        rb = cb.run_piv_batch(passes)
+        for q, p in enumerate(pairs):
+            cs.set_pair(p.A, p.B)
+            rs = cs.run_piv(passes)
+            assert rs["mask"].any()
+            assert _same(rb[q], rs), f"pair {q}"
+        if not dynamic:
+            # the same mask again keeps the plan: the next run replays its graph
+            n0 = cb.graph_replays()
+            cb.set_mask(m.copy(), 0.5)
+            rb2 = cb.run_piv_batch(passes)
+            assert cb.graph_replays() == n0 + 1
+            assert all(_same(a, b) for a, b in zip(rb, rb2))
+        # removing the mask gives the unmasked result
+        cb.set_mask(None)
+        ru = cb.run_piv_batch(passes)
+        assert not any(r["mask"].any() for r in ru)
+
+
+def test_run_piv_sequence_with_mask(hh):
+    pairs = hh.synthetic.config_batch(npairs=3, seed0=206000, sizes=((160, 192),))
+    m = np.zeros((160, 192), bool)
+    m[40:100, 50:120] = True
+    p = hh.PIVParameters(window_size=32, overlap=16, padding=True, apodization="gauss")
+    seq = hh.run_piv_sequence([(q.A, q.B) for q in pairs], p, mask=m, mask_threshold=0.4)
+    for r, q in zip(seq, pairs):
+        single = hh.run_piv(q.A, q.B, p, mask=m, mask_threshold=0.4)
+        assert r.mask.any() and np.array_equal(r.mask, single.mask)
+        assert np.array_equal(r.u, single.u, equal_nan=True)
+        assert np.array_equal(r.outliers, single.outliers)
