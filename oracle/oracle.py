@@ -384,9 +384,11 @@ def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask
     return res
 
 
-def run_piv_ensemble(pairs, passes, predictor_smoothing=True, nthreads=8, prec="f64", planes=False):
-    """run_piv_ensemble(pairs, passes): dict with the final PIVResult fields;
-    with ``planes`` also the last pass's FP64 plane sums (m1 x m2 x njobs)."""
+def run_piv_ensemble(pairs, passes, predictor_smoothing=True, nthreads=8, prec="f64", planes=False, mask=None,
+                     mask_threshold=0.5):
+    """run_piv_ensemble(pairs, passes; mask, mask_threshold): dict with the
+    final PIVResult fields (``mask``: the final grid mask); with ``planes``
+    also the last pass's FP64 plane sums (m1 x m2 per unmasked window)."""
     dt = _dt(prec)
     As = [_f(A, dt) for A, _ in pairs]
     Bs = [_f(B, dt) for _, B in pairs]
@@ -397,7 +399,8 @@ def run_piv_ensemble(pairs, passes, predictor_smoothing=True, nthreads=8, prec="
     g = lambda: np.zeros((ny, nx), dtype=dt, order="F")
     res = dict(x=np.zeros(nx, dtype=dt), y=np.zeros(ny, dtype=dt), u=g(), v=g(), peak_ratio=g(),
                correlation_moment=g(), uncertainty_u=g(), uncertainty_v=g(),
-               outliers=np.zeros((ny, nx), dtype=np.uint8, order="F"))
+               outliers=np.zeros((ny, nx), dtype=np.uint8, order="F"),
+               mask=np.zeros((ny, nx), dtype=np.uint8, order="F"))
     last = passes[-1]
     m1, m2 = (2 * s if last.padding else s for s in last.search_area_size)
     pl = np.zeros((m1, m2, ny * nx), dtype=np.float64, order="F") if planes else None
@@ -408,12 +411,14 @@ def run_piv_ensemble(pairs, passes, predictor_smoothing=True, nthreads=8, prec="
                                            C.c_int(nthreads), _p(res["x"]), _p(res["y"]), _p(res["u"]),
                                            _p(res["v"]), _p(res["peak_ratio"]), _p(res["correlation_moment"]),
                                            _p(res["uncertainty_u"]), _p(res["uncertainty_v"]),
-                                           _p(res["outliers"]), _p(pl))
+                                           _p(res["outliers"]), _p(pl), _p(_mask(mask)),
+                                           C.c_double(mask_threshold), _p(res["mask"]))
     if rc != 0:
         raise ValueError("invalid pass schedule or empty pair list")
     res["outliers"] = res["outliers"].astype(bool)
+    res["mask"] = res["mask"].astype(bool)
     if planes:
-        res["planes"] = pl
+        res["planes"] = pl[:, :, :int((~res["mask"]).sum())]
     return res
 
 

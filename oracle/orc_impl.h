@@ -1530,7 +1530,7 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
 }
 
 /* ======================================================================
- * Ensemble (sum-of-correlation) PIV (src/ensemble.jl:47-343), unmasked
+ * Ensemble (sum-of-correlation) PIV (src/ensemble.jl:47-343)
  * ==================================================================== */
 typedef struct {
     const hh_pass *p;
@@ -1540,11 +1540,14 @@ typedef struct {
     const int32_t *jobs;
     double *acc;  /* np per job */
     double *ust;  /* 2 x 44 per job, nullable */
+    const uint8_t *mask; /* image-sized pixel mask, nullable */
 } F(orc_ens_add_args);
 
 /* accumulate_planes! (src/ensemble.jl:317-343): the pair's plane of each
  * window is added to that window's sum; with uncertainty, its Wieneke
- * statistics (window-sized B at the same origin) are pooled. */
+ * statistics (window-sized B at the same origin) are pooled. Windows with
+ * masked pixels load as in load_windows! (valid-pixel means, masked pixels
+ * zero). */
 static void F(orc_ens_add_range)(void *arg, int j0, int j1)
 {
     F(orc_ens_add_args) *a = (F(orc_ens_add_args) *)arg;
@@ -1557,12 +1560,12 @@ static void F(orc_ens_add_range)(void *arg, int j0, int j1)
     size_t np = (size_t)a->c->m1 * a->c->m2;
     for (int jb = j0; jb < j1; jb++) {
         int rs = a->jobs[4 * jb + 2], cs = a->jobs[4 * jb + 3];
-        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs, NULL);
+        F(orc_plane)(a->c, &s, a->A, a->B, a->nr, rs, cs, a->mask);
         double *dst = a->acc + np * jb;
         for (size_t e = 0; e < np; e++) dst[e] += (double)s.R[e];
         if (a->ust)
             F(orc_uq_accumulate)(a->ust + (size_t)2 * HH_UQ_NSTATS * jb, a->A, a->B, a->nr, rs, cs, wr, wc,
-                                 a->c->apod, wA, wB, dC, dCs, NULL);
+                                 a->c->apod, wA, wB, dC, dCs, a->mask);
     }
     free(wA); free(wB); free(dC); free(dCs);
     F(orc_scratch_free)(&s);
@@ -1614,19 +1617,23 @@ static void F(orc_ens_an_range)(void *arg, int j0, int j1)
     free(R);
 }
 
-/* run_piv_ensemble(pairs, passes) (src/ensemble.jl:47-84) with
- * ensemble_pass (src/ensemble.jl:133-265), unmasked. Each pass deforms
+/* run_piv_ensemble(pairs, passes; mask, mask_threshold) (src/ensemble.jl:
+ * 47-84) with ensemble_pass (src/ensemble.jl:133-265). Each pass deforms
  * every pair by the shared predictor built from the previous pass, sums
  * each window's correlation planes over the pairs, analyzes the sums once
  * and validates (replacement forced on all but the last pass). The plane
  * sums are FP64 for either REAL (the reference sums in T); each sum is
- * rounded to REAL before its analysis. planes_out (nullable) receives the
- * last pass's sums, m1 x m2 x njobs. Returns 0, or -1 for an invalid
- * schedule or an empty pair list. */
+ * rounded to REAL before its analysis. With a pixel mask (image-sized,
+ * nullable) the grid mask removes windows as in run_piv, their cells are
+ * NaN and validated as masked, and the next pass's predictor fills them
+ * (src/ensemble.jl:154,254-264). planes_out (nullable) receives the last
+ * pass's sums, m1 x m2 per job (the unmasked windows in column-major
+ * order); omask (nullable) the final grid mask. Returns 0, or -1 for an
+ * invalid schedule or an empty pair list. */
 int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, int npairs, int nr, int nc,
                             const hh_pass *passes, int npasses, int predictor_smoothing, int nthreads, REAL *ox,
                             REAL *oy, REAL *ou, REAL *ov, REAL *opr, REAL *ocm, REAL *ouu, REAL *ouv, uint8_t *oout,
-                            double *planes_out)
+                            double *planes_out, const uint8_t *mask, double mask_threshold, uint8_t *omask)
 {
     if (npasses < 1 || npairs < 1) return -1;
     size_t npx = (size_t)nr * nc;
@@ -1640,13 +1647,14 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
     F(orc_pred) pred = {0, 0, NULL, NULL, NULL, NULL};
     int pny = 0, pnx = 0;
     REAL *px = NULL, *py = NULL, *pu = NULL, *pv = NULL;
+    uint8_t *pmask = NULL; /* the previous pass's grid mask */
     int rc = 0;
     for (int k = 0; k < npasses; k++) {
         const hh_pass *p = &passes[k];
         int force_replace = k < npasses - 1;
         int have_pred = px != NULL;
         if (have_pred) {
-            /* build_predictor (src/pipeline.jl:454-464), unmasked */
+            /* build_predictor (src/pipeline.jl:454-464) */
             free(pred.x); free(pred.y); free(pred.u); free(pred.v);
             pred.ny = pny;
             pred.nx = pnx;
@@ -1654,15 +1662,10 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
             pred.y = py;
             pred.u = (REAL *)malloc(sizeof(REAL) * pny * pnx);
             pred.v = (REAL *)malloc(sizeof(REAL) * pny * pnx);
-            if (predictor_smoothing) {
-                F(orc_smooth)(pu, pny, pnx, pred.u);
-                F(orc_smooth)(pv, pny, pnx, pred.v);
-            } else {
-                memcpy(pred.u, pu, sizeof(REAL) * pny * pnx);
-                memcpy(pred.v, pv, sizeof(REAL) * pny * pnx);
-            }
-            free(pu); free(pv);
+            F(orc_build_predictor)(pu, pv, pmask, pny, pnx, predictor_smoothing, pred.u, pred.v);
+            free(pu); free(pv); free(pmask);
             px = py = pu = pv = NULL;
+            pmask = NULL;
         }
         int ny, nx;
         if (F(orc_pass_grid)(p, nr, nc, &ny, &nx, NULL, NULL, NULL) != 0) {
@@ -1673,6 +1676,19 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
         REAL *x = (REAL *)malloc(sizeof(REAL) * nx), *y = (REAL *)malloc(sizeof(REAL) * ny);
         int32_t *jobs = (int32_t *)malloc(sizeof(int32_t) * 4 * (ng > 0 ? ng : 1));
         F(orc_pass_grid)(p, nr, nc, &ny, &nx, x, y, jobs);
+        /* grid mask: masked windows produce no job (pass_grid, src/pipeline.jl:542-553) */
+        uint8_t *gmask = (uint8_t *)calloc(ng > 0 ? ng : 1, 1);
+        int anyg = 0, njobs = (int)ng;
+        if (mask) {
+            F(orc_grid_mask)(p, nr, nc, mask, mask_threshold, gmask);
+            njobs = 0;
+            for (size_t q = 0; q < ng; q++) {
+                anyg |= gmask[q];
+                if (gmask[q]) continue;
+                memmove(jobs + 4 * (size_t)njobs, jobs + 4 * q, 4 * sizeof(int32_t));
+                njobs++;
+            }
+        }
         F(orc_corr) c;
         F(orc_corr_init)(&c, p);
         size_t np = (size_t)c.m1 * c.m2;
@@ -1690,8 +1706,8 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
                 srcA = wA;
                 srcB = wB;
             }
-            F(orc_ens_add_args) aa = {p, &c, srcA, srcB, nr, jobs, acc, ust};
-            orc_parallel_for((int)ng, nthreads, F(orc_ens_add_range), &aa);
+            F(orc_ens_add_args) aa = {p, &c, srcA, srcB, nr, jobs, acc, ust, mask};
+            orc_parallel_for(njobs, nthreads, F(orc_ens_add_range), &aa);
         }
         /* u, v = the shared predictor at the nodes (zero without one) */
         if (have_pred) F(orc_node_values)(pred.y, pred.ny, pred.x, pred.nx, pred.u, pred.v, y, ny, x, nx, u, v);
@@ -1704,9 +1720,12 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
         if (au)
             for (size_t q = 0; q < ng * nalt; q++) au[q] = av[q] = (REAL)NAN;
         F(orc_ens_an_args) an = {p, &c, jobs, ny, nx, acc, ust, u, v, pr, cm, au, av, uu, uv};
-        orc_parallel_for((int)ng, nthreads, F(orc_ens_an_range), &an);
+        orc_parallel_for(njobs, nthreads, F(orc_ens_an_range), &an);
+        if (anyg)
+            for (size_t q = 0; q < ng; q++)
+                if (gmask[q]) u[q] = v[q] = pr[q] = cm[q] = (REAL)NAN;
         uint8_t *outl = (uint8_t *)calloc(ng > 0 ? ng : 1, 1);
-        F(orc_validate_and_replace)(p, ny, nx, force_replace, u, v, pr, cm, au, av, NULL, outl);
+        F(orc_validate_and_replace)(p, ny, nx, force_replace, u, v, pr, cm, au, av, anyg ? gmask : NULL, outl);
         if (k == npasses - 1) {
             if (ox) memcpy(ox, x, sizeof(REAL) * nx);
             if (oy) memcpy(oy, y, sizeof(REAL) * ny);
@@ -1717,17 +1736,21 @@ int F(orc_run_piv_ensemble)(const REAL *const *imgA, const REAL *const *imgB, in
             if (ouu) memcpy(ouu, uu, sizeof(REAL) * ng);
             if (ouv) memcpy(ouv, uv, sizeof(REAL) * ng);
             if (oout) memcpy(oout, outl, ng);
-            if (planes_out) memcpy(planes_out, acc, sizeof(double) * ng * np);
+            if (planes_out) memcpy(planes_out, acc, sizeof(double) * (size_t)njobs * np);
+            if (omask) memcpy(omask, gmask, ng);
             free(x); free(y); free(u); free(v);
+            free(gmask);
         } else {
             px = x; py = y; pu = u; pv = v;
             pny = ny; pnx = nx;
+            pmask = anyg ? gmask : NULL;
+            if (!anyg) free(gmask);
         }
         free(jobs); free(acc); free(ust); free(pr); free(cm); free(uu); free(uv); free(au); free(av); free(outl);
         F(orc_corr_free)(&c);
     }
     free(pred.x); free(pred.y); free(pred.u); free(pred.v);
-    free(px); free(py); free(pu); free(pv);
+    free(px); free(py); free(pu); free(pv); free(pmask);
     free(cA); free(cB); free(wA); free(wB);
     return rc;
 }

@@ -152,3 +152,42 @@ def test_masked_single_window_recovers_shift(orc, hh):
     jobs = np.array([[1], [1], [1], [1]], dtype=np.int32, order="F")
     r = orc.process_windows(p, A, B, jobs, 1, 1, mask=m)
     assert r["u"][0, 0] == pytest.approx(du, abs=0.1) and r["v"][0, 0] == pytest.approx(dv, abs=0.1)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ensemble_one_pair_with_mask_is_run_piv(orc, hh, prec):
+    # one pair: each ensemble pass is the single-pair pass, masks included
+    # (src/ensemble.jl:154,254-264 against src/pipeline.jl:655-832)
+    A, B, mask, du, dv = _garbage_scene()
+    passes = [p.to_struct() for p in hh.multipass_parameters([32, 16], padding=True, apodization="gauss",
+                                                             final=dict(uncertainty=True))]
+    e = orc.run_piv_ensemble([(A, B)], passes, prec=prec, mask=mask)
+    r = orc.run_piv(A, B, passes, prec=prec, mask=mask)
+    assert e["mask"].any()
+    for k in ("u", "v", "peak_ratio", "correlation_moment", "uncertainty_u", "uncertainty_v", "outliers", "mask"):
+        assert np.array_equal(e[k], r[k], equal_nan=True), k
+
+
+def test_ensemble_masked_plane_sums(orc, hh):
+    # the masked ensemble sums each unmasked window's masked-load planes
+    # over the pairs; masked cells are NaN and never outliers
+    from conftest import low_snr_pairs
+
+    pairs = low_snr_pairs(n=64, npairs=3)
+    m = np.zeros((64, 64), bool)
+    m[10:40, 20:50] = True
+    p = P(hh, window_size=32, overlap=16)
+    e = orc.run_piv_ensemble(pairs, [p], prec="f32", planes=True, mask=m)
+    gm = orc.grid_mask(p, 64, 64, m)
+    assert np.array_equal(e["mask"], gm) and gm.any() and not gm.all()
+    x, y, jobs = orc.pass_grid(p, 64, 64)
+    keep = [k for k in range(jobs.shape[1]) if not gm[jobs[0, k] - 1, jobs[1, k] - 1]]
+    jobs = np.asfortranarray(jobs[:, keep])
+    total = 0
+    for A, B in pairs:
+        r = orc.process_windows(p, A.astype(np.float32), B.astype(np.float32), jobs, y.size, x.size, prec="f32",
+                                planes=True, mask=m)
+        total = total + r["planes"].astype(np.float64)
+    assert np.array_equal(e["planes"], total)
+    assert np.all(np.isnan(e["u"][gm])) and not e["outliers"][gm].any()
+    assert np.all(np.isfinite(e["u"][~gm]))
