@@ -393,11 +393,20 @@ float quantile95(std::vector<float> &v)
     return (1 - gamma) * a + gamma * b;
 }
 
-void set_predictor_from_grid(hh_ctx *c, const PassGrid &g, const float *u, const float *v, bool smoothing)
+// gmask (nullable, device): masked cells, filled from their unmasked
+// neighbours (into the validation scratch) before the smoothing.
+void set_predictor_from_grid(hh_ctx *c, const PassGrid &g, const float *u, const float *v, bool smoothing,
+                             const uint8_t *gmask = nullptr)
 {
     const size_t n = (size_t)g.ny * g.nx;
     c->pu.ensure(n);
     c->pv.ensure(n);
+    if (gmask) {
+        float *fu = c->gsu.ensure(n), *fv = c->gsv.ensure(n);
+        launch_fill_masked(one_grid(g.ny, g.nx), u, v, gmask, c->gcount.ensure(1), fu, fv, c->stream);
+        u = fu;
+        v = fv;
+    }
     if (smoothing) {
         launch_smooth(one_grid(g.ny, g.nx), u, c->pu.p, c->stream);
         launch_smooth(one_grid(g.ny, g.nx), v, c->pv.p, c->stream);
@@ -417,6 +426,20 @@ void set_predictor_from_grid(hh_ctx *c, const PassGrid &g, const float *u, const
 }  // namespace hh
 
 namespace {
+
+template <typename T>
+T *upload(hh_ctx *c, DBuf<T> &b, const T *h, size_t n)
+{
+    T *p = b.ensure(std::max<size_t>(n, 1));
+    if (h && n) HH_HIP(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+    return p;
+}
+
+template <typename T>
+void download(hh_ctx *c, T *h, const T *d, size_t n)
+{
+    if (h && n) HH_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+}
 
 // ---- ensemble (sum-of-correlation) PIV: src/ensemble.jl:133-343 -----------
 
@@ -448,6 +471,9 @@ void ens_begin(hh_ctx *c, const hh_pass &p, const int32_t *djobs, int njobs, boo
     e.per = e.box ? (size_t)nb * nb : (size_t)e.cfg.M * e.cfg.M;
     e.njobs = njobs;
     e.jobs = djobs;
+    e.nclean = njobs;
+    e.npart = 0;
+    e.tjobs = e.mjobs = nullptr;
     e.unc = unc;
     e.npairs = 0;
     fill64(c, c->ensacc.ensure(std::max<size_t>(e.per * njobs, 1)), e.per * njobs, 0.0);
@@ -465,12 +491,29 @@ void ens_add(hh_ctx *c, const float *A, const float *B)
     out.acc = c->ensacc.p;
     out.ny = out.nx = 1;
     timed(c, "ensemble_add", e.cfg.w,
-          [&] { launch_correlate(e.cfg, A, B, c->nr, c->nc, e.jobs, e.njobs, out, c->stream); });
+          [&] { launch_correlate(e.cfg, A, B, c->nr, c->nc, e.jobs, e.nclean, out, c->stream); });
+    const int w = e.cfg.w;
+    const size_t tp = (size_t)e.npart * w * w;
+    float *tA = e.npart ? c->enstiles.p : nullptr, *tB = tA ? tA + tp : nullptr;
+    if (e.npart) {
+        // windows with masked pixels: centred tiles, summed into their own
+        // slots after the clean windows'
+        timed(c, "mask_tiles", w, [&] {
+            launch_mask_tiles(A, B, c->nr, c->eng.dmask.p, c->eng.mnr, e.mjobs, e.npart, w, tA, tB, c->stream);
+        });
+        CorrOut o2 = out;
+        o2.acc = c->ensacc.p + e.per * e.nclean;
+        timed(c, "ensemble_add_masked", w,
+              [&] { launch_correlate(e.cfg, tA, tB, w, 0, e.tjobs, e.npart, o2, c->stream); });
+    }
     if (e.unc) {
         const ConfigBufs &cb = get_config(c, e.pass);
         timed(c, "uq", e.cfg.w, [&] {
-            launch_uq(e.cfg.w, cb.apod.p, A, B, c->nr, e.jobs, e.njobs, 1, nullptr, nullptr, c->ensstats.p, true,
+            launch_uq(e.cfg.w, cb.apod.p, A, B, c->nr, e.jobs, e.nclean, 1, nullptr, nullptr, c->ensstats.p, true,
                       c->stream);
+            if (e.npart)
+                launch_uq(w, cb.apod.p, tA, tB, w, e.tjobs, e.npart, 1, nullptr, nullptr,
+                          c->ensstats.p + (size_t)2 * HH_UQ_NSTATS * e.nclean, true, c->stream);
         });
     }
     ++e.npairs;
@@ -499,9 +542,12 @@ void ens_analyze(hh_ctx *c, int ny, int nx, float *u, float *v, float *pr, float
     if (e.unc && (uu || uv)) launch_uq_finalize(c->ensstats.p, e.jobs, e.njobs, ny, uu, uv, c->stream);
 }
 
-// run_piv_ensemble(pairs, passes) (src/ensemble.jl:47-84) with ensemble_pass
-// (src/ensemble.jl:133-265), unmasked, entirely on the device. The pairs are
-// uploaded once and stay resident across the passes.
+// run_piv_ensemble(pairs, passes; mask, mask_threshold) (src/ensemble.jl:
+// 47-84) with ensemble_pass (src/ensemble.jl:133-265), entirely on the
+// device. The pairs are uploaded once and stay resident across the passes.
+// The context's pixel mask (hh_set_mask), when set, removes the windows of
+// the grid mask, loads partly masked windows from their valid pixels and
+// validates and fills the masked cells as run_piv does.
 void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int npairs, int nr, int nc,
                   const hh_pass *passes, int np, int smoothing)
 {
@@ -509,6 +555,12 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     if (np < 1) throw Error(HH_EINVAL, "at least one pass is required");
     if (nr < 2 || nc < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
     for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
+    const Engine &EM = c->eng;
+    const bool masked = !EM.hmask.empty();
+    if (masked && (EM.mnr != nr || EM.mnc != nc))
+        throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(EM.mnr) + ", " +
+                                 std::to_string(EM.mnc) + ") for images of (" + std::to_string(nr) + ", " +
+                                 std::to_string(nc) + ")");
     const size_t npx = (size_t)nr * nc;
     float *arena = nullptr;
     bool upload = false;
@@ -560,7 +612,8 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     c->gpx.ensure(std::max(maxnx, 1));
     c->gpy.ensure(std::max(maxny, 1));
     PassGrid prev;
-    bool have_pred = false;
+    bool have_pred = false, prev_anyg = false;
+    c->final_gm.clear();
     for (int k = 0; k < np; ++k) {
         const hh_pass &p = passes[k];
         const bool force_replace = k < np - 1;
@@ -569,12 +622,64 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
         const size_t ng = (size_t)ny * nx;
         if (ng == 0) throw Error(HH_EINVAL, "the pass grid is empty");
         if (k > 0) {
-            set_predictor_from_grid(c, prev, c->gu.p, c->gv.p, smoothing);
+            // c->gmask still holds the previous pass's grid mask
+            set_predictor_from_grid(c, prev, c->gu.p, c->gv.p, smoothing, prev_anyg ? c->gmask.p : nullptr);
             have_pred = true;
         }
-        const int32_t *djobs = device_jobs(c, p, g);
         const bool unc = p.uncertainty && !force_replace;
-        ens_begin(c, p, djobs, (int)ng, unc);
+        const int32_t *djobs = nullptr;
+        int nclean = (int)ng, npart = 0;
+        bool anyg = false;
+        std::vector<uint8_t> gm;
+        if (masked) {
+            // pass_grid's grid mask (src/pipeline.jl:542-549): the clean
+            // windows first, then those with masked pixels; masked ones
+            // have no job
+            const int w = p.window_size[0];
+            std::vector<int32_t> clean, part, tj, mj;
+            gm.assign(ng, 0);
+            for (size_t j = 0; j < ng; ++j) {
+                const int32_t *jb = &g.jobs[4 * j];
+                const int rs = jb[2], cs = jb[3];
+                long nm = 0;
+                for (int cc = 0; cc < w; ++cc) {
+                    const uint8_t *col = EM.hmask.data() + (size_t)(cs - 1 + cc) * nr + (rs - 1);
+                    for (int i = 0; i < w; ++i) nm += col[i];
+                }
+                const bool gmj = (double)nm / (double)(w * w) >= EM.mthr;
+                gm[j] = gmj;
+                anyg |= gmj;
+                if (gmj) continue;
+                if (nm == 0) {
+                    clean.insert(clean.end(), jb, jb + 4);
+                } else {
+                    const int t = (int)(tj.size() / 4);
+                    part.insert(part.end(), jb, jb + 4);
+                    tj.insert(tj.end(), {jb[0], jb[1], 1, t * w + 1});
+                    mj.insert(mj.end(), {rs - 1, cs - 1, rs - 1, cs - 1});
+                }
+            }
+            nclean = (int)(clean.size() / 4);
+            npart = (int)(part.size() / 4);
+            if ((size_t)npart * w >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows with masked pixels");
+            clean.insert(clean.end(), part.begin(), part.end());
+            tj.insert(tj.end(), mj.begin(), mj.end());
+            djobs = ::upload(c, c->ensjobs, clean.data(), clean.size());
+            const int32_t *pj = ::upload(c, c->enspart, tj.data(), tj.size());
+            uint8_t *dgm = c->gmask.ensure(ng);
+            HH_HIP(hipMemcpyAsync(dgm, gm.data(), ng, hipMemcpyHostToDevice, c->stream));
+            // the uploads read host vectors of this scope
+            HH_HIP(hipStreamSynchronize(c->stream));
+            if (npart) c->enstiles.ensure(2 * (size_t)npart * w * w);
+            ens_begin(c, p, djobs, nclean + npart, unc);
+            c->ens.nclean = nclean;
+            c->ens.npart = npart;
+            c->ens.tjobs = pj;
+            c->ens.mjobs = pj + 4 * (size_t)npart;
+        } else {
+            djobs = device_jobs(c, p, g);
+            ens_begin(c, p, djobs, (int)ng, unc);
+        }
         for (int q = 0; q < npairs; ++q) {
             c->A = arena + 2 * npx * q;
             c->B = arena + 2 * npx * q + npx;
@@ -620,6 +725,7 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
         fill(c, c->guu.p, ng, NAN);
         fill(c, c->guv.p, ng, NAN);
         ens_analyze(c, ny, nx, u, v, c->gpr.p, c->gcm.p, au, av, c->guu.p, c->guv.p);
+        if (anyg) launch_mask_nan(one_grid(ny, nx), c->gmask.p, u, v, c->gpr.p, c->gcm.p, c->stream);
         ValidateArgs va{};
         va.grids = one_grid(ny, nx);
         va.pass = &p;
@@ -635,8 +741,11 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
         va.sv = c->gsv.p;
         va.sflag = c->gsflag.p;
         va.count = c->gcount.ensure(1);
+        va.mask = anyg ? c->gmask.p : nullptr;
         timed(c, "validate", p.window_size[0], [&] { launch_validate(va, c->stream); });
         prev = std::move(g);
+        prev_anyg = anyg;
+        if (k == np - 1 && anyg) c->final_gm = gm;
     }
     HH_HIP(hipGetLastError());
     c->final_grid = prev;
@@ -827,21 +936,12 @@ void fetch(hh_ctx *c, hh_field *r)
     HH_HIP(hipStreamSynchronize(c->stream));
     if (r->x) std::memcpy(r->x, g.x.data(), sizeof(float) * g.nx);
     if (r->y) std::memcpy(r->y, g.y.data(), sizeof(float) * g.ny);
-    if (r->mask) std::memset(r->mask, 0, ng);
-}
-
-template <typename T>
-T *upload(hh_ctx *c, DBuf<T> &b, const T *h, size_t n)
-{
-    T *p = b.ensure(std::max<size_t>(n, 1));
-    if (h && n) HH_HIP(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
-    return p;
-}
-
-template <typename T>
-void download(hh_ctx *c, T *h, const T *d, size_t n)
-{
-    if (h && n) HH_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+    if (r->mask) {
+        if (c->final_gm.size() == ng)
+            std::memcpy(r->mask, c->final_gm.data(), ng);
+        else
+            std::memset(r->mask, 0, ng);
+    }
 }
 
 }  // namespace

@@ -221,8 +221,16 @@ struct hh_ctx {
         int njobs = 0, npairs = 0;
         const int32_t *jobs = nullptr;       // device
         std::vector<int32_t> hjobs;          // host copy for bounds checks
+        // with a pixel mask: the first nclean jobs read the images, the
+        // npart after them are windows with masked pixels, accumulated from
+        // centred tiles (tjobs: their jobs on the tile atlas, mjobs: the
+        // tile builder's list)
+        int nclean = 0, npart = 0;
+        const int32_t *tjobs = nullptr, *mjobs = nullptr;
     } ens;
-    hh::host::DBuf<int32_t> ensjobs;
+    hh::host::DBuf<int32_t> ensjobs, enspart;
+    hh::host::DBuf<float> enstiles;
+    std::vector<uint8_t> final_gm;  // final grid mask of hh_run_piv_ensemble (empty: none)
     hh::host::DBuf<double> ensacc, ensstats;
     hh::host::DBuf<float> arena;  // the pairs of hh_run_piv_ensemble, resident in HBM
     int arena_pairs = 0, arena_nr = 0, arena_nc = 0;

@@ -161,7 +161,8 @@ def run_piv_ensemble(pairs: Sequence, params, *, effort=None, backend: str = "mi
     sum; with a schedule, the previous pass's ensemble field is the shared
     deformation predictor of every pair (replacement is forced on all but the
     last pass). With ``uncertainty`` on the last pass, the Wieneke statistics
-    are pooled over the pairs."""
+    are pooled over the pairs. ``mask`` and ``mask_threshold`` act as in
+    run_piv (src/ensemble.jl:151-154,254-264)."""
     if effort is not None:
         raise ArgumentError("effort cannot be combined with explicit PIVParameters or pass schedules")
     unknown = set(kwargs) - {"threaded", "mask_threshold", "scale", "progress", "preprocess", "image_type"}
@@ -186,18 +187,14 @@ def run_piv_ensemble(pairs: Sequence, params, *, effort=None, backend: str = "mi
             raise DimensionMismatch(f"all pairs must share the image size {arrays[0][0].shape}, got {A.shape}")
         arrays.append((A, B))
     shape = arrays[0][0].shape
-    if mask is not None:
-        m = np.asarray(mask, dtype=bool)
-        if m.shape != shape:
-            raise DimensionMismatch(f"mask must have the same size as the images, got {m.shape}")
-        if m.any():
-            raise ArgumentError("backend :mi355x does not support masks yet")
+    m, thr = _check_mask(mask, kwargs.get("mask_threshold", 0.5), shape)
     ws = workspace or PIVWorkspace(be)
     try:
         ctx = ws.ctx
         ctx.check_passes(passes)
         for p in passes:
             pass_grid_counts(p, *shape)
+        ctx.set_mask(m, thr)
         r = ctx.run_piv_ensemble(arrays, passes, predictor_smoothing)
     finally:
         if workspace is None:

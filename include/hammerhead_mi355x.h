@@ -219,10 +219,11 @@ int hh_run_piv(hh_ctx *ctx, const hh_pass *passes, int npasses,
 int hh_fetch_result(hh_ctx *ctx, hh_field *result);
 
 /* The mask and mask_threshold keywords of run_piv (src/pipeline.jl:361-449,
- * checks at 393-396) and run_piv_sequence (src/io.jl:229-247): an nr x nc
- * column-major pixel mask (nonzero = masked) applied to every following
- * hh_run_piv / hh_run_piv_batch, whose pairs must all be nr x nc (HH_EDIM at
- * the run otherwise). Windows whose masked-pixel fraction reaches
+ * checks at 393-396), run_piv_sequence (src/io.jl:229-247) and
+ * run_piv_ensemble (src/ensemble.jl:47-84,151-154): an nr x nc column-major
+ * pixel mask (nonzero = masked) applied to every following hh_run_piv /
+ * hh_run_piv_batch / hh_run_piv_ensemble, whose pairs must all be nr x nc
+ * (HH_EDIM at the run otherwise). Windows whose masked-pixel fraction reaches
  * mask_threshold produce no vector (pass_grid, src/pipeline.jl:542-549):
  * u, v, peak_ratio, correlation_moment and the uncertainties hold NaN there,
  * they are never outliers and result->mask flags them; windows with fewer
@@ -231,7 +232,8 @@ int hh_fetch_result(hh_ctx *ctx, hh_field *result);
  * (or one with no pixel set) removes the mask; the threshold must be in
  * (0, 1] (HH_EINVAL). The mask stays set across hh_set_pair* and
  * hh_set_pairs; setting an identical mask again keeps the planned schedule.
- * The ensemble (hh_run_piv_ensemble) does not use it. */
+ * The per-hook entry points (hh_process_windows, hh_uncertainty_sweep,
+ * hh_ensemble_*) do not use it. */
 int hh_set_mask(hh_ctx *ctx, const uint8_t *mask, int nr, int nc, double mask_threshold);
 
 /* ---- batches of pairs of uneven sizes ----------------------------------- */
@@ -305,8 +307,10 @@ int hh_ensemble_planes(hh_ctx *ctx, double *planes);
  * (src/ensemble.jl:47-84) on the device: A[q], B[q] are host column-major
  * float32 nr x nc frames of pair q, uploaded once and kept resident in HBM
  * for every pass. With A = B = NULL the pairs of the previous call (same
- * npairs, nr, nc) are used again without an upload. The final result is
- * copied to `result` when non-NULL, and stays available to hh_fetch_result. */
+ * npairs, nr, nc) are used again without an upload. The context's pixel
+ * mask (hh_set_mask), when set, applies as the mask keyword does. The final
+ * result is copied to `result` when non-NULL, and stays available to
+ * hh_fetch_result. */
 int hh_run_piv_ensemble(hh_ctx *ctx, const float *const *A, const float *const *B, int npairs, int nr, int nc,
                         const hh_pass *passes, int npasses, int predictor_smoothing, hh_field *result);
 

@@ -44,9 +44,11 @@ def disk_scene(hh, n=256, seed=61, speckle=0.0):
 
 def run_masked(hh, orc, ctx, A, B, m, passes, thr=0.5):
     ctx.set_mask(m, thr)
-    ctx.set_pair(A, B)
-    g = ctx.run_piv(passes)
-    ctx.set_mask(None)
+    try:
+        ctx.set_pair(A, B)
+        g = ctx.run_piv(passes)
+    finally:
+        ctx.set_mask(None)
     o = orc.run_piv(A.astype(np.float64), B.astype(np.float64), [p.to_struct() for p in passes], mask=m,
                     mask_threshold=thr)
     return g, o
@@ -191,3 +193,50 @@ def test_run_piv_sequence_with_mask(hh):
         assert r.mask.any() and np.array_equal(r.mask, single.mask)
         assert np.array_equal(r.u, single.u, equal_nan=True)
         assert np.array_equal(r.outliers, single.outliers)
+
+
+@pytest.mark.parametrize("form", ["fft32", "mfma16"])
+def test_ensemble_mask_vs_oracle(hh, orc, form):
+    """run_piv_ensemble with a mask (src/ensemble.jl:151-154,254-264): the
+    low-SNR ensemble scene with a masked block and scattered masked pixels,
+    two passes with pooled uncertainty on the last."""
+    from conftest import low_snr_pairs
+
+    pairs = low_snr_pairs(n=128, npairs=6)
+    rng = np.random.default_rng(3)
+    m = np.zeros((128, 128), bool)
+    m[30:75, 60:110] = True
+    m |= rng.random((128, 128)) < 0.01
+    passes = hh.multipass_parameters([32, 16], padding=True, apodization="gauss", final=dict(uncertainty=True))
+    say("ensemble mask", form)
+    f32 = [(A.astype(np.float32), B.astype(np.float32)) for A, B in pairs]
+    with hh.Context(0) as c:
+        c.set_form(hh.HH_FORM_MFMA16 if form == "mfma16" else hh.HH_FORM_FFT32)
+        c.set_mask(m, 0.5)
+        g = c.run_piv_ensemble(f32, passes)
+    o = orc.run_piv_ensemble(pairs, [p.to_struct() for p in passes], mask=m)
+    check_masked(g, o, uq=True)
+    if form == "fft32":
+        # the public keyword interface gives the same result
+        r = hh.run_piv_ensemble(f32, passes, mask=m)
+        assert np.array_equal(r.mask, g["mask"]) and np.array_equal(r.u, g["u"], equal_nan=True)
+
+
+def test_ensemble_one_pair_with_mask_matches_run_piv(hh, ctx):
+    A, B, m = disk_scene(hh, n=192, seed=71, speckle=0.005)
+    passes = hh.multipass_parameters([32, 16], padding=True, apodization="gauss", final=dict(uncertainty=True))
+    ctx.set_mask(m, 0.5)
+    try:
+        ctx.set_pair(A, B)
+        r = ctx.run_piv(passes)
+        e = ctx.run_piv_ensemble([(A, B)], passes)
+    finally:
+        ctx.set_mask(None)
+    assert r["mask"].any()
+    for k in FIELDS + ("uncertainty_u", "uncertainty_v", "outliers", "mask"):
+        fin = np.isfinite(r[k]) if r[k].dtype != bool else None
+        if fin is None:
+            assert np.array_equal(e[k], r[k]), k
+        else:
+            assert np.array_equal(np.isfinite(e[k]), fin), k
+            assert np.abs(e[k][fin] - r[k][fin]).max() <= 1e-5 * max(1.0, np.abs(r[k][fin]).max()), k
