@@ -441,6 +441,47 @@ void download(hh_ctx *c, T *h, const T *d, size_t n)
     if (h && n) HH_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
 }
 
+// The windows a per-hook entry point correlates. Without a pixel mask
+// (hh_set_mask) they are read from the images; with one, every job's window
+// is first copied into a centred tile (k_mask_tiles: the mean of its valid
+// pixels subtracted, masked pixels 0, as load_windows! loads a window with a
+// mask, src/correlators.jl:199-227) and the launch reads the tiles through
+// jobs (gi, gj, 1, t w + 1) in the caller's order, so every per-job output
+// keeps its index.
+struct HookSource {
+    const float *A, *B;
+    int ld;
+    const int32_t *jobs;
+};
+
+HookSource hook_source(hh_ctx *c, int w, const int32_t *hjobs, int njobs, const float *dA, const float *dB,
+                       const int32_t *djobs)
+{
+    const Engine &E = c->eng;
+    if (E.hmask.empty() || njobs <= 0) return HookSource{dA, dB, c->nr, djobs};
+    if (E.mnr != c->nr || E.mnc != c->nc)
+        throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(E.mnr) + ", " +
+                                 std::to_string(E.mnc) + ")");
+    if ((size_t)njobs * w >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows for the masked tiles");
+    std::vector<int32_t> t((size_t)8 * njobs);
+    for (int k = 0; k < njobs; ++k) {
+        const int32_t *j = hjobs + 4 * (size_t)k;
+        int32_t *tj = &t[4 * (size_t)k], *mj = &t[4 * ((size_t)njobs + k)];
+        tj[0] = j[0];
+        tj[1] = j[1];
+        tj[2] = 1;
+        tj[3] = k * w + 1;
+        mj[0] = mj[2] = j[2] - 1;
+        mj[1] = mj[3] = j[3] - 1;
+    }
+    const int32_t *d = upload(c, c->hookmask, t.data(), t.size());
+    const size_t tp = (size_t)njobs * w * w;
+    float *tiles = c->hooktiles.ensure(2 * tp);
+    launch_mask_tiles(dA, dB, c->nr, E.dmask.p, E.mnr, d + 4 * (size_t)njobs, njobs, w, tiles, tiles + tp, c->stream);
+    HH_HIP(hipStreamSynchronize(c->stream));  // the upload reads `t`
+    return HookSource{tiles, tiles + tp, w, d};
+}
+
 // ---- ensemble (sum-of-correlation) PIV: src/ensemble.jl:133-343 -----------
 
 __global__ void k_fill_f64(double *p, size_t n, double v)
@@ -482,16 +523,21 @@ void ens_begin(hh_ctx *c, const hh_pass &p, const int32_t *djobs, int njobs, boo
     e.active = true;
 }
 
-// accumulate_planes! (src/ensemble.jl:317-343) of one pair (device images).
-void ens_add(hh_ctx *c, const float *A, const float *B)
+// accumulate_planes! (src/ensemble.jl:317-343) of one pair (device images;
+// with src given, the clean windows are read through src instead of A, B
+// and the ensemble's jobs).
+void ens_add(hh_ctx *c, const float *A, const float *B, const HookSource *src = nullptr)
 {
     auto &e = c->ens;
     if (!e.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
     CorrOut out{};
     out.acc = c->ensacc.p;
     out.ny = out.nx = 1;
+    const float *cA = src ? src->A : A, *cB = src ? src->B : B;
+    const int cld = src ? src->ld : c->nr;
+    const int32_t *cj = src ? src->jobs : e.jobs;
     timed(c, "ensemble_add", e.cfg.w,
-          [&] { launch_correlate(e.cfg, A, B, c->nr, c->nc, e.jobs, e.nclean, out, c->stream); });
+          [&] { launch_correlate(e.cfg, cA, cB, cld, 0, cj, e.nclean, out, c->stream); });
     const int w = e.cfg.w;
     const size_t tp = (size_t)e.npart * w * w;
     float *tA = e.npart ? c->enstiles.p : nullptr, *tB = tA ? tA + tp : nullptr;
@@ -509,7 +555,7 @@ void ens_add(hh_ctx *c, const float *A, const float *B)
     if (e.unc) {
         const ConfigBufs &cb = get_config(c, e.pass);
         timed(c, "uq", e.cfg.w, [&] {
-            launch_uq(e.cfg.w, cb.apod.p, A, B, c->nr, e.jobs, e.nclean, 1, nullptr, nullptr, c->ensstats.p, true,
+            launch_uq(e.cfg.w, cb.apod.p, cA, cB, cld, cj, e.nclean, 1, nullptr, nullptr, c->ensstats.p, true,
                       c->stream);
             if (e.npart)
                 launch_uq(w, cb.apod.p, tA, tB, w, e.tjobs, e.npart, 1, nullptr, nullptr,
@@ -1141,6 +1187,7 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
         const int K = std::max(pass->n_peaks, 2);
         CorrConfig cfg = make_corr(c, *pass);
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
+        const HookSource hs = hook_source(c, w, jobs, njobs, dA, dB, dj);
         CorrOut out{};
         out.u = upload(c, c->hk[0], u, ng);
         out.v = upload(c, c->hk[1], v, ng);
@@ -1164,12 +1211,12 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
         }
         bool fused = false;
         timed(c, out.unc_u ? "correlate_uq" : "correlate", w,
-              [&] { fused = launch_correlate(cfg, dA, dB, c->nr, c->nc, dj, njobs, out, c->stream); });
+              [&] { fused = launch_correlate(cfg, hs.A, hs.B, hs.ld, 0, hs.jobs, njobs, out, c->stream); });
         if ((unc_u || unc_v) && !fused) {
             const ConfigBufs &cb = get_config(c, *pass);
             timed(c, "uq", w, [&] {
-                launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr,
-                          nullptr, false, c->stream);
+                launch_uq(w, cb.apod.p, hs.A, hs.B, hs.ld, hs.jobs, njobs, ny, unc_u ? duu : nullptr,
+                          unc_v ? duv : nullptr, nullptr, false, c->stream);
             });
         }
         download(c, u, out.u, ng);
@@ -1208,6 +1255,7 @@ int hh_correlation_planes(hh_ctx *c, const hh_pass *pass, int src, const int32_t
         if (njobs <= 0) return;
         CorrConfig cfg = make_corr(c, *pass);
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
+        const HookSource hs = hook_source(c, w, jobs, njobs, dA, dB, dj);
         const size_t ng = (size_t)maxgi * maxgj;
         CorrOut out{};
         out.u = upload<float>(c, c->hk[0], nullptr, ng);
@@ -1217,7 +1265,7 @@ int hh_correlation_planes(hh_ctx *c, const hh_pass *pass, int src, const int32_t
         out.planes = upload<float>(c, c->hk[9], nullptr, (size_t)M * M * njobs);
         out.ny = maxgi;
         out.nx = maxgj;
-        launch_correlate(cfg, dA, dB, c->nr, c->nc, dj, njobs, out, c->stream);
+        launch_correlate(cfg, hs.A, hs.B, hs.ld, 0, hs.jobs, njobs, out, c->stream);
         download(c, planes, out.planes, (size_t)M * M * njobs);
         HH_HIP(hipStreamSynchronize(c->stream));
     });
@@ -1241,13 +1289,14 @@ int hh_uncertainty_sweep(hh_ctx *c, const hh_pass *pass, int src, const int32_t 
         if (njobs <= 0) return;
         const size_t ng = (size_t)ny * nx;
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
+        const HookSource hs = hook_source(c, w, jobs, njobs, dA, dB, dj);
         float *duu = upload(c, c->hk[7], unc_u, ng);
         float *duv = upload(c, c->hk[8], unc_v, ng);
         double *dst = stats ? c->hkd.ensure((size_t)2 * HH_UQ_NSTATS * njobs) : nullptr;
         const ConfigBufs &cb = get_config(c, *pass);
         timed(c, "uq", w, [&] {
-            launch_uq(w, cb.apod.p, dA, dB, c->nr, dj, njobs, ny, unc_u ? duu : nullptr, unc_v ? duv : nullptr, dst,
-                      false, c->stream);
+            launch_uq(w, cb.apod.p, hs.A, hs.B, hs.ld, hs.jobs, njobs, ny, unc_u ? duu : nullptr,
+                      unc_v ? duv : nullptr, dst, false, c->stream);
         });
         if (unc_u) download(c, unc_u, duu, ng);
         if (unc_v) download(c, unc_v, duv, ng);
@@ -1457,7 +1506,8 @@ int hh_ensemble_add(hh_ctx *c, int src)
             const int rs = c->ens.hjobs[4 * k + 2], cs = c->ens.hjobs[4 * k + 3];
             if (rs + w - 1 > c->nr || cs + w - 1 > c->nc) throw Error(HH_EDIM, "window outside the image");
         }
-        ens_add(c, dA, dB);
+        const HookSource hs = hook_source(c, w, c->ens.hjobs.data(), c->ens.njobs, dA, dB, c->ens.jobs);
+        ens_add(c, dA, dB, &hs);
         HH_HIP(hipStreamSynchronize(c->stream));
     });
 }

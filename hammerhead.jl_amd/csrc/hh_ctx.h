@@ -206,7 +206,8 @@ struct hh_ctx {
     hh::host::DBuf<float> gu, gv, gpr, gcm, galtu, galtv, guu, guv, gsu, gsv, gpx, gpy;
     hh::host::DBuf<uint8_t> gout, gsflag, gmask;
     hh::host::DBuf<int> gcount;
-    hh::host::DBuf<int32_t> hookjobs;
+    hh::host::DBuf<int32_t> hookjobs, hookmask;
+    hh::host::DBuf<float> hooktiles;  // masked-window tiles of the per-hook entry points
     hh::host::DBuf<float> hk[12];
     hh::host::DBuf<double> hkd;
     hh::host::DBuf<int32_t> hki[2];

@@ -232,8 +232,10 @@ int hh_fetch_result(hh_ctx *ctx, hh_field *result);
  * (or one with no pixel set) removes the mask; the threshold must be in
  * (0, 1] (HH_EINVAL). The mask stays set across hh_set_pair* and
  * hh_set_pairs; setting an identical mask again keeps the planned schedule.
- * The per-hook entry points (hh_process_windows, hh_uncertainty_sweep,
- * hh_ensemble_*) do not use it. */
+ * The per-hook entry points (hh_process_windows, hh_correlation_planes,
+ * hh_uncertainty_sweep, hh_ensemble_add) load every job's window with it as
+ * load_windows! does (valid-pixel means, masked pixels zero); their jobs are
+ * the caller's (pass_grid already left the masked windows out). */
 int hh_set_mask(hh_ctx *ctx, const uint8_t *mask, int nr, int nc, double mask_threshold);
 
 /* ---- batches of pairs of uneven sizes ----------------------------------- */

@@ -240,3 +240,49 @@ def test_ensemble_one_pair_with_mask_matches_run_piv(hh, ctx):
         else:
             assert np.array_equal(np.isfinite(e[k]), fin), k
             assert np.abs(e[k][fin] - r[k][fin]).max() <= 1e-5 * max(1.0, np.abs(r[k][fin]).max()), k
+
+
+def test_hooks_with_mask_vs_oracle(hh, orc, ctx):
+    """The per-hook entry points (process_windows!, correlation planes,
+    uncertainty_sweep!, the ensemble's accumulate_planes!) with the
+    context's mask: every job's window loaded as load_windows! loads it with
+    the mask (src/correlators.jl:199-227), jobs as pass_grid leaves them."""
+    n = 160
+    A, B, m = disk_scene(hh, n=n, seed=73, speckle=0.01)
+    p = hh.PIVParameters(window_size=32, overlap=16, padding=True, apodization="gauss", uncertainty=True)
+    gm = orc.grid_mask(p.to_struct(), n, n, m)
+    x, y, jobs = hh.pass_grid(p, n, n)
+    keep = [k for k in range(jobs.shape[1]) if not gm[jobs[0, k] - 1, jobs[1, k] - 1]]
+    jobs = np.asfortranarray(jobs[:, keep])
+    ny, nx = y.size, x.size
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    o = orc.process_windows(p.to_struct(), A64, B64, jobs, ny, nx, uncertainty=True, planes=True, mask=m)
+    ctx.set_pair(A, B)
+    ctx.set_mask(m, 0.5)
+    try:
+        say("hooks with mask")
+        r = ctx.process_windows(p, jobs, ny, nx, src=hh.HH_SRC_RAW, uncertainty=True)
+        planes = ctx.correlation_planes(p, jobs)
+        uu, uv = ctx.uncertainty_sweep(p, jobs, ny, nx, src=hh.HH_SRC_RAW)
+        ctx.ensemble_begin(p, jobs, uncertainty=True)
+        for _ in range(2):
+            ctx.ensemble_add(hh.HH_SRC_RAW)
+        eplanes = ctx.ensemble_planes()
+    finally:
+        ctx.set_mask(None)
+    cells = (jobs[0] - 1, jobs[1] - 1)
+    du = np.abs(r["u"][cells] - o["u"][cells])
+    dv = np.abs(r["v"][cells] - o["v"][cells])
+    say("hook u/v max", du.max(), dv.max())
+    assert np.count_nonzero((du > 1e-3) | (dv > 1e-3)) <= 1
+    scale = np.abs(o["planes"]).max(axis=(0, 1))
+    rel = (np.abs(planes - o["planes"]).max(axis=(0, 1)) / scale).max()
+    erel = (np.abs(eplanes - 2 * o["planes"]).max(axis=(0, 1)) / (2 * scale)).max()
+    say("hook planes max rel", rel, "ensemble", erel)
+    assert rel < 1e-5 and erel < 1e-5
+    for got in ((r["unc_u"], r["unc_v"]), (uu, uv)):
+        for g, ref in zip(got, (o["unc_u"], o["unc_v"])):
+            fin = np.isfinite(ref[cells])
+            assert np.array_equal(np.isfinite(g[cells]), fin)
+            rr = np.abs(g[cells][fin] - ref[cells][fin]) / np.maximum(np.abs(ref[cells][fin]), 1e-6)
+            assert rr.max() < 0.01
