@@ -276,8 +276,7 @@ def main(argv=None):
         print(json.dumps({"calibration_copy_bytes_read": n * 4, "calibration_copy_bytes_written": n * 4}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding')",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path; --gpus must be 1", file=sys.stderr)
         return 2
 
     import __graft_entry__ as ge

@@ -628,7 +628,7 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 // window, one row per lane, windows formed exactly as k_corrL<1> forms them
 // (same loads, means and apodization), then uq_fused16. Blocks of 4 windows
 // in XCD bands.
-__global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
                                                  const float *__restrict__ B, int nr,
                                                  const int32_t *__restrict__ jobs, int njobs, int nblocks,
                                                  CorrOut out)
@@ -683,7 +683,7 @@ __global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *_
 // 2D transform, G = 16 L^2 lanes per window. HW: half-width of the analysis
 // box [M/2 - HW, M/2 + HW]^2, which holds the gain support plus a zero margin.
 template <int L, int K, bool PHASE, int HW, bool UQ>
-__global__ __launch_bounds__(L == 4 ? 256 : 64) void k_corrL(CorrConfig cfg, const float *__restrict__ A,
+__global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_eu(L >= 2 ? 2 : 1))) void k_corrL(CorrConfig cfg, const float *__restrict__ A,
                                                            const float *__restrict__ B, int nr,
                                                            const int32_t *__restrict__ jobs, int njobs, int nblocks,
                                                            CorrOut out)
