@@ -628,7 +628,7 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 // window, one row per lane, windows formed exactly as k_corrL<1> forms them
 // (same loads, means and apodization), then uq_fused16. Blocks of 4 windows
 // in XCD bands.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
+__global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
                                                  const float *__restrict__ B, int nr,
                                                  const int32_t *__restrict__ jobs, int njobs, int nblocks,
                                                  CorrOut out)

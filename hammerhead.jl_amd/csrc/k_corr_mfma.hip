@@ -144,7 +144,7 @@ constexpr int kS3P = 12;  // stage-3 operand scale (phase spectrum)
 constexpr int kS4 = -6;   // stage-4 operand scale
 
 template <int K, bool PHASE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
+__global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
                                                      const float *__restrict__ B, int nr,
                                                      const int32_t *__restrict__ jobs, int njobs, int chunk,
                                                      CorrOut out)

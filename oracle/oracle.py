@@ -65,9 +65,14 @@ def apodization(wr, wc, gauss=True, prec="f64"):
     return out
 
 
+def _plane_shape(pass_struct):
+    """FFT side of a pass: the search area, doubled when padded."""
+    sr, sc = pass_struct.search_area_size[0], pass_struct.search_area_size[1]
+    return (2 * sr, 2 * sc) if pass_struct.padding else (sr, sc)
+
+
 def gain(pass_struct, prec="f64"):
-    wr, wc = pass_struct.window_size[0], pass_struct.window_size[1]
-    m1, m2 = (2 * wr, 2 * wc) if pass_struct.padding else (wr, wc)
+    m1, m2 = _plane_shape(pass_struct)
     out = np.zeros((m1, m2), dtype=_dt(prec), order="F")
     _fn("orc_gain", prec)(C.byref(pass_struct), _p(out))
     return out
@@ -75,8 +80,7 @@ def gain(pass_struct, prec="f64"):
 
 def correlation_plane(pass_struct, subA, subB, prec="f64"):
     dt = _dt(prec)
-    wr, wc = pass_struct.window_size[0], pass_struct.window_size[1]
-    m1, m2 = (2 * wr, 2 * wc) if pass_struct.padding else (wr, wc)
+    m1, m2 = _plane_shape(pass_struct)
     a, b = _f(subA, dt), _f(subB, dt)
     R = np.zeros((m1, m2), dtype=dt, order="F")
     _fn("orc_correlation_plane", prec)(C.byref(pass_struct), _p(a), _p(b), _p(R))
@@ -289,8 +293,7 @@ def process_windows(pass_struct, A, B, jobs, ny, nx, uncertainty=False, debug=Fa
     locs = np.zeros((2, K, nj), dtype=np.int32, order="F") if debug else None
     vals = np.zeros((K, nj), dtype=dt, order="F") if debug else None
     found = np.zeros(nj, dtype=np.int32) if debug else None
-    wr, wc = pass_struct.window_size[0], pass_struct.window_size[1]
-    m1, m2 = (2 * wr, 2 * wc) if pass_struct.padding else (wr, wc)
+    m1, m2 = _plane_shape(pass_struct)
     pl = np.zeros((m1, m2, nj), dtype=dt, order="F") if planes else None
     _fn("orc_process_windows", prec)(C.byref(pass_struct), _p(A), _p(B), C.c_int(A.shape[0]), C.c_int(A.shape[1]),
                                      _p(jobs), C.c_int(nj), C.c_int(ny), C.c_int(nx), _p(res["u"]), _p(res["v"]),

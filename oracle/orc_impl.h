@@ -118,6 +118,7 @@ static void F(orc_fft2)(const F(orc_fftplan) * p1, const F(orc_fftplan) * p2, RE
  * ==================================================================== */
 typedef struct {
     int wr, wc, m1, m2;
+    int sr, sc; /* search area (frame B): the window size unless enlarged */
     int method, padding;
     REAL *apod; /* wr x wc                                  */
     REAL *gain; /* m1 x m2, NULL when not padded            */
@@ -186,13 +187,16 @@ static void F(orc_overlap_gain)(F(orc_corr) * c)
 }
 
 /* CrossCorrelator{T} / PhaseCorrelator{T} constructors
- * (src/correlators.jl:65-76,110-127), equal search area only. */
+ * (src/correlators.jl:65-76,110-127): FFT side = search area (twice it when
+ * padded); the overlap gain only for padded equal areas. */
 static void F(orc_corr_init)(F(orc_corr) * c, const hh_pass *p)
 {
     c->wr = p->window_size[0];
     c->wc = p->window_size[1];
-    c->m1 = p->padding ? 2 * c->wr : c->wr;
-    c->m2 = p->padding ? 2 * c->wc : c->wc;
+    c->sr = p->search_area_size[0];
+    c->sc = p->search_area_size[1];
+    c->m1 = p->padding ? 2 * c->sr : c->sr;
+    c->m2 = p->padding ? 2 * c->sc : c->sc;
     c->method = p->correlation_method;
     c->padding = p->padding;
     F(orc_fftplan_init)(&c->p1, c->m1);
@@ -200,7 +204,7 @@ static void F(orc_corr_init)(F(orc_corr) * c, const hh_pass *p)
     c->apod = (REAL *)malloc(sizeof(REAL) * c->wr * c->wc);
     F(orc_apodization)(c->wr, c->wc, p->apodization, c->apod);
     c->gain = NULL;
-    if (p->padding) F(orc_overlap_gain)(c);
+    if (p->padding && c->sr == c->wr && c->sc == c->wc) F(orc_overlap_gain)(c);
     c->W = NULL;
     if (p->correlation_method == HH_PHASE) {
         double sigma = (c->m1 < c->m2 ? c->m1 : c->m2) / 8.0;
@@ -291,25 +295,81 @@ static int F(orc_window_means)(const REAL *A, const REAL *B, int nr, int rs, int
     return anym;
 }
 
-static void F(orc_plane)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *A, const REAL *B, int nr,
-                         int rs, int cs, const uint8_t *mask)
+/* Mean of an h x w block at 1-based (r0, c0) of an image with leading
+ * dimension ld, over its unmasked pixels when the block holds a masked one
+ * (mask nullable, same coordinates); *anym tells which. */
+static REAL F(orc_block_mean)(const REAL *X, int ld, int r0, int c0, int h, int w, const uint8_t *mask, int *anym)
+{
+    int any = 0;
+    if (mask)
+        for (int j = 0; j < w && !any; j++)
+            for (int i = 0; i < h; i++)
+                if (mask[(size_t)(r0 - 1 + i) + (size_t)(c0 - 1 + j) * ld]) {
+                    any = 1;
+                    break;
+                }
+    REAL sum = 0;
+    int n = 0;
+    for (int j = 0; j < w; j++)
+        for (int i = 0; i < h; i++) {
+            size_t g = (size_t)(r0 - 1 + i) + (size_t)(c0 - 1 + j) * ld;
+            if (any && mask[g]) continue;
+            sum += X[g];
+            n++;
+        }
+    *anym = any;
+    if (any) return n > 0 ? sum / (REAL)n : (REAL)0;
+    return sum / (REAL)(h * w);
+}
+
+/* correlation_plane! (src/correlators.jl:326-341) of the frame-A window at
+ * 1-based (ra, ca) of A (leading dimension lda) and the frame-B search area
+ * at (rb, cb) of B (ldb); mask (nullable) in the coordinates of both images
+ * (lda = ldb then). Equal areas load as above; an enlarged search area loads
+ * as the general path of load_windows! (src/correlators.jl:230-262): the
+ * apodized window centred at offset ((sr - wr)/2, (sc - wc)/2) of C1, the
+ * mean-subtracted search area (not apodized) in C2, each mean over its own
+ * valid pixels; after the magnitude, restrict_search_lags!
+ * (src/correlators.jl:347-355) keeps only lags within the margins. */
+static void F(orc_plane_at)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *A, int lda, int ra, int ca,
+                            const REAL *B, int ldb, int rb, int cb, const uint8_t *mask)
 {
     int wr = c->wr, wc = c->wc, m1 = c->m1;
     size_t np = (size_t)c->m1 * c->m2;
-    REAL meanA, meanB;
-    const int anym = F(orc_window_means)(A, B, nr, rs, cs, wr, wc, mask, &meanA, &meanB);
     memset(s->c1r, 0, sizeof(REAL) * np);
     memset(s->c1i, 0, sizeof(REAL) * np);
     memset(s->c2r, 0, sizeof(REAL) * np);
     memset(s->c2i, 0, sizeof(REAL) * np);
-    for (int j = 0; j < wc; j++)
-        for (int i = 0; i < wr; i++) {
-            size_t g = (size_t)(rs - 1 + i) + (size_t)(cs - 1 + j) * nr;
-            if (anym && mask[g]) continue;  /* masked pixels stay zero */
-            REAL a = c->apod[i + (size_t)j * wr];
-            s->c1r[i + (size_t)j * m1] = a * (A[g] - meanA);
-            s->c2r[i + (size_t)j * m1] = a * (B[g] - meanB);
-        }
+    const int equal = c->sr == wr && c->sc == wc;
+    if (equal) {
+        REAL meanA, meanB;
+        const int anym = F(orc_window_means)(A, B, lda, ra, ca, wr, wc, mask, &meanA, &meanB);
+        for (int j = 0; j < wc; j++)
+            for (int i = 0; i < wr; i++) {
+                size_t g = (size_t)(ra - 1 + i) + (size_t)(ca - 1 + j) * lda;
+                if (anym && mask[g]) continue; /* masked pixels stay zero */
+                REAL a = c->apod[i + (size_t)j * wr];
+                s->c1r[i + (size_t)j * m1] = a * (A[g] - meanA);
+                s->c2r[i + (size_t)j * m1] = a * (B[g] - meanB);
+            }
+    } else {
+        const int offr = (c->sr - wr) / 2, offc = (c->sc - wc) / 2;
+        int anyA = 0, anyB = 0;
+        const REAL meanA = F(orc_block_mean)(A, lda, ra, ca, wr, wc, mask, &anyA);
+        const REAL meanB = F(orc_block_mean)(B, ldb, rb, cb, c->sr, c->sc, mask, &anyB);
+        for (int j = 0; j < wc; j++)
+            for (int i = 0; i < wr; i++) {
+                size_t g = (size_t)(ra - 1 + i) + (size_t)(ca - 1 + j) * lda;
+                if (anyA && mask[g]) continue;
+                s->c1r[(offr + i) + (size_t)(offc + j) * m1] = c->apod[i + (size_t)j * wr] * (A[g] - meanA);
+            }
+        for (int j = 0; j < c->sc; j++)
+            for (int i = 0; i < c->sr; i++) {
+                size_t g = (size_t)(rb - 1 + i) + (size_t)(cb - 1 + j) * ldb;
+                if (anyB && mask[g]) continue;
+                s->c2r[i + (size_t)j * m1] = B[g] - meanB;
+            }
+    }
     F(orc_fft2)(&c->p1, &c->p2, s->c1r, s->c1i, 0, s->lr, s->li, s->tr, s->ti);
     F(orc_fft2)(&c->p1, &c->p2, s->c2r, s->c2i, 0, s->lr, s->li, s->tr, s->ti);
     for (size_t k = 0; k < np; k++) {
@@ -335,6 +395,20 @@ static void F(orc_plane)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *
     F(orc_fftshift_abs)(s->c1r, s->c1i, c->m1, c->m2, s->R);
     if (c->gain)
         for (size_t k = 0; k < np; k++) s->R[k] *= c->gain[k];
+    if (!equal) {
+        const int mr = (c->sr - wr) / 2, mc = (c->sc - wc) / 2, cr = c->m1 / 2, cc = c->m2 / 2;
+        for (int j = 0; j < c->m2; j++)
+            for (int i = 0; i < c->m1; i++)
+                if (!(abs(i - cr) <= mr && abs(j - cc) <= mc)) s->R[i + (size_t)j * c->m1] = 0;
+    }
+}
+
+/* The plane of job (rs, cs) (window origin, 1-based) of an image pair with
+ * leading dimension nr: frame-B search area at (rs - mr, cs - mc). */
+static void F(orc_plane)(const F(orc_corr) * c, F(orc_scratch) * s, const REAL *A, const REAL *B, int nr,
+                         int rs, int cs, const uint8_t *mask)
+{
+    F(orc_plane_at)(c, s, A, nr, rs, cs, B, nr, rs - (c->sr - c->wr) / 2, cs - (c->sc - c->wc) / 2, mask);
 }
 
 /* ======================================================================
@@ -1160,14 +1234,15 @@ void F(orc_process_windows)(const hh_pass *p, const REAL *A, const REAL *B, int 
     F(orc_corr_free)(&c);
 }
 
-/* correlation_plane! of one window pair (subA, subB: wr x wc). */
+/* correlation_plane! of one window pair (subA: wr x wc, subB: the search
+ * area, sr x sc). */
 void F(orc_correlation_plane)(const hh_pass *p, const REAL *subA, const REAL *subB, REAL *R)
 {
     F(orc_corr) c;
     F(orc_corr_init)(&c, p);
     F(orc_scratch) s;
     F(orc_scratch_init)(&s, &c);
-    F(orc_plane)(&c, &s, subA, subB, c.wr, 1, 1, NULL);
+    F(orc_plane_at)(&c, &s, subA, c.wr, 1, 1, subB, c.sr, 1, 1, NULL);
     memcpy(R, s.R, sizeof(REAL) * c.m1 * c.m2);
     F(orc_scratch_free)(&s);
     F(orc_corr_free)(&c);
