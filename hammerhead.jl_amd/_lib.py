@@ -254,8 +254,8 @@ class Context:
     def correlation_planes(self, p, jobs, src=HH_SRC_RAW):
         st = p.to_struct() if hasattr(p, "to_struct") else p
         jobs = np.asfortranarray(jobs, dtype=np.int32)
-        w = st.window_size[0]
-        M = 2 * w if st.padding else w
+        sz = st.search_area_size[0]
+        M = 2 * sz if st.padding else sz
         planes = np.zeros((M, M, jobs.shape[1]), dtype=np.float32, order="F")
         _check(self.h, load().hh_correlation_planes(self.h, C.byref(st), src, _ptr(jobs), jobs.shape[1],
                                                      _ptr(planes)), "correlation_planes")
@@ -428,7 +428,7 @@ class Context:
 
     def ensemble_planes(self):
         st, nj, _ = self._ens
-        M = 2 * st.window_size[0] if st.padding else st.window_size[0]
+        M = 2 * st.search_area_size[0] if st.padding else st.search_area_size[0]
         pl = np.zeros((M, M, nj), dtype=np.float64, order="F")
         _check(self.h, load().hh_ensemble_planes(self.h, _ptr(pl)), "ensemble_planes")
         return pl

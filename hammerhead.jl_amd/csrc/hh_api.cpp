@@ -130,13 +130,12 @@ void check_pass_device(const hh_pass &p, bool for_schedule)
 {
     check_pass_valid(p);
     auto uns = [](const std::string &m) { throw Error(HH_EUNSUPPORTED, "backend :mi355x " + m); };
-    if (p.search_area_size[0] != p.window_size[0] || p.search_area_size[1] != p.window_size[1])
-        uns("does not support enlarged search_area_size yet; use backend = :cpu");
     if (p.window_size[0] != p.window_size[1]) uns("supports square windows only");
-    const int w = p.window_size[0];
-    const int M = p.padding ? 2 * w : w;
+    if (p.search_area_size[0] != p.search_area_size[1]) uns("supports square search areas only");
+    const int w = p.window_size[0], sz = p.search_area_size[0];
+    const int M = p.padding ? 2 * sz : sz;
     if (!(M == 16 || M == 32 || M == 64 || M == 128))
-        uns("supports FFT sides 16, 32, 64 and 128 (window 8-64 padded, 16-128 unpadded)");
+        uns("supports FFT sides 16, 32, 64 and 128 (the search area, doubled when padded)");
     if (p.subpixel_method == HH_GAUSS2D) uns("supports subpixel_method :gauss3, :gauss9 or :none only");
     if (p.n_peaks > kMaxPeaks) uns("supports n_peaks up to 8");
     if (p.uod_neighborhood > 3) uns("supports uod_neighborhood 1, 2 or 3");
@@ -196,18 +195,19 @@ std::vector<float> host_apod(int w, int apod)
 
 ConfigBufs &get_config(hh_ctx *c, const hh_pass &p)
 {
-    const int w = p.window_size[0];
-    auto key = std::make_tuple(w, (int)p.padding, (int)p.apodization, (int)p.correlation_method);
+    const int w = p.window_size[0], sz = p.search_area_size[0];
+    auto key = std::make_tuple(w, sz, (int)p.padding, (int)p.apodization, (int)p.correlation_method);
     auto it = c->configs.find(key);
     if (it != c->configs.end()) return *it->second;
     auto cb = std::make_unique<ConfigBufs>();
-    const int M = p.padding ? 2 * w : w;
+    const int M = p.padding ? 2 * sz : sz;
     std::vector<float> apod = host_apod(w, p.apodization);
     cb->apod.ensure(apod.size());
     HH_HIP(hipMemcpy(cb->apod.p, apod.data(), sizeof(float) * apod.size(), hipMemcpyHostToDevice));
     cb->box_lo = 0;
     cb->box_hi = M - 1;
-    if (p.padding) {
+    // the overlap gain belongs to padded equal areas (src/correlators.jl:73)
+    if (p.padding && sz == w) {
         // overlap_gain! (src/correlators.jl:170-181): the autocorrelation of the
         // apodization (direct sums in Float64 of the float32 window), in the
         // fftshifted layout; wmax / w where w >= wmax / 2, else 0.
@@ -261,7 +261,8 @@ CorrConfig make_corr(hh_ctx *c, const hh_pass &p)
     ConfigBufs &cb = get_config(c, p);
     CorrConfig cfg{};
     cfg.w = p.window_size[0];
-    cfg.M = p.padding ? 2 * cfg.w : cfg.w;
+    cfg.s = p.search_area_size[0];
+    cfg.M = p.padding ? 2 * cfg.s : cfg.s;
     cfg.padded = p.padding;
     cfg.method = p.correlation_method;
     cfg.subpixel = p.subpixel_method;
@@ -271,7 +272,7 @@ CorrConfig make_corr(hh_ctx *c, const hh_pass &p)
     cfg.box_lo = cb.box_lo;
     cfg.box_hi = cb.box_hi;
     cfg.apod = cb.apod.p;
-    cfg.gain = p.padding ? cb.gain.p : nullptr;
+    cfg.gain = p.padding && cfg.s == cfg.w ? cb.gain.p : nullptr;
     cfg.wfilt = p.correlation_method == HH_PHASE ? cb.wfilt.p : nullptr;
     cfg.form = c->form;
     return cfg;
@@ -279,7 +280,7 @@ CorrConfig make_corr(hh_ctx *c, const hh_pass &p)
 
 const int32_t *device_jobs(hh_ctx *c, const hh_pass &p, const PassGrid &g)
 {
-    auto key = std::make_tuple(p.window_size[0], p.window_size[1], p.overlap[0], p.overlap[1], c->nr, c->nc);
+    auto key = std::make_tuple(p.window_size[0], p.search_area_size[0], p.overlap[0], p.overlap[1], c->nr, c->nc);
     auto it = c->jobcache.find(key);
     if (it != c->jobcache.end()) return it->second->p;
     auto b = std::make_unique<DBuf<int32_t>>();
@@ -294,6 +295,16 @@ const int32_t *device_jobs(hh_ctx *c, const hh_pass &p, const PassGrid &g)
 void need_pair(hh_ctx *c)
 {
     if (!c->A) throw Error(HH_ESTATE, "no image pair: call hh_set_pair first");
+}
+
+// The frame-A window and the frame-B search area of a job (1-based window
+// origin rs, cs; the search area has the pass's margins around it) lie in
+// the current pair.
+void check_job_area(hh_ctx *c, const hh_pass &p, int rs, int cs)
+{
+    const int sz = p.search_area_size[0], m = (sz - p.window_size[0]) / 2;
+    if (rs - m < 1 || cs - m < 1 || rs - m + sz - 1 > c->nr || cs - m + sz - 1 > c->nc)
+        throw Error(HH_EDIM, m ? "search area outside the image" : "window outside the image");
 }
 
 const float *src_image(hh_ctx *c, int src, bool a)
@@ -454,11 +465,14 @@ struct HookSource {
     const int32_t *jobs;
 };
 
-HookSource hook_source(hh_ctx *c, int w, const int32_t *hjobs, int njobs, const float *dA, const float *dB,
-                       const int32_t *djobs)
+HookSource hook_source(hh_ctx *c, const hh_pass &p, const int32_t *hjobs, int njobs, const float *dA,
+                       const float *dB, const int32_t *djobs)
 {
     const Engine &E = c->eng;
+    const int w = p.window_size[0];
     if (E.hmask.empty() || njobs <= 0) return HookSource{dA, dB, c->nr, djobs};
+    if (p.search_area_size[0] != w)
+        throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks with an enlarged search_area_size");
     if (E.mnr != c->nr || E.mnc != c->nc)
         throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(E.mnr) + ", " +
                                  std::to_string(E.mnc) + ")");
@@ -603,6 +617,9 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
     const Engine &EM = c->eng;
     const bool masked = !EM.hmask.empty();
+    for (int k = 0; k < np && masked; ++k)
+        if (passes[k].search_area_size[0] != passes[k].window_size[0])
+            throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks with an enlarged search_area_size");
     if (masked && (EM.mnr != nr || EM.mnc != nc))
         throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(EM.mnr) + ", " +
                                  std::to_string(EM.mnc) + ") for images of (" + std::to_string(nr) + ", " +
@@ -1178,8 +1195,7 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
         for (int k = 0; k < njobs; ++k) {
             const int gi = jobs[4 * k], gj = jobs[4 * k + 1], rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
             if (gi < 1 || gi > ny || gj < 1 || gj > nx) throw Error(HH_EDIM, "job grid index out of range");
-            if (rs < 1 || cs < 1 || rs + w - 1 > c->nr || cs + w - 1 > c->nc)
-                throw Error(HH_EDIM, "window outside the image");
+            check_job_area(c, *pass, rs, cs);
         }
         if (njobs == 0) return;
         const size_t ng = (size_t)ny * nx;
@@ -1187,7 +1203,7 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
         const int K = std::max(pass->n_peaks, 2);
         CorrConfig cfg = make_corr(c, *pass);
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
-        const HookSource hs = hook_source(c, w, jobs, njobs, dA, dB, dj);
+        const HookSource hs = hook_source(c, *pass, jobs, njobs, dA, dB, dj);
         CorrOut out{};
         out.u = upload(c, c->hk[0], u, ng);
         out.v = upload(c, c->hk[1], v, ng);
@@ -1242,20 +1258,18 @@ int hh_correlation_planes(hh_ctx *c, const hh_pass *pass, int src, const int32_t
         if (!pass || !planes) throw Error(HH_EINVAL, "null argument");
         check_pass_device(*pass, false);
         const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
-        const int w = pass->window_size[0];
-        const int M = pass->padding ? 2 * w : w;
+        const int M = pass->padding ? 2 * pass->search_area_size[0] : pass->search_area_size[0];
         int maxgi = 1, maxgj = 1;
         for (int k = 0; k < njobs; ++k) {
             const int rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
-            if (rs < 1 || cs < 1 || rs + w - 1 > c->nr || cs + w - 1 > c->nc)
-                throw Error(HH_EDIM, "window outside the image");
+            check_job_area(c, *pass, rs, cs);
             maxgi = std::max(maxgi, jobs[4 * k]);
             maxgj = std::max(maxgj, jobs[4 * k + 1]);
         }
         if (njobs <= 0) return;
         CorrConfig cfg = make_corr(c, *pass);
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
-        const HookSource hs = hook_source(c, w, jobs, njobs, dA, dB, dj);
+        const HookSource hs = hook_source(c, *pass, jobs, njobs, dA, dB, dj);
         const size_t ng = (size_t)maxgi * maxgj;
         CorrOut out{};
         out.u = upload<float>(c, c->hk[0], nullptr, ng);
@@ -1289,7 +1303,7 @@ int hh_uncertainty_sweep(hh_ctx *c, const hh_pass *pass, int src, const int32_t 
         if (njobs <= 0) return;
         const size_t ng = (size_t)ny * nx;
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
-        const HookSource hs = hook_source(c, w, jobs, njobs, dA, dB, dj);
+        const HookSource hs = hook_source(c, *pass, jobs, njobs, dA, dB, dj);
         float *duu = upload(c, c->hk[7], unc_u, ng);
         float *duv = upload(c, c->hk[8], unc_v, ng);
         double *dst = stats ? c->hkd.ensure((size_t)2 * HH_UQ_NSTATS * njobs) : nullptr;
@@ -1501,12 +1515,9 @@ int hh_ensemble_add(hh_ctx *c, int src)
     return guarded(c, [&] {
         if (!c->ens.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
         const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
-        const int w = c->ens.pass.window_size[0];
-        for (int k = 0; k < c->ens.njobs; ++k) {
-            const int rs = c->ens.hjobs[4 * k + 2], cs = c->ens.hjobs[4 * k + 3];
-            if (rs + w - 1 > c->nr || cs + w - 1 > c->nc) throw Error(HH_EDIM, "window outside the image");
-        }
-        const HookSource hs = hook_source(c, w, c->ens.hjobs.data(), c->ens.njobs, dA, dB, c->ens.jobs);
+        for (int k = 0; k < c->ens.njobs; ++k)
+            check_job_area(c, c->ens.pass, c->ens.hjobs[4 * k + 2], c->ens.hjobs[4 * k + 3]);
+        const HookSource hs = hook_source(c, c->ens.pass, c->ens.hjobs.data(), c->ens.njobs, dA, dB, c->ens.jobs);
         ens_add(c, dA, dB, &hs);
         HH_HIP(hipStreamSynchronize(c->stream));
     });

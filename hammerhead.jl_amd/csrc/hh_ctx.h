@@ -199,8 +199,8 @@ struct hh_ctx {
     hh::host::DBuf<float> pu, pv;
     int gny = 0, gnx = 0;
     float y0 = 0, ysp = 1, x0 = 0, xsp = 1;
-    std::map<std::tuple<int, int, int, int>, std::unique_ptr<hh::host::ConfigBufs>> configs;
-    // job lists per (w, overlap, nr, nc)
+    std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<hh::host::ConfigBufs>> configs;
+    // job lists per (w, search side, overlap, nr, nc)
     std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<hh::host::DBuf<int32_t>>> jobcache;
     // chain buffers of the ensemble driver
     hh::host::DBuf<float> gu, gv, gpr, gcm, galtu, galtv, guu, guv, gsu, gsv, gpx, gpy;

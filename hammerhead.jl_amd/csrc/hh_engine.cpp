@@ -414,6 +414,10 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
                 throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(E.mnr) +
                                          ", " + std::to_string(E.mnc) + ") for images of (" +
                                          std::to_string(L.nr[q]) + ", " + std::to_string(L.nc[q]) + ")");
+        for (int k = 0; k < np; ++k)
+            if (passes[k].search_area_size[0] != passes[k].window_size[0])
+                throw Error(HH_EUNSUPPORTED,
+                            "backend :mi355x does not support masks with an enlarged search_area_size");
         mc.build(E.hmask, E.mnr, E.mnc);
     }
     E.maxtile = 0;
@@ -431,9 +435,11 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
         P->cfg_uq = P->cfg;
         const hh_pass &p = passes[k];
         const int w = p.window_size[0];
-        P->r0 = 1;
+        // first window origin: past the search margin (pass_grid,
+        // src/pipeline.jl:529-538)
+        P->r0 = 1 + (p.search_area_size[0] - w) / 2;
         P->rstep = w - p.overlap[0];
-        P->c0 = 1;
+        P->c0 = 1 + (p.search_area_size[1] - p.window_size[1]) / 2;
         P->cstep = p.window_size[1] - p.overlap[1];
         P->half = (float)((w - 1) / 2.0);
         P->fixed = p.max_iterations == 1 || p.convergence_tol == 0;

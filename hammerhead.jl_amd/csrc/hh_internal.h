@@ -37,7 +37,8 @@ constexpr int kMaxM = 128;     // largest FFT side
 
 struct CorrConfig {
     int w;          // square window side
-    int M;          // FFT side (2w when padded)
+    int s;          // square search-area side (frame B), s >= w; s = w: equal areas
+    int M;          // FFT side: s, or 2s when padded
     int padded;
     int method;     // HH_CROSS / HH_PHASE
     int subpixel;   // HH_GAUSS3 / HH_GAUSS9 / HH_SUBPIX_NONE

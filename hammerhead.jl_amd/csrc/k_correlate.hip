@@ -228,35 +228,51 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
     const int tid = threadIdx.x;
     const int job = blockIdx.x;
     const int gi = jobs[4 * job], gj = jobs[4 * job + 1], rs = jobs[4 * job + 2], cs = jobs[4 * job + 3];
-    const int w = cfg.w;
+    const int w = cfg.w, sa_ = cfg.s, mg = (sa_ - w) / 2;
     const float *Ab = A + (size_t)(cs - 1) * nr + (rs - 1);
-    const float *Bb = B + (size_t)(cs - 1) * nr + (rs - 1);
+    // frame B: the search area, margins mg around the window (mg = 0: the window)
+    const float *Bb = B + (size_t)(cs - 1 - mg) * nr + (rs - 1 - mg);
 
     for (int k = tid; k < M; k += kThreads) stw[k] = tw[k];
 
-    // ---- load_windows!: means (sum in T over the window) ----
+    // ---- load_windows!: means (sum in T over the window and the search area) ----
     float sa = 0.f, sb = 0.f;
     for (int e = tid; e < w * w; e += kThreads) {
         const int i = e % w, j = e / w;
         sa += Ab[i + (size_t)j * nr];
+    }
+    for (int e = tid; e < sa_ * sa_; e += kThreads) {
+        const int i = e % sa_, j = e / sa_;
         sb += Bb[i + (size_t)j * nr];
     }
     const float2 s = block_sum2(sa, sb, red);
-    const float meanA = s.x / (float)(w * w), meanB = s.y / (float)(w * w);
+    const float meanA = s.x / (float)(w * w), meanB = s.y / (float)(sa_ * sa_);
 
     for (int e = tid; e < M * P; e += kThreads) pl[e] = make_float2(0.f, 0.f);
     __syncthreads();
-    for (int e = tid; e < w * w; e += kThreads) {
-        const int i = e % w, j = e / w;
-        const float a = cfg.apod[e];
-        pl[i + j * P] = make_float2(a * (Ab[i + (size_t)j * nr] - meanA), a * (Bb[i + (size_t)j * nr] - meanB));
+    if (mg == 0) {
+        for (int e = tid; e < w * w; e += kThreads) {
+            const int i = e % w, j = e / w;
+            const float a = cfg.apod[e];
+            pl[i + j * P] = make_float2(a * (Ab[i + (size_t)j * nr] - meanA), a * (Bb[i + (size_t)j * nr] - meanB));
+        }
+    } else {
+        // src/correlators.jl:254-261: the apodized window at offset mg, the
+        // search area as loaded (mean-subtracted, not apodized)
+        for (int e = tid; e < sa_ * sa_; e += kThreads) {
+            const int i = e % sa_, j = e / sa_;
+            const int iw = i - mg, jw = j - mg;
+            const bool inw = iw >= 0 && iw < w && jw >= 0 && jw < w;
+            const float xa = inw ? cfg.apod[iw + jw * w] * (Ab[iw + (size_t)jw * nr] - meanA) : 0.f;
+            pl[i + j * P] = make_float2(xa, Bb[i + (size_t)j * nr] - meanB);
+        }
     }
     __syncthreads();
 
-    // ---- forward DIF along dim 1 (only the w nonzero columns) ----
+    // ---- forward DIF along dim 1 (only the nonzero columns) ----
     for (int len = M; len >= 2; len >>= 1) {
         const int half = len >> 1, step = M / len;
-        for (int b = tid; b < w * H; b += kThreads) {
+        for (int b = tid; b < sa_ * H; b += kThreads) {
             const int j = b / H, t = b % H;
             const int i0 = (t / half) * len + (t % half), i1 = i0 + half;
             const float2 x0 = pl[i0 + j * P], x1 = pl[i1 + j * P];
@@ -328,13 +344,16 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
         __syncthreads();
     }
 
-    // ---- fftshift |.| * gain into the .y lanes (the .x lanes stay live) ----
+    // ---- fftshift |.| * gain into the .y lanes (the .x lanes stay live);
+    // restrict_search_lags! (src/correlators.jl:347-355): with a search
+    // margin only lags |l| <= mg survive ----
     const float scale = 1.0f / (float)(M * M);
     for (int e = tid; e < M * M; e += kThreads) {
         const int i = e % M, j = e / M;
         const int ip = (i + H) & (M - 1), jp = (j + H) & (M - 1);
         float val = fabsf(pl[i + j * P].x * scale);
         if (cfg.gain) val *= cfg.gain[ip + jp * M];
+        if (mg > 0 && (abs(ip - H) > mg || abs(jp - H) > mg)) val = 0.f;
         pl[ip + jp * P].y = val;
     }
     __syncthreads();

@@ -131,7 +131,11 @@ int hh_set_form(hh_ctx *ctx, int form);
 /* ---- parameter checks (no device work) --------------------------------- */
 
 /* _check_backend_params(b, passes) (src/types.jl:110, src/ka_backend.jl:13-29):
- * HH_OK, or HH_EINVAL / HH_EUNSUPPORTED with a message, before any work. */
+ * HH_OK, or HH_EINVAL / HH_EUNSUPPORTED with a message, before any work.
+ * The device scope: square windows and search areas, FFT side (the search
+ * area, doubled when padded) 16, 32, 64 or 128; an enlarged search area
+ * (search_area_size > window_size) takes the general window load of
+ * src/correlators.jl:230-262 and restrict_search_lags! (:347-355). */
 int hh_check_passes(hh_ctx *ctx, const hh_pass *passes, int npasses);
 /* Grid counts of pass_grid (src/pipeline.jl:529-555); *ny = #row starts. */
 int hh_grid_size(const hh_pass *pass, int nr, int nc, int *ny, int *nx);
