@@ -134,8 +134,8 @@ void check_pass_device(const hh_pass &p, bool for_schedule)
     if (p.search_area_size[0] != p.search_area_size[1]) uns("supports square search areas only");
     const int w = p.window_size[0], sz = p.search_area_size[0];
     const int M = p.padding ? 2 * sz : sz;
-    if (!(M == 16 || M == 32 || M == 64 || M == 128))
-        uns("supports FFT sides 16, 32, 64 and 128 (the search area, doubled when padded)");
+    if (!(M == 16 || M == 24 || M == 32 || M == 48 || M == 64 || M == 96 || M == 128))
+        uns("supports FFT sides 16, 24, 32, 48, 64, 96 and 128 (the search area, doubled when padded)");
     if (p.subpixel_method == HH_GAUSS2D) uns("supports subpixel_method :gauss3, :gauss9 or :none only");
     if (p.n_peaks > kMaxPeaks) uns("supports n_peaks up to 8");
     if (p.uod_neighborhood > 3) uns("supports uod_neighborhood 1, 2 or 3");

@@ -211,31 +211,23 @@ template <int M>
 __device__ void analyze_lds(const CorrConfig &cfg, const PlaneView<M> &R, int job, int gi, int gj,
                             const CorrOut &out, Cand *scand, Cand *ssel);
 
+// load_windows! into an M x (M + 1)-pitched complex plane: frame A in .x,
+// frame B in .y, zero elsewhere. Equal areas (src/correlators.jl:199-207):
+// both apodized windows at the origin; an enlarged search area (:230-262):
+// the apodized window at offset mg = (s - w) / 2, the search area
+// mean-subtracted and not apodized. Means are block sums in T divided by
+// the pixel counts. Ends with a block barrier.
 template <int M>
-__global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const float2 *__restrict__ tw,
-                                                        const float *__restrict__ A, const float *__restrict__ B,
-                                                        int nr, const int32_t *__restrict__ jobs, CorrOut out)
+__device__ void load_windows_lds(const CorrConfig &cfg, const float *__restrict__ A, const float *__restrict__ B,
+                                 int nr, int rs, int cs, float2 *pl, float *red)
 {
 #pragma clang fp contract(off)
     constexpr int P = M + 1;
-    constexpr int H = M / 2;
-    __shared__ float2 pl[M * P];
-    __shared__ float2 stw[M];
-    __shared__ float red[2 * (kThreads / 64)];
-    __shared__ Cand scand[kThreads / 64];
-    __shared__ Cand ssel[kMaxPeaks];
-
     const int tid = threadIdx.x;
-    const int job = blockIdx.x;
-    const int gi = jobs[4 * job], gj = jobs[4 * job + 1], rs = jobs[4 * job + 2], cs = jobs[4 * job + 3];
     const int w = cfg.w, sa_ = cfg.s, mg = (sa_ - w) / 2;
     const float *Ab = A + (size_t)(cs - 1) * nr + (rs - 1);
     // frame B: the search area, margins mg around the window (mg = 0: the window)
     const float *Bb = B + (size_t)(cs - 1 - mg) * nr + (rs - 1 - mg);
-
-    for (int k = tid; k < M; k += kThreads) stw[k] = tw[k];
-
-    // ---- load_windows!: means (sum in T over the window and the search area) ----
     float sa = 0.f, sb = 0.f;
     for (int e = tid; e < w * w; e += kThreads) {
         const int i = e % w, j = e / w;
@@ -257,8 +249,6 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
             pl[i + j * P] = make_float2(a * (Ab[i + (size_t)j * nr] - meanA), a * (Bb[i + (size_t)j * nr] - meanB));
         }
     } else {
-        // src/correlators.jl:254-261: the apodized window at offset mg, the
-        // search area as loaded (mean-subtracted, not apodized)
         for (int e = tid; e < sa_ * sa_; e += kThreads) {
             const int i = e % sa_, j = e / sa_;
             const int iw = i - mg, jw = j - mg;
@@ -268,6 +258,29 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
         }
     }
     __syncthreads();
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const float2 *__restrict__ tw,
+                                                        const float *__restrict__ A, const float *__restrict__ B,
+                                                        int nr, const int32_t *__restrict__ jobs, CorrOut out)
+{
+#pragma clang fp contract(off)
+    constexpr int P = M + 1;
+    constexpr int H = M / 2;
+    __shared__ float2 pl[M * P];
+    __shared__ float2 stw[M];
+    __shared__ float red[2 * (kThreads / 64)];
+    __shared__ Cand scand[kThreads / 64];
+    __shared__ Cand ssel[kMaxPeaks];
+
+    const int tid = threadIdx.x;
+    const int job = blockIdx.x;
+    const int gi = jobs[4 * job], gj = jobs[4 * job + 1], rs = jobs[4 * job + 2], cs = jobs[4 * job + 3];
+    const int sa_ = cfg.s, mg = (sa_ - cfg.w) / 2;
+
+    for (int k = tid; k < M; k += kThreads) stw[k] = tw[k];
+    load_windows_lds<M>(cfg, A, B, nr, rs, cs, pl, red);
 
     // ---- forward DIF along dim 1 (only the nonzero columns) ----
     for (int len = M; len >= 2; len >>= 1) {
@@ -507,6 +520,157 @@ __device__ void analyze_lds(const CorrConfig &cfg, const PlaneView<M> &R, int jo
     }
 }
 
+
+// ---- direct-DFT correlation for FFT sides that are not powers of two ----
+// Search areas such as 24 or 48 px (M = s, or 2s when padded) have no radix-2
+// transform. The window load, spectrum, magnitude, gain, lag restriction and
+// analysis are those of k_corr_lds; each 1D transform is a direct O(M^2)
+// DFT with the twiddle W^(k n mod M) of a table of M entries. Outputs of a
+// pass are accumulated in registers (at most ceil(M^2 / 256) per thread),
+// then written back after a barrier, so one LDS plane suffices. Only the
+// nonzero input lines are transformed forward, and only the output lines
+// whose lags survive restrict_search_lags! inversely.
+
+// Lags kept along one axis, as plane indices before the fftshift: all M
+// when mg = 0, else 0..mg and M - mg..M - 1 (2 mg + 1 of them).
+__device__ __forceinline__ int kept_count(int M, int mg) { return mg == 0 ? M : 2 * mg + 1; }
+__device__ __forceinline__ int kept_index(int M, int mg, int t) { return (mg == 0 || t <= mg) ? t : M - (2 * mg + 1) + t; }
+
+// One DFT pass over the plane. Along dim 1 (DIM = 1) output (k, line) sums
+// input rows n < nin of column `line`; along dim 2, output (line, k) sums
+// input columns n < nin of row `line`. Output lines: nl of them, line index
+// from lmap; outputs per line: nk, output index from kmap. INV: e^{+...}.
+template <int M, int DIM, bool INV, typename LMap, typename KMap>
+__device__ __forceinline__ void dft_pass(float2 *pl, const float2 *tw, int nin, int nl, int nk, const LMap &lmap,
+                                         const KMap &kmap)
+{
+#pragma clang fp contract(off)
+    constexpr int P = M + 1;
+    constexpr int NO = (M * M + kThreads - 1) / kThreads;
+    const int tid = threadIdx.x;
+    const int nout = nl * nk;
+    float2 acc[NO];
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+        const int e = tid + q * kThreads;
+        // four interleaved partial sums of fused multiply-adds: the rounding
+        // error grows with nin / 4 terms instead of nin
+        float2 part[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
+                          make_float2(0.f, 0.f)};
+        if (e < nout) {
+            const int k = kmap(e % nk), line = lmap(e / nk);
+            int idx = 0;  // k n mod M
+            for (int n = 0; n < nin; ++n) {
+                const float2 x = DIM == 1 ? pl[n + line * P] : pl[line + n * P];
+                const float2 t = tw[idx];
+                const float ti = INV ? -t.y : t.y;
+                float2 &sm = part[n & 3];
+                sm.x = __builtin_fmaf(x.x, t.x, __builtin_fmaf(-x.y, ti, sm.x));
+                sm.y = __builtin_fmaf(x.x, ti, __builtin_fmaf(x.y, t.x, sm.y));
+                idx += k;
+                if (idx >= M) idx -= M;
+            }
+        }
+        acc[q] = make_float2((part[0].x + part[1].x) + (part[2].x + part[3].x),
+                             (part[0].y + part[1].y) + (part[2].y + part[3].y));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+        const int e = tid + q * kThreads;
+        if (e < nout) {
+            const int k = kmap(e % nk), line = lmap(e / nk);
+            if (DIM == 1)
+                pl[k + line * P] = acc[q];
+            else
+                pl[line + k * P] = acc[q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void k_corr_dft(CorrConfig cfg, const float2 *__restrict__ tw,
+                                                        const float *__restrict__ A, const float *__restrict__ B,
+                                                        int nr, const int32_t *__restrict__ jobs, CorrOut out)
+{
+#pragma clang fp contract(off)
+    constexpr int P = M + 1;
+    constexpr int H = M / 2;
+    __shared__ float2 pl[M * P];
+    __shared__ float2 stw[M];
+    __shared__ float red[2 * (kThreads / 64)];
+    __shared__ Cand scand[kThreads / 64];
+    __shared__ Cand ssel[kMaxPeaks];
+
+    const int tid = threadIdx.x;
+    const int job = blockIdx.x;
+    const int gi = jobs[4 * job], gj = jobs[4 * job + 1], rs = jobs[4 * job + 2], cs = jobs[4 * job + 3];
+    const int S = cfg.s, mg = (S - cfg.w) / 2;
+
+    for (int k = tid; k < M; k += kThreads) stw[k] = tw[k];
+    load_windows_lds<M>(cfg, A, B, nr, rs, cs, pl, red);
+
+    const auto ident = [](int t) { return t; };
+    const int nkept = kept_count(M, mg);
+    const auto kept = [&](int t) { return kept_index(M, mg, t); };
+    // forward: dim 1 over the S nonzero columns (rows n < S), then dim 2
+    // over all rows (columns n < S)
+    dft_pass<M, 1, false>(pl, stw, S, S, M, ident, ident);
+    dft_pass<M, 2, false>(pl, stw, S, M, M, ident, ident);
+
+    // ---- Hermitian separation + cross/phase spectrum, natural order ----
+    for (int e = tid; e < M * M; e += kThreads) {
+        const int k1 = e % M, k2 = e / M;
+        const int q1 = (M - k1) % M, q2 = (M - k2) % M;
+        const int e2 = q1 + q2 * M;
+        if (e2 < e) continue;
+        const float2 Z = pl[k1 + k2 * P], Zm = pl[q1 + q2 * P];
+        const float2 FA = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+        const float2 FB = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+        float2 Pk = cmulc(FB, FA);  // conj(FA) * FB
+        if (cfg.method == HH_PHASE) {
+            const float den = hypotf(Pk.x, Pk.y) + FLT_EPSILON;
+            const float wv = cfg.wfilt[k1 + k2 * M];
+            Pk = make_float2(wv * Pk.x / den, wv * Pk.y / den);
+        }
+        pl[k1 + k2 * P] = Pk;
+        if (e2 != e) pl[q1 + q2 * P] = make_float2(Pk.x, -Pk.y);
+    }
+    __syncthreads();
+
+    // inverse: dim 2 into the kept columns (all rows), then dim 1 into the
+    // kept rows of the kept columns
+    dft_pass<M, 2, true>(pl, stw, M, M, nkept, ident, kept);
+    dft_pass<M, 1, true>(pl, stw, M, nkept, nkept, kept, kept);
+
+    // ---- fftshift |.| * gain into the .y lanes; lags beyond the margin 0 ----
+    const float scale = 1.0f / (float)(M * M);
+    for (int e = tid; e < M * M; e += kThreads) {
+        const int i = e % M, j = e / M;
+        const int ip = (i + H) % M, jp = (j + H) % M;
+        float val = 0.f;
+        if (mg == 0 || (abs(ip - H) <= mg && abs(jp - H) <= mg)) {
+            val = fabsf(pl[i + j * P].x * scale);
+            if (cfg.gain) val *= cfg.gain[ip + jp * M];
+        }
+        pl[ip + jp * P].y = val;
+    }
+    __syncthreads();
+    PlaneView<M> R{pl};
+
+    if (out.planes) {
+        float *dst = out.planes + (size_t)job * M * M;
+        for (int e = tid; e < M * M; e += kThreads) dst[e] = R(e % M, e / M);
+    }
+    if (out.acc) {
+        double *dst = out.acc + (size_t)job * M * M;
+        for (int e = tid; e < M * M; e += kThreads) dst[e] += (double)R(e % M, e / M);
+        return;
+    }
+    analyze_lds<M>(cfg, R, job, gi, gj, out, scand, ssel);
+}
+
 // ensemble_analyze! (src/ensemble.jl:284-313): one block per window; the
 // FP64 plane sum is rounded to float and analyzed like a single plane.
 template <int M>
@@ -575,7 +739,7 @@ const char *correlate_family(const CorrConfig &cfg, bool uq)
 void correlate_prepare(const CorrConfig &cfg, hipStream_t s)
 {
     (void)device_cu_count();
-    if (cfg.M == 16 || cfg.M == 32 || cfg.M == 64 || cfg.M == 128) (void)twiddle_table(cfg.M, s);
+    if (cfg.M >= 2 && cfg.M <= kMaxM) (void)twiddle_table(cfg.M, s);
 }
 
 bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
@@ -602,7 +766,10 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     case 32: hipLaunchKernelGGL(k_corr_lds<32>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
     case 64: hipLaunchKernelGGL(k_corr_lds<64>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
     case 128: hipLaunchKernelGGL(k_corr_lds<128>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
-    default: throw Error(HH_EUNSUPPORTED, "FFT size must be 16, 32, 64 or 128");
+    case 24: hipLaunchKernelGGL(k_corr_dft<24>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
+    case 48: hipLaunchKernelGGL(k_corr_dft<48>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
+    case 96: hipLaunchKernelGGL(k_corr_dft<96>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;
+    default: throw Error(HH_EUNSUPPORTED, "FFT size must be 16, 24, 32, 48, 64, 96 or 128");
     }
     HH_HIP(hipGetLastError());
     return false;
@@ -620,7 +787,10 @@ void launch_plane_analyze(const CorrConfig &cfg, const double *acc, bool box, co
     case 128:
         hipLaunchKernelGGL(k_plane_analyze<128>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out);
         break;
-    default: throw Error(HH_EUNSUPPORTED, "FFT size must be 16, 32, 64 or 128");
+    case 24: hipLaunchKernelGGL(k_plane_analyze<24>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out); break;
+    case 48: hipLaunchKernelGGL(k_plane_analyze<48>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out); break;
+    case 96: hipLaunchKernelGGL(k_plane_analyze<96>, dim3(njobs), dim3(kThreads), 0, s, cfg, acc, b, jobs, out); break;
+    default: throw Error(HH_EUNSUPPORTED, "FFT size must be 16, 24, 32, 48, 64, 96 or 128");
     }
     HH_HIP(hipGetLastError());
 }

@@ -133,7 +133,8 @@ int hh_set_form(hh_ctx *ctx, int form);
 /* _check_backend_params(b, passes) (src/types.jl:110, src/ka_backend.jl:13-29):
  * HH_OK, or HH_EINVAL / HH_EUNSUPPORTED with a message, before any work.
  * The device scope: square windows and search areas, FFT side (the search
- * area, doubled when padded) 16, 32, 64 or 128; an enlarged search area
+ * area, doubled when padded) 16, 32, 64 or 128 (radix-2) or 24, 48 or 96
+ * (direct DFT); an enlarged search area
  * (search_area_size > window_size) takes the general window load of
  * src/correlators.jl:230-262 and restrict_search_lags! (:347-355). */
 int hh_check_passes(hh_ctx *ctx, const hh_pass *passes, int npasses);

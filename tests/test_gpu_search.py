@@ -1,9 +1,12 @@
 """Enlarged search areas (search_area_size > window_size) on the device
 against the oracle, through the C ABI: the general path of load_windows!
 (src/correlators.jl:230-262) and restrict_search_lags! (:347-355) in the
-generic correlation kernel, for FFT sides 16-128.
+generic correlation kernels, radix-2 for FFT sides 16/32/64/128 and a
+direct DFT for 24/48/96 (also equal-area windows of 24 or 48 px).
 
-Bars: planes within 1e-5 of their maximum; u, v within 1e-3 px per window
+Bars: planes within 3e-5 of their maximum (FP32 transforms; the unapodized
+search area of a small apodized window is the least favourable case, about
+1.3e-5 for an 8-px window in a 24-px area); u, v within 1e-3 px per window
 (integer lags exact with subpixel :none); run_piv's integer medians of the
 reference's search-area scene (test/runtests.jl:66-80); multi-pass and
 ensemble runs within the run_piv bars (u/v RMS < 5e-3 px).
@@ -32,7 +35,10 @@ def shifted_scene(n=128, seed=71, dv=3, du=10):
 
 
 @pytest.mark.parametrize("w,s,padding,apod", [(16, 32, False, "gauss"), (16, 32, True, "none"),
-                                              (32, 64, False, "gauss"), (8, 16, True, "gauss")])
+                                              (32, 64, False, "gauss"), (8, 16, True, "gauss"),
+                                              (16, 48, False, "none"), (16, 24, True, "gauss"),
+                                              (24, 24, True, "gauss"), (8, 24, False, "gauss"),
+                                              (32, 48, True, "none")])
 def test_search_planes_and_windows_vs_oracle(hh, orc, ctx, w, s, padding, apod):
     pair = hh.synthetic.particle_pair(160, 160, hh.synthetic.uniform_flow(2.6, -1.3), seed=w + s, noise=0.02)
     p = hh.PIVParameters(window_size=w, search_area_size=s, overlap=w // 2, padding=padding, apodization=apod)
@@ -51,15 +57,17 @@ def test_search_planes_and_windows_vs_oracle(hh, orc, ctx, w, s, padding, apod):
     du = np.abs(r["u"][cells] - o["u"][cells])
     dv = np.abs(r["v"][cells] - o["v"][cells])
     say("plane rel", rel, "u/v max", du.max(), dv.max())
-    assert rel < 1e-5
+    assert rel < 3e-5
     assert np.count_nonzero((du > 1e-3) | (dv > 1e-3)) <= 1
 
 
-def test_run_piv_search_area_scene(hh, orc, ctx):
-    # the reference's scene (a 48-px search area there; FFT sides are powers
-    # of two on the device, so 64 here): integer medians (10, 3)
+@pytest.mark.parametrize("s", [48, 64])
+def test_run_piv_search_area_scene(hh, orc, ctx, s):
+    # the reference's scene (test/runtests.jl:66-80: a 48-px search area, a
+    # direct-DFT side on the device; 64 takes the radix-2 kernel): integer
+    # medians (10, 3)
     A, B = shifted_scene()
-    p = hh.PIVParameters(window_size=16, search_area_size=64, overlap=8, subpixel_method="none", n_peaks=1,
+    p = hh.PIVParameters(window_size=16, search_area_size=s, overlap=8, subpixel_method="none", n_peaks=1,
                          uod_enable=False)
     ctx.set_pair(A, B)
     g = ctx.run_piv([p])
