@@ -465,35 +465,38 @@ struct HookSource {
     const int32_t *jobs;
 };
 
+// window_only: the uncertainty's load (window-sized second frame at the
+// window origin, src/pipeline.jl:984-989) instead of the search area. The
+// tiles go to buffer `slot` (0 or 1), so both loads of one call can coexist.
 HookSource hook_source(hh_ctx *c, const hh_pass &p, const int32_t *hjobs, int njobs, const float *dA,
-                       const float *dB, const int32_t *djobs)
+                       const float *dB, const int32_t *djobs, bool window_only = false, int slot = 0)
 {
     const Engine &E = c->eng;
     const int w = p.window_size[0];
     if (E.hmask.empty() || njobs <= 0) return HookSource{dA, dB, c->nr, djobs};
-    if (p.search_area_size[0] != w)
-        throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks with an enlarged search_area_size");
     if (E.mnr != c->nr || E.mnc != c->nc)
         throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(E.mnr) + ", " +
                                  std::to_string(E.mnc) + ")");
-    if ((size_t)njobs * w >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows for the masked tiles");
+    const int sz = window_only ? w : p.search_area_size[0], mg = (sz - w) / 2;
+    if ((size_t)njobs * sz >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows for the masked tiles");
     std::vector<int32_t> t((size_t)8 * njobs);
     for (int k = 0; k < njobs; ++k) {
         const int32_t *j = hjobs + 4 * (size_t)k;
         int32_t *tj = &t[4 * (size_t)k], *mj = &t[4 * ((size_t)njobs + k)];
         tj[0] = j[0];
         tj[1] = j[1];
-        tj[2] = 1;
-        tj[3] = k * w + 1;
+        tj[2] = 1 + mg;
+        tj[3] = k * sz + 1 + mg;
         mj[0] = mj[2] = j[2] - 1;
         mj[1] = mj[3] = j[3] - 1;
     }
-    const int32_t *d = upload(c, c->hookmask, t.data(), t.size());
-    const size_t tp = (size_t)njobs * w * w;
-    float *tiles = c->hooktiles.ensure(2 * tp);
-    launch_mask_tiles(dA, dB, c->nr, E.dmask.p, E.mnr, d + 4 * (size_t)njobs, njobs, w, tiles, tiles + tp, c->stream);
+    const int32_t *d = upload(c, c->hookmask[slot], t.data(), t.size());
+    const size_t tp = (size_t)njobs * sz * sz;
+    float *tiles = c->hooktiles[slot].ensure(2 * tp);
+    launch_mask_tiles(dA, dB, c->nr, E.dmask.p, E.mnr, d + 4 * (size_t)njobs, njobs, w, sz, tiles, tiles + tp,
+                      c->stream);
     HH_HIP(hipStreamSynchronize(c->stream));  // the upload reads `t`
-    return HookSource{tiles, tiles + tp, w, d};
+    return HookSource{tiles, tiles + tp, sz, d};
 }
 
 // ---- ensemble (sum-of-correlation) PIV: src/ensemble.jl:133-343 -----------
@@ -528,7 +531,7 @@ void ens_begin(hh_ctx *c, const hh_pass &p, const int32_t *djobs, int njobs, boo
     e.jobs = djobs;
     e.nclean = njobs;
     e.npart = 0;
-    e.tjobs = e.mjobs = nullptr;
+    e.tjobs = e.mjobs = e.ujobs = nullptr;
     e.unc = unc;
     e.npairs = 0;
     fill64(c, c->ensacc.ensure(std::max<size_t>(e.per * njobs, 1)), e.per * njobs, 0.0);
@@ -538,9 +541,10 @@ void ens_begin(hh_ctx *c, const hh_pass &p, const int32_t *djobs, int njobs, boo
 }
 
 // accumulate_planes! (src/ensemble.jl:317-343) of one pair (device images;
-// with src given, the clean windows are read through src instead of A, B
-// and the ensemble's jobs).
-void ens_add(hh_ctx *c, const float *A, const float *B, const HookSource *src = nullptr)
+// with src given, the clean windows are correlated through src, and their
+// statistics taken through usrc, instead of A, B and the ensemble's jobs).
+void ens_add(hh_ctx *c, const float *A, const float *B, const HookSource *src = nullptr,
+             const HookSource *usrc = nullptr)
 {
     auto &e = c->ens;
     if (!e.active) throw Error(HH_ESTATE, "no ensemble: call hh_ensemble_begin first");
@@ -552,28 +556,36 @@ void ens_add(hh_ctx *c, const float *A, const float *B, const HookSource *src = 
     const int32_t *cj = src ? src->jobs : e.jobs;
     timed(c, "ensemble_add", e.cfg.w,
           [&] { launch_correlate(e.cfg, cA, cB, cld, 0, cj, e.nclean, out, c->stream); });
-    const int w = e.cfg.w;
-    const size_t tp = (size_t)e.npart * w * w;
+    const int w = e.cfg.w, sz = e.cfg.s;
+    const size_t tp = (size_t)e.npart * sz * sz;
     float *tA = e.npart ? c->enstiles.p : nullptr, *tB = tA ? tA + tp : nullptr;
     if (e.npart) {
         // windows with masked pixels: centred tiles, summed into their own
         // slots after the clean windows'
         timed(c, "mask_tiles", w, [&] {
-            launch_mask_tiles(A, B, c->nr, c->eng.dmask.p, c->eng.mnr, e.mjobs, e.npart, w, tA, tB, c->stream);
+            launch_mask_tiles(A, B, c->nr, c->eng.dmask.p, c->eng.mnr, e.mjobs, e.npart, w, sz, tA, tB, c->stream);
         });
         CorrOut o2 = out;
         o2.acc = c->ensacc.p + e.per * e.nclean;
         timed(c, "ensemble_add_masked", w,
-              [&] { launch_correlate(e.cfg, tA, tB, w, 0, e.tjobs, e.npart, o2, c->stream); });
+              [&] { launch_correlate(e.cfg, tA, tB, sz, 0, e.tjobs, e.npart, o2, c->stream); });
     }
     if (e.unc) {
         const ConfigBufs &cb = get_config(c, e.pass);
+        const float *uA = usrc ? usrc->A : cA, *uB = usrc ? usrc->B : cB;
+        const int uld = usrc ? usrc->ld : cld;
+        const int32_t *uj = usrc ? usrc->jobs : cj;
         timed(c, "uq", e.cfg.w, [&] {
-            launch_uq(e.cfg.w, cb.apod.p, cA, cB, cld, cj, e.nclean, 1, nullptr, nullptr, c->ensstats.p, true,
+            launch_uq(e.cfg.w, cb.apod.p, uA, uB, uld, uj, e.nclean, 1, nullptr, nullptr, c->ensstats.p, true,
                       c->stream);
-            if (e.npart)
-                launch_uq(w, cb.apod.p, tA, tB, w, e.tjobs, e.npart, 1, nullptr, nullptr,
+            if (e.npart) {
+                // the statistics read window tiles of both frames
+                if (sz != w)
+                    launch_mask_tiles(A, B, c->nr, c->eng.dmask.p, c->eng.mnr, e.mjobs, e.npart, w, w, tA, tB,
+                                      c->stream);
+                launch_uq(w, cb.apod.p, tA, tB, w, sz != w ? e.ujobs : e.tjobs, e.npart, 1, nullptr, nullptr,
                           c->ensstats.p + (size_t)2 * HH_UQ_NSTATS * e.nclean, true, c->stream);
+            }
         });
     }
     ++e.npairs;
@@ -617,9 +629,6 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
     for (int k = 0; k < np; ++k) check_pass_device(passes[k], true);
     const Engine &EM = c->eng;
     const bool masked = !EM.hmask.empty();
-    for (int k = 0; k < np && masked; ++k)
-        if (passes[k].search_area_size[0] != passes[k].window_size[0])
-            throw Error(HH_EUNSUPPORTED, "backend :mi355x does not support masks with an enlarged search_area_size");
     if (masked && (EM.mnr != nr || EM.mnc != nc))
         throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(EM.mnr) + ", " +
                                  std::to_string(EM.mnc) + ") for images of (" + std::to_string(nr) + ", " +
@@ -698,47 +707,54 @@ void run_ensemble(hh_ctx *c, const float *const *hA, const float *const *hB, int
             // pass_grid's grid mask (src/pipeline.jl:542-549): the clean
             // windows first, then those with masked pixels; masked ones
             // have no job
-            const int w = p.window_size[0];
-            std::vector<int32_t> clean, part, tj, mj;
+            const int w = p.window_size[0], sz = p.search_area_size[0], mg = (sz - w) / 2;
+            std::vector<int32_t> clean, part, tj, mj, uj;
             gm.assign(ng, 0);
+            auto count = [&](int r0, int c0, int side) {  // masked pixels of a block (1-based origin)
+                long n = 0;
+                for (int cc = 0; cc < side; ++cc) {
+                    const uint8_t *col = EM.hmask.data() + (size_t)(c0 - 1 + cc) * nr + (r0 - 1);
+                    for (int i = 0; i < side; ++i) n += col[i];
+                }
+                return n;
+            };
             for (size_t j = 0; j < ng; ++j) {
                 const int32_t *jb = &g.jobs[4 * j];
                 const int rs = jb[2], cs = jb[3];
-                long nm = 0;
-                for (int cc = 0; cc < w; ++cc) {
-                    const uint8_t *col = EM.hmask.data() + (size_t)(cs - 1 + cc) * nr + (rs - 1);
-                    for (int i = 0; i < w; ++i) nm += col[i];
-                }
-                const bool gmj = (double)nm / (double)(w * w) >= EM.mthr;
+                const long nm = count(rs, cs, w), ns = sz == w ? nm : count(rs - mg, cs - mg, sz);
+                const bool gmj = (double)nm / (double)(w * w) >= EM.mthr || (double)ns / (double)(sz * sz) >= EM.mthr;
                 gm[j] = gmj;
                 anyg |= gmj;
                 if (gmj) continue;
-                if (nm == 0) {
+                if (nm == 0 && ns == 0) {
                     clean.insert(clean.end(), jb, jb + 4);
                 } else {
                     const int t = (int)(tj.size() / 4);
                     part.insert(part.end(), jb, jb + 4);
-                    tj.insert(tj.end(), {jb[0], jb[1], 1, t * w + 1});
+                    tj.insert(tj.end(), {jb[0], jb[1], 1 + mg, t * sz + 1 + mg});
+                    uj.insert(uj.end(), {jb[0], jb[1], 1, t * w + 1});
                     mj.insert(mj.end(), {rs - 1, cs - 1, rs - 1, cs - 1});
                 }
             }
             nclean = (int)(clean.size() / 4);
             npart = (int)(part.size() / 4);
-            if ((size_t)npart * w >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows with masked pixels");
+            if ((size_t)npart * sz >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows with masked pixels");
             clean.insert(clean.end(), part.begin(), part.end());
             tj.insert(tj.end(), mj.begin(), mj.end());
+            tj.insert(tj.end(), uj.begin(), uj.end());
             djobs = ::upload(c, c->ensjobs, clean.data(), clean.size());
             const int32_t *pj = ::upload(c, c->enspart, tj.data(), tj.size());
             uint8_t *dgm = c->gmask.ensure(ng);
             HH_HIP(hipMemcpyAsync(dgm, gm.data(), ng, hipMemcpyHostToDevice, c->stream));
             // the uploads read host vectors of this scope
             HH_HIP(hipStreamSynchronize(c->stream));
-            if (npart) c->enstiles.ensure(2 * (size_t)npart * w * w);
+            if (npart) c->enstiles.ensure(2 * (size_t)npart * sz * sz);
             ens_begin(c, p, djobs, nclean + npart, unc);
             c->ens.nclean = nclean;
             c->ens.npart = npart;
             c->ens.tjobs = pj;
             c->ens.mjobs = pj + 4 * (size_t)npart;
+            c->ens.ujobs = pj + 8 * (size_t)npart;
         } else {
             djobs = device_jobs(c, p, g);
             ens_begin(c, p, djobs, (int)ng, unc);
@@ -1230,8 +1246,10 @@ int hh_process_windows(hh_ctx *c, const hh_pass *pass, int src, const int32_t *j
               [&] { fused = launch_correlate(cfg, hs.A, hs.B, hs.ld, 0, hs.jobs, njobs, out, c->stream); });
         if ((unc_u || unc_v) && !fused) {
             const ConfigBufs &cb = get_config(c, *pass);
+            const HookSource us = pass->search_area_size[0] == w ? hs
+                                                                 : hook_source(c, *pass, jobs, njobs, dA, dB, dj, true, 1);
             timed(c, "uq", w, [&] {
-                launch_uq(w, cb.apod.p, hs.A, hs.B, hs.ld, hs.jobs, njobs, ny, unc_u ? duu : nullptr,
+                launch_uq(w, cb.apod.p, us.A, us.B, us.ld, us.jobs, njobs, ny, unc_u ? duu : nullptr,
                           unc_v ? duv : nullptr, nullptr, false, c->stream);
             });
         }
@@ -1303,7 +1321,7 @@ int hh_uncertainty_sweep(hh_ctx *c, const hh_pass *pass, int src, const int32_t 
         if (njobs <= 0) return;
         const size_t ng = (size_t)ny * nx;
         const int32_t *dj = upload(c, c->hookjobs, jobs, (size_t)4 * njobs);
-        const HookSource hs = hook_source(c, *pass, jobs, njobs, dA, dB, dj);
+        const HookSource hs = hook_source(c, *pass, jobs, njobs, dA, dB, dj, true);
         float *duu = upload(c, c->hk[7], unc_u, ng);
         float *duv = upload(c, c->hk[8], unc_v, ng);
         double *dst = stats ? c->hkd.ensure((size_t)2 * HH_UQ_NSTATS * njobs) : nullptr;
@@ -1517,8 +1535,12 @@ int hh_ensemble_add(hh_ctx *c, int src)
         const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
         for (int k = 0; k < c->ens.njobs; ++k)
             check_job_area(c, c->ens.pass, c->ens.hjobs[4 * k + 2], c->ens.hjobs[4 * k + 3]);
-        const HookSource hs = hook_source(c, c->ens.pass, c->ens.hjobs.data(), c->ens.njobs, dA, dB, c->ens.jobs);
-        ens_add(c, dA, dB, &hs);
+        const hh_pass &ep = c->ens.pass;
+        const HookSource hs = hook_source(c, ep, c->ens.hjobs.data(), c->ens.njobs, dA, dB, c->ens.jobs);
+        const HookSource us = ep.search_area_size[0] == ep.window_size[0]
+                                  ? hs
+                                  : hook_source(c, ep, c->ens.hjobs.data(), c->ens.njobs, dA, dB, c->ens.jobs, true, 1);
+        ens_add(c, dA, dB, &hs, &us);
         HH_HIP(hipStreamSynchronize(c->stream));
     });
 }

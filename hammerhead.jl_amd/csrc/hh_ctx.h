@@ -97,9 +97,10 @@ struct PassTables {
     DBuf<int> count;          // one valid-cell counter per segment (saturating at 3)
     // windows with masked pixels below the grid-mask threshold: the tile
     // builder's list (4 x npart, 0-based image row / column and mask row /
-    // column) and the correlation jobs on the tile atlas (cell + 1, 1, 1,
-    // t w + 1)
-    DBuf<int32_t> mjobs, tjobs;
+    // column of the window), the correlation jobs on the s x s tiles (cell +
+    // 1, 1, 1 + m, t s + 1 + m; m the search margin) and the uncertainty's
+    // jobs on w x w window tiles (cell + 1, 1, 1, t w + 1)
+    DBuf<int32_t> mjobs, tjobs, ujobs;
     int npart = 0;
 };
 
@@ -206,8 +207,8 @@ struct hh_ctx {
     hh::host::DBuf<float> gu, gv, gpr, gcm, galtu, galtv, guu, guv, gsu, gsv, gpx, gpy;
     hh::host::DBuf<uint8_t> gout, gsflag, gmask;
     hh::host::DBuf<int> gcount;
-    hh::host::DBuf<int32_t> hookjobs, hookmask;
-    hh::host::DBuf<float> hooktiles;  // masked-window tiles of the per-hook entry points
+    hh::host::DBuf<int32_t> hookjobs, hookmask[2];
+    hh::host::DBuf<float> hooktiles[2];  // masked-window tiles of the per-hook entry points (two loads)
     hh::host::DBuf<float> hk[12];
     hh::host::DBuf<double> hkd;
     hh::host::DBuf<int32_t> hki[2];
@@ -227,7 +228,7 @@ struct hh_ctx {
         // centred tiles (tjobs: their jobs on the tile atlas, mjobs: the
         // tile builder's list)
         int nclean = 0, npart = 0;
-        const int32_t *tjobs = nullptr, *mjobs = nullptr;
+        const int32_t *tjobs = nullptr, *mjobs = nullptr, *ujobs = nullptr;  // ujobs: window tiles (statistics)
     } ens;
     hh::host::DBuf<int32_t> ensjobs, enspart;
     hh::host::DBuf<float> enstiles;

@@ -220,9 +220,9 @@ std::vector<DeformPair> deform_table(hh_ctx *c, const std::vector<int> &pairs, c
 void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vector<int> &pairs, const HostJobs &H,
                   PassTables &T)
 {
-    std::vector<int32_t> jobs, mjobs, tjobs;
+    std::vector<int32_t> jobs, mjobs, tjobs, ujobs;
     std::vector<GridSeg> segs;
-    const int w = P.p.window_size[0];
+    const int w = P.p.window_size[0], sz = P.p.search_area_size[0], mg = (sz - w) / 2;
     int start = 0;
     for (size_t s = 0; s < pairs.size(); ++s) {
         const int q = pairs[s];
@@ -232,7 +232,8 @@ void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vecto
         for (int t = 0; t < H.pcnt[q]; ++t) {
             const int32_t *e = &H.part[5 * ((size_t)H.poff[q] + t)];
             const int k = (int)(tjobs.size() / 4);
-            tjobs.insert(tjobs.end(), {e[0] + 1, 1, 1, k * w + 1});
+            tjobs.insert(tjobs.end(), {e[0] + 1, 1, 1 + mg, k * sz + 1 + mg});
+            ujobs.insert(ujobs.end(), {e[0] + 1, 1, 1, k * w + 1});
             mjobs.insert(mjobs.end(), {e[1], e[2], e[3], e[4]});
         }
         segs.push_back(GridSeg{start, P.off[q], P.ny[q], P.nx[q], (int)s, {0, 0, 0}});
@@ -243,6 +244,7 @@ void build_tables(hh_ctx *c, PassPlan &P, const PassPlan *prev, const std::vecto
     upload_sync(c, T.jobs, jobs);
     upload_sync(c, T.mjobs, mjobs);
     upload_sync(c, T.tjobs, tjobs);
+    upload_sync(c, T.ujobs, ujobs);
     upload_sync(c, T.segs, segs);
     T.grids = Grids{T.segs.p, (int)segs.size(), 0, 0, start, P.total};
     // one grid starting at buffer cell 0: no table lookups in the kernels
@@ -414,10 +416,6 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
                 throw Error(HH_EDIM, "mask must have the same size as the images, got (" + std::to_string(E.mnr) +
                                          ", " + std::to_string(E.mnc) + ") for images of (" +
                                          std::to_string(L.nr[q]) + ", " + std::to_string(L.nc[q]) + ")");
-        for (int k = 0; k < np; ++k)
-            if (passes[k].search_area_size[0] != passes[k].window_size[0])
-                throw Error(HH_EUNSUPPORTED,
-                            "backend :mi355x does not support masks with an enlarged search_area_size");
         mc.build(E.hmask, E.mnr, E.mnc);
     }
     E.maxtile = 0;
@@ -452,17 +450,21 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
             if (ng == 0) throw Error(HH_EINVAL, "the pass grid is empty");
             if ((long long)P->total + (long long)ng > (1LL << 30)) throw Error(HH_EINVAL, "too many vectors in one batch");
             if (masked && kind.empty()) {
-                // pass_grid (src/pipeline.jl:542-549): the masked fraction of
-                // the window (the search area is the window on the device)
+                // pass_grid (src/pipeline.jl:542-549): the masked fractions of
+                // the window and of its search area
                 const int wr = p.window_size[0], wc = p.window_size[1];
+                const int sr = p.search_area_size[0], sc = p.search_area_size[1];
+                const int mr = (sr - wr) / 2, mcm = (sc - wc) / 2;
                 kind.assign(ng, 0);
                 P->gm.assign(ng, 0);
                 for (size_t j = 0; j < ng; ++j) {
-                    const uint32_t nm = mc.rect(g.jobs[4 * j + 2] - 1, g.jobs[4 * j + 3] - 1, wr, wc);
-                    const bool gm = (double)nm / (double)(wr * wc) >= E.mthr;
+                    const int r0 = g.jobs[4 * j + 2] - 1, c0 = g.jobs[4 * j + 3] - 1;
+                    const uint32_t nm = mc.rect(r0, c0, wr, wc);
+                    const uint32_t ns = mc.rect(r0 - mr, c0 - mcm, sr, sc);
+                    const bool gm = (double)nm / (double)(wr * wc) >= E.mthr || (double)ns / (double)(sr * sc) >= E.mthr;
                     P->gm[j] = gm;
                     P->anyg |= gm;
-                    kind[j] = gm ? 2 : (nm > 0 ? 1 : 0);
+                    kind[j] = gm ? 2 : ((nm > 0 || ns > 0) ? 1 : 0);
                 }
             }
             P->ny.push_back(g.ny);
@@ -488,9 +490,9 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
             P->total += (int)ng;
         }
         {
-            const size_t npart = H.part.size() / 5, w = (size_t)p.window_size[0];
-            if (npart * w >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows with masked pixels");
-            E.maxtile = std::max(E.maxtile, npart * w * w);
+            const size_t npart = H.part.size() / 5, sz = (size_t)p.search_area_size[0];
+            if (npart * sz >= (size_t)1 << 31) throw Error(HH_EINVAL, "too many windows with masked pixels");
+            E.maxtile = std::max(E.maxtile, npart * sz * sz);
         }
         E.maxcells = std::max(E.maxcells, (size_t)P->total);
         E.maxalt = std::max(E.maxalt, (size_t)P->total * std::max(p.n_peaks - 1, 0));
@@ -641,10 +643,11 @@ void record(hh_ctx *c, bool dynamic)
             if (T->npart > 0) {
                 // windows with masked pixels, as centred tiles
                 timed(c, "mask_tiles", P.cfg.w, [&] {
-                    launch_mask_tiles(srcA, srcB, L.ld, E.dmask.p, E.mnr, T->mjobs.p, T->npart, P.cfg.w, tA, tB, s);
+                    launch_mask_tiles(srcA, srcB, L.ld, E.dmask.p, E.mnr, T->mjobs.p, T->npart, P.cfg.w, P.cfg.s, tA,
+                                      tB, s);
                 });
                 timed(c, "correlate_masked", P.cfg.w, [&] {
-                    const bool f = launch_correlate(cc, tA, tB, P.cfg.w, 0, T->tjobs.p, T->npart, out, s);
+                    const bool f = launch_correlate(cc, tA, tB, P.cfg.s, 0, T->tjobs.p, T->npart, out, s);
                     fused = fused && f;
                 });
             }
@@ -711,9 +714,10 @@ void record(hh_ctx *c, bool dynamic)
                 if (P.all.npart > 0) {
                     // every pair's masked windows from its final warp (the
                     // tiles of the last sweep may cover a subset of pairs)
-                    launch_mask_tiles(srcA, srcB, L.ld, E.dmask.p, E.mnr, P.all.mjobs.p, P.all.npart, P.cfg.w, tA,
-                                      tB, s);
-                    launch_uq(P.cfg.w, cb.apod.p, tA, tB, P.cfg.w, P.all.tjobs.p, P.all.npart, P.total, E.uu.p,
+                    // window tiles of both frames: the uncertainty's load
+                    launch_mask_tiles(srcA, srcB, L.ld, E.dmask.p, E.mnr, P.all.mjobs.p, P.all.npart, P.cfg.w,
+                                      P.cfg.w, tA, tB, s);
+                    launch_uq(P.cfg.w, cb.apod.p, tA, tB, P.cfg.w, P.all.ujobs.p, P.all.npart, P.total, E.uu.p,
                               E.uv.p, nullptr, false, s);
                 }
             });

@@ -229,14 +229,17 @@ void launch_mask_nan(const Grids &g, const uint8_t *mask, float *u, float *v, fl
 // count: grids.nseg device ints of scratch.
 void launch_fill_masked(const Grids &g, const float *u, const float *v, const uint8_t *mask, int *count, float *ou,
                         float *ov, hipStream_t s);
-// Windows with masked pixels (load_windows!, src/correlators.jl:214-227):
+// Windows with masked pixels (load_windows!, src/correlators.jl:214-262):
 // for each of the nw entries of mjobs (4 ints, 0-based: image row, image
-// column, mask row, mask column of the window's first pixel) a w x w tile
-// of both frames, at column t w of the tile atlases tA, tB (leading
-// dimension w): valid pixels minus the mean of the window's valid pixels,
-// masked pixels 0. mask: column-major, leading dimension mld.
+// column, mask row, mask column of the window's first pixel) an sz x sz
+// tile of each frame at column t sz of the tile atlases tA, tB (leading
+// dimension sz): frame A's w x w window at offset (sz - w) / 2, frame B's
+// sz x sz area around it, valid pixels minus the mean of their region's
+// valid pixels, masked pixels (and the rest of tile A) 0. sz = w gives
+// window tiles for both frames (the uncertainty's load). mask: column-major,
+// leading dimension mld.
 void launch_mask_tiles(const float *A, const float *B, int ld, const uint8_t *mask, int mld, const int32_t *mjobs,
-                       int nw, int w, float *tA, float *tB, hipStream_t s);
+                       int nw, int w, int sz, float *tA, float *tB, hipStream_t s);
 void launch_smooth(const Grids &g, const float *f, float *out, hipStream_t s);
 // Two fields (u and v of a predictor) in one launch.
 void launch_smooth2(const Grids &g, const float *f, const float *f2, float *out, float *out2, hipStream_t s);

@@ -286,3 +286,68 @@ def test_hooks_with_mask_vs_oracle(hh, orc, ctx):
             assert np.array_equal(np.isfinite(g[cells]), fin)
             rr = np.abs(g[cells][fin] - ref[cells][fin]) / np.maximum(np.abs(ref[cells][fin]), 1e-6)
             assert rr.max() < 0.01
+
+
+@pytest.mark.parametrize("form", ["fft32", "mfma16"])
+def test_run_piv_mask_with_search_area_vs_oracle(hh, orc, form):
+    """Masks with an enlarged search area (load_windows!, src/correlators.jl:
+    230-262: window and search area each with its own valid-pixel mean; the
+    grid mask counts both fractions): a 16-px window in a 32-px area on the
+    first pass, equal areas with uncertainty on the last."""
+    A, B, m = disk_scene(hh, seed=79, speckle=0.01)
+    passes = [hh.PIVParameters(window_size=16, search_area_size=32, overlap=8, padding=True, apodization="gauss"),
+              hh.PIVParameters(window_size=16, overlap=8, padding=True, apodization="gauss", max_iterations=2,
+                               convergence_tol=0.0, uncertainty=True)]
+    say("mask + search area", form)
+    with hh.Context(0) as c:
+        c.set_form(hh.HH_FORM_MFMA16 if form == "mfma16" else hh.HH_FORM_FFT32)
+        g, o = run_masked(hh, orc, c, A, B, m, passes)
+    check_masked(g, o, uq=True)
+
+
+def test_single_pass_mask_with_search_area_uq_vs_oracle(hh, orc, ctx):
+    # the uncertainty of an enlarged-search pass with a mask reads window
+    # tiles of both frames (src/pipeline.jl:984-989)
+    A, B, m = disk_scene(hh, seed=83, speckle=0.01)
+    p = hh.PIVParameters(window_size=16, search_area_size=32, overlap=8, uncertainty=True)
+    g, o = run_masked(hh, orc, ctx, A, B, m, [p])
+    check_masked(g, o, max_tol=1e-3, uq=True)
+
+
+def test_hooks_and_ensemble_mask_with_search_area_vs_oracle(hh, orc, ctx):
+    n = 160
+    A, B, m = disk_scene(hh, n=n, seed=89, speckle=0.01)
+    p = hh.PIVParameters(window_size=16, search_area_size=32, overlap=8, padding=True, apodization="gauss",
+                         uncertainty=True)
+    gm = orc.grid_mask(p.to_struct(), n, n, m)
+    x, y, jobs = hh.pass_grid(p, n, n)
+    keep = [k for k in range(jobs.shape[1]) if not gm[jobs[0, k] - 1, jobs[1, k] - 1]]
+    jobs = np.asfortranarray(jobs[:, keep])
+    ny, nx = y.size, x.size
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    o = orc.process_windows(p.to_struct(), A64, B64, jobs, ny, nx, uncertainty=True, planes=True, mask=m)
+    ctx.set_pair(A, B)
+    ctx.set_mask(m, 0.5)
+    try:
+        r = ctx.process_windows(p, jobs, ny, nx, src=hh.HH_SRC_RAW, uncertainty=True)
+        planes = ctx.correlation_planes(p, jobs)
+        uu, uv = ctx.uncertainty_sweep(p, jobs, ny, nx, src=hh.HH_SRC_RAW)
+        e = ctx.run_piv_ensemble([(A, B), (A, B)], [p])
+    finally:
+        ctx.set_mask(None)
+    cells = (jobs[0] - 1, jobs[1] - 1)
+    du = np.abs(r["u"][cells] - o["u"][cells])
+    dv = np.abs(r["v"][cells] - o["v"][cells])
+    scale = np.abs(o["planes"]).max(axis=(0, 1))
+    rel = (np.abs(planes - o["planes"]).max(axis=(0, 1)) / scale).max()
+    say("mask + search hooks: u/v max", du.max(), dv.max(), "planes rel", rel)
+    assert np.count_nonzero((du > 1e-3) | (dv > 1e-3)) <= 1 and rel < 3e-5
+    for got in ((r["unc_u"], r["unc_v"]), (uu, uv)):
+        for gq, ref in zip(got, (o["unc_u"], o["unc_v"])):
+            fin = np.isfinite(ref[cells])
+            assert np.array_equal(np.isfinite(gq[cells]), fin)
+            rr = np.abs(gq[cells][fin] - ref[cells][fin]) / np.maximum(np.abs(ref[cells][fin]), 1e-6)
+            assert rr.max() < 0.01
+    # the ensemble of two identical pairs under the same mask
+    oe = orc.run_piv_ensemble([(A64, B64), (A64, B64)], [p.to_struct()], mask=m)
+    check_masked(e, oe, uq=True)

@@ -75,9 +75,20 @@ def test_run_piv_search_area_scene(hh, orc, ctx, s):
     assert np.array_equal(g["x"], o["x"]) and np.array_equal(g["y"], o["y"])
     assert np.median(g["u"]) == 10 and np.median(g["v"]) == 3
     assert np.array_equal(g["u"], o["u"], equal_nan=True) and np.array_equal(g["v"], o["v"], equal_nan=True)
-    # a mask with an enlarged search area is outside the device scope
-    with pytest.raises(hh.ArgumentError):
-        hh.run_piv(A, B, p, mask=np.ones_like(A, dtype=bool))
+    # test/runtests.jl:98-102: a search-only masked fringe drops the node
+    # whose window is clean
+    mask = np.zeros_like(A, dtype=bool)
+    mask[:16, :48] = True
+    ctx.set_mask(mask, 0.3)
+    try:
+        gm = ctx.run_piv([p])
+    finally:
+        ctx.set_mask(None)
+    om = orc.run_piv(A.astype(np.float64), B.astype(np.float64), [p.to_struct()], mask=mask, mask_threshold=0.3)
+    assert np.array_equal(gm["mask"], om["mask"])
+    if s == 48:  # 16 x 48 of the 48 x 48 search area: a third, above 0.3
+        assert gm["mask"][0, 0]
+    assert np.array_equal(gm["u"], om["u"], equal_nan=True) and np.array_equal(gm["v"], om["v"], equal_nan=True)
 
 
 def test_run_piv_search_multipass_vs_oracle(hh, orc, ctx):
