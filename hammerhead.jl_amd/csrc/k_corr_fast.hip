@@ -705,7 +705,11 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_
     constexpr int NB = 2 * HW + 1;
     constexpr int LO = H - HW;
     constexpr int U = W + 1;                 // unit pitch of the kept rows
-    constexpr int TSZ = (H + 1) * W;         // float4 per window
+    // row-frequency spectra: column k2 of the window's lines at pitch TP =
+    // W + 1 float4, so the column reads of step 3 (lanes of different units,
+    // same row) and the row writes of step 2 fall in distinct LDS banks
+    constexpr int TP = W + 1;
+    constexpr int TSZ = (H + 1) * TP;        // float4 per window
     constexpr int NPAIR = (NB + 1) / 2;
     constexpr int NWAVE = NT / 64;
     static_assert(NB * U * 2 + NB * NB <= TSZ * 4, "box does not fit the LDS slice");
@@ -833,7 +837,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_
             else
                 zm = mirror<L>(Y[32 - k]);
             if (k2 <= H)
-                T[k2 * W + line] = make_float4(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y), 0.5f * (z.y + zm.y),
+                T[k2 * TP + line] = make_float4(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y), 0.5f * (z.y + zm.y),
                                                -0.5f * (z.x - zm.x));
         });
     }
@@ -848,7 +852,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_
 #pragma unroll
         for (int n = 0; n < 16; ++n) {
             const int i = L * n + l;
-            const float4 v = T[u * W + i], vh = T[H * W + i];
+            const float4 v = T[u * TP + i], vh = T[H * TP + i];
             fa[n] = u != 0 ? make_float2(v.x, v.y) : make_float2(v.x, vh.x);
             fb[n] = u != 0 ? make_float2(v.z, v.w) : make_float2(v.z, vh.z);
         }
