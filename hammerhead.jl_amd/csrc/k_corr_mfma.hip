@@ -139,9 +139,14 @@ constexpr int kNB = 2 * kHW + 1;  // 15 box rows and columns
 // Operand scales. The input is scaled to |x| < 2^-1, so the stage-2 operand
 // (16-term sums) stays below 2^3, Z below 2^7 and the cross spectrum below
 // 2^14: stages 2 and 3 need no rescaling. The phase spectrum (|P| <= 1) is
-// raised to 2^12; the stage-3 result (34 weighted terms) is lowered by 2^-6.
+// raised to 2^12. The stage-3 tables (weights <= 2) carry 2^kS4, so that the
+// stage-3 result enters stage 4 without a rescale: by Parseval the cross
+// spectrum of one k1 sums to at most sum |Z_A|^2 = 1024 * 256 * 2^-2 = 2^16
+// in magnitude, which bounds the result by 2 * 2^16 * 2^-4 = 2^13 (phase:
+// 17 bins of 2^12 give 2^13.1), inside FP16. The smallest table entries
+// still split to within 2^-25 absolute.
 constexpr int kS3P = 12;  // stage-3 operand scale (phase spectrum)
-constexpr int kS4 = -6;   // stage-4 operand scale
+constexpr int kS4 = -4;   // stage-3 table scale
 
 template <int K, bool PHASE>
 __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
@@ -182,13 +187,14 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
             const int k2 = rmap(h, s);
             const float cw = k2 == 0 ? 1.f : 2.f;
             const int t = (k2 * l2) & 31;
-            put(T3[0], s, vb ? (im ? cw * tg.s[t] : cw * tg.c[t]) : 0.f);
-            put(T3[1], s, vb ? (im ? cw * tg.c[t] : -cw * tg.s[t]) : 0.f);
+            // scaled by 2^kS4: the stage-3 result then enters stage 4 as is
+            put(T3[0], s, ldexpf(vb ? (im ? cw * tg.s[t] : cw * tg.c[t]) : 0.f, kS4));
+            put(T3[1], s, ldexpf(vb ? (im ? cw * tg.c[t] : -cw * tg.s[t]) : 0.f, kS4));
             const int t16 = (16 * l2) & 31;
             float e = 0.f;
             if (vb && h == 0 && s == 0) e = im ? tg.s[t16] : tg.c[t16];   // Re P(k1, 16)
             if (vb && h == 0 && s == 1) e = im ? tg.c[t16] : -tg.s[t16];  // Im P(k1, 16)
-            put(T3[2], s, e);
+            put(T3[2], s, ldexpf(e, kS4));
             // stage 4: K = k1 = 16 q + rmap(h, s), column a = n, l1 = a - HW
             const bool va = n < kNB;
             const int l1 = n - kHW;
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int s = 0; s < 8; s += 2)
-                put2(a4[q], s, ldexpf(d3[8 * q + s], kS4), ldexpf(d3[8 * q + s + 1], kS4), neg1);
+                put2(a4[q], s, d3[8 * q + s], d3[8 * q + s + 1], neg1);
         f16v oc = mfma3(a4[0], tabs[6][tl], zero);
         oc = mfma3(a4[1], tabs[7][tl], oc);
         f16v os = mfma3(a4[0], tabs[8][tl], zero);
