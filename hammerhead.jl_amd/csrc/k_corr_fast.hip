@@ -408,14 +408,15 @@ __device__ __forceinline__ void uq_init(UQAcc &a)
 // s = 0, o = 0..4 (o = 0 is S00) and s = 1..4, o = -4..4. Up to the point
 // reflection S(-a, -b) = S(a, b) this is the half-plane offset set of the
 // ring sums (the ring of an offset is max(|s|, |o|)). The cross-lane totals
-// go through LDS: each lane writes its row partials of a group of lags (16 rows x kUQP doubles
+// go through LDS: each lane writes its row partials of a group of lags (16 rows x kUQP floats
 // per window, the row pitch chosen so 16 lanes hit distinct banks), then lane
 // r adds the 16 row partials of lags r and r + 16 and folds its totals into
 // per-ring maximum and sum, which DPP reductions finish over the window:
 // about a third of the instructions of a 16-lane DPP reduction per lag. Lags
 // are numbered s = 0, o = 0..4 (k = o; k = 0 is S00), then s = 1..4,
 // o = -4..4 (k = 5 + 9 (s - 1) + o + 4), in two groups (s <= 2: 23 lags;
-// s >= 3: 18) so the scratch is 3200 bytes per window.
+// s >= 3: 18) so the scratch is 1600 bytes per window. The partials are FP32
+// dot products, so holding them as FP32 loses nothing; the totals are FP64.
 constexpr int kUQP = 25;
 
 __device__ __forceinline__ void lds_wave_sync()
@@ -448,7 +449,7 @@ __device__ __forceinline__ double row_max_d(double v)  // a NaN never replaces t
 
 // Totals of the N lags of one group (global numbers kbase + 0..N-1).
 template <int N>
-__device__ __forceinline__ void uq_collect(const double *red, int r, int kbase, double (&rmax)[4], double (&rsum)[4],
+__device__ __forceinline__ void uq_collect(const float *red, int r, int kbase, double (&rmax)[4], double (&rsum)[4],
                                            double &s00)
 {
 #pragma unroll
@@ -457,7 +458,7 @@ __device__ __forceinline__ void uq_collect(const double *red, int r, int kbase, 
         if (k < N) {
             double tot = 0.0;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) tot += red[q * kUQP + k];
+            for (int q = 0; q < 16; ++q) tot += (double)red[q * kUQP + k];
             const int kg = kbase + k;
             const int ring = uq_ring(kg);
             if (kg == 0) s00 = tot;
@@ -477,7 +478,7 @@ __device__ __forceinline__ void uq_collect(const double *red, int r, int kbase, 
 // row's |terms|, far inside the statistics' use) and taken to FP64 before
 // every cross-row sum; FP32 products halve the cost of the FP64 ones.
 template <int NC>
-__device__ __forceinline__ void uq_lags_lds(const double (&ed)[NC], UQAcc &acc, double *red, int r)
+__device__ __forceinline__ void uq_lags_lds(const double (&ed)[NC], UQAcc &acc, float *red, int r)
 {
     float e[NC];
 #pragma unroll
@@ -488,7 +489,7 @@ __device__ __forceinline__ void uq_lags_lds(const double (&ed)[NC], UQAcc &acc, 
         rmax[q] = -INFINITY;
         rsum[q] = 0.0;
     }
-    double *row = red + r * kUQP;
+    float *row = red + r * kUQP;
     auto shifted = [&](auto ss, int base) {
         constexpr int s = decltype(ss)::value;
         float p[9];
@@ -504,7 +505,7 @@ __device__ __forceinline__ void uq_lags_lds(const double (&ed)[NC], UQAcc &acc, 
             }
         }
 #pragma unroll
-        for (int oi = 0; oi < 9; ++oi) row[base + oi] = (double)p[oi];
+        for (int oi = 0; oi < 9; ++oi) row[base + oi] = p[oi];
     };
     // group 1: s = 0, 1, 2
 #pragma unroll
@@ -512,7 +513,7 @@ __device__ __forceinline__ void uq_lags_lds(const double (&ed)[NC], UQAcc &acc, 
         float p = 0.f;
 #pragma unroll
         for (int c = 0; c + o < NC; ++c) p = __builtin_fmaf(e[c], e[c + o], p);
-        row[o] = (double)p;
+        row[o] = p;
     }
     shifted(std::integral_constant<int, 1>{}, 5);
     shifted(std::integral_constant<int, 2>{}, 14);
@@ -537,7 +538,7 @@ __device__ __forceinline__ void uq_lags_lds(const double (&ed)[NC], UQAcc &acc, 
 // window whose apodized, mean-subtracted rows sit one per lane (x[j] =
 // (wA[r, j], wB[r, j]), r = lane of the row). u: shift along columns
 // (in-lane); v: the transposed windows, shift along rows (across lanes).
-__device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &sig_v, double *red)
+__device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &sig_v, float *red)
 {
 #pragma clang fp contract(off)
     const float q = 0.25f;
@@ -635,13 +636,13 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 // window, one row per lane, windows formed exactly as k_corrL<1> forms them
 // (same loads, means and apodization), then uq_fused16. Blocks of 4 windows
 // in XCD bands.
-__global__ __launch_bounds__(64) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
                                                  const float *__restrict__ B, int nr,
                                                  const int32_t *__restrict__ jobs, int njobs, int nblocks,
                                                  CorrOut out)
 {
 #pragma clang fp contract(off)
-    __shared__ double ured[4][16 * kUQP];
+    __shared__ float ured[4][16 * kUQP];
     const int tid = threadIdx.x, line = tid & 15, slot_w = tid >> 4;
     const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
@@ -820,7 +821,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_
     // post-loop uncertainty_sweep! reads (src/pipeline.jl:810-824)
     float sig_u = NAN, sig_v = NAN;
     // (the window's LDS slice is free until the transpose below)
-    if constexpr (UQ && L == 1) uq_fused16(x, line, sig_u, sig_v, reinterpret_cast<double *>(T));
+    if constexpr (UQ && L == 1) uq_fused16(x, line, sig_u, sig_v, reinterpret_cast<float *>(T));
 
     // ---- 2. row DFT (dim 2) and Hermitian split, k2 = 0..H ----
     const int mblk = block_of<L>(l);
