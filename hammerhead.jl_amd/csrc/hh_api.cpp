@@ -852,6 +852,23 @@ struct EnergySetup {
 
 // Node grid, windows and node-interval pixel ranges of one energy call; the
 // node field itself is uploaded by energy_nodes.
+// A predictor or node value that is not finite would give NaN sample
+// positions; they are rejected before anything reaches the device.
+void check_finite(const float *a, size_t n, const char *what)
+{
+    for (size_t k = 0; k < n; ++k)
+        if (!std::isfinite(a[k])) throw Error(HH_EINVAL, std::string(what) + " holds a non-finite value");
+}
+
+// Uniform knots (the first two define the spacing): finite and increasing.
+void check_knots(const float *py, int gny, const float *px, int gnx)
+{
+    check_finite(py, (size_t)gny, "row knots");
+    check_finite(px, (size_t)gnx, "column knots");
+    if ((gny > 1 && !(py[1] > py[0])) || (gnx > 1 && !(px[1] > px[0])))
+        throw Error(HH_EINVAL, "node knots must increase");
+}
+
 EnergySetup energy_setup(hh_ctx *c, const hh_pass &p, const float *py, int gny, const float *px, int gnx)
 {
     need_pair(c);
@@ -863,8 +880,7 @@ EnergySetup energy_setup(hh_ctx *c, const hh_pass &p, const float *py, int gny, 
     if (w != 8 && w != 16 && w != 32)
         throw Error(HH_EUNSUPPORTED, "the correlation-energy adjoint supports 8, 16 and 32 px windows");
     if (gny < 1 || gnx < 1 || !py || !px) throw Error(HH_EINVAL, "empty node grid");
-    if ((gny > 1 && !(py[1] > py[0])) || (gnx > 1 && !(px[1] > px[0])))
-        throw Error(HH_EINVAL, "node knots must increase");
+    check_knots(py, gny, px, gnx);
     if (!c->coefs_valid) do_prefilter(c);
     PassGrid grid = make_grid(p, c->nr, c->nc);
     if (grid.ny * grid.nx == 0) throw Error(HH_EINVAL, "the pass grid is empty");
@@ -931,6 +947,8 @@ void energy_nodes(hh_ctx *c, EnergySetup &s, const float *pu, const float *pv)
 {
     if (!pu || !pv) throw Error(HH_EINVAL, "null node field");
     const size_t n = (size_t)s.g.gny * s.g.gnx;
+    check_finite(pu, n, "node field u");
+    check_finite(pv, n, "node field v");
     HH_HIP(hipMemcpyAsync(c->enode.p, pu, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     HH_HIP(hipMemcpyAsync(c->enode.p + n, pv, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     s.g.pu = c->enode.p;
@@ -1163,6 +1181,9 @@ int hh_deform(hh_ctx *c, const float *py, int gny, const float *px, int gnx, con
         if (!c->coefs_valid) throw Error(HH_ESTATE, "no interpolant: call hh_prefilter first");
         if (gny < 1 || gnx < 1 || !py || !px || !pu || !pv) throw Error(HH_EINVAL, "empty predictor grid");
         const size_t n = (size_t)gny * gnx;
+        check_knots(py, gny, px, gnx);
+        check_finite(pu, n, "predictor u");
+        check_finite(pv, n, "predictor v");
         upload(c, c->pu, pu, n);
         upload(c, c->pv, pv, n);
         c->gny = gny;
@@ -1275,11 +1296,13 @@ int hh_correlation_planes(hh_ctx *c, const hh_pass *pass, int src, const int32_t
     return guarded(c, [&] {
         if (!pass || !planes) throw Error(HH_EINVAL, "null argument");
         check_pass_device(*pass, false);
+        if (njobs < 0 || (njobs > 0 && !jobs)) throw Error(HH_EINVAL, "invalid job list");
         const float *dA = src_image(c, src, true), *dB = src_image(c, src, false);
         const int M = pass->padding ? 2 * pass->search_area_size[0] : pass->search_area_size[0];
         int maxgi = 1, maxgj = 1;
         for (int k = 0; k < njobs; ++k) {
             const int rs = jobs[4 * k + 2], cs = jobs[4 * k + 3];
+            if (jobs[4 * k] < 1 || jobs[4 * k + 1] < 1) throw Error(HH_EDIM, "job grid index out of range");
             check_job_area(c, *pass, rs, cs);
             maxgi = std::max(maxgi, jobs[4 * k]);
             maxgj = std::max(maxgj, jobs[4 * k + 1]);

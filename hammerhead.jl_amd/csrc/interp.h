@@ -16,7 +16,9 @@ __device__ __forceinline__ void lin_axis(float q0, float qstep, int n, float q, 
         return;
     }
     float t = (q - q0) / qstep;
-    t = t < 0.f ? 0.f : t;
+    // written so that a NaN position clamps to the first knot instead of
+    // reaching the float-to-int conversion
+    t = t >= 0.f ? t : 0.f;
     t = t > (float)(n - 1) ? (float)(n - 1) : t;
     int ii = (int)floorf(t);
     if (ii > n - 2) ii = n - 2;
@@ -66,7 +68,9 @@ __device__ __forceinline__ void cubw6(float d, float w[4])
 
 __device__ __forceinline__ float cubic(const float *__restrict__ C, int nr, int nc, float yy, float xx)
 {
-    if (yy < 1.f || yy > (float)nr || xx < 1.f || xx > (float)nc) return 0.f;
+    // negated in-domain test: a NaN coordinate is outside (sample 0), never
+    // an index
+    if (!(yy >= 1.f && yy <= (float)nr && xx >= 1.f && xx <= (float)nc)) return 0.f;
     int iy = (int)floorf(yy);
     if (iy > nr - 1) iy = nr - 1;
     int ix = (int)floorf(xx);
@@ -74,6 +78,52 @@ __device__ __forceinline__ float cubic(const float *__restrict__ C, int nr, int 
     float wy[4], wx[4];
     cubw6(yy - (float)iy, wy);
     cubw6(xx - (float)ix, wx);
+    const int N1 = nr + 2;
+    float s = 0.f;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        s += wx[n] * (wy[0] * col[0] + wy[1] * col[1] + wy[2] * col[2] + wy[3] * col[3]);
+    }
+    return s / 36.f;
+}
+
+// One axis of a sample position given as an integer pixel k plus a small
+// offset o (the half displacement): the base index and fraction of k + o
+// without forming k + o in float32, whose rounding (ulp(2048) = 2.4e-4 px)
+// would move the sample. Returns false outside [1, n] (or for a non-finite
+// offset); otherwise i in [1, n-1] and d in [0, 1] as the spline needs.
+__device__ __forceinline__ bool split_axis(int k, float o, int n, int &i, float &d)
+{
+    if (!(fabsf(o) < 1.0e6f)) return false;
+    const float f = floorf(o);
+    int ii = k + (int)f;
+    float dd = o - f;  // exact for |o| >= 1; may round up to 1 for tiny negative o
+    if (dd >= 1.f) {
+        ii += 1;
+        dd = 0.f;
+    }
+    if (ii < 1 || ii > n || (ii == n && dd > 0.f)) return false;
+    if (ii > n - 1) {  // the position n itself: base n - 1, fraction 1
+        ii = n - 1;
+        dd += 1.f;
+    }
+    i = ii;
+    d = dd;
+    return true;
+}
+
+// cubic() at (r + oy, c + ox) for integer r, c and small offsets, with the
+// fractions taken from the offsets exactly (split_axis).
+__device__ __forceinline__ float cubic_at(const float *__restrict__ C, int nr, int nc, int r, float oy, int c,
+                                          float ox)
+{
+    int iy, ix;
+    float fy, fx;
+    if (!split_axis(r, oy, nr, iy, fy) || !split_axis(c, ox, nc, ix, fx)) return 0.f;
+    float wy[4], wx[4];
+    cubw6(fy, wy);
+    cubw6(fx, wx);
     const int N1 = nr + 2;
     float s = 0.f;
 #pragma unroll
@@ -99,7 +149,7 @@ __device__ __forceinline__ void cubw_d(float d, float g[4])
 __device__ __forceinline__ float cubic_d(const float *__restrict__ C, int nr, int nc, float yy, float xx, float &sy,
                                          float &sx)
 {
-    if (yy < 1.f || yy > (float)nr || xx < 1.f || xx > (float)nc) {
+    if (!(yy >= 1.f && yy <= (float)nr && xx >= 1.f && xx <= (float)nc)) {
         sy = sx = 0.f;
         return 0.f;
     }

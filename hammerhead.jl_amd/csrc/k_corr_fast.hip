@@ -817,6 +817,29 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_
         }
     }
 
+    // A frame whose loaded window is identically zero has an exactly zero
+    // plane in the reference's separate transforms; the packed A + iB
+    // transform would leak the other frame's round-off into it, so the box
+    // is scaled by keep (0 for such a window, else 1).
+    float keep;
+    {
+        bool nza = false, nzb = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            nza |= x[j].x != 0.f;
+            nzb |= x[j].y != 0.f;
+        }
+        if constexpr (G >= 64) {
+            const int anya = __syncthreads_or(nza);
+            const int anyb = __syncthreads_or(nzb);
+            keep = anya && anyb ? 1.f : 0.f;
+        } else {
+            const uint64_t grp = ((1ull << G) - 1) << ((tid & 63) / G * G);
+            const uint64_t ba = __ballot(nza), bb = __ballot(nzb);
+            keep = (ba & grp) && (bb & grp) ? 1.f : 0.f;
+        }
+    }
+
     // fused uncertainty: the final sweep's windows are the ones the
     // post-loop uncertainty_sweep! reads (src/pipeline.jl:810-824)
     float sig_u = NAN, sig_v = NAN;
@@ -933,7 +956,7 @@ __global__ __launch_bounds__(L == 4 ? 256 : 64) __attribute__((amdgpu_waves_per_
             c[k] = cv;
         }
         line_inv<L>(c, xl);  // c[n] = r(qa) + i r(qb) at l2 = L n + l
-        constexpr float scale = 1.0f / (float)(M * M);
+        const float scale = keep / (float)(M * M);
         const int ipa = LO + qa, ipb = LO + qb;
         sfor<32>([&](auto nn) {
             constexpr int n = decltype(nn)::value;

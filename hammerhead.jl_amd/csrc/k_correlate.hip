@@ -216,10 +216,14 @@ __device__ void analyze_lds(const CorrConfig &cfg, const PlaneView<M> &R, int jo
 // both apodized windows at the origin; an enlarged search area (:230-262):
 // the apodized window at offset mg = (s - w) / 2, the search area
 // mean-subtracted and not apodized. Means are block sums in T divided by
-// the pixel counts. Ends with a block barrier.
+// the pixel counts. Ends with a block barrier. Returns 1, or 0 when either
+// frame's loaded window is identically zero: the reference's separate
+// transforms then give an exactly zero plane, while the packed A + iB
+// transform leaks round-off of the other frame into it, so the caller
+// scales the plane by the returned factor.
 template <int M>
-__device__ void load_windows_lds(const CorrConfig &cfg, const float *__restrict__ A, const float *__restrict__ B,
-                                 int nr, int rs, int cs, float2 *pl, float *red)
+__device__ float load_windows_lds(const CorrConfig &cfg, const float *__restrict__ A, const float *__restrict__ B,
+                                  int nr, int rs, int cs, float2 *pl, float *red)
 {
 #pragma clang fp contract(off)
     constexpr int P = M + 1;
@@ -242,11 +246,15 @@ __device__ void load_windows_lds(const CorrConfig &cfg, const float *__restrict_
 
     for (int e = tid; e < M * P; e += kThreads) pl[e] = make_float2(0.f, 0.f);
     __syncthreads();
+    bool nza = false, nzb = false;
     if (mg == 0) {
         for (int e = tid; e < w * w; e += kThreads) {
             const int i = e % w, j = e / w;
             const float a = cfg.apod[e];
-            pl[i + j * P] = make_float2(a * (Ab[i + (size_t)j * nr] - meanA), a * (Bb[i + (size_t)j * nr] - meanB));
+            const float2 v = make_float2(a * (Ab[i + (size_t)j * nr] - meanA), a * (Bb[i + (size_t)j * nr] - meanB));
+            nza |= v.x != 0.f;
+            nzb |= v.y != 0.f;
+            pl[i + j * P] = v;
         }
     } else {
         for (int e = tid; e < sa_ * sa_; e += kThreads) {
@@ -254,10 +262,15 @@ __device__ void load_windows_lds(const CorrConfig &cfg, const float *__restrict_
             const int iw = i - mg, jw = j - mg;
             const bool inw = iw >= 0 && iw < w && jw >= 0 && jw < w;
             const float xa = inw ? cfg.apod[iw + jw * w] * (Ab[iw + (size_t)jw * nr] - meanA) : 0.f;
-            pl[i + j * P] = make_float2(xa, Bb[i + (size_t)j * nr] - meanB);
+            const float xb = Bb[i + (size_t)j * nr] - meanB;
+            nza |= xa != 0.f;
+            nzb |= xb != 0.f;
+            pl[i + j * P] = make_float2(xa, xb);
         }
     }
-    __syncthreads();
+    const int anya = __syncthreads_or(nza);
+    const int anyb = __syncthreads_or(nzb);
+    return anya && anyb ? 1.f : 0.f;
 }
 
 template <int M>
@@ -280,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
     const int sa_ = cfg.s, mg = (sa_ - cfg.w) / 2;
 
     for (int k = tid; k < M; k += kThreads) stw[k] = tw[k];
-    load_windows_lds<M>(cfg, A, B, nr, rs, cs, pl, red);
+    const float keep = load_windows_lds<M>(cfg, A, B, nr, rs, cs, pl, red);
 
     // ---- forward DIF along dim 1 (only the nonzero columns) ----
     for (int len = M; len >= 2; len >>= 1) {
@@ -364,7 +377,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_lds(CorrConfig cfg, const flo
     for (int e = tid; e < M * M; e += kThreads) {
         const int i = e % M, j = e / M;
         const int ip = (i + H) & (M - 1), jp = (j + H) & (M - 1);
-        float val = fabsf(pl[i + j * P].x * scale);
+        float val = fabsf(pl[i + j * P].x * scale) * keep;
         if (cfg.gain) val *= cfg.gain[ip + jp * M];
         if (mg > 0 && (abs(ip - H) > mg || abs(jp - H) > mg)) val = 0.f;
         pl[ip + jp * P].y = val;
@@ -609,7 +622,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_dft(CorrConfig cfg, const flo
     const int S = cfg.s, mg = (S - cfg.w) / 2;
 
     for (int k = tid; k < M; k += kThreads) stw[k] = tw[k];
-    load_windows_lds<M>(cfg, A, B, nr, rs, cs, pl, red);
+    const float keep = load_windows_lds<M>(cfg, A, B, nr, rs, cs, pl, red);
 
     const auto ident = [](int t) { return t; };
     const int nkept = kept_count(M, mg);
@@ -651,7 +664,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_dft(CorrConfig cfg, const flo
         const int ip = (i + H) % M, jp = (j + H) % M;
         float val = 0.f;
         if (mg == 0 || (abs(ip - H) <= mg && abs(jp - H) <= mg)) {
-            val = fabsf(pl[i + j * P].x * scale);
+            val = fabsf(pl[i + j * P].x * scale) * keep;
             if (cfg.gain) val *= cfg.gain[ip + jp * M];
         }
         pl[ip + jp * P].y = val;

@@ -19,6 +19,10 @@
 // coarse vector grid, and warpA(r, c) = A(r - dv/2, c - du/2),
 // warpB(r, c) = B(r + dv/2, c + du/2), sampled with the cubic B-spline
 // weights of _pk_cubic (src/ka_backend.jl:265-291), zero outside [1, n].
+// The sample's base index and fraction come from the pixel index and the
+// half displacement separately (cubic_at), so the position is as exact as
+// the displacement itself rather than rounded to float32 at the pixel's
+// magnitude.
 
 #include "hh_internal.h"
 #include "interp.h"
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__
     const float dv2 = bilinear_at(P.pv, P.gny, P.gnx, iy, ix, ly, lx) / 2.f;
     const size_t o = (size_t)P.out_off + (size_t)(r - 1) + (size_t)(c - 1) * ld;
     // both gathers before either store: all 32 coefficient loads in flight
-    const float va = cubic(cA, nr, nc, R - dv2, Cc - du2);
-    const float vb = cubic(cB, nr, nc, R + dv2, Cc + du2);
+    const float va = cubic_at(cA, nr, nc, r, -dv2, c, -du2);
+    const float vb = cubic_at(cB, nr, nc, r, dv2, c, du2);
     wA[o] = va;
     wB[o] = vb;
 }

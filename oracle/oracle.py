@@ -359,9 +359,16 @@ def grid_mask(pass_struct, nr, nc, mask, threshold=0.5):
     return gm.astype(bool)
 
 
-def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask=None, mask_threshold=0.5):
+def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask=None, mask_threshold=0.5,
+            uq_pert=False):
     """run_piv(imgA, imgB, passes; mask, mask_threshold): dict with the final
-    PIVResult fields (``mask``: the final grid mask)."""
+    PIVResult fields (``mask``: the final grid mask). With ``uq_pert`` and
+    uncertainty on the final pass, ``uq_pert_u`` / ``uq_pert_v`` hold the
+    final uncertainty recomputed on its input images rounded to float32 and
+    scaled by 1 + 2^-21 h per pixel (h a fixed hash in [-1, 1]): sigma's own
+    sensitivity to float32-level input error, and ``uq_stats`` the FP64
+    Wieneke statistics behind each cell's sigma, shape (44, 2, ny, nx) (u then
+    v)."""
     dt = _dt(prec)
     A, B = _f(A, dt), _f(B, dt)
     nr, nc = A.shape
@@ -374,14 +381,19 @@ def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask
                outliers=np.zeros((ny, nx), dtype=np.uint8, order="F"),
                mask=np.zeros((ny, nx), dtype=np.uint8, order="F"))
     m = _mask(mask)
+    st = np.zeros((ny, nx, 2), dtype=dt, order="F") if uq_pert else None
+    ss = np.zeros((44, 2, ny, nx), dtype=np.float64, order="F") if uq_pert else None
     rc = _fn("orc_run_piv", prec)(_p(A), _p(B), C.c_int(nr), C.c_int(nc), arr, C.c_int(len(passes)),
                                   C.c_int(1 if predictor_smoothing else 0), C.c_int(nthreads), _p(res["x"]),
                                   _p(res["y"]), _p(res["u"]), _p(res["v"]), _p(res["peak_ratio"]),
                                   _p(res["correlation_moment"]), _p(res["uncertainty_u"]),
                                   _p(res["uncertainty_v"]), _p(res["outliers"]), _p(m), C.c_double(mask_threshold),
-                                  _p(res["mask"]))
+                                  _p(res["mask"]), _p(st), _p(ss))
     if rc != 0:
         raise ValueError("invalid pass schedule")
+    if uq_pert:
+        res["uq_pert_u"], res["uq_pert_v"] = st[:, :, 0], st[:, :, 1]
+        res["uq_stats"] = ss
     res["outliers"] = res["outliers"].astype(bool)
     res["mask"] = res["mask"].astype(bool)
     return res

@@ -110,4 +110,15 @@ static int orc_cmpd(const void *a, const void *b)
     return x < y ? -1 : (x > y ? 1 : 0);
 }
 
+/* A fixed pseudo-random value in [-1, 1) for index k (splitmix64 finalizer):
+ * the deterministic perturbation of the test diagnostics. */
+static double orc_hash_unit(size_t k)
+{
+    uint64_t z = (uint64_t)k * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
 #endif

@@ -1414,11 +1414,20 @@ static void F(orc_build_predictor)(const REAL *u, const REAL *v, const uint8_t *
  * 361-449) with piv_pass (src/pipeline.jl:655-832). mask: image-sized pixel
  * mask, nullable; omask (nullable) receives the final grid mask. Outputs the
  * final pass's PIVResult into caller arrays sized by the final grid (any may
- * be NULL except the ones needed). Returns 0, or -1 for an invalid schedule. */
+ * be NULL except the ones needed). ouqpert (nullable, test diagnostics):
+ * the final uncertainty again (sigma_u grid, then sigma_v grid), computed on
+ * the final pass's images re-deformed with every predictor node moved by up
+ * to 1e-5 px (when the pass deforms), rounded to float32 and scaled by
+ * 1 + 2^-21 h per pixel (h in [-1, 1], fixed hashes of the node and pixel
+ * indices): how far sigma moves when its inputs carry the error of a
+ * float32 pipeline. ouqstats (nullable, test diagnostics): the FP64 Wieneke
+ * statistics behind the final uncertainty, 2 x 44 per grid cell (u then v;
+ * zero at cells without a job).
+ * Returns 0, or -1 for an invalid schedule. */
 int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_pass *passes, int npasses,
                    int predictor_smoothing, int nthreads, REAL *ox, REAL *oy, REAL *ou, REAL *ov, REAL *opr,
                    REAL *ocm, REAL *ouu, REAL *ouv, uint8_t *oout, const uint8_t *mask, double mask_threshold,
-                   uint8_t *omask)
+                   uint8_t *omask, REAL *ouqpert, double *ouqstats)
 {
     if (npasses < 1) return -1;
     int deforms = npasses > 1;
@@ -1571,6 +1580,43 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
         }
         if (unc && !fused_unc)
             F(orc_uncertainty_sweep)(p, srcA, srcB, nr, jobs, njobs, ny, uu, uv, NULL, mask, nthreads);
+        if (unc && ouqstats && k == npasses - 1) {
+            double *st = (double *)calloc((size_t)2 * HH_UQ_NSTATS * (njobs > 0 ? njobs : 1), sizeof(double));
+            F(orc_uncertainty_sweep)(p, srcA, srcB, nr, jobs, njobs, ny, NULL, NULL, st, mask, nthreads);
+            memset(ouqstats, 0, sizeof(double) * 2 * HH_UQ_NSTATS * ng);
+            for (int jb = 0; jb < njobs; jb++) {
+                size_t g = (size_t)(jobs[4 * jb] - 1) + (size_t)(jobs[4 * jb + 1] - 1) * ny;
+                memcpy(ouqstats + (size_t)2 * HH_UQ_NSTATS * g, st + (size_t)2 * HH_UQ_NSTATS * jb,
+                       sizeof(double) * 2 * HH_UQ_NSTATS);
+            }
+            free(st);
+        }
+        if (unc && ouqpert && k == npasses - 1) {
+            /* the final images again, from a predictor moved by up to 1e-5 px
+             * per node (when the pass deforms), rounded to float32 and scaled
+             * by 1 + 2^-21 h per pixel */
+            REAL *qA = (REAL *)malloc(sizeof(REAL) * npx), *qB = (REAL *)malloc(sizeof(REAL) * npx);
+            const REAL *sA = srcA, *sB = srcB;
+            if (have_pred) {
+                size_t nn = (size_t)pred.ny * pred.nx;
+                REAL *tu = (REAL *)malloc(sizeof(REAL) * nn), *tv = (REAL *)malloc(sizeof(REAL) * nn);
+                for (size_t q = 0; q < nn; q++) {
+                    tu[q] = pred.u[q] + (REAL)(1e-5 * orc_hash_unit(2 * q + 7));
+                    tv[q] = pred.v[q] + (REAL)(1e-5 * orc_hash_unit(2 * q + 8));
+                }
+                F(orc_deform)(cA, cB, nr, nc, pred.y, pred.ny, pred.x, pred.nx, tu, tv, qA, qB, nthreads);
+                free(tu); free(tv);
+                sA = qA;
+                sB = qB;
+            }
+            for (size_t q = 0; q < npx; q++) {
+                qA[q] = (REAL)((double)(float)sA[q] * (1.0 + ldexp(orc_hash_unit(2 * q), -21)));
+                qB[q] = (REAL)((double)(float)sB[q] * (1.0 + ldexp(orc_hash_unit(2 * q + 1), -21)));
+            }
+            for (size_t q = 0; q < 2 * ng; q++) ouqpert[q] = (REAL)NAN;
+            F(orc_uncertainty_sweep)(p, qA, qB, nr, jobs, njobs, ny, ouqpert, ouqpert + ng, NULL, mask, nthreads);
+            free(qA); free(qB);
+        }
         if (keep_measured)
             for (size_t q = 0; q < ng; q++)
                 if (outl[q]) {

@@ -6,11 +6,12 @@ what run_piv gives on that pair alone, since both run the same per-window,
 per-pixel and per-cell arithmetic; a graph replay gives exactly the eager
 launch sequence's result; and a batch pair against the FP64 oracle meets the
 run_piv bars (flags within the near-tie allowance, u/v RMS < 5e-3 px)."""
-import math
 import sys
 
 import numpy as np
 import pytest
+
+from parity_bars import field_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -134,16 +135,9 @@ def test_batch_pair_vs_oracle(hh, orc, small_pairs):
         cb.set_pairs([(p.A, p.B) for p in pairs])
         rb = cb.run_piv_batch(passes)
     for q, p in enumerate(pairs):
-        o = orc.run_piv(p.A.astype(np.float64), p.B.astype(np.float64), [s.to_struct() for s in passes],
+        o = orc.run_piv(p.A.astype(np.float64), p.B.astype(np.float64), [s.to_struct() for s in passes], uq_pert=True,
                         prec="f64")
-        g = rb[q]
-        dflag = int((g["outliers"] != o["outliers"]).sum())
-        ok = (~g["outliers"]) & (~o["outliers"])
-        d = np.concatenate([(g["u"] - o["u"])[ok], (g["v"] - o["v"])[ok]])
-        rms = math.sqrt(np.mean(d ** 2))
-        say("pair", q, p.A.shape, "flag diffs", dflag, "rms", rms)
-        assert dflag <= 0.002 * g["outliers"].size + 2
-        assert rms < 5e-3
+        field_parity(rb[q], o, label=f"batch pair {q} {p.A.shape}", rms_tol=5e-3)
 
 
 def test_run_piv_sequence_api(hh, small_pairs):

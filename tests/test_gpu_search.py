@@ -11,11 +11,12 @@ search area of a small apodized window is the least favourable case, about
 reference's search-area scene (test/runtests.jl:66-80); multi-pass and
 ensemble runs within the run_piv bars (u/v RMS < 5e-3 px).
 """
-import math
 import sys
 
 import numpy as np
 import pytest
+
+from parity_bars import field_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -98,14 +99,8 @@ def test_run_piv_search_multipass_vs_oracle(hh, orc, ctx):
                                convergence_tol=0.0, uncertainty=True)]
     ctx.set_pair(pair.A, pair.B)
     g = ctx.run_piv(passes)
-    o = orc.run_piv(pair.A.astype(np.float64), pair.B.astype(np.float64), [p.to_struct() for p in passes])
-    dflag = int((g["outliers"] != o["outliers"]).sum())
-    same = ~g["outliers"] & ~o["outliers"]
-    d = np.concatenate([np.abs(g["u"] - o["u"])[same], np.abs(g["v"] - o["v"])[same]])
-    rms = math.sqrt(np.mean(d ** 2))
-    say("search multipass flag diffs", dflag, "rms", rms)
-    assert dflag <= 0.002 * g["outliers"].size + 2
-    assert rms < 5e-3
+    o = orc.run_piv(pair.A.astype(np.float64), pair.B.astype(np.float64), [p.to_struct() for p in passes], uq_pert=True)
+    field_parity(g, o, label="search multipass", rms_tol=5e-3)
 
 
 def test_ensemble_search_area_vs_oracle(hh, orc, ctx):
