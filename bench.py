@@ -61,19 +61,101 @@ def read_traffic(path: pathlib.Path):
     return None
 
 
-def cpu_baseline(pair, passes, threads: int):
-    """The oracle's Float32 restatement of the reference run_piv, timed on the
-    host for one vector field of the same workload."""
+def cpu_model() -> str:
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def window_correlations(hh, passes, nr, nc) -> int:
+    n = 0
+    for ps in passes:
+        ny, nx = hh.grid_size(ps, nr, nc)
+        n += ny * nx * ps.max_iterations
+    return n
+
+
+def reference_anchor(hh, passes, nr, nc):
+    """The reference's published serial CPU costs of this path on a 1024^2
+    pair (reference/archive/FEEDBACK_PLAN.md:36-38: deformation 77 ms per
+    sweep; all-window correlation 225 ms padded + Gaussian, 347 ms with the
+    uncertainty, i.e. 3969 windows of 32 px at 50 % overlap), scaled to this
+    workload: deformation by pixels, correlation per window by
+    M^2 log2 M^2 (M = FFT side) from M = 64, uncertainty per window by w^2.
+    An estimate, not a measurement."""
+    base_win = 0.225 / 3969
+    base_uq = (0.347 - 0.225) / 3969
+    s = 0.0
+    deform_sweeps = sum(ps.max_iterations for ps in passes) - (1 if passes[0].max_iterations == 1 else 0)
+    s += deform_sweeps * 0.077 * (nr * nc) / (1024 * 1024)
+    for ps in passes:
+        ny, nx = hh.grid_size(ps, nr, nc)
+        M = 2 * ps.window_size[0] if ps.padding else ps.window_size[0]
+        s += ny * nx * ps.max_iterations * base_win * (M * M * math.log2(M * M)) / (64 * 64 * 12)
+        if ps.uncertainty:
+            s += ny * nx * base_uq * (ps.window_size[0] / 32) ** 2
+    return {"value": round(1.0 / s, 5), "unit": "vector fields/s", "cores": 1, "kind": "reference",
+            "estimated_s_per_field": round(s, 2),
+            "source": "reference/archive/FEEDBACK_PLAN.md:36-38 (serial 1024^2 per-sweep costs), scaled"}
+
+
+def cpu_leg(hh, pair, passes, threads: int, log):
+    """The CPU leg: the oracle (the C restatement of the reference run_piv)
+    timed on the host, and its FP64 result as the checker of the device
+    fields. Returns (cpu_baseline dict, FP64 oracle result)."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle  # noqa: E402  (checker only; never the measured GPU path)
+    import oracle  # noqa: E402  (checker and CPU baseline only; never the measured GPU path)
 
     oracle.lib()
+    st = [p.to_struct() for p in passes]
+    nr, nc = pair.A.shape
     t0 = time.perf_counter()
-    oracle.run_piv(pair.A, pair.B, [p.to_struct() for p in passes], nthreads=threads, prec="f32")
+    oracle.run_piv(pair.A, pair.B, st, nthreads=threads, prec="f32")
     dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "vector fields/s", "cores": threads, "kind": "port",
+    log(f"cpu baseline: {threads} threads {dt:.1f} s")
+    # one thread on a bounded sample: the same schedule on the first 512
+    # columns of the pair, scaled by the window-correlation count
+    strip = 512
+    A1, B1 = np.asfortranarray(pair.A[:, :strip]), np.asfortranarray(pair.B[:, :strip])
+    t0 = time.perf_counter()
+    oracle.run_piv(A1, B1, st, nthreads=1, prec="f32")
+    d1 = time.perf_counter() - t0
+    scale = window_correlations(hh, passes, nr, nc) / window_correlations(hh, passes, nr, strip)
+    log(f"cpu baseline: 1 thread, {nr}x{strip} strip {d1:.1f} s")
+    t0 = time.perf_counter()
+    o64 = oracle.run_piv(pair.A.astype(np.float64), pair.B.astype(np.float64), st, nthreads=threads, prec="f64")
+    log(f"fp64 oracle: {time.perf_counter() - t0:.1f} s")
+    base = {"value": 1.0 / dt, "unit": "vector fields/s", "cores": threads, "kind": "port",
             "sample": f"one full 2048x2048 vector field (797341 window correlations + UQ) on {threads} threads, "
-                      f"{dt:.1f} s; C restatement of the reference's Float32 run_piv"}
+                      f"{dt:.1f} s; C restatement of the reference's Float32 run_piv (the Julia reference cannot "
+                      f"run here)",
+            "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
+            "one_thread": {"value": round(1.0 / (d1 * scale), 5), "unit": "vector fields/s", "cores": 1,
+                           "sample": f"the same schedule on the {nr}x{strip} column strip ({d1:.1f} s), scaled by "
+                                     f"the window-correlation count ({scale:.3f})"},
+            "reference_anchor": reference_anchor(hh, passes, nr, nc)}
+    return base, o64
+
+
+def oracle_deviation(res, o64):
+    """Deviation of a device field from the FP64 oracle: RMS and max of the
+    u/v differences, flag differences, peak-ratio and sigma errors."""
+    fin = np.isfinite(res["u"]) & np.isfinite(o64["u"])
+    d = np.concatenate([(res["u"] - o64["u"])[fin], (res["v"] - o64["v"])[fin]]).astype(np.float64)
+    pr = np.isfinite(res["peak_ratio"]) & np.isfinite(o64["peak_ratio"])
+    rpr = np.abs(res["peak_ratio"][pr] - o64["peak_ratio"][pr]) / np.abs(o64["peak_ratio"][pr])
+    su = np.isfinite(res["uncertainty_u"]) & np.isfinite(o64["uncertainty_u"])
+    rsu = np.abs(res["uncertainty_u"][su] - o64["uncertainty_u"][su]) / np.maximum(np.abs(o64["uncertainty_u"][su]),
+                                                                                   1e-6)
+    return {"rms_px_vs_fp64_oracle": float(np.sqrt(np.mean(d ** 2))), "max_px_vs_fp64_oracle": float(np.abs(d).max()),
+            "flag_diffs_vs_fp64_oracle": int((res["outliers"] != o64["outliers"]).sum()),
+            "peak_ratio_rel_err_median": float(np.median(rpr)), "peak_ratio_rel_err_max": float(rpr.max()),
+            "sigma_u_rel_err_median": float(np.median(rsu)),
+            "sigma_u_cells_over_1pct": int((rsu > 0.01).sum()), "sigma_u_cells": int(su.sum())}
 
 
 def mfma_flops_per_window_m32() -> float:
@@ -302,21 +384,29 @@ def main(argv=None):
     nwin = ny3 * nx3
     ms3 = kernels["correlate_w16"]["avg_ms"]
     fft_flops = nwin * fft_flops_per_window(32)
+    achieved = fft_flops / (ms3 * 1e-3) / 1e12
     if args.form == "mfma16":
+        # algorithmic FLOPs (the FFT-form count, the same for both forms) over
+        # the FP16 matrix-core peak; the issued hi/lo-split MFMA FLOPs, a
+        # measure of matrix-pipe occupancy, are reported beside it
         issued = nwin * mfma_flops_per_window_m32()
-        achieved = issued / (ms3 * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / PEAK_FP16_TFLOPS, 5), "traffic": None, "kernel": "correlate_w16 (mfma)",
-                    "algorithmic_flops_per_launch": issued, "flops_convention": "issued MFMA FLOPs (hi/lo split)",
-                    "fft_convention_tflops": round(fft_flops / (ms3 * 1e-3) / 1e12, 3)}
+                    "algorithmic_flops_per_launch": fft_flops,
+                    "flops_convention": "algorithmic: 3 x 5 M^2 log2 M^2 + 12 M^2 per window (M = 32)",
+                    "issued_mfma_flops_per_launch": issued,
+                    "issued_mfma_tflops": round(issued / (ms3 * 1e-3) / 1e12, 3),
+                    "issued_mfma_frac": round(issued / (ms3 * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 5)}
         tfile = args.traffic_file or str(ROOT / "profiles" / "correlate_w16_mfma_traffic.txt")
     else:
-        achieved = fft_flops / (ms3 * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None, "kernel": "correlate_w16 (fft32)",
                     "algorithmic_flops_per_launch": fft_flops,
-                    "flops_convention": "FFT form: 3 x 5 M^2 log2 M^2 + 12 M^2 per window, vs the FP32 VALU peak"}
+                    "flops_convention": "algorithmic: 3 x 5 M^2 log2 M^2 + 12 M^2 per window (M = 32), vs the FP32 "
+                                        "VALU peak"}
         tfile = args.traffic_file or str(ROOT / "profiles" / "correlate_w16_traffic.txt")
+    roofline["algorithmic_bytes_per_launch"] = nwin * 36 + nr * nc * 8
+    roofline["avg_launch_ms"] = round(ms3, 5)
     roofline["traffic"] = read_traffic(pathlib.Path(tfile))
 
     ms_per_step = elapsed / args.steps * 1e3
@@ -362,7 +452,10 @@ def main(argv=None):
         line["ensemble"] = time_ensemble(hh, args.ensemble_pairs, 3, log)
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(pair, passes, args.cpu_threads)
+        line["cpu_baseline"], o64 = cpu_leg(hh, pair, passes, args.cpu_threads, log)
+        line["accuracy"].update(oracle_deviation(res, o64))
+        if not args.no_compare:
+            line["compare"]["accuracy"] = oracle_deviation(r2, o64)
     else:
         line["cpu_baseline"] = None
     if tune_path and not tune_path.exists():
