@@ -649,3 +649,74 @@ def test_energy_gradient_central_differences(hh, orc, interp):
             # line inside +-h adds an O(h) kink term
             tol = 1e-6 if interp == "cubic" else 2e-3
             assert abs(fd - g) <= tol * scale, (field, i, j, fd, g)
+
+
+# ---------------------------------------------------------------- phase filter and node values
+@pytest.mark.parametrize("w,padding,apod", [(16, False, "none"), (32, True, "gauss"), (24, True, "none"),
+                                            (16, True, "gauss")])
+def test_phase_plane_equals_numpy(orc, hh, w, padding, apod):
+    # PhaseCorrelator (src/correlators.jl:110-127) and spectrum!
+    # (:273-279): W = exp(-(fi^2 + fj^2) / (2 sigma^2)) over fftfreq(M, M)
+    # bins in FFT order, sigma = min(M) / 8, P = W s / (|s| + eps), s =
+    # conj(FA) FB; |ifft| fftshifted to M/2 + 1, times the overlap gain when
+    # padded. Restated here with numpy's FFT and the closed-form apodization.
+    rng = np.random.default_rng(w + padding)
+    A = rng.random((w, w))
+    B = np.roll(A, (2, -1), axis=(0, 1)) + 0.3 * rng.random((w, w))
+    p = P(hh, window_size=w, overlap=w // 2, padding=padding, apodization=apod, correlation_method="phase")
+    R = orc.correlation_plane(p, A, B)
+    M = 2 * w if padding else w
+    i = np.arange(1, w + 1)
+    c = (w + 1) / 2
+    s = w / 4
+    ap = np.exp(-((i[:, None] - c) ** 2 / (2 * s * s) + (i[None, :] - c) ** 2 / (2 * s * s))) if apod == "gauss" \
+        else np.ones((w, w))
+    a = np.zeros((M, M))
+    b = np.zeros((M, M))
+    a[:w, :w] = ap * (A - A.mean())
+    b[:w, :w] = ap * (B - B.mean())
+    sp = np.conj(np.fft.fft2(a)) * np.fft.fft2(b)
+    f = np.fft.fftfreq(M, 1.0 / M)
+    sigma = M / 8
+    Wf = np.exp(-(f[:, None] ** 2 + f[None, :] ** 2) / (2 * sigma ** 2))
+    Rn = np.fft.fftshift(np.abs(np.fft.ifft2(Wf * sp / (np.abs(sp) + np.finfo(np.float64).eps))))
+    if padding:
+        Rn = Rn * orc.gain(p)
+    assert np.allclose(R, Rn, rtol=0, atol=1e-11 * Rn.max())
+    # the filter's width and bin order matter: a sigma of min(M)/4, or the
+    # filter in fftshifted order, changes the plane well above the tolerance
+    for Wbad in (np.exp(-(f[:, None] ** 2 + f[None, :] ** 2) / (2 * (M / 4) ** 2)), np.fft.fftshift(Wf)):
+        Rb = np.fft.fftshift(np.abs(np.fft.ifft2(Wbad * sp / (np.abs(sp) + np.finfo(np.float64).eps))))
+        if padding:
+            Rb = Rb * orc.gain(p)
+        assert not np.allclose(R, Rb, rtol=0, atol=1e-6 * Rn.max())
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_node_values_equal_linear_interpolation(orc, prec):
+    # predictor_node_values (src/pipeline.jl:561-569): Gridded(Linear()) over
+    # the (y, x) knots with Flat() extrapolation = scipy's linear
+    # RegularGridInterpolator at coordinates clamped into the knot range, on
+    # a non-linear node field and points inside, on and outside the knots
+    from scipy.interpolate import RegularGridInterpolator
+
+    rng = np.random.default_rng(5)
+    py = 16.5 + 16.0 * np.arange(9)
+    px = 24.5 + 12.0 * np.arange(11)
+    Y, X = np.meshgrid(py, px, indexing="ij")
+    pu = np.sin(X / 17.0) * np.cos(Y / 23.0) + 0.3 * rng.standard_normal(Y.shape)
+    pv = (X - 60.0) ** 2 / 900.0 - Y / 50.0
+    y = np.concatenate([[-5.0, 1.0, 16.5], rng.uniform(0, 170, 20), [144.5, 150.0, 400.0]])
+    x = np.concatenate([[0.0, 24.5], rng.uniform(-10, 170, 25), [144.5, 160.0]])
+    gu, gv = orc.node_values(py, px, pu, pv, y, x, prec=prec)
+    yy = np.clip(y, py[0], py[-1])
+    xx = np.clip(x, px[0], px[-1])
+    pts = np.stack(np.meshgrid(yy, xx, indexing="ij"), -1).reshape(-1, 2)
+    ru = RegularGridInterpolator((py, px), pu)(pts).reshape(y.size, x.size)
+    rv = RegularGridInterpolator((py, px), pv)(pts).reshape(y.size, x.size)
+    tol = 1e-12 if prec == "f64" else 2e-6
+    assert np.allclose(gu, ru, rtol=0, atol=tol * 3) and np.allclose(gv, rv, rtol=0, atol=tol * 10)
+    # linear extrapolation instead of Flat() would differ outside the knots
+    lin = RegularGridInterpolator((py, px), pu, bounds_error=False, fill_value=None)
+    pts_raw = np.stack(np.meshgrid(y, x, indexing="ij"), -1).reshape(-1, 2)
+    assert not np.allclose(gu, lin(pts_raw).reshape(y.size, x.size), atol=1e-3)
