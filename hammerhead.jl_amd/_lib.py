@@ -25,6 +25,7 @@ EXPORTS = (
     "hh_create", "hh_destroy", "hh_last_error", "hh_version", "hh_set_form", "hh_check_passes", "hh_grid_size",
     "hh_set_pair", "hh_set_pair_device", "hh_prefilter", "hh_deform", "hh_get_images", "hh_get_coefs",
     "hh_process_windows", "hh_correlation_planes", "hh_uncertainty_sweep", "hh_validate_and_replace",
+    "hh_validate_and_replace_noalloc",
     "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
@@ -78,6 +79,7 @@ def load():
         "hh_correlation_planes": (i32, [vp, pp, i32, vp, i32, vp]),
         "hh_uncertainty_sweep": (i32, [vp, pp, i32, vp, i32, i32, i32, vp, vp, vp]),
         "hh_validate_and_replace": (i32, [vp, pp, i32, i32, i32] + [vp] * 8),
+        "hh_validate_and_replace_noalloc": (i32, [vp, pp, i32, i32, i32] + [vp] * 8),
         "hh_smooth_field": (i32, [vp, i32, i32, vp, vp]),
         "hh_run_piv": (i32, [vp, pp, i32, i32, C.POINTER(hh_field)]),
         "hh_fetch_result": (i32, [vp, C.POINTER(hh_field)]),
@@ -285,6 +287,17 @@ class Context:
                                                        _ptr(cm), _ptr(au), _ptr(av), _ptr(m), _ptr(out)),
                "validate_and_replace")
         return u, v, out.astype(bool)
+
+    def validate_and_replace_noalloc(self, p, ny, nx, d_u, d_v, d_pr, d_cm, d_alt_u=None, d_alt_v=None,
+                                     d_mask=None, d_outliers=None, force_replace=False):
+        """hh_validate_and_replace_noalloc on device pointers (ints); enqueued
+        on the context stream, returns without waiting."""
+        st = p.to_struct() if hasattr(p, "to_struct") else p
+        vp = lambda x: None if x is None else C.c_void_p(x)
+        _check(self.h, load().hh_validate_and_replace_noalloc(self.h, C.byref(st), ny, nx, 1 if force_replace else 0,
+                                                               vp(d_u), vp(d_v), vp(d_pr), vp(d_cm), vp(d_alt_u),
+                                                               vp(d_alt_v), vp(d_mask), vp(d_outliers)),
+               "validate_and_replace_noalloc")
 
     def smooth_field(self, f):
         f = _f32(f)

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -49,6 +50,38 @@ hipEvent_t get_event(hh_ctx *c)
 namespace {
 
 thread_local std::string g_create_err;
+
+// ---- scratch of hh_validate_and_replace_noalloc ---------------------------
+// Device memory the validation chain needs beyond its inputs and outputs:
+// the valid-cell counter (256 bytes), the substituted / replaced u and v
+// (float each) and flags (one byte) of up to HH_NOALLOC_MAX_CELLS cells. One
+// copy per GPU for the life of the process; in host code `scratch` only
+// names the symbol, its device address comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocCells = HH_NOALLOC_MAX_CELLS;
+constexpr size_t kScratchCount = 256;
+constexpr size_t kScratchBytes = kScratchCount + kNoAllocCells * (2 * sizeof(float) + sizeof(uint8_t));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char *> g_scratch_ptr;  // per device, resolved once
+
+// The device address of this GPU's scratch (the calling thread's current
+// device must be `device`); resolved on the first call per device.
+unsigned char *scratch_on(int device, std::string &err)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("scratch symbol lookup failed: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 void drain_timing(hh_ctx *c)
 {
@@ -1076,6 +1109,14 @@ int hh_create(hh_ctx **out, int device)
         delete c;
         return HH_EDEVICE;
     }
+    // this device's no-allocation scratch, resolved at set-up with the
+    // device current
+    c->scratch = scratch_on(device, g_create_err);
+    if (!c->scratch) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return HH_EDEVICE;
+    }
     *out = c;
     return HH_OK;
 }
@@ -1406,6 +1447,44 @@ int hh_validate_and_replace(hh_ctx *c, const hh_pass *pass, int ny, int nx, int 
         download(c, v, va.v, ng);
         download(c, outliers, va.outliers, ng);
         HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_validate_and_replace_noalloc(hh_ctx *c, const hh_pass *pass, int ny, int nx, int force_replace, float *u,
+                                    float *v, const float *peak_ratio, const float *correlation_moment,
+                                    const float *alt_u, const float *alt_v, const uint8_t *mask, uint8_t *outliers)
+{
+    return guarded(c, [&] {
+        if (!pass || !u || !v || !peak_ratio || !correlation_moment || !outliers)
+            throw Error(HH_EINVAL, "null argument");
+        check_pass_valid(*pass);
+        if (pass->uod_neighborhood > 3) throw Error(HH_EUNSUPPORTED, "uod_neighborhood up to 3 on the device");
+        if (ny < 1 || nx < 1) throw Error(HH_EDIM, "empty grid");
+        if ((size_t)ny * nx > kNoAllocCells)
+            throw Error(HH_EINVAL, "grid of " + std::to_string((size_t)ny * nx) + " cells exceeds the " +
+                                       std::to_string(kNoAllocCells) + "-cell scratch of the no-allocation entry point");
+        if (!c->scratch) throw Error(HH_ESTATE, "no scratch on this device");
+        ValidateArgs va{};
+        va.grids = one_grid(ny, nx);
+        va.pass = pass;
+        va.force_replace = force_replace;
+        va.u = u;
+        va.v = v;
+        va.pr = peak_ratio;
+        va.cm = correlation_moment;
+        if (alt_u && alt_v && pass->n_peaks > 1) {
+            va.alt_u = alt_u;
+            va.alt_v = alt_v;
+        }
+        va.outliers = outliers;
+        // the counter is cleared by the chain's first kernel; the other
+        // scratch cells are written before they are read
+        va.count = reinterpret_cast<int *>(c->scratch);
+        va.su = reinterpret_cast<float *>(c->scratch + kScratchCount);
+        va.sv = va.su + kNoAllocCells;
+        va.sflag = reinterpret_cast<uint8_t *>(va.sv + kNoAllocCells);
+        va.mask = mask;
+        launch_validate(va, c->stream);
     });
 }
 

@@ -187,6 +187,8 @@ struct hh_ctx {
     int device = 0;
     int ncu = 256;
     hipStream_t stream = nullptr;
+    // device address of this GPU's no-allocation scratch (hh_create)
+    unsigned char *scratch = nullptr;
     std::string err;
     int form = HH_FORM_FFT32;
     int nr = 0, nc = 0;

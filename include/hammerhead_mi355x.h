@@ -208,6 +208,22 @@ int hh_validate_and_replace(hh_ctx *ctx, const hh_pass *pass, int ny, int nx,
                             const float *peak_ratio, const float *correlation_moment,
                             const float *alt_u, const float *alt_v,
                             const uint8_t *mask, uint8_t *outliers);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). hh_validate_and_replace on DEVICE-resident grids: every
+ * pointer is device memory (u, v in/out; peak_ratio, correlation_moment in;
+ * alt_u/alt_v nullable, ny x nx x (n_peaks-1); mask nullable, one byte per
+ * cell; outliers out), the work is enqueued on the context stream
+ * (hh_stream) and the call returns without waiting. Its scratch (a counter
+ * and the substituted/replaced u, v and flags) is a fixed device array of the
+ * library, one per GPU, for grids of at most HH_NOALLOC_MAX_CELLS cells
+ * (larger grids: HH_EINVAL); it allocates nothing. Calls that overlap on one
+ * GPU (from several contexts or streams) share that array and race; order
+ * them on one stream. */
+#define HH_NOALLOC_MAX_CELLS (1 << 20)
+int hh_validate_and_replace_noalloc(hh_ctx *ctx, const hh_pass *pass, int ny, int nx, int force_replace, float *u,
+                                    float *v, const float *peak_ratio, const float *correlation_moment,
+                                    const float *alt_u, const float *alt_v, const uint8_t *mask, uint8_t *outliers);
 /* smooth_field (src/quality.jl:123-136). */
 int hh_smooth_field(hh_ctx *ctx, int ny, int nx, const float *f, float *out);
 
