@@ -28,7 +28,7 @@ EXPORTS = (
     "hh_validate_and_replace_noalloc",
     "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
-    "hh_refine_nodes", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
+    "hh_refine_nodes", "hh_set_energy_sampling", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
     "hh_set_autotune", "hh_tune_report", "hh_tune_set", "hh_set_mask",
 )
@@ -91,6 +91,7 @@ def load():
         "hh_energy": (i32, [vp, pp, vp, i32, vp, i32, vp, vp, C.POINTER(C.c_double)]),
         "hh_grad_energy": (i32, [vp, pp, vp, i32, vp, i32, vp, vp, C.POINTER(C.c_double), vp, vp]),
         "hh_refine_nodes": (i32, [vp, pp, vp, i32, vp, i32, vp, vp, i32, vp]),
+        "hh_set_energy_sampling": (i32, [vp, i32]),
         "hh_stream": (vp, [vp]),
         "hh_synchronize": (i32, [vp]),
         "hh_set_timing": (i32, [vp, i32]),
@@ -498,6 +499,12 @@ class Context:
         if pu.shape != (py.size, px.size) or pv.shape != pu.shape:
             raise DimensionMismatch(f"node field must be {(py.size, px.size)}, got {pu.shape} and {pv.shape}")
         return py, px, pu, pv
+
+    def set_energy_sampling(self, sampling: str):
+        """Image sampling of the correlation energy: "cubic" (the
+        deformation path's interpolant) or "bilinear"."""
+        code = {"cubic": 0, "bilinear": 1}[sampling]
+        _check(self.h, load().hh_set_energy_sampling(self.h, code), "set_energy_sampling")
 
     def energy(self, p, py, px, pu, pv):
         st = p.to_struct() if hasattr(p, "to_struct") else p

@@ -914,13 +914,16 @@ EnergySetup energy_setup(hh_ctx *c, const hh_pass &p, const float *py, int gny, 
         throw Error(HH_EUNSUPPORTED, "the correlation-energy adjoint supports 8, 16 and 32 px windows");
     if (gny < 1 || gnx < 1 || !py || !px) throw Error(HH_EINVAL, "empty node grid");
     check_knots(py, gny, px, gnx);
-    if (!c->coefs_valid) do_prefilter(c);
+    if (!c->energy_bilinear && !c->coefs_valid) do_prefilter(c);
     PassGrid grid = make_grid(p, c->nr, c->nc);
     if (grid.ny * grid.nx == 0) throw Error(HH_EINVAL, "the pass grid is empty");
     EnergySetup s;
     EnergyArgs &g = s.g;
-    g.cA = c->coefA.p;
-    g.cB = c->coefB.p;
+    g.bilinear = c->energy_bilinear ? 1 : 0;
+    g.cA = c->energy_bilinear ? nullptr : c->coefA.p;
+    g.cB = c->energy_bilinear ? nullptr : c->coefB.p;
+    g.A = c->A;
+    g.B = c->B;
     g.nr = c->nr;
     g.nc = c->nc;
     g.gny = gny;
@@ -1720,6 +1723,15 @@ int hh_run_piv_ensemble(hh_ctx *c, const float *const *A, const float *const *B,
         if (!passes) throw Error(HH_EINVAL, "null pass schedule");
         run_ensemble(c, A, B, npairs, nr, nc, passes, npasses, predictor_smoothing);
         if (result) fetch(c, result);
+    });
+}
+
+int hh_set_energy_sampling(hh_ctx *c, int sampling)
+{
+    return guarded(c, [&] {
+        if (sampling != HH_SAMPLE_CUBIC && sampling != HH_SAMPLE_BILINEAR)
+            throw Error(HH_EINVAL, "sampling must be HH_SAMPLE_CUBIC or HH_SAMPLE_BILINEAR");
+        c->energy_bilinear = sampling == HH_SAMPLE_BILINEAR;
     });
 }
 

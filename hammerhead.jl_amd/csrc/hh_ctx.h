@@ -189,6 +189,8 @@ struct hh_ctx {
     hipStream_t stream = nullptr;
     // device address of this GPU's no-allocation scratch (hh_create)
     unsigned char *scratch = nullptr;
+    // image sampling of the correlation energy (hh_set_energy_sampling)
+    bool energy_bilinear = false;
     std::string err;
     int form = HH_FORM_FFT32;
     int nr = 0, nc = 0;

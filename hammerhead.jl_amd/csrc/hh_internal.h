@@ -254,7 +254,9 @@ void launch_uq_finalize(const double *stats, const int32_t *jobs, int njobs, int
 
 // ---- correlation energy and its adjoint (k_adjoint.hip) ---------------------
 struct EnergyArgs {
-    const float *cA, *cB;  // prefiltered pair, (nr+2) x (nc+2) each
+    const float *cA, *cB;  // prefiltered pair, (nr+2) x (nc+2) each (cubic sampling)
+    const float *A, *B;    // the pair itself, nr x nc each (bilinear sampling)
+    int bilinear;          // image sampling: 0 cubic B-spline, 1 bilinear
     int nr, nc;
     const float *pu, *pv;  // node field gny x gnx on uniform knots
     int gny, gnx;

@@ -134,6 +134,26 @@ __device__ __forceinline__ float cubic_at(const float *__restrict__ C, int nr, i
     return s / 36.f;
 }
 
+// Bilinear sample of an nr x nc image at (r + oy, c + ox) and its partial
+// derivatives (sy along dim 1, sx along dim 2), positions split as in
+// cubic_at; zero with zero slope outside [1, nr] x [1, nc]. The slope is that
+// of the interval the sample's base index selects (base clamped to n - 1).
+__device__ __forceinline__ float bilinear_d_at(const float *__restrict__ I, int nr, int nc, int r, float oy, int c,
+                                               float ox, float &sy, float &sx)
+{
+    int iy, ix;
+    float fy, fx;
+    if (!split_axis(r, oy, nr, iy, fy) || !split_axis(c, ox, nc, ix, fx)) {
+        sy = sx = 0.f;
+        return 0.f;
+    }
+    const float *q = I + (size_t)(iy - 1) + (size_t)(ix - 1) * nr;
+    const float a = q[0], b = q[nr], cc = q[1], d = q[nr + 1];  // (iy,ix) (iy,ix+1) (iy+1,ix) (iy+1,ix+1)
+    sy = (1.f - fx) * (cc - a) + fx * (d - b);
+    sx = (1.f - fy) * (b - a) + fy * (d - cc);
+    return (1.f - fy) * ((1.f - fx) * a + fx * b) + fy * ((1.f - fx) * cc + fx * d);
+}
+
 // Derivative of the cubic B-spline weights of cubw with respect to d.
 __device__ __forceinline__ void cubw_d(float d, float g[4])
 {
@@ -143,6 +163,11 @@ __device__ __forceinline__ void cubw_d(float d, float g[4])
     g[2] = 0.5f + d - 1.5f * d * d;
     g[3] = d * d / 2.f;
 }
+
+// cubic_d at (r + oy, c + ox) for integer r, c (positions split as in
+// cubic_at).
+__device__ __forceinline__ float cubic_d_at(const float *__restrict__ C, int nr, int nc, int r, float oy, int c,
+                                            float ox, float &sy, float &sx);
 
 // cubic() and its partial derivatives (sy along dim 1, sx along dim 2);
 // zero with zero slope outside [1, nr] x [1, nc].
@@ -162,6 +187,37 @@ __device__ __forceinline__ float cubic_d(const float *__restrict__ C, int nr, in
     cubw(xx - (float)ix, wx);
     cubw_d(yy - (float)iy, gy);
     cubw_d(xx - (float)ix, gx);
+    const int N1 = nr + 2;
+    float s = 0.f, s_y = 0.f, s_x = 0.f;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        const float c0 = col[0], c1 = col[1], c2 = col[2], c3 = col[3];
+        const float cv = wy[0] * c0 + wy[1] * c1 + wy[2] * c2 + wy[3] * c3;
+        const float cd = gy[0] * c0 + gy[1] * c1 + gy[2] * c2 + gy[3] * c3;
+        s += wx[n] * cv;
+        s_y += wx[n] * cd;
+        s_x += gx[n] * cv;
+    }
+    sy = s_y;
+    sx = s_x;
+    return s;
+}
+
+__device__ __forceinline__ float cubic_d_at(const float *__restrict__ C, int nr, int nc, int r, float oy, int c,
+                                            float ox, float &sy, float &sx)
+{
+    int iy, ix;
+    float dy, dx;
+    if (!split_axis(r, oy, nr, iy, dy) || !split_axis(c, ox, nc, ix, dx)) {
+        sy = sx = 0.f;
+        return 0.f;
+    }
+    float wy[4], wx[4], gy[4], gx[4];
+    cubw(dy, wy);
+    cubw(dx, wx);
+    cubw_d(dy, gy);
+    cubw_d(dx, gx);
     const int N1 = nr + 2;
     float s = 0.f, s_y = 0.f, s_x = 0.f;
 #pragma unroll

@@ -7,8 +7,9 @@
 // (apply_predictor / deform_images, src/pipeline.jl:575-600,936-965): pixel
 // displacements are the bilinear Flat() values of the node field (pu, pv),
 // A^(r,c) = A(r - v/2, c - u/2) and B^(r,c) = B(r + v/2, c + u/2) are
-// sampled from the cubic B-spline interpolants, and over the windows of a
-// pass (pass_grid, apodization p)
+// sampled from the cubic B-spline interpolants or bilinearly from the images
+// (zero outside [1, n] either way), and over the windows of a pass
+// (pass_grid, apodization p)
 //     E = - sum_w sum_{x in w} p_x (A^_x - mA_w) (B^_x - mB_w).
 // With N = w^2, PA_w = sum p (A^ - mA), PB_w = sum p (B^ - mB):
 //     dE/dA^_x = sum_{w contains x} (PB_w / N - p_x (B^_x - mB_w))
@@ -42,6 +43,16 @@ __device__ __forceinline__ void node_uv(const EnergyArgs &g, float r, float c, f
     v = bilinear(g.pv, g.gny, g.gnx, g.y0, g.ysp, g.x0, g.xsp, r, c);
 }
 
+// The warped samples A^(r, c) (s = -1) or B^(r, c) (s = +1) and their
+// slopes, by the configured sampling (g.bilinear is uniform per launch).
+__device__ __forceinline__ float warp_sample(const EnergyArgs &g, bool frame_b, int r, int c, float u, float v,
+                                             float &sy, float &sx)
+{
+    const float h = frame_b ? 0.5f : -0.5f;
+    if (g.bilinear) return bilinear_d_at(frame_b ? g.B : g.A, g.nr, g.nc, r, h * v, c, h * u, sy, sx);
+    return cubic_d_at(frame_b ? g.cB : g.cA, g.nr, g.nc, r, h * v, c, h * u, sy, sx);
+}
+
 // One wave per window: recompute the warped window, then its statistics.
 template <int W>
 __global__ __launch_bounds__(64) void k_energy_windows(EnergyArgs g, float4 *__restrict__ stats,
@@ -59,10 +70,10 @@ __global__ __launch_bounds__(64) void k_energy_windows(EnergyArgs g, float4 *__r
     for (int q = 0; q < PPL; ++q) {
         const int e = lane + 64 * q;
         const float r = (float)(rs + e % W), c = (float)(cs + e / W);
-        float u, v;
+        float u, v, sy, sx;
         node_uv(g, r, c, u, v);
-        a[q] = cubic(g.cA, g.nr, g.nc, r - v / 2.f, c - u / 2.f);
-        b[q] = cubic(g.cB, g.nr, g.nc, r + v / 2.f, c + u / 2.f);
+        a[q] = warp_sample(g, false, rs + e % W, cs + e / W, u, v, sy, sx);
+        b[q] = warp_sample(g, true, rs + e % W, cs + e / W, u, v, sy, sx);
         sa += a[q];
         sb += b[q];
     }
@@ -110,8 +121,8 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(EnergyArgs g, const floa
         float u, v;
         node_uv(g, (float)r, (float)c, u, v);
         float ay, ax, by, bx;
-        const float ah = cubic_d(g.cA, g.nr, g.nc, (float)r - v / 2.f, (float)c - u / 2.f, ay, ax);
-        const float bh = cubic_d(g.cB, g.nr, g.nc, (float)r + v / 2.f, (float)c + u / 2.f, by, bx);
+        const float ah = warp_sample(g, false, r, c, u, v, ay, ax);
+        const float bh = warp_sample(g, true, r, c, u, v, by, bx);
         // windows containing (r, c): starts r0 + k st within [r - w + 1, r]
         const int kilo = max(0, (r - g.w + 1 - g.r0 + g.st_r - 1) / g.st_r);
         const int kihi = min(g.ny - 1, (r - g.r0) / g.st_r);

@@ -343,12 +343,19 @@ int hh_run_piv_ensemble(hh_ctx *ctx, const float *const *A, const float *const *
  * node field pu, pv (gny x gnx, column-major) on uniform knots py, px, the
  * current pair is warped symmetrically as in deform_images
  * (src/pipeline.jl:936-965: bilinear Flat() node values, cubic B-spline
- * samples A(r - v/2, c - u/2), B(r + v/2, c + u/2)), and over the windows of
+ * samples A(r - v/2, c - u/2), B(r + v/2, c + u/2); bilinear samples of the
+ * pair instead after hh_set_energy_sampling(HH_SAMPLE_BILINEAR)), and over the windows of
  * `pass` (square, 8/16/32 px; grid and apodization as pass_grid)
  *     E = - sum_w sum_{x in w} p_x (A^_x - mean_w A^) (B^_x - mean_w B^).
  * The prefiltered pair is computed on first use (hh_prefilter). */
 int hh_energy(hh_ctx *ctx, const hh_pass *pass, const float *py, int gny, const float *px, int gnx,
               const float *pu, const float *pv, double *energy);
+/* Image sampling of the warped windows of hh_energy, hh_grad_energy and
+ * hh_refine_nodes: HH_SAMPLE_CUBIC (default; the cubic B-spline interpolant
+ * of the deformation path) or HH_SAMPLE_BILINEAR (bilinear sampling of the
+ * pair itself, zero outside [1, n]). No counterpart in the reference. */
+enum { HH_SAMPLE_CUBIC = 0, HH_SAMPLE_BILINEAR = 1 };
+int hh_set_energy_sampling(hh_ctx *ctx, int sampling);
 /* E and its gradient dE/dpu, dE/dpv (gny x gnx) by the hand-written adjoint,
  * which recomputes the warped windows instead of storing them. energy is
  * nullable. */
