@@ -325,6 +325,39 @@ def time_ensemble(hh, npairs, steps, log):
             "rms_px_vs_truth": round(rms, 4), "outliers": int(res["outliers"].sum())}
 
 
+def time_dewarp(hh, steps, log):
+    """The stereo dewarp kernel (dewarp!, src/dewarp.jl:233-248): both
+    2048 x 2048 frames of one camera's pair resampled onto a 2048 x 2048
+    world grid at a smooth projective 16.16 fixed-point map, HIP events around
+    the dewarp launches only. Algorithmic bytes per launch: the map (8 B per
+    node), both outputs (8 B per node) and both frames' coefficients read
+    once (8 B per (nr + 2)(nc + 2))."""
+    n = 2048
+    i = np.arange(1, n + 1, dtype=np.float64)[:, None]
+    j = np.arange(1, n + 1, dtype=np.float64)[None, :]
+    rows = 8.0 + 0.985 * i + 0.004 * j + 2e-6 * i * j
+    cols = 6.0 + 0.003 * i + 0.99 * j - 1e-6 * j * j
+    pair = hh.synthetic.particle_pair(n, n, hh.synthetic.uniform_flow(1.2, -0.6), 107)
+    ctx = hh.Context(0)
+    ctx.set_dewarp_map(rows, cols, None, (n, n))
+    ctx.dewarp_pair(pair.A, pair.B)
+    ctx.synchronize()
+    ctx.set_timing(True, only=("dewarp", 0))
+    for _ in range(steps):
+        ctx.dewarp_pair(pair.A, pair.B)
+    ctx.synchronize()
+    ms, k = ctx.kernel_time("dewarp", 0)
+    ctx.close()
+    avg = ms / k
+    nbytes = n * n * (8 + 8) + 2 * 4 * (n + 2) * (n + 2)
+    gbs = nbytes / (avg * 1e-3) / 1e9
+    log(f"dewarp: {avg * 1e3:.1f} us per pair launch, {gbs:.0f} GB/s")
+    return {"workload": "2 x 2048x2048 frames onto a 2048x2048 grid (projective map)", "avg_launch_ms": round(avg, 5),
+            "launches": k, "algorithmic_bytes_per_launch": nbytes,
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(gbs / PEAK_HBM_GBS, 4)}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +368,7 @@ def main(argv=None):
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-end-to-end", action="store_true", help="skip the PCIe-inclusive rate")
+    ap.add_argument("--no-dewarp", action="store_true", help="skip the stereo dewarp kernel")
     ap.add_argument("--batch-pairs", type=int, default=64, help="pairs of the batching workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--ensemble-pairs", type=int, default=500, help="pairs of the ensemble workload (0: skip)")
@@ -450,6 +484,8 @@ def main(argv=None):
         line["batch"] = time_batch(hh, passes, forms[args.form], args.batch_pairs, args.batch_steps, log)
     if args.ensemble_pairs > 0:
         line["ensemble"] = time_ensemble(hh, args.ensemble_pairs, 3, log)
+    if not args.no_dewarp:
+        line["dewarp"] = time_dewarp(hh, 10, log)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"], o64 = cpu_leg(hh, pair, passes, args.cpu_threads, log)

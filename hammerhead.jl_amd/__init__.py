@@ -12,10 +12,12 @@ from . import synthetic
 from ._lib import Context, DeviceError, HH_FORM_FFT32, HH_FORM_MFMA16, HH_SRC_RAW, HH_SRC_WARP, grid_size
 from .pipeline import (PIVResult, PIVWorkspace, pass_grid, piv_workspace, run_piv, run_piv_ensemble,
                        run_piv_sequence)
+from .stereo import ImageDewarper, StereoCameraResults, dewarp, run_piv_stereo
 
 __all__ = [
     "ArgumentError", "DimensionMismatch", "PIVParameters", "multipass_parameters", "run_piv", "run_piv_ensemble",
     "run_piv_sequence",
     "PIVResult",
     "PIVWorkspace", "piv_workspace", "pass_grid", "Context", "DeviceError", "synthetic",
+    "ImageDewarper", "StereoCameraResults", "dewarp", "run_piv_stereo",
 ]

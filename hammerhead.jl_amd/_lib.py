@@ -30,7 +30,8 @@ EXPORTS = (
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_set_energy_sampling", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
-    "hh_set_autotune", "hh_tune_report", "hh_tune_set", "hh_set_mask",
+    "hh_set_autotune", "hh_tune_report", "hh_tune_set", "hh_set_mask", "hh_set_dewarp_map", "hh_dewarp",
+    "hh_dewarp_pair",
 )
 
 
@@ -107,6 +108,9 @@ def load():
         "hh_tune_report": (i32, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "hh_tune_set": (i32, [vp, C.c_char_p, i32, i32]),
         "hh_set_mask": (i32, [vp, vp, i32, i32, C.c_double]),
+        "hh_set_dewarp_map": (i32, [vp, vp, vp, vp, i32, i32, i32, i32]),
+        "hh_dewarp": (i32, [vp, vp, i32, i32, vp]),
+        "hh_dewarp_pair": (i32, [vp, vp, vp, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -202,6 +206,34 @@ class Context:
         m = np.asfortranarray(np.asarray(mask, dtype=bool), dtype=np.uint8)
         _check(self.h, load().hh_set_mask(self.h, _ptr(m), m.shape[0], m.shape[1], float(mask_threshold)),
                "set_mask")
+
+    # ---- dewarping ---------------------------------------------------------
+    def set_dewarp_map(self, rows, cols, mask=None, image_size=None):
+        """The precomputed fields of an ImageDewarper: rows, cols (ny x nx,
+        1-based source coordinates), out-of-view mask, camera frame size."""
+        rows = np.asfortranarray(rows, dtype=np.float64)
+        cols = np.asfortranarray(cols, dtype=np.float64)
+        if rows.shape != cols.shape:
+            raise DimensionMismatch(f"rows {rows.shape} and cols {cols.shape} differ")
+        m = None if mask is None else np.asfortranarray(np.asarray(mask, dtype=bool), dtype=np.uint8)
+        nr, nc = image_size
+        self.dw_shape = rows.shape
+        _check(self.h, load().hh_set_dewarp_map(self.h, _ptr(rows), _ptr(cols), _ptr(m), rows.shape[0],
+                                                 rows.shape[1], nr, nc), "set_dewarp_map")
+
+    def dewarp(self, img):
+        img = _f32(img)
+        out = np.zeros(self.dw_shape, dtype=np.float32, order="F")
+        _check(self.h, load().hh_dewarp(self.h, _ptr(img), img.shape[0], img.shape[1], _ptr(out)), "dewarp")
+        return out
+
+    def dewarp_pair(self, A, B):
+        """Dewarp both frames; the result becomes the current (device) pair."""
+        A, B = _f32(A), _f32(B)
+        if A.shape != B.shape:
+            raise DimensionMismatch(f"images must have the same size, got {A.shape} and {B.shape}")
+        _check(self.h, load().hh_dewarp_pair(self.h, _ptr(A), _ptr(B), A.shape[0], A.shape[1]), "dewarp_pair")
+        self.nr, self.nc = self.dw_shape
 
     def prefilter(self):
         _check(self.h, load().hh_prefilter(self.h), "prefilter")

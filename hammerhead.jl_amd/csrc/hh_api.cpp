@@ -1213,6 +1213,94 @@ int hh_set_pair_device(hh_ctx *c, const float *dA, const float *dB, int nr, int 
     });
 }
 
+int hh_set_dewarp_map(hh_ctx *c, const double *rows, const double *cols, const uint8_t *mask, int ny, int nx, int nr,
+                      int nc)
+{
+    return guarded(c, [&] {
+        if (!rows || !cols) throw Error(HH_EINVAL, "null coordinate map");
+        if (ny < 1 || nx < 1) throw Error(HH_EDIM, "empty dewarp grid");
+        if (nr < 2 || nc < 2 || nr > 32767 || nc > 32767)
+            throw Error(HH_EDIM, "camera frames must be 2 to 32767 pixels per side");
+        const size_t n = (size_t)ny * nx;
+        if (n >= (size_t)1 << 31) throw Error(HH_EDIM, "dewarp grid too large");
+        std::vector<int2> m(n);
+        for (size_t e = 0; e < n; ++e) {
+            const double r = rows[e], q = cols[e];
+            // ImageDewarper's view test (src/dewarp.jl:205-215): finite and
+            // within [1, nr] x [1, nc]; the mask marks the others too
+            const bool in = !(mask && mask[e]) && std::isfinite(r) && std::isfinite(q) && r >= 1.0 &&
+                            r <= (double)nr && q >= 1.0 && q <= (double)nc;
+            m[e] = in ? make_int2((int)std::llround(r * 65536.0), (int)std::llround(q * 65536.0))
+                      : make_int2(INT32_MIN, INT32_MIN);
+        }
+        c->dwmap.ensure(n);
+        HH_HIP(hipMemcpyAsync(c->dwmap.p, m.data(), sizeof(int2) * n, hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipStreamSynchronize(c->stream));
+        c->dw_ny = ny;
+        c->dw_nx = nx;
+        c->dw_nr = nr;
+        c->dw_nc = nc;
+    });
+}
+
+namespace {
+
+// Upload nframes host frames of the dewarper's camera size, prefilter them
+// on the device and resample them onto the grid into oA (and oB).
+void dewarp_frames(hh_ctx *c, const float *A, const float *B, int nr, int nc, float *oA, float *oB)
+{
+    if (c->dw_ny == 0) throw Error(HH_ESTATE, "no dewarp map: call hh_set_dewarp_map first");
+    if (nr != c->dw_nr || nc != c->dw_nc) throw Error(HH_EDIM, "frame size does not match the dewarp map's camera");
+    const size_t npx = (size_t)nr * nc, ncf = (size_t)(nr + 2) * (nc + 2);
+    float *raw = c->dwraw.ensure(2 * npx);
+    float *coef = c->dwcoef.ensure(2 * ncf);
+    float *tmp = c->dwtmp.ensure((size_t)2 * (nr + 2) * nc);
+    HH_HIP(hipMemcpyAsync(raw, A, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
+    if (B) HH_HIP(hipMemcpyAsync(raw + npx, B, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
+    timed(c, "prefilter", 0, [&] {
+        launch_prefilter_pair(raw, B ? raw + npx : raw, nr, nr, nc, coef, coef + ncf, tmp, c->stream);
+    });
+    timed(c, "dewarp", 0, [&] {
+        launch_dewarp(c->dwmap.p, c->dw_ny, c->dw_nx, coef, B ? coef + ncf : nullptr, nr, nc, oA, B ? oB : nullptr,
+                      c->stream);
+    });
+}
+
+}  // namespace
+
+int hh_dewarp(hh_ctx *c, const float *img, int nr, int nc, float *out)
+{
+    return guarded(c, [&] {
+        if (!img || !out) throw Error(HH_EINVAL, "null argument");
+        const size_t n = (size_t)c->dw_ny * c->dw_nx;
+        float *o = c->dwout.ensure(n > 0 ? n : 1);
+        dewarp_frames(c, img, nullptr, nr, nc, o, nullptr);
+        HH_HIP(hipMemcpyAsync(out, o, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        HH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hh_dewarp_pair(hh_ctx *c, const float *A, const float *B, int nr, int nc)
+{
+    return guarded(c, [&] {
+        if (!A || !B) throw Error(HH_EINVAL, "null image");
+        const int ny = c->dw_ny, nx = c->dw_nx;
+        if (ny > 0 && (ny < 2 || nx < 2)) throw Error(HH_EDIM, "the dewarped pair must be at least 2 x 2");
+        const size_t n = (size_t)ny * nx;
+        c->ownA.ensure(n > 0 ? n : 1);
+        c->ownB.ensure(n > 0 ? n : 1);
+        dewarp_frames(c, A, B, nr, nc, c->ownA.p, c->ownB.p);
+        HH_HIP(hipStreamSynchronize(c->stream));
+        c->A = c->ownA.p;
+        c->B = c->ownB.p;
+        if (ny != c->nr || nx != c->nc) c->jobcache.clear();
+        c->nr = ny;
+        c->nc = nx;
+        c->coefs_valid = c->warp_valid = c->have_result = false;
+        c->eng.have_result = false;
+    });
+}
+
 int hh_prefilter(hh_ctx *c)
 {
     return guarded(c, [&] { do_prefilter(c); });

@@ -189,6 +189,12 @@ struct hh_ctx {
     hipStream_t stream = nullptr;
     // device address of this GPU's no-allocation scratch (hh_create)
     unsigned char *scratch = nullptr;
+    // dewarping (hh_set_dewarp_map): 16.16 fixed-point source map of the
+    // dw_ny x dw_nx world grid into dw_nr x dw_nc camera frames; raw frames,
+    // their coefficients and the prefilter scratch
+    hh::host::DBuf<int2> dwmap;
+    int dw_ny = 0, dw_nx = 0, dw_nr = 0, dw_nc = 0;
+    hh::host::DBuf<float> dwraw, dwcoef, dwtmp, dwout;
     // image sampling of the correlation energy (hh_set_energy_sampling)
     bool energy_bilinear = false;
     std::string err;

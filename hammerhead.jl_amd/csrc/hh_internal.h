@@ -271,4 +271,9 @@ void launch_energy(const EnergyArgs &g, float4 *stats, float *ew, double *esum, 
 void launch_energy_adjoint(const EnergyArgs &g, const float4 *stats, const int2 *rows, const int2 *cols, int ncy,
                            int ncx, float *part, float *gu, float *gv, hipStream_t s);
 
+// dewarp! of one or two prefiltered frames (cA/cB: (nr+2) x (nc+2)) at the
+// 16.16 fixed-point source map (ny x nx int2, INT32_MIN = out of view) into
+// oA/oB (ny x nx); cB/oB nullable.
+void launch_dewarp(const int2 *map, int ny, int nx, const float *cA, const float *cB, int nr, int nc, float *oA,
+                   float *oB, hipStream_t s);
 }  // namespace hh

@@ -210,3 +210,27 @@ def rms_error(result, flow, midpoint: bool = True) -> float:
         ok &= ~np.asarray(result["mask"], bool)
     e2 = (np.asarray(result["u"], float) - tu) ** 2 + (np.asarray(result["v"], float) - tv) ** 2
     return float(np.sqrt(np.mean(e2[ok])))
+
+
+def stereo_maps(ny: int = 256, nx: int = 320, image_size=(300, 360)):
+    """Two cameras' dewarp maps onto one ny x nx world grid: smooth,
+    mildly projective source coordinates (1-based camera row, col per grid
+    node), each camera leaving part of the grid out of view (the first
+    camera's map runs past its frame's last row, the second's past its last
+    column)."""
+    i = np.arange(1, ny + 1, dtype=np.float64)[:, None]
+    j = np.arange(1, nx + 1, dtype=np.float64)[None, :]
+    r1 = 18.0 + 1.08 * i + 0.015 * j + 6e-5 * i * j
+    c1 = 12.0 + 0.01 * i + 1.03 * j - 4e-5 * j * j
+    r2 = 22.0 + 0.99 * i - 0.02 * j - 5e-5 * i * j
+    c2 = 30.0 + 0.012 * i + 1.04 * j + 3e-5 * j * j
+    return (np.asfortranarray(r1), np.asfortranarray(c1)), (np.asfortranarray(r2), np.asfortranarray(c2)), image_size
+
+
+def stereo_frames(image_size=(300, 360), seed: int = 106):
+    """Camera frames of a stereo pair: each camera's pair is a particle image
+    pair of image_size with its own uniform shift (camera pixels)."""
+    nr, nc = image_size
+    p1 = particle_pair(nr, nc, uniform_flow(1.6, -0.7), seed, ppp=0.03, noise=0.02)
+    p2 = particle_pair(nr, nc, uniform_flow(1.1, -0.9), seed + 1, ppp=0.03, noise=0.02)
+    return (p1.A, p1.B), (p2.A, p2.B)

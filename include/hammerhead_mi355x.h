@@ -150,6 +150,30 @@ int hh_set_pair(hh_ctx *ctx, const float *A, const float *B, int nr, int nc);
  * nr x nc). The pointers must stay valid while ctx uses them. */
 int hh_set_pair_device(hh_ctx *ctx, const float *dA, const float *dB, int nr, int nc);
 
+/* ---- dewarping (stereo) ------------------------------------------------ */
+
+/* ImageDewarper(cam, grid, image_size) (src/dewarp.jl:196-221) as its
+ * precomputed fields: for every node of the ny x nx world grid the 1-based
+ * source row and column in the camera frame (rows, cols: column-major
+ * float64, as the reference stores them) and the out-of-view mask (nullable;
+ * nonzero = out of view). Nodes that are masked, non-finite or outside
+ * [1, nr] x [1, nc] are out of view. nr x nc: the camera's frame size (at
+ * most 32767 per side); the map is kept on the device with each coordinate
+ * resolved to 2^-16 px. */
+int hh_set_dewarp_map(hh_ctx *ctx, const double *rows, const double *cols, const uint8_t *mask, int ny, int nx,
+                      int nr, int nc);
+/* dewarp!(out, dw, img) (src/dewarp.jl:233-248) on the device: the frame
+ * (host column-major float32, nr x nc = the map's camera size, HH_EDIM
+ * otherwise) is prefiltered (cubic B-spline, Line(OnGrid())) and sampled at
+ * the map; out (ny x nx host) holds 0 at out-of-view nodes. */
+int hh_dewarp(hh_ctx *ctx, const float *img, int nr, int nc, float *out);
+/* Both frames of one camera's pair dewarped as hh_dewarp does, the result
+ * kept on the device as the context's current pair (ny x nx), as
+ * _run_piv_stereo! (src/stereo.jl:186-192) dewarps before each camera's
+ * run_piv: a following hh_run_piv (or the hooks) correlates it without a
+ * host round trip. */
+int hh_dewarp_pair(hh_ctx *ctx, const float *A, const float *B, int nr, int nc);
+
 /* ---- deformation ------------------------------------------------------- */
 
 /* Cubic B-spline prefilter of both images on the device: the coefficients of

@@ -247,6 +247,20 @@ def deform(coefA, coefB, nr, nc, py, px, pu, pv, nthreads=8, prec="f64"):
     return wA, wB
 
 
+def dewarp(img, rows, cols, mask=None, prec="f64"):
+    """dewarp!(out, dw, img): the frame resampled at the dewarper's source
+    coordinates (ny x nx, 1-based); masked nodes 0."""
+    dt = _dt(prec)
+    img = _f(img, dt)
+    rows = np.asfortranarray(rows, dtype=np.float64)
+    cols = np.asfortranarray(cols, dtype=np.float64)
+    ny, nx = rows.shape
+    out = np.zeros((ny, nx), dtype=dt, order="F")
+    _fn("orc_dewarp", prec)(_p(img), C.c_int(img.shape[0]), C.c_int(img.shape[1]), _p(rows), _p(cols),
+                            _p(_mask(mask)), C.c_int(ny), C.c_int(nx), _p(out))
+    return out
+
+
 def node_values(py, px, pu, pv, y, x, prec="f64"):
     dt = _dt(prec)
     py, px, y, x = (np.asarray(a, dtype=dt) for a in (py, px, y, x))

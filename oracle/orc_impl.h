@@ -1098,6 +1098,21 @@ REAL F(orc_cubic)(const REAL *C, int nr, int nc, REAL yy, REAL xx)
     return s;
 }
 
+/* dewarp!(out, dw, img) (src/dewarp.jl:233-248): the frame's cubic B-spline
+ * interpolant (BSpline(Cubic(Line(OnGrid()))), extrapolated with zero)
+ * sampled at the dewarper's precomputed source coordinates rows, cols
+ * (ny x nx, 1-based, column-major, converted to the image precision T as
+ * itp(T(rows), T(cols)) does); masked nodes (mask nullable) are 0. */
+void F(orc_dewarp)(const REAL *img, int nr, int nc, const double *rows, const double *cols, const uint8_t *mask,
+                   int ny, int nx, REAL *out)
+{
+    REAL *coef = (REAL *)malloc(sizeof(REAL) * (size_t)(nr + 2) * (nc + 2));
+    F(orc_prefilter)(img, nr, nc, coef);
+    for (size_t e = 0; e < (size_t)ny * nx; e++)
+        out[e] = (mask && mask[e]) ? (REAL)0 : F(orc_cubic)(coef, nr, nc, (REAL)rows[e], (REAL)cols[e]);
+    free(coef);
+}
+
 typedef struct {
     const REAL *cA, *cB, *pu, *pv;
     int nr, nc, gny, gnx;
