@@ -23,7 +23,7 @@ UQ_NSTATS = 44
 # every entry point of include/hammerhead_mi355x.h
 EXPORTS = (
     "hh_create", "hh_destroy", "hh_last_error", "hh_version", "hh_set_form", "hh_check_passes", "hh_grid_size",
-    "hh_set_pair", "hh_set_pair_device", "hh_prefilter", "hh_deform", "hh_get_images", "hh_get_coefs",
+    "hh_set_pair", "hh_set_pair_device", "hh_prefilter", "hh_set_coefs", "hh_deform", "hh_get_images", "hh_get_coefs",
     "hh_process_windows", "hh_correlation_planes", "hh_uncertainty_sweep", "hh_validate_and_replace",
     "hh_validate_and_replace_noalloc",
     "hh_smooth_field", "hh_run_piv", "hh_fetch_result", "hh_ensemble_begin", "hh_ensemble_add",
@@ -73,6 +73,7 @@ def load():
         "hh_set_pair": (i32, [vp, vp, vp, i32, i32]),
         "hh_set_pair_device": (i32, [vp, vp, vp, i32, i32]),
         "hh_prefilter": (i32, [vp]),
+        "hh_set_coefs": (i32, [vp, vp, vp, i32, i32]),
         "hh_deform": (i32, [vp, vp, i32, vp, i32, vp, vp]),
         "hh_get_images": (i32, [vp, i32, vp, vp]),
         "hh_get_coefs": (i32, [vp, vp, vp]),
@@ -237,6 +238,14 @@ class Context:
 
     def prefilter(self):
         _check(self.h, load().hh_prefilter(self.h), "prefilter")
+
+    def set_coefs(self, cA, cB):
+        """Stage prefiltered coefficient arrays ((nr+2) x (nc+2) each)."""
+        cA, cB = _f32(cA), _f32(cB)
+        if cA.shape != cB.shape:
+            raise DimensionMismatch(f"coefficient arrays differ: {cA.shape} and {cB.shape}")
+        self.nr, self.nc = cA.shape[0] - 2, cA.shape[1] - 2
+        _check(self.h, load().hh_set_coefs(self.h, _ptr(cA), _ptr(cB), self.nr, self.nc), "set_coefs")
 
     def coefs(self):
         cA = np.zeros((self.nr + 2, self.nc + 2), dtype=np.float32, order="F")

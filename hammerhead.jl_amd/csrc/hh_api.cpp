@@ -342,11 +342,11 @@ void check_job_area(hh_ctx *c, const hh_pass &p, int rs, int cs)
 
 const float *src_image(hh_ctx *c, int src, bool a)
 {
-    need_pair(c);
     if (src == HH_SRC_WARP) {
         if (!c->warp_valid) throw Error(HH_ESTATE, "no warped pair: call hh_deform first");
         return a ? c->warpA.p : c->warpB.p;
     }
+    need_pair(c);
     if (src != HH_SRC_RAW) throw Error(HH_EINVAL, "src must be HH_SRC_RAW or HH_SRC_WARP");
     return a ? c->A : c->B;
 }
@@ -1306,11 +1306,34 @@ int hh_prefilter(hh_ctx *c)
     return guarded(c, [&] { do_prefilter(c); });
 }
 
+int hh_set_coefs(hh_ctx *c, const float *cA, const float *cB, int nr, int nc)
+{
+    return guarded(c, [&] {
+        if (!cA || !cB) throw Error(HH_EINVAL, "null coefficient array");
+        if (nr < 2 || nc < 2) throw Error(HH_EDIM, "images must be at least 2 x 2");
+        const size_t n = (size_t)(nr + 2) * (nc + 2);
+        c->coefA.ensure(n);
+        c->coefB.ensure(n);
+        HH_HIP(hipMemcpyAsync(c->coefA.p, cA, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipMemcpyAsync(c->coefB.p, cB, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HH_HIP(hipStreamSynchronize(c->stream));
+        if (nr != c->nr || nc != c->nc) {
+            // a raw pair of another size no longer describes these images
+            c->A = c->B = nullptr;
+            c->jobcache.clear();
+        }
+        c->nr = nr;
+        c->nc = nc;
+        c->coefs_valid = true;
+        c->warp_valid = c->have_result = false;
+        c->eng.have_result = false;
+    });
+}
+
 int hh_deform(hh_ctx *c, const float *py, int gny, const float *px, int gnx, const float *pu, const float *pv)
 {
     return guarded(c, [&] {
-        need_pair(c);
-        if (!c->coefs_valid) throw Error(HH_ESTATE, "no interpolant: call hh_prefilter first");
+        if (!c->coefs_valid) throw Error(HH_ESTATE, "no interpolant: call hh_prefilter or hh_set_coefs first");
         if (gny < 1 || gnx < 1 || !py || !px || !pu || !pv) throw Error(HH_EINVAL, "empty predictor grid");
         const size_t n = (size_t)gny * gnx;
         check_knots(py, gny, px, gnx);

@@ -180,6 +180,14 @@ int hh_dewarp_pair(hh_ctx *ctx, const float *A, const float *B, int nr, int nc);
  * Interpolations' BSpline(Cubic(Line(OnGrid()))) (image_interpolant,
  * src/pipeline.jl:909-915), padded (nr+2) x (nc+2). */
 int hh_prefilter(hh_ctx *ctx);
+/* _deform_context(b, workspace, itpA, itpB, imgsize, T) (src/pipeline.jl:
+ * 602-611; device form _ka_deform_context, src/ka_backend.jl:335-351): stage
+ * the host-prefiltered interpolants' coefficients, parent(itp.itp.coefs) of
+ * image_interpolant (src/pipeline.jl:909-915), (nr+2) x (nc+2) column-major
+ * float32 each, logical index k at parent k + 1. Replaces the context's
+ * coefficients (as hh_prefilter does) and fixes the image size for hh_deform;
+ * a raw pair of another size is dropped. */
+int hh_set_coefs(hh_ctx *ctx, const float *cA, const float *cB, int nr, int nc);
 /* Symmetric deformation by a predictor grid (apply_predictor/deform_images,
  * src/pipeline.jl:575-600,936-965; _ka_deform!, src/ka_backend.jl:293-312).
  * py (gny knots, uniform), px (gnx knots, uniform), pu/pv gny x gnx host
