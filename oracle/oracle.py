@@ -309,16 +309,19 @@ def process_windows(pass_struct, A, B, jobs, ny, nx, uncertainty=False, debug=Fa
     found = np.zeros(nj, dtype=np.int32) if debug else None
     m1, m2 = _plane_shape(pass_struct)
     pl = np.zeros((m1, m2, nj), dtype=dt, order="F") if planes else None
+    tie = np.ones((ny, nx), dtype=dt, order="F") if debug else None
     _fn("orc_process_windows", prec)(C.byref(pass_struct), _p(A), _p(B), C.c_int(A.shape[0]), C.c_int(A.shape[1]),
                                      _p(jobs), C.c_int(nj), C.c_int(ny), C.c_int(nx), _p(res["u"]), _p(res["v"]),
                                      _p(res["peak_ratio"]), _p(res["correlation_moment"]),
                                      _p(res["alt_u"]) if pass_struct.n_peaks > 1 else None,
                                      _p(res["alt_v"]) if pass_struct.n_peaks > 1 else None, _p(uu), _p(uv),
-                                     _p(locs), _p(vals), _p(found), _p(pl), _p(_mask(mask)), C.c_int(nthreads))
+                                     _p(locs), _p(vals), _p(found), _p(pl), _p(_mask(mask)), C.c_int(nthreads),
+                                     _p(tie))
     if uncertainty:
         res["unc_u"], res["unc_v"] = uu, uv
     if debug:
         res["locs"], res["vals"], res["found"] = locs, vals, found
+        res["tie"] = tie
     if planes:
         res["planes"] = pl
     return res
@@ -374,7 +377,7 @@ def grid_mask(pass_struct, nr, nc, mask, threshold=0.5):
 
 
 def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask=None, mask_threshold=0.5,
-            uq_pert=False):
+            uq_pert=False, tie_rel=1e-5, max_ties=100000):
     """run_piv(imgA, imgB, passes; mask, mask_threshold): dict with the final
     PIVResult fields (``mask``: the final grid mask). With ``uq_pert`` and
     uncertainty on the final pass, ``uq_pert_u`` / ``uq_pert_v`` hold the
@@ -382,7 +385,9 @@ def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask
     scaled by 1 + 2^-21 h per pixel (h a fixed hash in [-1, 1]): sigma's own
     sensitivity to float32-level input error, and ``uq_stats`` the FP64
     Wieneke statistics behind each cell's sigma, shape (44, 2, ny, nx) (u then
-    v)."""
+    v). ``ties`` lists the windows of every pass and sweep whose primary peak
+    is a near-tie (relative margin to a neighbour or to the secondary below
+    ``tie_rel``) as rows (pass, sweep, y, x)."""
     dt = _dt(prec)
     A, B = _f(A, dt), _f(B, dt)
     nr, nc = A.shape
@@ -397,17 +402,22 @@ def run_piv(A, B, passes, predictor_smoothing=True, nthreads=8, prec="f64", mask
     m = _mask(mask)
     st = np.zeros((ny, nx, 2), dtype=dt, order="F") if uq_pert else None
     ss = np.zeros((44, 2, ny, nx), dtype=np.float64, order="F") if uq_pert else None
+    ties = np.zeros((max_ties, 4), dtype=np.float64)
+    nties = C.c_int(0)
     rc = _fn("orc_run_piv", prec)(_p(A), _p(B), C.c_int(nr), C.c_int(nc), arr, C.c_int(len(passes)),
                                   C.c_int(1 if predictor_smoothing else 0), C.c_int(nthreads), _p(res["x"]),
                                   _p(res["y"]), _p(res["u"]), _p(res["v"]), _p(res["peak_ratio"]),
                                   _p(res["correlation_moment"]), _p(res["uncertainty_u"]),
                                   _p(res["uncertainty_v"]), _p(res["outliers"]), _p(m), C.c_double(mask_threshold),
-                                  _p(res["mask"]), _p(st), _p(ss))
+                                  _p(res["mask"]), _p(st), _p(ss), C.c_double(tie_rel), _p(ties), C.c_int(max_ties),
+                                  C.byref(nties))
     if rc != 0:
         raise ValueError("invalid pass schedule")
     if uq_pert:
         res["uq_pert_u"], res["uq_pert_v"] = st[:, :, 0], st[:, :, 1]
         res["uq_stats"] = ss
+    res["ties"] = ties[:min(nties.value, max_ties)]
+    res["ties_total"] = nties.value
     res["outliers"] = res["outliers"].astype(bool)
     res["mask"] = res["mask"].astype(bool)
     return res

@@ -1177,7 +1177,33 @@ typedef struct {
     int32_t *found_out;
     REAL *planes;
     const uint8_t *mask; /* image-sized pixel mask, nullable */
+    REAL *tie;           /* per grid cell: near-tie margin of the primary peak, nullable */
 } F(orc_pw_args);
+
+/* Near-tie margin of a window's primary peak (test diagnostics): the
+ * smallest relative gap (P1 - R(q)) / P1 between the primary value and any
+ * 8-neighbour q of its location or the secondary peak value. A margin
+ * below the float32 resolution of the plane means another computation of
+ * the same window may take the neighbour (or the secondary) as the peak. */
+static REAL F(orc_tie_margin)(const REAL *R, int m1, int m2, const REAL *vals, const int32_t *locs, int found)
+{
+    if (found < 1 || !(vals[0] > 0)) return (REAL)1;
+    REAL P1 = vals[0], g = (REAL)1;
+    int r = locs[0] - 1, c = locs[1] - 1;
+    for (int dc = -1; dc <= 1; dc++)
+        for (int dr = -1; dr <= 1; dr++) {
+            if (!dr && !dc) continue;
+            int rr = r + dr, cc = c + dc;
+            if (rr < 0 || rr >= m1 || cc < 0 || cc >= m2) continue;
+            REAL q = (P1 - R[rr + (size_t)cc * m1]) / P1;
+            if (q < g) g = q;
+        }
+    if (found >= 2) {
+        REAL q = (P1 - vals[1]) / P1;
+        if (q < g) g = q;
+    }
+    return g;
+}
 
 static void F(orc_pw_range)(void *arg, int j0, int j1)
 {
@@ -1214,6 +1240,7 @@ static void F(orc_pw_range)(void *arg, int j0, int j1)
             }
         }
         if (a->found_out) a->found_out[jb] = found;
+        if (a->tie) a->tie[g] = F(orc_tie_margin)(s.R, a->c->m1, a->c->m2, vals, locs, found);
         if (a->locs_out)
             for (int m = 0; m < K; m++) {
                 a->locs_out[2 * (K * jb + m)] = m < found ? locs[2 * m] : 0;
@@ -1238,13 +1265,13 @@ static void F(orc_pw_range)(void *arg, int j0, int j1)
 void F(orc_process_windows)(const hh_pass *p, const REAL *A, const REAL *B, int nr, int nc, const int32_t *jobs,
                             int njobs, int ny, int nx, REAL *u, REAL *v, REAL *pr, REAL *cm, REAL *alt_u,
                             REAL *alt_v, REAL *unc_u, REAL *unc_v, int32_t *locs, REAL *vals, int32_t *found,
-                            REAL *planes, const uint8_t *mask, int nthreads)
+                            REAL *planes, const uint8_t *mask, int nthreads, REAL *tie)
 {
     (void)nc;
     F(orc_corr) c;
     F(orc_corr_init)(&c, p);
     F(orc_pw_args) a = {p, &c, A, B, nr, jobs, ny, nx, u, v, pr, cm, alt_u, alt_v, unc_u, unc_v,
-                        locs, vals, found, planes, mask};
+                        locs, vals, found, planes, mask, tie};
     orc_parallel_for(njobs, nthreads, F(orc_pw_range), &a);
     F(orc_corr_free)(&c);
 }
@@ -1437,13 +1464,19 @@ static void F(orc_build_predictor)(const REAL *u, const REAL *v, const uint8_t *
  * indices): how far sigma moves when its inputs carry the error of a
  * float32 pipeline. ouqstats (nullable, test diagnostics): the FP64 Wieneke
  * statistics behind the final uncertainty, 2 x 44 per grid cell (u then v;
- * zero at cells without a job).
+ * zero at cells without a job). oties (nullable, test diagnostics): the
+ * windows of every pass and sweep whose primary peak is a near-tie (margin
+ * orc_tie_margin below tie_rel), as (pass, sweep, y, x) rows, at most
+ * maxties; *nties receives their count.
  * Returns 0, or -1 for an invalid schedule. */
 int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_pass *passes, int npasses,
                    int predictor_smoothing, int nthreads, REAL *ox, REAL *oy, REAL *ou, REAL *ov, REAL *opr,
                    REAL *ocm, REAL *ouu, REAL *ouv, uint8_t *oout, const uint8_t *mask, double mask_threshold,
-                   uint8_t *omask, REAL *ouqpert, double *ouqstats)
+                   uint8_t *omask, REAL *ouqpert, double *ouqstats, double tie_rel, double *oties, int maxties,
+                   int *nties)
 {
+    if (nties) *nties = 0;
+    if (!nties) oties = NULL;
     if (npasses < 1) return -1;
     int deforms = npasses > 1;
     for (int k = 0; k < npasses; k++) deforms |= passes[k].max_iterations > 1;
@@ -1536,9 +1569,26 @@ int F(orc_run_piv)(const REAL *imgA, const REAL *imgB, int nr, int nc, const hh_
             memset(rv, 0, sizeof(REAL) * ng);
             if (au)
                 for (size_t q = 0; q < ng * nalt; q++) au[q] = av[q] = (REAL)NAN;
+            REAL *tie = oties ? (REAL *)malloc(sizeof(REAL) * (ng > 0 ? ng : 1)) : NULL;
+            if (tie)
+                for (size_t q = 0; q < ng; q++) tie[q] = (REAL)1;
             F(orc_process_windows)(p, srcA, srcB, nr, nc, jobs, njobs, ny, nx, ru, rv, pr, cm, au, av,
                                    fused_unc ? uu : NULL, fused_unc ? uv : NULL, NULL, NULL, NULL, NULL, mask,
-                                   nthreads);
+                                   nthreads, tie);
+            if (tie) {
+                for (size_t q = 0; q < ng; q++)
+                    if ((double)tie[q] < tie_rel) {
+                        if (*nties < maxties) {
+                            double *row = oties + 4 * (size_t)*nties;
+                            row[0] = k;
+                            row[1] = it;
+                            row[2] = (double)y[q % ny];
+                            row[3] = (double)x[q / ny];
+                        }
+                        (*nties)++;
+                    }
+                free(tie);
+            }
             if (au)
                 for (int m = 0; m < nalt; m++)
                     for (size_t q = 0; q < ng; q++) {

@@ -25,6 +25,13 @@ Bars (the acceptance criteria of the device path):
     class), or whose ring truncation decides within 1e-4 S00 of its
     threshold, is ill-conditioned: counted and reported, and exempt from the
     1 % bar.
+  * FP64 near-ties are reported separately: a window whose primary peak is
+    within 1e-5 relative of a neighbouring plane value or of the secondary
+    peak (the oracle lists them for every pass and sweep as ``ties``) may
+    legitimately resolve the other way in float32, and the difference then
+    travels through the predictor into the later passes. Cells within
+    ``tie_radius`` (128 px) of such a window are counted and reported and
+    exempt from the bars above; everything else must meet them.
 Every statistic is printed before any assert so a failing run still reports
 all of them.
 """
@@ -85,6 +92,16 @@ def sigma_ill_conditioned(o, rel=5e-3, margin=1e-4, rings=None):
     return out
 
 
+def tie_region(o, radius=128.0):
+    """Final-grid cells within ``radius`` px (Chebyshev) of a near-tie window
+    of any pass or sweep of the oracle run."""
+    X, Y = np.meshgrid(np.asarray(o["x"], np.float64), np.asarray(o["y"], np.float64))
+    near = np.zeros(X.shape, bool)
+    for _, _, ty, tx in np.asarray(o.get("ties", np.zeros((0, 4)))):
+        near |= (np.abs(Y - ty) <= radius) & (np.abs(X - tx) <= radius)
+    return near
+
+
 def field_stats(g, o, cells=None):
     """Deviation statistics of device result g from oracle result o.
     ``cells`` (bool grid, optional) restricts every statistic."""
@@ -125,8 +142,16 @@ def field_stats(g, o, cells=None):
 
 
 def field_parity(g, o, label="", tol_px=1e-3, rms_tol=None, quality_tol=1e-2, sigma_tol=1e-2, uncertainty=True,
-                 cells=None, sigma_abs=1e-5):
+                 cells=None, sigma_abs=1e-5, tie_radius=128.0):
     """Assert the parity bars; returns the statistics."""
+    if "ties" in o and len(o["ties"]):
+        near = tie_region(o, tie_radius)
+        base = np.ones(o["u"].shape, bool) if cells is None else np.asarray(cells, bool)
+        inside = field_stats(g, o, base & near)
+        say("near-tie region", label, {"ties": int(o.get("ties_total", len(o["ties"]))),
+                                       "cells": inside["cells"], "flag_diffs": inside["flag_diffs"],
+                                       "max_du": inside["max_du"], "max_dv": inside["max_dv"]})
+        cells = base & ~near
     st = field_stats(g, o, cells)
     ill = None
     if uncertainty and "uq_pert_u" in o:

@@ -77,3 +77,19 @@ def test_ring_margin_marks_the_truncation_edge(orc):
     nudged = s[:, 0, 0, 0].copy()
     nudged[4:4 + rings[1]] = (0.05 - 1e-7) * 1e-3
     assert abs(orc.uq_finalize(nudged) - base) > 0.05 * abs(base)
+
+
+def test_near_tie_listed_with_its_plateau(hh, orc):
+    # batch pair 1 (768 x 1024 shear): in the 32/16 pass the window centred
+    # at (752.5, 656.5) has two adjacent plane values 3e-7 apart at its
+    # peak; the oracle lists exactly that window as a near-tie
+    p = hh.synthetic.config_batch(npairs=2)[1]
+    passes = hh.multipass_parameters([64, 32], padding=True, apodization="gauss")
+    o = orc.run_piv(p.A.astype(np.float64), p.B.astype(np.float64), [q.to_struct() for q in passes])
+    assert o["ties_total"] == 1
+    k, it, y, x = o["ties"][0]
+    assert (k, it, y, x) == (1, 1, 752.5, 656.5)
+    from parity_bars import tie_region
+
+    near = tie_region(o, 32.0)
+    assert near.sum() == 5 * 3  # rows 44..46 (y 720.5..752.5) x cols 38..42 of the 47 x 63 grid
