@@ -30,7 +30,7 @@ EXPORTS = (
     "hh_ensemble_analyze", "hh_ensemble_planes", "hh_run_piv_ensemble", "hh_energy", "hh_grad_energy",
     "hh_refine_nodes", "hh_set_energy_sampling", "hh_stream", "hh_synchronize", "hh_set_timing", "hh_set_timing_only", "hh_kernel_time",
     "hh_set_pairs", "hh_run_piv_batch", "hh_fetch_pair", "hh_pair_grid", "hh_set_graphs", "hh_graph_replays",
-    "hh_set_autotune", "hh_tune_report", "hh_tune_set", "hh_set_mask", "hh_set_dewarp_map", "hh_dewarp",
+    "hh_set_autotune", "hh_tune_report", "hh_tune_set", "hh_tune_set_family", "hh_set_mask", "hh_set_dewarp_map", "hh_dewarp",
     "hh_dewarp_pair",
 )
 
@@ -108,6 +108,7 @@ def load():
         "hh_set_autotune": (i32, [vp, i32]),
         "hh_tune_report": (i32, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "hh_tune_set": (i32, [vp, C.c_char_p, i32, i32]),
+        "hh_tune_set_family": (i32, [vp, C.c_char_p, i32]),
         "hh_set_mask": (i32, [vp, vp, i32, i32, C.c_double]),
         "hh_set_dewarp_map": (i32, [vp, vp, vp, vp, i32, i32, i32, i32]),
         "hh_dewarp": (i32, [vp, vp, i32, i32, vp]),
@@ -427,8 +428,11 @@ class Context:
     def set_graphs(self, enable: bool):
         _check(self.h, load().hh_set_graphs(self.h, 1 if enable else 0), "set_graphs")
 
-    def set_autotune(self, enable: bool):
-        _check(self.h, load().hh_set_autotune(self.h, 1 if enable else 0), "set_autotune")
+    def set_autotune(self, enable, level: int = None):
+        """enable: False (off) / True (launch geometry, level 1); level 2 also
+        chooses the kernel family per FFT side."""
+        lv = (0 if not enable else 1) if level is None else int(level)
+        _check(self.h, load().hh_set_autotune(self.h, lv), "set_autotune")
 
     def tune_report(self) -> dict:
         import json
@@ -445,6 +449,8 @@ class Context:
             ch = e["choice"]
             _check(self.h, load().hh_tune_set(self.h, key.encode(), int(ch["waves_per_simd"]),
                                               int(ch["linear_order"])), "tune_set")
+            _check(self.h, load().hh_tune_set_family(self.h, key.encode(), int(ch.get("family", 0))),
+                   "tune_set_family")
 
     def graph_replays(self) -> int:
         n = C.c_int64()

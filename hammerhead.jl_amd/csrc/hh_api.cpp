@@ -1691,8 +1691,24 @@ int hh_set_graphs(hh_ctx *c, int enable)
 int hh_set_autotune(hh_ctx *c, int enable)
 {
     return guarded(c, [&] {
+        if (enable < 0 || enable > 2) throw Error(HH_EINVAL, "autotune level must be 0, 1 or 2");
+        const int level = enable == 0 ? c->tune.level : enable;
+        if (!enable || level != c->tune.level) {
+            c->tune.entries.clear();
+            c->eng.key.clear();  // re-plan with the new choices
+        }
         c->tune.enabled = enable != 0;
-        if (!enable) c->tune.entries.clear();
+        c->tune.level = level;
+    });
+}
+
+int hh_tune_set_family(hh_ctx *c, const char *key, int family)
+{
+    return guarded(c, [&] {
+        if (!key) throw Error(HH_EINVAL, "null key");
+        if (family < 0 || family > 2) throw Error(HH_EINVAL, "family must be 0, 1 or 2");
+        c->tune.entries[key].family = family;
+        c->eng.key.clear();
     });
 }
 
@@ -1714,10 +1730,10 @@ int hh_tune_set(hh_ctx *c, const char *key, int waves_per_simd, int linear_order
     return guarded(c, [&] {
         if (!key) throw Error(HH_EINVAL, "null key");
         if (waves_per_simd < 0 || waves_per_simd > 8) throw Error(HH_EINVAL, "waves_per_simd must be 0-8");
-        TuneEntry e;
+        TuneEntry &e = c->tune.entries[key];  // keeps a preloaded family
         e.waves_per_simd = waves_per_simd;
         e.linear_order = linear_order != 0;
-        c->tune.entries[key] = e;
+        e.ms.clear();
         c->eng.key.clear();  // re-plan with the choice
     });
 }

@@ -137,10 +137,15 @@ struct HostJobs {
 // chosen candidate and the measured milliseconds of every candidate.
 struct TuneEntry {
     int waves_per_simd = 0, linear_order = 0;
+    int family = 0;  // CorrConfig::family of the choice
     std::vector<std::pair<std::string, float>> ms;
 };
 struct Tuner {
     bool enabled = true;
+    // 1: launch geometry only (results do not depend on the choice);
+    // 2: also the kernel family per FFT side (results may then differ at
+    // the arithmetic's round-off between choices)
+    int level = 1;
     std::map<std::string, TuneEntry> entries;
 };
 

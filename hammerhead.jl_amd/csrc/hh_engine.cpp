@@ -311,23 +311,47 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
 {
     Engine &E = c->eng;
     const char *fam = correlate_family(base, uq);
-    const bool mfma = std::string(fam).rfind("mfma", 0) == 0, reg = std::string(fam).rfind("reg", 0) == 0;
-    const bool mfma64 = std::string(fam) == "mfma64";
+    const std::string f0(fam);
+    const bool mfma = f0.rfind("mfma", 0) == 0, reg = f0.rfind("reg", 0) == 0;
     CorrConfig cfg = base;
-    if (!c->tune.enabled || !(mfma || reg) || P.all.njobs <= 0) return cfg;
-    const std::string key = std::string(fam) + (uq && !mfma ? "+uq" : "") + " w" + std::to_string(base.w) + " M" +
+    const bool families = c->tune.level >= 2;
+    if (!c->tune.enabled || !(mfma || reg || families) || P.all.njobs <= 0) return cfg;
+    const std::string key = f0 + (uq && !mfma ? "+uq" : "") + " w" + std::to_string(base.w) + " M" +
                             std::to_string(base.M) + " K" + std::to_string(base.K) + " m" +
                             std::to_string(base.method) + " n" + std::to_string(P.all.njobs);
     auto it = c->tune.entries.find(key);
     if (it == c->tune.entries.end()) {
-        std::vector<std::pair<int, int>> cand;
-        for (int order = 0; order < 2; ++order) {
-            if (mfma64)
-                for (int w : {2, 1}) cand.push_back({w, order});
-            else if (mfma)
-                for (int w : {3, 2, 4, 1}) cand.push_back({w, order});
-            else
-                cand.push_back({0, order});
+        // candidates (family, waves per SIMD, window order): the form's own
+        // family first, then, at level 2, the register FFT and the LDS
+        // kernels wherever they differ from it
+        struct Cand {
+            int family, w, order;
+        };
+        std::vector<Cand> cand;
+        std::vector<int> fams = {0};
+        if (families) {
+            CorrConfig t = base;
+            t.family = 1;
+            const std::string f1 = correlate_family(t, uq);
+            t.family = 2;
+            const std::string f2 = correlate_family(t, uq);
+            if (f1 != f0) fams.push_back(1);
+            if (f2 != f0 && f2 != f1) fams.push_back(2);
+        }
+        for (int fm : fams) {
+            CorrConfig t = base;
+            t.family = fm;
+            const std::string f = correlate_family(t, uq);
+            for (int order = 0; order < 2; ++order) {
+                if (f == "mfma64")
+                    for (int w : {2, 1}) cand.push_back({fm, w, order});
+                else if (f.rfind("mfma", 0) == 0)
+                    for (int w : {3, 2, 4, 1}) cand.push_back({fm, w, order});
+                else if (f.rfind("reg", 0) == 0)
+                    cand.push_back({fm, 0, order});
+                else if (order == 0)
+                    cand.push_back({fm, 0, 0});  // LDS / DFT kernels: one block per window, no geometry
+            }
         }
         CorrOut out{};
         out.u = E.u.p;
@@ -342,38 +366,52 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
             out.unc_u = E.uu.p;
             out.unc_v = E.uv.p;
         }
+        const ConfigBufs &cb = get_config(c, P.p);
         hipEvent_t a = get_event(c), b = get_event(c);
         TuneEntry best;
         float best_ms = 1e30f;
-        for (auto [w, order] : cand) {
+        for (const Cand &cd : cand) {
             CorrConfig t = base;
-            t.waves_per_simd = w;
-            t.linear_order = order;
+            t.family = cd.family;
+            t.waves_per_simd = cd.w;
+            t.linear_order = cd.order;
+            const std::string f = correlate_family(t, uq);
             float ms_min = 1e30f;
             for (int rep = 0; rep < 4; ++rep) {
                 HH_HIP(hipEventRecord(a, c->stream));
-                launch_correlate(t, E.lay.A, E.lay.B, E.lay.ld, 0, P.all.jobs.p, P.all.njobs, out, c->stream);
+                const bool fused = launch_correlate(t, E.lay.A, E.lay.B, E.lay.ld, 0, P.all.jobs.p, P.all.njobs, out,
+                                                    c->stream);
+                // a family that does not fuse the statistics is timed with
+                // the standalone launch the engine then makes
+                if (uq && !fused)
+                    launch_uq(base.w, cb.apod.p, E.lay.A, E.lay.B, E.lay.ld, P.all.jobs.p, P.all.njobs, P.total,
+                              E.uu.p, E.uv.p, nullptr, false, c->stream);
                 HH_HIP(hipEventRecord(b, c->stream));
                 HH_HIP(hipEventSynchronize(b));
                 float ms = 0.f;
                 HH_HIP(hipEventElapsedTime(&ms, a, b));
                 if (rep > 0) ms_min = std::min(ms_min, ms);  // the first run warms caches
             }
-            const std::string name = (mfma ? "waves_per_simd=" + std::to_string(w) + " " : std::string()) +
-                                     (order ? "order=linear" : "order=xcd_bands");
+            std::string name = f.rfind("mfma", 0) == 0 ? "waves_per_simd=" + std::to_string(cd.w) + " " : "";
+            if (f.rfind("mfma", 0) == 0 || f.rfind("reg", 0) == 0)
+                name += cd.order ? "order=linear" : "order=xcd_bands";
+            if (families) name = "family=" + f + (name.empty() ? "" : " " + name);
             best.ms.push_back({name, ms_min});
             // a later candidate must win by more than timing noise (2 %):
-            // the first listed (XCD bands) also keeps the L2 traffic lowest
+            // the first listed (the form's own family, XCD bands) also keeps
+            // the L2 traffic lowest
             if (ms_min < 0.98f * best_ms) {
                 best_ms = ms_min;
-                best.waves_per_simd = w;
-                best.linear_order = order;
+                best.family = cd.family;
+                best.waves_per_simd = cd.w;
+                best.linear_order = cd.order;
             }
         }
         c->event_pool.push_back(a);
         c->event_pool.push_back(b);
         it = c->tune.entries.emplace(key, best).first;
     }
+    cfg.family = it->second.family;
     cfg.waves_per_simd = it->second.waves_per_simd;
     cfg.linear_order = it->second.linear_order;
     return cfg;
@@ -940,7 +978,7 @@ std::string engine_tune_report(hh_ctx *c)
     for (auto &[key, e] : c->tune.entries) {
         j += (first ? "\"" : ", \"") + key + "\": {\"choice\": {\"waves_per_simd\": " +
              std::to_string(e.waves_per_simd) + ", \"linear_order\": " + std::to_string(e.linear_order) +
-             "}, \"ms\": {";
+             ", \"family\": " + std::to_string(e.family) + "}, \"ms\": {";
         for (size_t q = 0; q < e.ms.size(); ++q) {
             char buf[64];
             std::snprintf(buf, sizeof buf, "%.4f", e.ms[q].second);

@@ -56,6 +56,11 @@ struct CorrConfig {
     // results do not depend on it
     int waves_per_simd; // persistent MFMA kernel: resident waves per SIMD
     int linear_order;   // 1: windows in launch order instead of XCD bands
+    // kernel family (the autotuner's algorithm choice): 0 the form's own
+    // (matrix-core DFT where it covers the pass in the MFMA form, else the
+    // register FFT, else the LDS kernels), 1 the register FFT (no matrix
+    // cores), 2 the LDS radix-2 FFT / direct DFT kernels
+    int family;
 };
 
 // Kernel family a correlation launch of cfg takes ("mfma32", "regL1",

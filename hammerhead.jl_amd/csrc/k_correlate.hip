@@ -743,10 +743,10 @@ bool launch_uq16_rows(const CorrConfig &cfg, const float *A, const float *B, int
 
 const char *correlate_family(const CorrConfig &cfg, bool uq)
 {
-    if (cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg) && !(uq && cfg.w != 16))
+    if (cfg.family == 0 && cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg) && !(uq && cfg.w != 16))
         return cfg.M == 64 ? "mfma64" : (uq ? "mfma32+rows" : "mfma32");
-    const char *f = fast_family(cfg);
-    return f ? f : "lds";
+    const char *f = cfg.family == 2 ? nullptr : fast_family(cfg);
+    return f ? f : (cfg.M == 24 || cfg.M == 48 || cfg.M == 96 ? "dft" : "lds");
 }
 
 void correlate_prepare(const CorrConfig &cfg, hipStream_t s)
@@ -761,8 +761,8 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     (void)nc;
     if (njobs <= 0) return out.unc_u != nullptr;
     bool uq_done = false;
-    if (cfg.form == HH_FORM_MFMA16 && out.unc_u && out.unc_v && !out.planes && !out.acc && mfma_covers(cfg) &&
-        cfg.w == 16) {
+    const bool mfma_ok = cfg.family == 0 && cfg.form == HH_FORM_MFMA16, fast_ok = cfg.family != 2;
+    if (mfma_ok && out.unc_u && out.unc_v && !out.planes && !out.acc && mfma_covers(cfg) && cfg.w == 16) {
         // the sweep with uncertainty: matrix-core correlation, then the
         // statistics of the same windows by the row kernel
         CorrOut o = out;
@@ -771,8 +771,8 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
             launch_uq16_rows(cfg, A, B, nr, jobs, njobs, out, s))
             return true;
     }
-    if (cfg.form == HH_FORM_MFMA16 && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
-    if (launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
+    if (mfma_ok && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
+    if (fast_ok && launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
     const float2 *tw = twiddle_table(cfg.M, s);
     switch (cfg.M) {
     case 16: hipLaunchKernelGGL(k_corr_lds<16>, dim3(njobs), dim3(kThreads), 0, s, cfg, tw, A, B, nr, jobs, out); break;

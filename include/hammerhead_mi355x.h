@@ -321,19 +321,29 @@ int hh_pair_grid(hh_ctx *ctx, int q, int *ny, int *nx);
 int hh_set_graphs(hh_ctx *ctx, int enable);
 int hh_graph_replays(hh_ctx *ctx, int64_t *replays);
 
-/* Autotuner of the correlation launch geometry (on by default): while a
+/* Autotuner of the correlation kernels (level 1 by default): while a
  * schedule is planned, each pass's correlation kernel is timed over its own
- * job list for every candidate geometry (persistent MFMA waves per SIMD,
- * window-to-XCD order) and the fastest is kept per context. Results do not
- * depend on the choice. hh_tune_report writes the choices and timings as a
- * JSON object (NUL-terminated, truncated to len; *needed = full size). No
- * counterpart in the reference (its device batch size is a fixed keyword,
- * src/ka_backend.jl). */
+ * job list for every candidate and the fastest is kept per context.
+ * Level 1 tunes the launch geometry: resident MFMA waves per SIMD (which
+ * sets the windows each persistent wave and workgroup walks), workgroup
+ * size of the 32-px matrix-core kernel (4 or 8 waves), window-to-XCD order;
+ * results do not depend on the choice. Level 2 also chooses the kernel
+ * family per FFT side among the form's own (the split-FP16 matrix-core DFT
+ * in the MFMA form), the register FFT and the LDS radix-2 FFT / direct DFT
+ * kernels, the uncertainty launch included in each family's time; results
+ * then differ between choices at the arithmetic's round-off (each family
+ * meets the same parity bars). 0 disables tuning. hh_tune_report writes the
+ * choices and timings as a JSON object (NUL-terminated, truncated to len;
+ * *needed = full size). No counterpart in the reference (its device batch
+ * size is a fixed keyword, src/ka_backend.jl). */
 int hh_set_autotune(hh_ctx *ctx, int enable);
 int hh_tune_report(hh_ctx *ctx, char *buf, size_t len, size_t *needed);
 /* Preload one autotuner choice (a key of hh_tune_report) so that plans use it
  * without timing candidates, e.g. choices saved by an earlier process. */
 int hh_tune_set(hh_ctx *ctx, const char *key, int waves_per_simd, int linear_order);
+/* Preload the kernel family of a key (0 the form's own, 1 register FFT,
+ * 2 LDS FFT / direct DFT), as the "family" of hh_tune_report. */
+int hh_tune_set_family(hh_ctx *ctx, const char *key, int family);
 
 /* ---- ensemble (sum-of-correlation) PIV --------------------------------- */
 

@@ -186,3 +186,41 @@ def test_ensemble_resident_rerun_matches(hh):
         assert same(first, again), diff_report(first, again)
         with pytest.raises(hh.DeviceError):
             c.run_piv_ensemble_resident(len(pairs) + 1, pairs[0].A.shape, [p])
+
+
+@pytest.mark.parametrize("form", ["fft32", "mfma16"])
+def test_autotune_kernel_families(hh, orc, small_pairs, form):
+    """Level-2 tuning chooses the kernel family per FFT side as well: every
+    pass's report lists the family candidates with their times; a run with
+    each pass forced to each of its families meets the parity bars against
+    the FP64 oracle; the tuned run equals a run with its choices preloaded."""
+    passes = schedule(hh)
+    fm = hh.HH_FORM_MFMA16 if form == "mfma16" else hh.HH_FORM_FFT32
+    p = small_pairs[0]
+    o = orc.run_piv(p.A.astype(np.float64), p.B.astype(np.float64), [s.to_struct() for s in passes],
+                    uq_pert=True)
+    with hh.Context(0) as c:
+        c.set_form(fm)
+        c.set_autotune(True, level=2)
+        c.set_pair(p.A, p.B)
+        tuned = c.run_piv(passes)
+        rep = c.tune_report()
+    say("level-2 report", rep)
+    assert rep
+    for key, e in rep.items():
+        fams = {n.split()[0] for n in e["ms"]}
+        assert all(n.startswith("family=") for n in fams) and len(fams) >= 2, (key, fams)
+    with hh.Context(0) as c:
+        c.set_form(fm)
+        c.tune_load(rep)
+        c.set_pair(p.A, p.B)
+        again = c.run_piv(passes)
+    assert same(tuned, again), diff_report(tuned, again)
+    for fam in (0, 1, 2):
+        forced = {k: dict(e, choice=dict(e["choice"], family=fam, waves_per_simd=0)) for k, e in rep.items()}
+        with hh.Context(0) as c:
+            c.set_form(fm)
+            c.tune_load(forced)
+            c.set_pair(p.A, p.B)
+            g = c.run_piv(passes)
+        field_parity(g, o, label=f"{form} family {fam}", rms_tol=5e-3)
