@@ -64,17 +64,24 @@ __device__ __forceinline__ void put(Split &a, int s, float x)
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 
-// Slots s, s + 1 (s even) from two values: hi by one packed conversion,
-// the residuals x - f32(hi) (exact in FP32) by mixed-precision FMAs that
-// read hi as FP16, lo by one packed conversion. neg1 is -1 held in a
-// register the compiler cannot see through, so x + hi * neg1 stays an FMA
-// (a constant -1 would fold it into a subtraction of a converted hi).
+// Slots s, s + 1 (s even) from two values: hi by one packed conversion;
+// each lo = f16(x + hi * -1) by one mixed-precision FMA that reads hi as
+// FP16 from its half of the packed register and writes the rounded
+// residual straight into its half of lo (v_fma_mixlo/mixhi_f16): the
+// residual x - f32(hi) is exact in FP32 inside the FMA, so lo is the same
+// value an FP32 FMA followed by a conversion gives, in one instruction per
+// value instead of one and a half.
 __device__ __forceinline__ void put2(Split &a, int s, float x0, float x1, float neg1)
 {
     const f2v x = {x0, x1};
     const h2v hi = __builtin_convertvector(x, h2v);
-    const f2v r = {__builtin_fmaf((float)hi[0], neg1, x0), __builtin_fmaf((float)hi[1], neg1, x1)};
-    const h2v lo = __builtin_convertvector(r, h2v);
+    const unsigned hp = __builtin_bit_cast(unsigned, hi);
+    unsigned lp;
+    asm("v_fma_mixlo_f16 %0, %1, %2, %3 op_sel_hi:[1,0,0]\n\t"
+        "v_fma_mixhi_f16 %0, %1, %2, %4 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=&v"(lp)
+        : "v"(hp), "v"(neg1), "v"(x0), "v"(x1));
+    const h2v lo = __builtin_bit_cast(h2v, lp);
     a.hi[s] = hi[0];
     a.hi[s + 1] = hi[1];
     a.lo[s] = lo[0];
