@@ -16,7 +16,12 @@
 //      lane and combine into Z(k1, k2), k2 = 0..16 (Hermitian half);
 //   3. the spectrum conj(ZA) ZB (or the filtered phase spectrum) per lane,
 //      then P (rows k1, K = Re/Im of k2) x table: the inverse along k2 for
-//      the 15 box columns l2, Hermitian weights folded into the table;
+//      the 15 box columns l2, Hermitian weights folded into the table. The
+//      columns k2 = 0 and 16 are Hermitian in k1, so each enters as the one
+//      real number H = Re P - Im P (in the K slots of Re and Im k2 = 0)
+//      against table columns that make it H (1 - i) e^(i th k2 l2): summed
+//      over k1 in stage 4 that gives sum H (cos + sin) = sum (Re P cos -
+//      Im P sin), the same contribution, and K = 32 fits two K steps;
 //   4. Q^T (rows = (l2, Re/Im), K = k1) x [cos | sin](k1, l1): the inverse
 //      along k1 for the 15 box rows l1; Re parts minus Im parts give R.
 // Every stage feeds the next from registers: an MFMA result holds one
@@ -145,7 +150,8 @@ constexpr int kHW = 7;            // box half-width (Gaussian apodization)
 constexpr int kNB = 2 * kHW + 1;  // 15 box rows and columns
 // Operand scales. The input is scaled to |x| < 2^-1, so the stage-2 operand
 // (16-term sums) stays below 2^3, Z below 2^7 and the cross spectrum below
-// 2^14: stages 2 and 3 need no rescaling. The phase spectrum (|P| <= 1) is
+// 2^14 (H = Re P - Im P below 2^14.5): stages 2 and 3 need no rescaling.
+// The phase spectrum (|P| <= 1) is
 // raised to 2^12. The stage-3 tables (weights <= 2) carry 2^kS4, so that the
 // stage-3 result enters stage 4 without a rescale: by Parseval the cross
 // spectrum of one k1 sums to at most sum |Z_A|^2 = 1024 * 256 * 2^-2 = 2^16
@@ -169,9 +175,9 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 
     // ---- constant operand tables: one copy per block in LDS, read per use
     // (registers are left to the accumulators) ----
-    __shared__ Split tabs[10][64];
+    __shared__ Split tabs[9][64];
     if (wave == 0) {
-        Split F1, WrT, WiT, T3[3], C4[2], S4[2];
+        Split F1, WrT, WiT, T3[2], C4[2], S4[2];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int c = 8 * h + s;  // stage 1: K = c, column n
@@ -186,22 +192,26 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
             const int r = rmap(h, s);  // stage 2: K = r, column k1 = n
             put(WrT, s, tg.c[(n * r) & 31]);
             put(WiT, s, -tg.s[(n * r) & 31]);
-            // stage 3: K = (Re | Im of k2 = rmap(h, s)) and (k2 = 16), column
-            // n = (b = n & 15, Re for n < 16 | Im), l2 = b - HW
+            // stage 3: K = (Re | Im of k2 = rmap(h, s)), column n = (b = n & 15,
+            // Re for n < 16 | Im), l2 = b - HW; the K slots of k2 = 0 carry
+            // H(k1, 0) (Re table) and H(k1, 16) (Im table)
             const int b = n & 15, im = n >> 4;
             const bool vb = b < kNB;
             const int l2 = b - kHW;
             const int k2 = rmap(h, s);
-            const float cw = k2 == 0 ? 1.f : 2.f;
             const int t = (k2 * l2) & 31;
+            const float sgn16 = tg.c[(16 * l2) & 31];  // (-1)^l2 = e^(i pi l2)
+            float t3r, t3i;
+            if (k2 == 0) {
+                t3r = im ? -1.f : 1.f;        // H(k1, 0) (1 - i)
+                t3i = im ? -sgn16 : sgn16;    // H(k1, 16) (1 - i) (-1)^l2
+            } else {
+                t3r = im ? 2.f * tg.s[t] : 2.f * tg.c[t];   // 2 Re P e^(i th k2 l2)
+                t3i = im ? 2.f * tg.c[t] : -2.f * tg.s[t];  // 2 Im P ...
+            }
             // scaled by 2^kS4: the stage-3 result then enters stage 4 as is
-            put(T3[0], s, ldexpf(vb ? (im ? cw * tg.s[t] : cw * tg.c[t]) : 0.f, kS4));
-            put(T3[1], s, ldexpf(vb ? (im ? cw * tg.c[t] : -cw * tg.s[t]) : 0.f, kS4));
-            const int t16 = (16 * l2) & 31;
-            float e = 0.f;
-            if (vb && h == 0 && s == 0) e = im ? tg.s[t16] : tg.c[t16];   // Re P(k1, 16)
-            if (vb && h == 0 && s == 1) e = im ? tg.c[t16] : -tg.s[t16];  // Im P(k1, 16)
-            put(T3[2], s, ldexpf(e, kS4));
+            put(T3[0], s, ldexpf(vb ? t3r : 0.f, kS4));
+            put(T3[1], s, ldexpf(vb ? t3i : 0.f, kS4));
             // stage 4: K = k1 = 16 q + rmap(h, s), column a = n, l1 = a - HW
             const bool va = n < kNB;
             const int l1 = n - kHW;
@@ -218,11 +228,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         tabs[2][l] = WiT;
         tabs[3][l] = T3[0];
         tabs[4][l] = T3[1];
-        tabs[5][l] = T3[2];
-        tabs[6][l] = C4[0];
-        tabs[7][l] = C4[1];
-        tabs[8][l] = S4[0];
-        tabs[9][l] = S4[1];
+        tabs[5][l] = C4[0];
+        tabs[6][l] = C4[1];
+        tabs[7][l] = S4[0];
+        tabs[8][l] = S4[1];
     }
     __syncthreads();
     const f16v zero = {};
@@ -334,7 +343,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         }
 
         // ---- 3. spectrum and inverse along k2 ----
-        Split a3[3];
+        Split a3[2];
         const int es = eA + eB;  // P scale: P_true 2^es
         float sp_r[9], sp_i[9];
 #pragma unroll
@@ -353,23 +362,20 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
             sp_r[v] = pr;
             sp_i[v] = pi;
         }
+        // k2 = 0 and 16 (lanes h = 0, slots 0 and 8): H = Re P - Im P into
+        // the K slots of Re and Im k2 = 0
+        if (h == 0) {
+            const float h0 = sp_r[0] - sp_i[0], h16 = sp_r[8] - sp_i[8];
+            sp_r[0] = h0;
+            sp_i[0] = h16;
+        }
 #pragma unroll
         for (int s = 0; s < 8; s += 2) {
             put2(a3[0], s, sp_r[s], sp_r[s + 1], neg1);
             put2(a3[1], s, sp_i[s], sp_i[s + 1], neg1);
         }
-        {
-            const bool k16 = h == 0;
-            put2(a3[2], 0, k16 ? sp_r[8] : 0.f, k16 ? sp_i[8] : 0.f, neg1);
-#pragma unroll
-            for (int s = 2; s < 8; ++s) {
-                a3[2].hi[s] = (_Float16)0.f;
-                a3[2].lo[s] = (_Float16)0.f;
-            }
-        }
         f16v d3 = mfma3(a3[0], tabs[3][tl], zero);
         d3 = mfma3(a3[1], tabs[4][tl], d3);
-        d3 = mfma3(a3[2], tabs[5][tl], d3);
 
         // ---- 4. inverse along k1 for the box rows ----
         Split a4[2];
@@ -378,10 +384,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 #pragma unroll
             for (int s = 0; s < 8; s += 2)
                 put2(a4[q], s, d3[8 * q + s], d3[8 * q + s + 1], neg1);
-        f16v oc = mfma3(a4[0], tabs[6][tl], zero);
-        oc = mfma3(a4[1], tabs[7][tl], oc);
-        f16v os = mfma3(a4[0], tabs[8][tl], zero);
-        os = mfma3(a4[1], tabs[9][tl], os);
+        f16v oc = mfma3(a4[0], tabs[5][tl], zero);
+        oc = mfma3(a4[1], tabs[6][tl], oc);
+        f16v os = mfma3(a4[0], tabs[7][tl], zero);
+        os = mfma3(a4[1], tabs[8][tl], os);
 
         // R(l1 = n - HW, l2 = b - HW) = oc[Re row b] - os[Im row b]
         const int etot = (PHASE ? kS3P : es) + kS4;
