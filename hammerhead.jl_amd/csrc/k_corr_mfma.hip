@@ -425,9 +425,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 //   3. at once its contribution to Q[nt][ct] += P (K = Re/Im of k2) x table,
 //      the inverse along k2 for the 27 box columns (2 column tiles ct of
 //      (l2, Re/Im)) with the Hermitian weights in the table, so the spectrum
-//      is never held whole;
+//      is never held whole; k2 = 0 and the Nyquist column k2 = 32 enter as
+//      H = Re P - Im P in the K slots of k2 = 0, as in the 16-px kernel;
 //   4. R = Q^T (K = k1 in 4 steps) x [cos | sin](k1, l1) per column tile.
-// Constant operand tables (30 split 32 x 16 operands, 61 KB) live in LDS,
+// Constant operand tables (28 split 32 x 16 operands, 57 KB) live in LDS,
 // one copy per block; boxes of 2 windows are analyzed together, 32 lanes
 // per window.
 struct Trig64 {
@@ -443,10 +444,9 @@ enum {
     T64_WS = 8,    // [kk][nt]
     T64_T3R = 12,  // [J][ct]
     T64_T3I = 16,  // [J][ct]
-    T64_T3N = 20,  // [ct]
-    T64_C4 = 22,   // [nt][kk]
-    T64_S4 = 26,   // [nt][kk]
-    T64_N = 30
+    T64_C4 = 20,   // [nt][kk]
+    T64_S4 = 24,   // [nt][kk]
+    T64_N = 28
 };
 
 // Operand table q for lane l (slot s of half h holds K index k(h, s)).
@@ -471,23 +471,23 @@ __device__ void build_table64(int q, int l, const Trig64 &tg, Split &out)
             const int kk = i >> 1, nt = i & 1;
             const int t = ((32 * nt + n) * (16 * kk + rmap(h, s))) & 63;
             f = sn ? -tg.s[t] : tg.c[t];
-        } else if (q < T64_T3N) {  // T3re / T3im[J][ct]: K = k2, column (b, Re | Im)
+        } else if (q < T64_C4) {  // T3re / T3im[J][ct]: K = k2, column (b, Re | Im)
             const bool imk = q >= T64_T3I;
             const int i = q - (imk ? T64_T3I : T64_T3R);
             const int J = i >> 1, ct = i & 1;
             const int k2 = 16 * J + rmap(h, s);
             const int b = 16 * ct + (n & 15), imc = n >> 4;
             if (b < kNB64) {
-                const float cw = k2 == 0 ? 1.f : 2.f;
                 const int t = (k2 * (b - kHW64)) & 63;
-                f = imk ? (imc ? cw * tg.c[t] : -cw * tg.s[t]) : (imc ? cw * tg.s[t] : cw * tg.c[t]);
-            }
-        } else if (q < T64_C4) {  // T3n[ct]: Re / Im of P(k1, 32) in slots 0 / 1 of half 0
-            const int ct = q - T64_T3N;
-            const int b = 16 * ct + (n & 15), imc = n >> 4;
-            if (b < kNB64 && h == 0 && s < 2) {
-                const int t = (32 * (b - kHW64)) & 63;
-                f = s == 0 ? (imc ? tg.s[t] : tg.c[t]) : (imc ? tg.c[t] : -tg.s[t]);
+                if (k2 == 0) {
+                    // the K slots of k2 = 0 carry H(k1, 0) (Re table) and
+                    // H(k1, 32) (Im table), H = Re P - Im P: columns of
+                    // H (1 - i) and H (1 - i) (-1)^l2
+                    const float sg = imk ? tg.c[(32 * (b - kHW64)) & 63] : 1.f;
+                    f = imc ? -sg : sg;
+                } else {
+                    f = imk ? (imc ? 2.f * tg.c[t] : -2.f * tg.s[t]) : (imc ? 2.f * tg.s[t] : 2.f * tg.c[t]);
+                }
             }
         } else {  // C4 / S4[nt][kk]: K = k1, column l1
             const bool sn = q >= T64_S4;
@@ -644,6 +644,14 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                         sp_r[v] = pr;
                         sp_i[v] = pi;
                     }
+                    // k2 = 0 and the Nyquist column k2 = 32 (tile 0, half 0,
+                    // slots 0 and 8), Hermitian in k1: H = Re P - Im P into the
+                    // K slots of Re and Im k2 = 0
+                    if (J == 0 && h == 0) {
+                        const float h0 = sp_r[0] - sp_i[0], h32 = sp_r[8] - sp_i[8];
+                        sp_r[0] = h0;
+                        sp_i[0] = h32;
+                    }
                     Split a3r, a3i;
 #pragma unroll
                     for (int s = 0; s < 8; s += 2) {
@@ -654,19 +662,6 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                     for (int ct = 0; ct < 2; ++ct) {
                         q3[nt][ct] = mfma3(a3r, tabs[T64_T3R + J * 2 + ct][tl], q3[nt][ct]);
                         q3[nt][ct] = mfma3(a3i, tabs[T64_T3I + J * 2 + ct][tl], q3[nt][ct]);
-                    }
-                    if (J == 0) {  // the Nyquist column k2 = 32 (half 0, slots 0 / 1)
-                        Split a3n;
-                        const bool k32 = h == 0;
-                        put2(a3n, 0, k32 ? sp_r[8] : 0.f, k32 ? sp_i[8] : 0.f, neg1);
-#pragma unroll
-                        for (int s = 2; s < 8; ++s) {
-                            a3n.hi[s] = (_Float16)0.f;
-                            a3n.lo[s] = (_Float16)0.f;
-                        }
-#pragma unroll
-                        for (int ct = 0; ct < 2; ++ct)
-                            q3[nt][ct] = mfma3(a3n, tabs[T64_T3N + ct][tl], q3[nt][ct]);
                     }
                 }
             }
