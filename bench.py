@@ -386,8 +386,12 @@ def main(argv=None):
         n = args.calibrate_copy_mib * (1 << 20) // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
+        # two copies of known size: the runtime's blit (copyBuffer) and an
+        # elementwise kernel with ordinary vector loads and stores
         for _ in range(5):
             y.copy_(x)
+        for _ in range(5):
+            torch.mul(x, 2.0, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"calibration_copy_bytes_read": n * 4, "calibration_copy_bytes_written": n * 4}))
         return 0

@@ -369,7 +369,7 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
         const ConfigBufs &cb = get_config(c, P.p);
         hipEvent_t a = get_event(c), b = get_event(c);
         TuneEntry best;
-        float best_ms = 1e30f;
+        float best_score = 1e30f;
         for (const Cand &cd : cand) {
             CorrConfig t = base;
             t.family = cd.family;
@@ -397,11 +397,16 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
                 name += cd.order ? "order=linear" : "order=xcd_bands";
             if (families) name = "family=" + f + (name.empty() ? "" : " " + name);
             best.ms.push_back({name, ms_min});
-            // a later candidate must win by more than timing noise (2 %):
-            // the first listed (the form's own family, XCD bands) also keeps
-            // the L2 traffic lowest
-            if (ms_min < 0.98f * best_ms) {
-                best_ms = ms_min;
+            // a later candidate must win by more than timing noise (2 %);
+            // the first listed is the form's own family in XCD bands. Launch
+            // order deals neighbouring windows to different XCDs, so each L2
+            // fetches most of the pair (2x the HBM bytes of XCD bands for the
+            // 16-px matrix-core kernel, 5x for the statistics kernel), and
+            // back-to-back launches here time it more favourably than the
+            // schedule does: it must win by 5 % more
+            const float score = ms_min * (cd.order ? 1.05f : 1.f);
+            if (score < 0.98f * best_score) {
+                best_score = score;
                 best.family = cd.family;
                 best.waves_per_simd = cd.w;
                 best.linear_order = cd.order;

@@ -31,8 +31,8 @@
 // written for the peak search. Operands are scaled by powers of two to sit
 // in FP16 range (a per-window scale at the input, fixed ones after) and the
 // scale is removed exactly before the magnitude and gain.
-// One wave per window; persistent waves walk contiguous window ranges, the
-// constant operand tables staged once per block in LDS.
+// One wave per window; persistent waves walk the windows at a stride (see
+// wave_windows), the constant operand tables staged once per block in LDS.
 
 #include "hh_internal.h"
 #include "box_peaks.h"
@@ -146,6 +146,38 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The windows a persistent wave takes: base + k * stride, k = 0 .. n - 1
+// (wv waves per block). XCD bands: block b runs on XCD b % 8; XCD x takes a
+// contiguous band of the windows in proportion to its blocks, and its waves
+// take every (waves on the XCD)-th window of the band, wave i the windows
+// i, i + stride, ...: at any moment the waves of one XCD work on adjacent
+// windows, so their 75 % overlap is fetched into that XCD's L2 once (with a
+// contiguous range per wave the XCD's waves read as many separate regions,
+// and the overlap of neighbouring ranges is evicted before it is reused).
+// Launch order: the same striding over all windows and all waves.
+struct WaveWindows {
+    int base, stride, n;
+};
+__device__ __forceinline__ WaveWindows wave_windows(int linear, int nb, int b, int wv, int wave, int njobs)
+{
+    WaveWindows r;
+    int end = njobs;
+    if (linear) {
+        r.base = b * wv + wave;
+        r.stride = nb * wv;
+    } else {
+        const int xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
+        const int before = xcd * q8 + min(xcd, r8);  // blocks on the lower XCDs
+        const int mine = q8 + (xcd < r8 ? 1 : 0);    // blocks on this XCD
+        const int xs = (int)((long long)njobs * before / nb);
+        end = (int)((long long)njobs * (before + mine) / nb);
+        r.base = xs + (b >> 3) * wv + wave;
+        r.stride = mine * wv;
+    }
+    r.n = r.base < end ? (end - r.base + r.stride - 1) / r.stride : 0;
+    return r;
+}
+
 constexpr int kHW = 7;            // box half-width (Gaussian apodization)
 constexpr int kNB = 2 * kHW + 1;  // 15 box rows and columns
 // Operand scales. The input is scaled to |x| < 2^-1, so the stage-2 operand
@@ -164,8 +196,7 @@ constexpr int kS4 = -4;   // stage-3 table scale
 template <int K, bool PHASE>
 __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
                                                      const float *__restrict__ B, int nr,
-                                                     const int32_t *__restrict__ jobs, int njobs, int chunk,
-                                                     CorrOut out)
+                                                     const int32_t *__restrict__ jobs, int njobs, CorrOut out)
 {
 #pragma clang fp contract(off)
     constexpr int M = 32, H = 16, LO = H - kHW;
@@ -249,21 +280,13 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         gain8[v] = (n < kNB && bc < kNB) ? cfg.gain[(LO + n) + (LO + bc) * M] : 0.f;
     }
 
-    // XCD-aware order: block b runs on XCD b % 8; each XCD takes a contiguous
-    // range of window chunks, so the 75 %-overlapping windows of neighbouring
-    // chunks are fetched into one L2 instead of several
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
-    const int blk = cfg.linear_order ? b : xcd * q8 + min(xcd, r8) + (b >> 3);
-    const int gw = blk * 4 + wave;
-    const int w0 = gw * chunk;
-    const int w1 = min(njobs, w0 + chunk);
-    if (w0 >= w1) return;
-    // software pipeline: the pixels of window win + 1 and the job of win + 2
-    // are in flight while window win is processed
+    const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, 4, wave, njobs);
+    if (ww.n <= 0) return;
+    // software pipeline: the pixels of window k + 1 and the job of k + 2 are
+    // in flight while window k is processed
     const int4 *jobs4 = reinterpret_cast<const int4 *>(jobs);
-    int4 jcur = jobs4[w0];
-    int4 jnext = jobs4[min(w0 + 1, w1 - 1)];
+    int4 jcur = jobs4[ww.base];
+    int4 jnext = jobs4[ww.base + min(1, ww.n - 1) * ww.stride];
     float pnext[8];
     {
         const float *p = src + (size_t)(jcur.w - 1 + 8 * h) * nr + (jcur.z - 1 + row);
@@ -272,10 +295,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     }
     // windows in groups of four: the boxes of a group are analyzed together,
     // 16 lanes per window
-    for (int g = w0; g < w1; g += 4) {
-        const int ng = min(4, w1 - g);
+    for (int g = 0; g < ww.n; g += 4) {
+        const int ng = min(4, ww.n - g);
         for (int j = 0; j < ng; ++j) {
-        const int win = g + j;
+        const int k = g + j;
         float *Rb = rbox[wave][j];
         // an opaque zero keeps the table reads inside the loop
         int z;
@@ -290,7 +313,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
             for (int s = 0; s < 8; ++s) pnext[s] = p[(size_t)s * nr];
         }
         jcur = jnext;
-        jnext = jobs4[min(win + 2, w1 - 1)];
+        jnext = jobs4[ww.base + min(k + 2, ww.n - 1) * ww.stride];
         // ---- 0. window: lane (image, row, 8 columns), mean, apodize ----
         float sum = 0.f;
 #pragma unroll
@@ -405,7 +428,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         }
         wave_sync();
         const int slot = l >> 4;
-        const int jb = g + min(slot, ng - 1);
+        const int jb = ww.base + (g + min(slot, ng - 1)) * ww.stride;
         const int4 jj = jobs4[jb];
         float sg = NAN;
         analyze_box<M, K, kNB, LO, 16, 1, false>(cfg, out, rbox[wave][slot], l & 15, l, nullptr, nullptr, slot < ng,
@@ -508,7 +531,7 @@ __device__ void build_table64(int q, int l, const Trig64 &tg, Split &out)
 template <int K, bool PHASE, int WV>
 __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4, WV / 4))) void k_corr_mfma64(
     CorrConfig cfg, Trig64 tg, const float *__restrict__ A, const float *__restrict__ B, int nr,
-    const int32_t *__restrict__ jobs, int njobs, int chunk, CorrOut out)
+    const int32_t *__restrict__ jobs, int njobs, CorrOut out)
 {
 #pragma clang fp contract(off)
     constexpr int M = 64, H = 32, LO = H - kHW64;
@@ -528,19 +551,13 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
     // per window from the cache rather than held in registers
     const float *__restrict__ apodl = cfg.apod + n + 32 * 8 * h;
 
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int xcd = b & 7, q8 = nb >> 3, r8 = nb & 7;
-    const int blk = cfg.linear_order ? b : xcd * q8 + min(xcd, r8) + (b >> 3);
-    const int gw = blk * WV + wave;
-    const int w0 = gw * chunk;
-    const int w1 = min(njobs, w0 + chunk);
-    if (w0 >= w1) return;
+    const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, WV, wave, njobs);
+    if (ww.n <= 0) return;
     const int4 *jobs4 = reinterpret_cast<const int4 *>(jobs);
-    for (int g = w0; g < w1; g += 2) {
-        const int ng = min(2, w1 - g);
+    for (int g = 0; g < ww.n; g += 2) {
+        const int ng = min(2, ww.n - g);
         for (int j = 0; j < ng; ++j) {
-            const int win = g + j;
-            const int4 jw = jobs4[win];
+            const int4 jw = jobs4[ww.base + (g + j) * ww.stride];
             float *Rb = rbox[wave][j];
             int z;
             asm volatile("v_mov_b32 %0, 0" : "=v"(z));
@@ -698,7 +715,7 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
         }
         wave_sync();
         const int slot = l >> 5;
-        const int jb = g + min(slot, ng - 1);
+        const int jb = ww.base + (g + min(slot, ng - 1)) * ww.stride;
         const int4 jj = jobs4[jb];
         float sg = NAN;
         analyze_box<M, K, kNB64, LO, 32, 1, false>(cfg, out, rbox[wave][slot], l & 31, l, nullptr, nullptr,
@@ -748,14 +765,12 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
         }();
         // one block per CU (tables and boxes take 84-108 KB of LDS):
         // waves_per_simd 1 -> 4-wave blocks, 2 (default) -> 8-wave blocks;
-        // persistent waves walk contiguous window ranges
+        // persistent waves walk the windows at a stride (wave_windows)
         const int wps = cfg.waves_per_simd == 1 ? 1 : 2;
         const int wv = 4 * wps;
         const int nblocks = std::max(1, std::min((njobs + wv - 1) / wv, ncu));
-        const int chunk = (njobs + nblocks * wv - 1) / (nblocks * wv);
         auto go = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * wv), 0, s, cfg, tg64, A, B, nr, jobs, njobs, chunk,
-                               out);
+            hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * wv), 0, s, cfg, tg64, A, B, nr, jobs, njobs, out);
         };
         const bool ph = cfg.method == HH_PHASE, k2 = cfg.K == 2;
         if (wps == 1) {
@@ -783,21 +798,20 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
     const int wps = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 3;
     const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 4 * wps));
     const int nblocks = (waves + 3) / 4;
-    const int chunk = (njobs + nblocks * 4 - 1) / (nblocks * 4);
     if (cfg.method == HH_PHASE) {
         if (cfg.K == 2)
             hipLaunchKernelGGL((k_corr_mfma32<2, true>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, chunk, out);
+                               njobs, out);
         else
             hipLaunchKernelGGL((k_corr_mfma32<3, true>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, chunk, out);
+                               njobs, out);
     } else {
         if (cfg.K == 2)
             hipLaunchKernelGGL((k_corr_mfma32<2, false>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, chunk, out);
+                               njobs, out);
         else
             hipLaunchKernelGGL((k_corr_mfma32<3, false>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, chunk, out);
+                               njobs, out);
     }
     HH_HIP(hipGetLastError());
     return true;
