@@ -22,8 +22,10 @@
 //      against table columns that make it H (1 - i) e^(i th k2 l2): summed
 //      over k1 in stage 4 that gives sum H (cos + sin) = sum (Re P cos -
 //      Im P sin), the same contribution, and K = 32 fits two K steps;
-//   4. Q^T (rows = (l2, Re/Im), K = k1) x [cos | sin](k1, l1): the inverse
-//      along k1 for the 15 box rows l1; Re parts minus Im parts give R.
+//   4. Q^T (rows = (l2, Re/Im), K = k1) x [cos | sin](k1, l1), the cos
+//      columns of the 15 box rows l1 in lanes 0-15 and their sin columns in
+//      lanes 16-31 of one product: the inverse along k1; the Re rows of the
+//      cos lanes minus the Im rows of the sin lanes (one lane exchange) give R.
 // Every stage feeds the next from registers: an MFMA result holds one
 // column per lane and 16 rows in a fixed pattern, which is exactly an
 // operand of the next product with K permuted the same way in the constant
@@ -206,9 +208,9 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 
     // ---- constant operand tables: one copy per block in LDS, read per use
     // (registers are left to the accumulators) ----
-    __shared__ Split tabs[9][64];
+    __shared__ Split tabs[7][64];
     if (wave == 0) {
-        Split F1, WrT, WiT, T3[2], C4[2], S4[2];
+        Split F1, WrT, WiT, T3[2], CS4[2];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int c = 8 * h + s;  // stage 1: K = c, column n
@@ -243,15 +245,16 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
             // scaled by 2^kS4: the stage-3 result then enters stage 4 as is
             put(T3[0], s, ldexpf(vb ? t3r : 0.f, kS4));
             put(T3[1], s, ldexpf(vb ? t3i : 0.f, kS4));
-            // stage 4: K = k1 = 16 q + rmap(h, s), column a = n, l1 = a - HW
-            const bool va = n < kNB;
-            const int l1 = n - kHW;
+            // stage 4: K = k1 = 16 q + rmap(h, s), column n = (a = n & 15,
+            // cos for n < 16 | sin), l1 = a - HW
+            const int a = n & 15, sn = n >> 4;
+            const bool va = a < kNB;
+            const int l1 = a - kHW;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int k1 = 16 * q + r;
                 const int t4 = (k1 * l1) & 31;
-                put(C4[q], s, va ? tg.c[t4] : 0.f);
-                put(S4[q], s, va ? tg.s[t4] : 0.f);
+                put(CS4[q], s, va ? (sn ? tg.s[t4] : tg.c[t4]) : 0.f);
             }
         }
         tabs[0][l] = F1;
@@ -259,10 +262,8 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         tabs[2][l] = WiT;
         tabs[3][l] = T3[0];
         tabs[4][l] = T3[1];
-        tabs[5][l] = C4[0];
-        tabs[6][l] = C4[1];
-        tabs[7][l] = S4[0];
-        tabs[8][l] = S4[1];
+        tabs[5][l] = CS4[0];
+        tabs[6][l] = CS4[1];
     }
     __syncthreads();
     const f16v zero = {};
@@ -407,12 +408,15 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 #pragma unroll
             for (int s = 0; s < 8; s += 2)
                 put2(a4[q], s, d3[8 * q + s], d3[8 * q + s + 1], neg1);
-        f16v oc = mfma3(a4[0], tabs[5][tl], zero);
-        oc = mfma3(a4[1], tabs[6][tl], oc);
-        f16v os = mfma3(a4[0], tabs[7][tl], zero);
-        os = mfma3(a4[1], tabs[8][tl], os);
+        f16v o4 = mfma3(a4[0], tabs[5][tl], zero);
+        o4 = mfma3(a4[1], tabs[6][tl], o4);
 
-        // R(l1 = n - HW, l2 = b - HW) = oc[Re row b] - os[Im row b]
+        // R(l1 = n - HW, l2 = b - HW) = (Re row b) x cos - (Im row b) x sin:
+        // the cos products sit in lanes n < 16, the sin products of the same
+        // l1 in lane n + 16 of the same half
+        float sim[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) sim[v] = __shfl_xor(o4[v + 8], 16);
         const int etot = (PHASE ? kS3P : es) + kS4;
         const float scale = ldexpf(1.f / (float)(M * M), -etot);
         if (n < kNB) {
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
             for (int v = 0; v < 8; ++v) {
                 const int b = rmap(h, v);
                 if (b < kNB) {
-                    const float rv = (oc[v] - os[v + 8]) * scale;
+                    const float rv = (o4[v] - sim[v]) * scale;
                     Rb[n + b * kNB] = fabsf(rv) * gain8[v];
                 }
             }
