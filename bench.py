@@ -159,10 +159,10 @@ def oracle_deviation(res, o64):
 
 
 def mfma_flops_per_window_m32() -> float:
-    """Issued matrix-core FLOPs of the 16-px MFMA kernel per window: 33
-    v_mfma_f32_32x32x16_f16 (stages 1-4 with the hi/lo split: 3 + 12 + 6 + 12),
+    """Issued matrix-core FLOPs of the 16-px MFMA kernel per window: 27
+    v_mfma_f32_32x32x16_f16 (stages 1-4 with the hi/lo split: 3 + 12 + 6 + 6),
     2 x 32 x 32 x 16 each."""
-    return 33 * 2 * 32 * 32 * 16
+    return 27 * 2 * 32 * 32 * 16
 
 
 TUNE = {}  # autotuner choices loaded with --tune-file (key -> entry)

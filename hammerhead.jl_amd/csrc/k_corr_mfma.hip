@@ -449,12 +449,15 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 //      the same k2 (rows 16-31; the Im slot of k2 = 0 carries Re k2 = 32);
 //   2. per (J, k1 tile nt): D2 = D1[I][J]^T (K = r) x [cos | -sin](r, k1),
 //      Z(k1, k2) for 8 k2 per lane, the cross (or phase) spectrum P, and
-//   3. at once its contribution to Q[nt][ct] += P (K = Re/Im of k2) x table,
-//      the inverse along k2 for the 27 box columns (2 column tiles ct of
-//      (l2, Re/Im)) with the Hermitian weights in the table, so the spectrum
-//      is never held whole; k2 = 0 and the Nyquist column k2 = 32 enter as
-//      H = Re P - Im P in the K slots of k2 = 0, as in the 16-px kernel;
-//   4. R = Q^T (K = k1 in 4 steps) x [cos | sin](k1, l1) per column tile.
+//   3. at once its contribution to Q_Re[nt], Q_Im[nt] += P (K = Re/Im of
+//      k2) x table, the inverse along k2 for the 27 box columns l2 (one
+//      column tile each for the Re and the Im parts of Q) with the Hermitian
+//      weights in the table, so the spectrum is never held whole; k2 = 0 and
+//      the Nyquist column k2 = 32 enter as H = Re P - Im P in the K slots of
+//      k2 = 0, as in the 16-px kernel;
+//   4. R = [cos | -sin](l1, k1) x [Q_Re; Q_Im] (K = k1 in 4 steps for each
+//      half): the tables as the A operand and the stage-3 results as B, so
+//      R accumulates whole in one tile.
 // Constant operand tables (28 split 32 x 16 operands, 57 KB) live in LDS,
 // one copy per block; boxes of 2 windows are analyzed together, 32 lanes
 // per window.
@@ -469,10 +472,9 @@ enum {
     T64_F1 = 0,    // [kk][J]
     T64_WC = 4,    // [kk][nt]
     T64_WS = 8,    // [kk][nt]
-    T64_T3R = 12,  // [J][ct]
-    T64_T3I = 16,  // [J][ct]
+    T64_T3 = 12,   // [J][source Re | Im][result Re | Im]
     T64_C4 = 20,   // [nt][kk]
-    T64_S4 = 24,   // [nt][kk]
+    T64_S4 = 24,   // [nt][kk] (-sin)
     T64_N = 28
 };
 
@@ -492,18 +494,17 @@ __device__ void build_table64(int q, int l, const Trig64 &tg, Split &out)
                 const int k2 = 16 * J + n - 16;
                 f = k2 == 0 ? ((c & 1) ? -1.f : 1.f) : -tg.s[(k2 * c) & 63];
             }
-        } else if (q < T64_T3R) {  // Wc / Ws[kk][nt]: K = r, column k1
+        } else if (q < T64_T3) {  // Wc / Ws[kk][nt]: K = r, column k1
             const bool sn = q >= T64_WS;
             const int i = q - (sn ? T64_WS : T64_WC);
             const int kk = i >> 1, nt = i & 1;
             const int t = ((32 * nt + n) * (16 * kk + rmap(h, s))) & 63;
             f = sn ? -tg.s[t] : tg.c[t];
-        } else if (q < T64_C4) {  // T3re / T3im[J][ct]: K = k2, column (b, Re | Im)
-            const bool imk = q >= T64_T3I;
-            const int i = q - (imk ? T64_T3I : T64_T3R);
-            const int J = i >> 1, ct = i & 1;
+        } else if (q < T64_C4) {  // T3[J][src][dst]: K = k2, column l2 = b - HW, b = n
+            const int i = q - T64_T3;
+            const int J = i >> 2, imk = (i >> 1) & 1, imc = i & 1;
             const int k2 = 16 * J + rmap(h, s);
-            const int b = 16 * ct + (n & 15), imc = n >> 4;
+            const int b = n;
             if (b < kNB64) {
                 const int t = (k2 * (b - kHW64)) & 63;
                 if (k2 == 0) {
@@ -516,13 +517,13 @@ __device__ void build_table64(int q, int l, const Trig64 &tg, Split &out)
                     f = imk ? (imc ? 2.f * tg.c[t] : -2.f * tg.s[t]) : (imc ? 2.f * tg.s[t] : 2.f * tg.c[t]);
                 }
             }
-        } else {  // C4 / S4[nt][kk]: K = k1, column l1
+        } else {  // C4 / S4[nt][kk]: the A operand of stage 4, row l1 = n - HW, K = k1
             const bool sn = q >= T64_S4;
             const int i = q - (sn ? T64_S4 : T64_C4);
             const int nt = i >> 1, kk = i & 1;
             if (n < kNB64) {
                 const int t = ((32 * nt + 16 * kk + rmap(h, s)) * (n - kHW64)) & 63;
-                f = sn ? tg.s[t] : tg.c[t];
+                f = sn ? -tg.s[t] : tg.c[t];
             }
         }
         put(out, s, f);
@@ -606,11 +607,11 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                              ldexpf(apodl[32 * (16 * kk + s + 1)] * px[I][8 * kk + s + 1], e0), neg1);
             }
             const int es = eI[0] + eI[1];
-            f16v q3[2][2];  // Q[nt][ct]
+            f16v q3[2][2];  // Q[nt][Re | Im]: rows k1 of tile nt, columns l2
 #pragma unroll
             for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct) q3[nt][ct] = zero;
+                for (int ri = 0; ri < 2; ++ri) q3[nt][ri] = zero;
 #pragma unroll
             for (int J = 0; J < 2; ++J) {
                 // ---- 1. column DFT of both windows, parts of tile J ----
@@ -680,40 +681,40 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                         put2(a3i, s, sp_i[s], sp_i[s + 1], neg1);
                     }
 #pragma unroll
-                    for (int ct = 0; ct < 2; ++ct) {
-                        q3[nt][ct] = mfma3(a3r, tabs[T64_T3R + J * 2 + ct][tl], q3[nt][ct]);
-                        q3[nt][ct] = mfma3(a3i, tabs[T64_T3I + J * 2 + ct][tl], q3[nt][ct]);
+                    for (int ri = 0; ri < 2; ++ri) {
+                        q3[nt][ri] = mfma3(a3r, tabs[T64_T3 + J * 4 + 0 * 2 + ri][tl], q3[nt][ri]);
+                        q3[nt][ri] = mfma3(a3i, tabs[T64_T3 + J * 4 + 1 * 2 + ri][tl], q3[nt][ri]);
                     }
                 }
             }
-            // ---- 4. inverse along k1 for the box rows, per column tile ----
+            // ---- 4. R = [cos | -sin](l1, k1) x [Q_Re; Q_Im](k1, l2): the
+            // tables as A (rows l1), the stage-3 results as B (columns l2,
+            // K = k1 in the slot order of their result rows) ----
             const int etot = (PHASE ? kS3P : es) + kS4_64;
             const float scale = ldexpf(1.f / (float)(M * M), -etot);
+            f16v r4 = zero;
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-                f16v oc = zero, os = zero;
+            for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-                for (int nt = 0; nt < 2; ++nt)
+                for (int kk = 0; kk < 2; ++kk) {
+                    Split br, bi;
 #pragma unroll
-                    for (int kk = 0; kk < 2; ++kk) {
-                        Split a4;
-#pragma unroll
-                        for (int s = 0; s < 8; s += 2)
-                            put2(a4, s, ldexpf(q3[nt][ct][8 * kk + s], kS4_64),
-                                 ldexpf(q3[nt][ct][8 * kk + s + 1], kS4_64), neg1);
-                        oc = mfma3(a4, tabs[T64_C4 + nt * 2 + kk][tl], oc);
-                        os = mfma3(a4, tabs[T64_S4 + nt * 2 + kk][tl], os);
+                    for (int s = 0; s < 8; s += 2) {
+                        put2(br, s, ldexpf(q3[nt][0][8 * kk + s], kS4_64), ldexpf(q3[nt][0][8 * kk + s + 1], kS4_64),
+                             neg1);
+                        put2(bi, s, ldexpf(q3[nt][1][8 * kk + s], kS4_64), ldexpf(q3[nt][1][8 * kk + s + 1], kS4_64),
+                             neg1);
                     }
-                // R(l1 = n - HW, l2 = b - HW) = oc[Re row] - os[Im row]
-                if (n < kNB64) {
+                    r4 = mfma3(tabs[T64_C4 + nt * 2 + kk][tl], br, r4);
+                    r4 = mfma3(tabs[T64_S4 + nt * 2 + kk][tl], bi, r4);
+                }
+            // lane column n = l2 index, rows l1 = rmap(h, v)
+            if (n < kNB64) {
 #pragma unroll
-                    for (int v = 0; v < 8; ++v) {
-                        const int bb = 16 * ct + rmap(h, v);
-                        if (bb < kNB64) {
-                            const float rv = (oc[v] - os[v + 8]) * scale;
-                            Rb[n + bb * kNB64] = fabsf(rv) * cfg.gain[(LO + n) + (LO + bb) * M];
-                        }
-                    }
+                for (int v = 0; v < 16; ++v) {
+                    const int a = rmap(h, v);
+                    if (a < kNB64)
+                        Rb[a + n * kNB64] = fabsf(r4[v] * scale) * cfg.gain[(LO + a) + (LO + n) * M];
                 }
             }
         }
