@@ -262,9 +262,16 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
         const int lin = q < N / 2 ? q : q + 1;
         const int dc = lin / W - NB, dr = lin % W - NB;
         const int r2 = r + dr, c2 = c + dc;
-        ok[q] = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !fg[r2 + c2 * ny] && !(mg && mg[r2 + c2 * ny]);
-        nu[q] = ok[q] ? (double)ug[r2 + c2 * ny] : INFINITY;
-        nv[q] = ok[q] ? (double)vg[r2 + c2 * ny] : INFINITY;
+        // clamped addresses: the flag, mask and vector loads of every
+        // neighbour are unconditional and in flight together (a load behind
+        // its flag test would wait for the flag)
+        const int rc = min(max(r2, 0), ny - 1), cc = min(max(c2, 0), nx - 1);
+        const int at = rc + cc * ny;
+        const uint8_t fq = fg[at], mq = mg ? mg[at] : (uint8_t)0;
+        const float uq = ug[at], vq = vg[at];
+        ok[q] = r2 == rc && c2 == cc && !fq && !mq;
+        nu[q] = ok[q] ? (double)uq : INFINITY;
+        nv[q] = ok[q] ? (double)vq : INFINITY;
         nanu |= ok[q] && isnan(nu[q]);
         nanv |= ok[q] && isnan(nv[q]);
         n += ok[q];
@@ -407,7 +414,6 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
         return;
     }
     if (!invalid_all[e]) return;
-    if (nvalid[k.si] < 3) return;
     const int ny = sg.ny, nx = sg.nx, r = k.r, c = k.c;
     const float *__restrict__ uref = uref_all + sg.off;
     const float *__restrict__ vref = vref_all + sg.off;
@@ -415,20 +421,28 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
     const uint8_t *__restrict__ mask = mask_all ? mask_all + sg.off : nullptr;
     auto bad = [&](int at) { return invalid[at] || (mask && mask[at]); };
     {
-        // radius 1 (the common case) from registers
+        // radius 1 (the common case) from registers; the segment's count and
+        // every neighbour's flag, mask and vector are loaded together at
+        // clamped addresses (a load behind a test would wait for it)
+        const int nval = nvalid[k.si];
         double nu[9], nv[9];
         int n = 0;
         bool nanu = false, nanv = false;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             const int r2 = r + q % 3 - 1, c2 = c + q / 3 - 1;
-            const bool ok = r2 >= 0 && r2 < ny && c2 >= 0 && c2 < nx && !bad(r2 + c2 * ny);
-            nu[q] = ok ? (double)uref[r2 + c2 * ny] : INFINITY;
-            nv[q] = ok ? (double)vref[r2 + c2 * ny] : INFINITY;
+            const int rc = min(max(r2, 0), ny - 1), cc = min(max(c2, 0), nx - 1);
+            const int at = rc + cc * ny;
+            const bool bq = invalid[at] || (mask && mask[at]);
+            const float uq = uref[at], vq = vref[at];
+            const bool ok = r2 == rc && c2 == cc && !bq;
+            nu[q] = ok ? (double)uq : INFINITY;
+            nv[q] = ok ? (double)vq : INFINITY;
             nanu |= ok && isnan(nu[q]);
             nanv |= ok && isnan(nv[q]);
             n += ok;
         }
+        if (nval < 3) return;
         if (n >= 3) {
             sort_net<double, 9>(nu);
             sort_net<double, 9>(nv);
