@@ -147,6 +147,50 @@ __device__ float moment_b(const V &R, float pr, float pc)
     return (float)sqrt((sdx + sdy) / sumC);
 }
 
+template <int R>
+__device__ __forceinline__ double row_ror_d(double v)  // DPP row_ror:R of a double
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x120 + R, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x120 + R, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double row_sum16_d(double v)  // over the 16 lanes of a DPP row, in every lane
+{
+    v += row_ror_d<8>(v);
+    v += row_ror_d<4>(v);
+    v += row_ror_d<2>(v);
+    v += row_ror_d<1>(v);
+    return v;
+}
+
+// moment_b with its (at most 9) points spread over lanes t = 0..8 of the
+// window's first DPP row (the window's lanes start on a 16-lane boundary):
+// lane t takes column c0 - 1 + t / 3, row r0 - 1 + t % 3, the three Float64
+// sums are reduced by row rotations. Every lane of the row calls it (no
+// early exit before the rotations); the result is valid on all of them.
+template <int M, typename V>
+__device__ __forceinline__ float moment_row(const V &R, float pr, float pc, int t)
+{
+    const bool fin = isfinite(pr) && isfinite(pc);
+    const int r0 = fin ? (int)rintf(pr) : 1, c0 = fin ? (int)rintf(pc) : 1;
+    const int rlo = max(1, r0 - 1), rhi = min(M, r0 + 1);
+    const int clo = max(1, c0 - 1), chi = min(M, c0 + 1);
+    const int c = c0 - 1 + t / 3, r = r0 - 1 + t % 3;
+    double w = 0.0, tx = 0.0, ty = 0.0;
+    if (fin && t < 9 && c >= clo && c <= chi && r >= rlo && r <= rhi) {
+        w = (double)R(r - 1, c - 1);
+        w = w > 0.0 ? w : 0.0;
+        const float dx = (float)c - pc, dy = (float)r - pr;
+        tx = (double)(dx * dx) * w;
+        ty = (double)(dy * dy) * w;
+    }
+    const double sumC = row_sum16_d(w), sdx = row_sum16_d(tx), sdy = row_sum16_d(ty);
+    if (!fin || rlo > rhi || clo > chi || sumC < 1e-9) return NAN;
+    return (float)sqrt((sdx + sdy) / sumC);
+}
+
 // Regional-max top-K over the box Rb (NB x NB column-major, plane rows and
 // columns LO .. LO + NB - 1, fftshifted) by the G lanes t = 0 .. G - 1 of
 // one window (NWAVE waves per window, exchanging through mv/ml), then the
@@ -330,6 +374,8 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             fits[m] = make_float2(NAN, NAN);
         }
     }
+    float mom_row = NAN;
+    if constexpr (G >= 16) mom_row = moment_row<M>(R, fits[0].x, fits[0].y, t);
     if (t != 0 || !active) return;
     const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
     const size_t ng = (size_t)out.ny * out.nx;
@@ -358,7 +404,11 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
     const float du = ref.y - cc, dv = ref.x - cc;
     const float v1 = sv[0];
     const float ratio = found >= 2 ? v1 / sv[1] : (v1 > 0.f ? INFINITY : NAN);
-    const float mom = moment_b<M>(R, ref.x, ref.y);
+    float mom;
+    if constexpr (G >= 16)
+        mom = mom_row;
+    else
+        mom = moment_b<M>(R, ref.x, ref.y);
     const float u0 = out.u0 ? out.u0[g] : 0.f, v0 = out.v0 ? out.v0[g] : 0.f;
     out.u[g] = out.u0 ? u0 + du : du;
     out.v[g] = out.v0 ? v0 + dv : dv;
