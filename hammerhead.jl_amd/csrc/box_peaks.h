@@ -22,47 +22,34 @@ __device__ __forceinline__ bool better(float av, int al, float bv, int bl)
     return av > bv || (av == bv && al < bl);
 }
 
-// A candidate (value > 0, location >= 0) as one integer whose order is that
-// of better(): value bits (monotonic for positive floats) above the
-// complement of the location; an empty slot (INT_MAX) is 0.
-__device__ __forceinline__ unsigned long long cand_key(float v, int l)
-{
-    return l == INT_MAX ? 0ull : ((unsigned long long)__float_as_uint(v) << 32) | (0xFFFFFFFFu - (unsigned)l);
-}
-
-template <int R>
-__device__ __forceinline__ unsigned long long key_ror(unsigned long long k)
-{
-    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)k, 0x120 + R, 0xF, 0xF, true);   // row_ror:R
-    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(k >> 32), 0x120 + R, 0xF, 0xF, true);
-    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
-}
-
-__device__ __forceinline__ unsigned long long key_xor(unsigned long long k, int o)
-{
-    const int lo = __shfl_xor((int)(unsigned)k, o), hi = __shfl_xor((int)(unsigned)(k >> 32), o);
-    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
-}
-
-// Maximum key over the G lanes of a window (groups of G lanes aligned to G;
-// at most the 64 lanes of a wave): rotations within each 16-lane DPP row
-// reduce the row, exchanges across rows finish it.
+// The best candidate over the G lanes of a window (groups of G lanes aligned
+// to G), in the order of better(): the largest value (positive floats order
+// as their bit patterns; an empty slot counts as 0), then, among the lanes
+// holding it, the smallest location. Two 32-bit reductions instead of one
+// over 64-bit keys.
 template <int G>
-__device__ __forceinline__ unsigned long long cand_key_max(unsigned long long k)
+__device__ __forceinline__ void cand_best(float v, int l, float &bv, int &bl)
 {
-    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
-    if constexpr (G >= 16) {
-        k = mx(k, key_ror<8>(k));
-        k = mx(k, key_ror<4>(k));
-        k = mx(k, key_ror<2>(k));
-        k = mx(k, key_ror<1>(k));
-        if constexpr (G >= 32) k = mx(k, key_xor(k, 16));
-        if constexpr (G >= 64) k = mx(k, key_xor(k, 32));
-    } else {
+    unsigned vb = l == INT_MAX ? 0u : __float_as_uint(v);
+    auto red = [](unsigned x, auto op) {
+        if constexpr (G >= 16) {
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, true));  // row_ror:8
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x124, 0xF, 0xF, true));  // row_ror:4
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x122, 0xF, 0xF, true));  // row_ror:2
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x121, 0xF, 0xF, true));  // row_ror:1
+            if constexpr (G >= 32) x = op(x, (unsigned)__shfl_xor((int)x, 16));
+            if constexpr (G >= 64) x = op(x, (unsigned)__shfl_xor((int)x, 32));
+        } else {
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) k = mx(k, key_xor(k, o));
-    }
-    return k;
+            for (int o = G / 2; o > 0; o >>= 1) x = op(x, (unsigned)__shfl_xor((int)x, o));
+        }
+        return x;
+    };
+    const unsigned vm = red(vb, [](unsigned a, unsigned b) { return a > b ? a : b; });
+    const unsigned lc = (vm != 0u && vb == vm) ? (unsigned)l : 0xFFFFFFFFu;
+    const unsigned lm = red(lc, [](unsigned a, unsigned b) { return a < b ? a : b; });
+    bv = vm ? __uint_as_float(vm) : 0.f;
+    bl = vm ? (int)lm : INT_MAX;
 }
 
 // Box plane accessor: plane (0-based, fftshifted) -> value; 0 outside box.
@@ -301,9 +288,9 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
     int nsel = 0;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-        const unsigned long long best = cand_key_max<G>(cand_key(tv[0], tl[0]));
-        float bv = best ? __uint_as_float((unsigned)(best >> 32)) : 0.f;
-        int bl = best ? (int)(0xFFFFFFFFu - (unsigned)best) : INT_MAX;
+        float bv;
+        int bl;
+        cand_best<G>(tv[0], tl[0], bv, bl);
         if constexpr (NWAVE > 1) {
             if ((tid & 63) == 0) {
                 mv[wave] = bv;
