@@ -11,7 +11,7 @@ for f in k_correlate k_corr_fast k_corr_mfma k_deform k_dewarp k_validate k_mask
   src="$HERE/$f.hip"; [ -f "$src" ] || src="$HERE/$f.cpp"
   extra=()
   # matrix-core accumulators in VGPRs: no AGPR round trips, higher occupancy
-  [ "$f" = k_corr_mfma ] && extra=(-mllvm -amdgpu-mfma-vgpr-form=1)
+  [ "$f" = k_corr_mfma ] || [ "$f" = k_corr_fast ] && extra=(-mllvm -amdgpu-mfma-vgpr-form=1)
   hipcc "${FLAGS[@]}" "${extra[@]}" -x hip -c "$src" -o "$OBJ/$f.o" 2> "$OBJ/$f.log" &
   pids+=($!)
 done

@@ -631,19 +631,194 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
     sig_v = rowsh<1>(sg);
 }
 
-// Wieneke statistics alone for 16-px windows (the uncertainty of a sweep
-// whose correlation runs elsewhere, e.g. on the matrix cores): 16 lanes per
-// window, one row per lane, windows formed exactly as k_corrL<1> forms them
-// (same loads, means and apodization), then uq_fused16. Blocks of 4 windows
-// in XCD bands.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_uq16_rows(CorrConfig cfg, const float *__restrict__ A,
-                                                 const float *__restrict__ B, int nr,
-                                                 const int32_t *__restrict__ jobs, int njobs, int nblocks,
-                                                 CorrOut out)
+// ---- Wieneke statistics with the covariance sums on the matrix cores ------
+// The uncertainty of a 16-px sweep whose correlation runs on the matrix cores
+// (the FFT form fuses uq_fused16 into its correlation kernel instead).
+// For one component with centred, smoothed values e(r, k) (u: r = window row,
+// k = column 0..14; v: r = window column, k = row 0..14) the 41 half-plane
+// covariance sums S(dr, dk) = sum e(r, k) e(r + dr, k + dk) (dk = 1..4 with
+// dr = -4..4, and dk = 0 with dr = 0..4, dr = dk = 0 being S00) are diagonal
+// sums of five 16 x 16 products G_dk(r, r') = sum_k e(r, k) e(r', k + dk):
+// S(dr, dk) = sum_r G_dk(r, r + dr). An entry of G is the row partial that
+// uq_lags_lds forms as an FP32 dot product of at most 15 FP32 products; here
+// v_mfma_f32_16x16x16_f16 forms it from e split into FP16 hi + lo parts
+// (hi*hi + hi*lo + lo*hi, FP32 accumulation, ~22-bit operands), after a
+// power-of-two scale that puts max |e| of the (window, component) in
+// [1/2, 1) and is removed exactly afterwards. The diagonal sums over r are
+// FP64, as the cross-row sums of uq_lags_lds.
+// One wave per block, 4 windows per wave. The fields are formed as in
+// uq_fused16 (16 lanes per window, one window row per lane; C0, C+, C- the
+// same FP64 sums of FP32 products) and go to LDS as FP32, centred by the
+// FP64 window mean rounded to FP32. Each (window, component) then takes its
+// 15 products; the entries of the nine diagonals |r' - r| <= 4 go to LDS
+// diagonal by diagonal, so that lane k < 41 reads the 16 entries of lag k
+// with four 16-byte loads and sums them in FP64; the lanes (window,
+// component) finish the rings and sigma.
+// LDS strides are chosen so that the field writes, the operand reads and the
+// diagonal stores and loads spread over the banks. The kernel is bound by
+// latency at its occupancy (14 KB of LDS and 153 VGPRs per wave), not by the
+// VALU or the matrix pipe: FP32 products on v_mfma_f32_16x16x4_f32 (the same
+// precision as the VALU row partials) ran no faster than the VALU, since
+// they do not co-execute with VALU work and G is twice the band of lags.
+constexpr int kUE = 20;              // e row pitch: k = 0..14, zero pads 15..19
+constexpr int kUC = 320;             // component stride of the fields (16 * kUE)
+constexpr int kUW = 656;             // window stride of the fields
+constexpr int kUD = 20;              // diagonal pitch (16 entries r = 0..15)
+constexpr int kUDK = 180;            // per dk: diagonals dr + 4 = 0..8
+constexpr int kUL = 41;              // lags: S00 and the 40 half-plane offsets
+
+// Lag k: k = 0..4 -> (dr, dk) = (k, 0); k = 5..40 -> dk = 1 + (k - 5) / 9,
+// dr = (k - 5) % 9 - 4. Ring max(|dr|, dk) (0 for S00).
+__host__ __device__ constexpr int uqm_dk(int k) { return k < 5 ? 0 : 1 + (k - 5) / 9; }
+__host__ __device__ constexpr int uqm_dr(int k) { return k < 5 ? k : (k - 5) % 9 - 4; }
+__host__ __device__ constexpr int uqm_ring(int k)
+{
+    const int a = uqm_dr(k) < 0 ? -uqm_dr(k) : uqm_dr(k);
+    return a > uqm_dk(k) ? a : uqm_dk(k);
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h2u __attribute__((ext_vector_type(2)));
+typedef float f2u __attribute__((ext_vector_type(2)));
+
+struct Split4 {
+    h4v hi, lo;
+};
+
+// x = hi + lo with hi = f16(x), lo = f16(x - hi): one packed conversion per
+// two values, and per value one mixed-precision FMA x + hi * (-1) (exact in
+// FP32 inside the FMA) written rounded into its half of lo.
+__device__ __forceinline__ Split4 split4(f4v x, float neg1)
+{
+    Split4 s;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const f2u xv = {x[2 * p], x[2 * p + 1]};
+        const h2u hi = __builtin_convertvector(xv, h2u);
+        const unsigned hp = __builtin_bit_cast(unsigned, hi);
+        unsigned lp;
+        asm("v_fma_mixlo_f16 %0, %1, %2, %3 op_sel_hi:[1,0,0]\n\t"
+            "v_fma_mixhi_f16 %0, %1, %2, %4 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+            : "=&v"(lp)
+            : "v"(hp), "v"(neg1), "v"(xv[0]), "v"(xv[1]));
+        const h2u lo = __builtin_bit_cast(h2u, lp);
+        s.hi[2 * p] = hi[0];
+        s.hi[2 * p + 1] = hi[1];
+        s.lo[2 * p] = lo[0];
+        s.lo[2 * p + 1] = lo[1];
+    }
+    return s;
+}
+
+__device__ __forceinline__ f4v mfma3_16(const Split4 &a, const Split4 &b, f4v acc)
+{
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, acc, 0, 0, 0);
+    return acc;
+}
+
+// max |e| over the 16 lanes of the window -> exponent p with max |e| 2^p in
+// [1/2, 1) (0 for an all-zero field)
+__device__ __forceinline__ int uqm_scale_exp(float amx)
+{
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) amx = fmaxf(amx, __shfl_xor(amx, o));
+    int ex = 0;
+    (void)frexpf(amx, &ex);
+    return amx > 0.f ? -ex : 0;
+}
+
+// The u and v fields of uq_component! (src/uncertainty.jl:104-151) for the
+// matrix-core kernel: C0, C+, C- as in uq_fused16; e = the smoothed dC values
+// minus the FP64 window mean rounded to FP32.
+__device__ __forceinline__ void uqm_field_u(const float2 (&x)[16], double (&C)[3], float (&e)[15])
 {
 #pragma clang fp contract(off)
-    __shared__ float ured[4][16 * kUQP];
+    const float q = 0.25f;
+    float dC[15];
+    double c0 = 0.0, cp = 0.0, cm = 0.0;
+#pragma unroll
+    for (int c = 0; c < 15; ++c) {
+        const float a0 = x[c].x, a1 = x[c + 1].x, b0 = x[c].y, b1 = x[c + 1].y;
+        const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
+        c0 += (double)p00;
+        cp += (double)p01;
+        cm += (double)p10;
+        dC[c] = p01 - p10;
+    }
+    double mu = 0.0;
+#pragma unroll
+    for (int c = 0; c < 15; ++c) {
+        const int cl = c > 0 ? c - 1 : 0, cr = c < 14 ? c + 1 : 14;
+        e[c] = q * ((dC[cl] + 2.f * dC[c]) + dC[cr]);
+        mu += (double)e[c];
+    }
+    C[0] = row_sum_d(c0);
+    C[1] = row_sum_d(cp);
+    C[2] = row_sum_d(cm);
+    const float mf = (float)(row_sum_d(mu) / 240.0);
+#pragma unroll
+    for (int c = 0; c < 15; ++c) e[c] -= mf;
+}
+
+// lane r = i holds e(i; j), j = 0..15, for i = 0..14 (lane 15 holds zeros)
+__device__ __forceinline__ void uqm_field_v(const float2 (&x)[16], int r, double (&C)[3], float (&e)[16])
+{
+#pragma clang fp contract(off)
+    const float q = 0.25f;
+    const bool live = r < 15;
+    float dC[16];
+    double c0 = 0.0, cp = 0.0, cm = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float a0 = x[j].x, b0 = x[j].y;
+        const float a1 = rowsh<1>(x[j].x), b1 = rowsh<1>(x[j].y);
+        const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
+        if (live) {
+            c0 += (double)p00;
+            cp += (double)p01;
+            cm += (double)p10;
+        }
+        dC[j] = live ? p01 - p10 : 0.f;
+    }
+    double mu = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float up = rowsh<-1>(dC[j]), dn = rowsh<1>(dC[j]);
+        const float lft = r > 0 ? up : dC[j];
+        const float rgt = r < 14 ? dn : dC[j];
+        e[j] = live ? q * ((lft + 2.f * dC[j]) + rgt) : 0.f;
+        mu += (double)e[j];
+    }
+    C[0] = row_sum_d(c0);
+    C[1] = row_sum_d(cp);
+    C[2] = row_sum_d(cm);
+    const float mf = (float)(row_sum_d(mu) / 240.0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) e[j] = live ? e[j] - mf : 0.f;
+}
+
+__global__ __launch_bounds__(64) void k_uq16_mfma(CorrConfig cfg, const float *__restrict__ A,
+                                                  const float *__restrict__ B, int nr,
+                                                  const int32_t *__restrict__ jobs, int njobs, int nblocks,
+                                                  CorrOut out)
+{
+#pragma clang fp contract(off)
+    // fields: (window w, component c) e(r, k) at w * kUW + c * kUC + r * kUE + k;
+    // once the products of (w, c) are formed its lag totals (doubles) take
+    // the same place
+    __shared__ __align__(16) float ef[4 * kUW];
+    __shared__ __align__(16) float gd[5 * kUDK];  // G_dk(r, r + dr) at dk * kUDK + (dr + 4) * kUD + r
+    __shared__ double cst[4][2][3];               // C0, C+, C-
+    __shared__ int pex[4][2];                     // field scale exponents
+    static_assert(kUC >= 16 * kUE && kUW >= 2 * kUC && kUC % 2 == 0 && kUW % 2 == 0 && kUC / 2 >= kUL,
+                  "field layout");
     const int tid = threadIdx.x, line = tid & 15, slot_w = tid >> 4;
+    // diagonal entries past the ends of G (r + dr outside 0..15) are never
+    // stored: zero once
+    for (int i = tid; i < 5 * kUDK; i += 64) gd[i] = 0.f;
     const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
@@ -678,14 +853,125 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const float a = cfg.apod[line + 16 * j];
         x[j] = make_float2(a * (ra[j] - meanA), a * (rb[j] - meanB));
     }
-    float sig_u, sig_v;
-    uq_fused16(x, line, sig_u, sig_v, ured[slot_w]);
-    if (active && line == 0) {
-        const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
-        out.unc_u[g] = sig_u;
-        out.unc_v[g] = sig_v;
+    float *ew = ef + slot_w * kUW;
+    {
+        double C[3];
+        float e[15];
+        uqm_field_u(x, C, e);
+        float amx = 0.f;
+#pragma unroll
+        for (int c = 0; c < 15; ++c) amx = fmaxf(amx, fabsf(e[c]));
+        const int p = uqm_scale_exp(amx);
+        float *er = ew + line * kUE;  // r = line, k = c
+#pragma unroll
+        for (int c = 0; c < 15; ++c) er[c] = ldexpf(e[c], p);
+#pragma unroll
+        for (int c = 15; c < kUE; ++c) er[c] = 0.f;
+        if (line == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) cst[slot_w][0][i] = C[i];
+            pex[slot_w][0] = p;
+        }
+    }
+    {
+        double C[3];
+        float e[16];
+        uqm_field_v(x, line, C, e);
+        float amx = 0.f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) amx = fmaxf(amx, fabsf(e[j]));
+        const int p = uqm_scale_exp(amx);
+        float *ec = ew + kUC;  // r = window column j, k = row i = line (line 15: zeros)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) ec[j * kUE + line] = ldexpf(e[j], p);
+#pragma unroll
+        for (int k = 16; k < kUE; ++k) ec[line * kUE + k] = 0.f;
+        if (line == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) cst[slot_w][1][i] = C[i];
+            pex[slot_w][1] = p;
+        }
+    }
+    lds_wave_sync();
+
+    // operands of v_mfma_f32_16x16x16_f16: lane l gives A(i = l % 16, k =
+    // 4 (l / 16) + 0..3) and B(k = 4 (l / 16) + 0..3, j = l % 16); it
+    // receives D(4 (l / 16) + v, l % 16)
+    const int l = tid, lq = l >> 4, lj = l & 15;
+    const int eo = lj * kUE + 4 * lq;
+    const float neg1 = -1.f;
+    // lag of this lane for the diagonal sums (lanes 41-63 read lag 0's)
+    const int lk = l < kUL ? l : 0;
+    const int gb = uqm_dk(lk) * kUDK + (uqm_dr(lk) + 4) * kUD;
+#pragma unroll 1
+    for (int wc = 0; wc < 8; ++wc) {
+        float *e = ef + (wc >> 1) * kUW + (wc & 1) * kUC;
+        // k = 4 lq + 0..3 (A) and 4 lq + 4..7: two aligned 16-byte loads; the
+        // B operand of dk takes k + dk = 4 lq + dk .. 4 lq + dk + 3 from the
+        // two split quads (the field's zero pads end each row)
+        const f4v *er = reinterpret_cast<const f4v *>(e + eo);
+        const Split4 q0 = split4(er[0], neg1), q1 = split4(er[1], neg1);
+        f4v g[5];
+#pragma unroll
+        for (int dk = 0; dk < 5; ++dk) {
+            Split4 bq;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                bq.hi[j] = j + dk < 4 ? q0.hi[j + dk] : q1.hi[j + dk - 4];
+                bq.lo[j] = j + dk < 4 ? q0.lo[j + dk] : q1.lo[j + dk - 4];
+            }
+            g[dk] = mfma3_16(q0, bq, (f4v){0.f, 0.f, 0.f, 0.f});
+        }
+        // entries (r, r') = (4 lq + v, lj) with |r' - r| <= 4, by diagonal
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 4 * lq + v, dr = lj - r;
+            if (dr >= -4 && dr <= 4) {
+                float *p = gd + (dr + 4) * kUD + r;
+#pragma unroll
+                for (int dk = 0; dk < 5; ++dk) p[dk * kUDK] = g[dk][v];
+            }
+        }
+        lds_wave_sync();
+        const f4v *dp = reinterpret_cast<const f4v *>(gd + gb);
+        double s4[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const f4v q = dp[i];
+            s4[i] = ((double)q[0] + (double)q[1]) + ((double)q[2] + (double)q[3]);
+        }
+        // the lag total without the field scale (exact)
+        const double tot = ldexp((s4[0] + s4[1]) + (s4[2] + s4[3]), -2 * pex[wc >> 1][wc & 1]);
+        lds_wave_sync();
+        // the field of (w, c) is consumed: its place takes the lag totals
+        if (l < kUL) reinterpret_cast<double *>(e)[l] = tot;
+    }
+    lds_wave_sync();
+    // lanes (window, component): rings and sigma (finalize_uncertainty,
+    // src/uncertainty.jl:162-179); a NaN never replaces a ring maximum
+    if (line < 2) {
+        UQAcc acc;
+        uq_init(acc);
+        const double *s = reinterpret_cast<const double *>(ew + line * kUC);
+        acc.C0 = cst[slot_w][line][0];
+        acc.Cp = cst[slot_w][line][1];
+        acc.Cm = cst[slot_w][line][2];
+        acc.S00 = s[0];
+#pragma unroll
+        for (int k = 1; k < kUL; ++k) {
+            const int q = uqm_ring(k) - 1;
+            const double t = s[k];
+            acc.rmax[q] = t > acc.rmax[q] ? t : acc.rmax[q];
+            acc.rsum[q] += t;
+        }
+        const float sg = uq_sigma(acc);
+        if (active) {
+            const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
+            (line ? out.unc_v : out.unc_u)[g] = sg;
+        }
     }
 }
+
 
 // Correlation of w = 16 L windows padded to M = 32 L. L lanes per line of the
 // 2D transform, G = 16 L^2 lanes per window. HW: half-width of the analysis
@@ -1043,13 +1329,13 @@ static int fast_halfwidth(const CorrConfig &cfg)
 
 bool correlate_box_layout(const CorrConfig &cfg) { return fast_halfwidth(cfg) >= 0; }
 
-bool launch_uq16_rows(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+bool launch_uq16_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
                       const CorrOut &out, hipStream_t s)
 {
     if (cfg.w != 16 || !out.unc_u || !out.unc_v) return false;
     if (njobs <= 0) return true;
     const int nblocks = (njobs + 3) / 4;
-    hipLaunchKernelGGL(k_uq16_rows, dim3(nblocks), dim3(64), 0, s, cfg, A, B, nr, jobs, njobs, nblocks, out);
+    hipLaunchKernelGGL(k_uq16_mfma, dim3(nblocks), dim3(64), 0, s, cfg, A, B, nr, jobs, njobs, nblocks, out);
     HH_HIP(hipGetLastError());
     return true;
 }

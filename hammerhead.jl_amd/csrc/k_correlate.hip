@@ -738,13 +738,13 @@ static float2 *twiddle_table(int M, hipStream_t s)
 
 const char *fast_family(const CorrConfig &cfg);
 bool mfma_covers(const CorrConfig &cfg);
-bool launch_uq16_rows(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
+bool launch_uq16_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs, int njobs,
                       const CorrOut &out, hipStream_t s);
 
 const char *correlate_family(const CorrConfig &cfg, bool uq)
 {
     if (cfg.family == 0 && cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg) && !(uq && cfg.w != 16))
-        return cfg.M == 64 ? "mfma64" : (uq ? "mfma32+rows" : "mfma32");
+        return cfg.M == 64 ? "mfma64" : (uq ? "mfma32+uq" : "mfma32");
     const char *f = cfg.family == 2 ? nullptr : fast_family(cfg);
     return f ? f : (cfg.M == 24 || cfg.M == 48 || cfg.M == 96 ? "dft" : "lds");
 }
@@ -764,11 +764,12 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
     const bool mfma_ok = cfg.family == 0 && cfg.form == HH_FORM_MFMA16, fast_ok = cfg.family != 2;
     if (mfma_ok && out.unc_u && out.unc_v && !out.planes && !out.acc && mfma_covers(cfg) && cfg.w == 16) {
         // the sweep with uncertainty: matrix-core correlation, then the
-        // statistics of the same windows by the row kernel
+        // statistics of the same windows with their covariance sums on the
+        // matrix cores
         CorrOut o = out;
         o.unc_u = o.unc_v = nullptr;
         if (launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, o, s) &&
-            launch_uq16_rows(cfg, A, B, nr, jobs, njobs, out, s))
+            launch_uq16_mfma(cfg, A, B, nr, jobs, njobs, out, s))
             return true;
     }
     if (mfma_ok && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
