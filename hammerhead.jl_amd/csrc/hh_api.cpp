@@ -1106,9 +1106,18 @@ int hh_create(hh_ctx **out, int device)
     c->device = device;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->pre_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         g_create_err = std::string("stream creation failed: ") + hipGetErrorString(e);
+        if (c->pre_join) (void)hipEventDestroy(c->pre_join);
+        if (c->side.join) (void)hipEventDestroy(c->side.join);
+        if (c->side.fork) (void)hipEventDestroy(c->side.fork);
+        if (c->side.stream) (void)hipStreamDestroy(c->side.stream);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return HH_EDEVICE;
     }
@@ -1116,6 +1125,10 @@ int hh_create(hh_ctx **out, int device)
     // device current
     c->scratch = scratch_on(device, g_create_err);
     if (!c->scratch) {
+        (void)hipEventDestroy(c->pre_join);
+        (void)hipEventDestroy(c->side.join);
+        (void)hipEventDestroy(c->side.fork);
+        (void)hipStreamDestroy(c->side.stream);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return HH_EDEVICE;
@@ -1140,6 +1153,11 @@ int hh_destroy(hh_ctx *c)
         (void)hipStreamDestroy(c->copy_stream);
     }
     for (auto e : c->upload_ev) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(c->side.stream);
+    (void)hipEventDestroy(c->pre_join);
+    (void)hipEventDestroy(c->side.join);
+    (void)hipEventDestroy(c->side.fork);
+    (void)hipStreamDestroy(c->side.stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HH_OK;

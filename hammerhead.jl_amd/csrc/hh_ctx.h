@@ -192,6 +192,11 @@ struct hh_ctx {
     int device = 0;
     int ncu = 256;
     hipStream_t stream = nullptr;
+    // the side stream of concurrent launches within a schedule (default
+    // priority), with its fork and join events, and the join event of the
+    // prefilter that runs beside the first pass
+    hh::SideStream side{};
+    hipEvent_t pre_join = nullptr;
     // device address of this GPU's no-allocation scratch (hh_create)
     unsigned char *scratch = nullptr;
     // dewarping (hh_set_dewarp_map): 16.16 fixed-point source map of the

@@ -380,7 +380,7 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
             for (int rep = 0; rep < 4; ++rep) {
                 HH_HIP(hipEventRecord(a, c->stream));
                 const bool fused = launch_correlate(t, E.lay.A, E.lay.B, E.lay.ld, 0, P.all.jobs.p, P.all.njobs, out,
-                                                    c->stream);
+                                                    c->stream, &c->side);
                 // a family that does not fuse the statistics is timed with
                 // the standalone launch the engine then makes
                 if (uq && !fused)
@@ -608,15 +608,36 @@ void record(hh_ctx *c, bool dynamic)
     hipStream_t s = c->stream;
     float *cA = E.coefs.p, *cB = E.coefs.p + L.coef_total;
     float *wA = E.warp.p, *wB = E.warp.p + (size_t)L.ld * L.ncols;
+    // The prefilter only feeds the first deformation: unless it is being
+    // timed, it runs on the side stream beside the first pass (which reads
+    // the raw pair) and is joined right before that deformation.
+    bool pre_pending = false;
     if (E.deforms) {
-        timed(c, "prefilter", 0, [&] {
+        auto prefilter = [&](hipStream_t st) {
             for (int q = 0; q < L.npairs; ++q) {
                 const size_t o = (size_t)L.roff[q] + (size_t)L.coff[q] * L.ld;
                 launch_prefilter_pair(L.A + o, L.B + o, L.ld, L.nr[q], L.nc[q], cA + L.coef_off[q],
-                                      cB + L.coef_off[q], E.ptmp.p, s);
+                                      cB + L.coef_off[q], E.ptmp.p, st);
             }
-        });
+        };
+        const bool timing_pre = c->timing && (c->timing_family.empty() || c->timing_family == "prefilter");
+        if (timing_pre) {
+            timed(c, "prefilter", 0, [&] { prefilter(s); });
+        } else {
+            HH_HIP(hipEventRecord(c->side.fork, s));
+            HH_HIP(hipStreamWaitEvent(c->side.stream, c->side.fork, 0));
+            prefilter(c->side.stream);
+            HH_HIP(hipEventRecord(c->pre_join, c->side.stream));
+            pre_pending = true;
+        }
     }
+    // the statistics of the final sweep run on the side stream beside the
+    // correlation, the validation and the rest of the schedule (they only
+    // read that sweep's warp, which nothing overwrites afterwards), joined
+    // at the end
+    SideStream side = c->side;
+    side.defer_join = true;
+    bool uq_pending = false;
     bool have_pred = false, pred_self = false;
     for (int k = 0; k < np; ++k) {
         PassPlan &P = *E.plan[k];
@@ -651,6 +672,10 @@ void record(hh_ctx *c, bool dynamic)
                 tab.npairs = T->grids.nseg;
                 tab.nblocks = T->blocks;
                 tab.one = pred_self ? T->hself : T->hprev;
+                if (pre_pending) {
+                    HH_HIP(hipStreamWaitEvent(s, c->pre_join, 0));
+                    pre_pending = false;
+                }
                 timed(c, "deform", 0, [&] { launch_deform_batch(tab, L.ld, wA, wB, s); });
                 srcA = wA;
                 srcB = wB;
@@ -681,7 +706,13 @@ void record(hh_ctx *c, bool dynamic)
             const CorrConfig &cc = out.unc_u ? P.cfg_uq : P.cfg;
             bool fused = true;
             timed(c, out.unc_u ? "correlate_uq" : "correlate", P.cfg.w, [&] {
-                fused = launch_correlate(cc, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs, out, s);
+                // the deferred join is only safe for the final sweep of the
+                // final pass (no later deformation overwrites its warp)
+                const bool timing_uq = c->timing && (c->timing_family.empty() || c->timing_family == "correlate_uq");
+                const bool defer = out.unc_u && last && !timing_uq;
+                fused = launch_correlate(cc, srcA, srcB, L.ld, 0, T->jobs.p, T->njobs, out, s,
+                                         defer ? &side : &c->side);
+                uq_pending = uq_pending || defer;
             });
             if (T->npart > 0) {
                 // windows with masked pixels, as centred tiles
@@ -769,6 +800,8 @@ void record(hh_ctx *c, bool dynamic)
             hipLaunchKernelGGL(k_grestore, cells_grid(P.all.grids), dim3(256), 0, s, E.u.p, E.v.p, E.mu.p, E.mv.p,
                                E.out.p, P.all.grids);
     }
+    if (pre_pending) HH_HIP(hipStreamWaitEvent(s, c->pre_join, 0));
+    if (uq_pending) HH_HIP(hipStreamWaitEvent(s, c->side.join, 0));
     HH_HIP(hipGetLastError());
 }
 

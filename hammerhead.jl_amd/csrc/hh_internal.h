@@ -85,10 +85,23 @@ struct CorrOut {
 };
 
 // ---- launchers (k_*.hip) ----------------------------------------------------
+// A second stream on which a launch may run independent work concurrently
+// with the caller's stream: forked after the work already queued on the
+// caller's stream and joined back before anything queued after the launch
+// (both streams at the default priority).
+struct SideStream {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+    // the caller makes its stream wait for `join` itself, later
+    bool defer_join;
+};
+
 // Returns true when out.unc_u/unc_v were filled by the same launch (fused
-// uncertainty); otherwise the caller runs launch_uq.
+// uncertainty); otherwise the caller runs launch_uq. With `side`, the
+// statistics of an MFMA-form sweep run on the side stream beside the
+// correlation.
 bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
-                      int njobs, const CorrOut &out, hipStream_t s);
+                      int njobs, const CorrOut &out, hipStream_t s, const SideStream *side = nullptr);
 
 // Builds the constant tables a correlation launch of cfg may need (device
 // allocations happen here, never inside a launch sequence).

@@ -756,7 +756,7 @@ void correlate_prepare(const CorrConfig &cfg, hipStream_t s)
 }
 
 bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int nr, int nc, const int32_t *jobs,
-                      int njobs, const CorrOut &out, hipStream_t s)
+                      int njobs, const CorrOut &out, hipStream_t s, const SideStream *side)
 {
     (void)nc;
     if (njobs <= 0) return out.unc_u != nullptr;
@@ -766,11 +766,23 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
         // the sweep with uncertainty: matrix-core correlation, then the
         // statistics of the same windows with their covariance sums on the
         // matrix cores
+        // matrix cores (the statistics kernel is latency-bound at its
+        // occupancy, the correlation VALU-bound: on a side stream they
+        // overlap); both only read the images and write disjoint outputs
         CorrOut o = out;
         o.unc_u = o.unc_v = nullptr;
-        if (launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, o, s) &&
-            launch_uq16_mfma(cfg, A, B, nr, jobs, njobs, out, s))
+        if (side) {
+            HH_HIP(hipEventRecord(side->fork, s));
+            HH_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+            const bool ok = launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, o, s) &&
+                            launch_uq16_mfma(cfg, A, B, nr, jobs, njobs, out, side->stream);
+            HH_HIP(hipEventRecord(side->join, side->stream));
+            if (!side->defer_join) HH_HIP(hipStreamWaitEvent(s, side->join, 0));
+            if (ok) return true;
+        } else if (launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, o, s) &&
+                   launch_uq16_mfma(cfg, A, B, nr, jobs, njobs, out, s)) {
             return true;
+        }
     }
     if (mfma_ok && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
     if (fast_ok && launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
