@@ -10,6 +10,7 @@
 #pragma once
 
 #include <climits>
+#include <type_traits>
 
 #include "hh_internal.h"
 
@@ -218,14 +219,15 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             cl = c ? sl_ : cl;
         }
     };
-    if constexpr ((G == 16 && NB <= 16) || (G == 32 && NB < 32)) {
+    if constexpr ((G == 16 && NB <= 16) || (G == 32 && NB < 32) || (G == 64 && NB < 64)) {
         // lane t walks box column t down the rows with its 3 x 3
         // neighbourhood in registers (left and right columns from lanes
         // t -/+ 1, zero past the box), one LDS read per point; neighbours
         // earlier in column order (left column, row above) disqualify when
         // strictly greater, later ones when >=. 16 lanes: DPP row shifts
-        // within the window's row; 32 lanes: wave shifts, whose reads across
-        // a window boundary land on lanes past the box, which hold zero
+        // within the window's row; 32 and 64 lanes: wave shifts, whose reads
+        // across a window boundary (or past lane 63) land on lanes past the
+        // box, which hold zero
         const bool col = t < NB;
         auto left = [](float v) {  // lane t gets lane t - 1 (0 at t = 0)
             constexpr int ctrl = G == 16 ? 0x111 : 0x138;  // row_shr:1 / wave_shr:1
@@ -241,8 +243,9 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
         // marked in a bit mask and inserted afterwards, lowest row first
         // (increasing location), so the insertion runs once per regional
         // maximum instead of once per point.
-        static_assert(NB <= 32, "one mask bit per box row");
-        unsigned okm = 0u;
+        static_assert(NB <= 64, "one mask bit per box row");
+        using Mask = std::conditional_t<(NB > 32), unsigned long long, unsigned>;
+        Mask okm = 0;
         float cp = 0.f;
         float cc = col ? Rb[t * NB] : 0.f;
         for (int bi = 0; bi < NB; ++bi) {
@@ -254,12 +257,12 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             const float me = fmaxf(left(V), cp);    // earlier: disqualify when >
             const float ml_ = fmaxf(right(V), cn);  // later: disqualify when >=
             const bool ok = I0 > 0.f && !(me > I0) && !(ml_ >= I0);
-            okm |= (ok ? 1u : 0u) << bi;
+            okm |= (Mask)(ok ? 1u : 0u) << bi;
             cp = cc;
             cc = cn;
         }
         while (okm) {
-            const int bi = __builtin_ctz(okm);
+            const int bi = NB > 32 ? __builtin_ctzll((unsigned long long)okm) : __builtin_ctz((unsigned)okm);
             okm &= okm - 1u;
             insert(Rb[bi + t * NB], (LO + bi) + (LO + t) * M);
         }

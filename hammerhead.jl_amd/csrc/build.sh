@@ -7,7 +7,7 @@ OBJ="$HERE/../_obj"
 mkdir -p "$OBJ"
 FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-function)
 pids=()
-for f in k_correlate k_corr_fast k_corr_mfma k_deform k_dewarp k_validate k_mask k_uq k_adjoint hh_api hh_engine; do
+for f in k_correlate k_corr_fast k_corr_mfma k_corr_mfma96 k_deform k_dewarp k_validate k_mask k_uq k_adjoint hh_api hh_engine; do
   src="$HERE/$f.hip"; [ -f "$src" ] || src="$HERE/$f.cpp"
   extra=()
   # matrix-core accumulators in VGPRs: no AGPR round trips, higher occupancy

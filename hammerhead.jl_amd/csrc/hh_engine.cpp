@@ -345,6 +345,8 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
             for (int order = 0; order < 2; ++order) {
                 if (f == "mfma64")
                     for (int w : {2, 1}) cand.push_back({fm, w, order});
+                else if (f == "mfma96")
+                    cand.push_back({fm, 1, order});  // one wave per SIMD (LDS and registers)
                 else if (f.rfind("mfma", 0) == 0)
                     for (int w : {3, 2, 4, 1}) cand.push_back({fm, w, order});
                 else if (f.rfind("reg", 0) == 0)

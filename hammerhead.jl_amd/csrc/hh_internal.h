@@ -126,6 +126,12 @@ int device_cu_count();
 // Split-FP16 MFMA DFT kernels (HH_FORM_MFMA16); false when not covered.
 bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
                            int njobs, const CorrOut &out, hipStream_t s);
+// The 64-px matrix-core kernel (circular length 96, cross correlation), its
+// coverage and its device tables (built by mfma96_prepare outside capture).
+bool mfma96_covers(const CorrConfig &cfg);
+void mfma96_prepare(hipStream_t s);
+bool launch_correlate_mfma96(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,
+                             int njobs, const CorrOut &out, hipStream_t s);
 
 // Specialized kernels; return false when the configuration is not covered.
 bool launch_correlate_fast(const CorrConfig &cfg, const float *A, const float *B, int nr, const int32_t *jobs,

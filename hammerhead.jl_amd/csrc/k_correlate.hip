@@ -745,6 +745,7 @@ const char *correlate_family(const CorrConfig &cfg, bool uq)
 {
     if (cfg.family == 0 && cfg.form == HH_FORM_MFMA16 && mfma_covers(cfg) && !(uq && cfg.w != 16))
         return cfg.M == 64 ? "mfma64" : (uq ? "mfma32+uq" : "mfma32");
+    if (cfg.family == 0 && cfg.form == HH_FORM_MFMA16 && mfma96_covers(cfg) && !uq) return "mfma96";
     const char *f = cfg.family == 2 ? nullptr : fast_family(cfg);
     return f ? f : (cfg.M == 24 || cfg.M == 48 || cfg.M == 96 ? "dft" : "lds");
 }
@@ -752,6 +753,7 @@ const char *correlate_family(const CorrConfig &cfg, bool uq)
 void correlate_prepare(const CorrConfig &cfg, hipStream_t s)
 {
     (void)device_cu_count();
+    if (cfg.form == HH_FORM_MFMA16 && mfma96_covers(cfg)) mfma96_prepare(s);
     if (cfg.M >= 2 && cfg.M <= kMaxM) (void)twiddle_table(cfg.M, s);
 }
 
@@ -785,6 +787,7 @@ bool launch_correlate(const CorrConfig &cfg, const float *A, const float *B, int
         }
     }
     if (mfma_ok && launch_correlate_mfma(cfg, A, B, nr, jobs, njobs, out, s)) return false;
+    if (mfma_ok && launch_correlate_mfma96(cfg, A, B, nr, jobs, njobs, out, s)) return false;
     if (fast_ok && launch_correlate_fast(cfg, A, B, nr, jobs, njobs, out, s, &uq_done)) return uq_done;
     const float2 *tw = twiddle_table(cfg.M, s);
     switch (cfg.M) {

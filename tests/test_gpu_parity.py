@@ -485,13 +485,14 @@ def ctx_mfma(hh):
 
 
 @pytest.mark.parametrize("method", ["cross", "phase"])
-@pytest.mark.parametrize("w", [16, 32])
+@pytest.mark.parametrize("w", [16, 32, 64])
 def test_mfma_correlate_vs_oracle(hh, orc, ctx_mfma, c1, method, w):
     # split-FP16 DFT products on the matrix cores, 16- and 32-px windows
-    # padded to 32 / 64
+    # padded to 32 / 64, 64-px windows at circular length 96 (cross; the
+    # phase filter keeps 64-px windows on the FFT kernels)
     A, B = c1.A.astype(np.float32), c1.B.astype(np.float32)
     ctx_mfma.set_pair(A, B)
-    for ov in ((8, 12) if w == 16 else (16, 24)):
+    for ov in {16: (8, 12), 32: (16, 24), 64: (32, 48)}[w]:
         p = P(hh, window_size=w, overlap=ov, padding=True, apodization="gauss", correlation_method=method)
         say("mfma correlate", method, w, ov)
         s, g, o = compare_windows(orc, hh, ctx_mfma, A.astype(np.float64), B.astype(np.float64), p)
