@@ -59,6 +59,11 @@ constexpr int kS4_96 = -8;            // stage-4 operand scale (96 K slots of |P
 
 struct Trig96 {
     float c[kN96], s[kN96];  // cos, sin of 2 pi t / 96
+    // the Gaussian apodization of a 64-px window is separable,
+    // exp(-((i - 32.5)^2 + (j - 32.5)^2) / (2 * 16^2)) = ap[i - 1] ap[j - 1]
+    // (apodization_window, src/correlators.jl:18-25): its row factor goes
+    // into the stage-2 tables and its column factor into the stage-1 tables
+    float ap[64];
 };
 
 enum {
@@ -89,8 +94,9 @@ __device__ void build_table96(int q, int l, const Trig96 &tg, Split &out)
             const bool sn = q >= T96_WS;
             const int i = q - (sn ? T96_WS : T96_WC);
             const int kk = i / 3, nt = i % 3;
-            const int t = ((32 * nt + n) * (16 * kk + rmap(h, s))) % kN96;
-            f = sn ? -tg.s[t] : tg.c[t];
+            const int r = 16 * kk + rmap(h, s);
+            const int t = ((32 * nt + n) * r) % kN96;
+            f = tg.ap[r] * (sn ? -tg.s[t] : tg.c[t]);
         } else {
             int m, cs, lt, src = -1;
             if (q < T96_J0) {
@@ -126,7 +132,8 @@ __device__ void build_table96(int q, int l, const Trig96 &tg, Split &out)
 }
 
 // Stage-1 tables F1[kk][J] (K = c = 16 kk + 8 h + s, column = part n of tile
-// J), written once per device into the buffer the kernel reads.
+// J) times the column apodization, written once per device into the buffer
+// the kernel reads.
 __global__ __launch_bounds__(64) void k_build_f1_96(Trig96 tg, Split *f1)
 {
     const int q = blockIdx.x, l = threadIdx.x;
@@ -143,7 +150,7 @@ __global__ __launch_bounds__(64) void k_build_f1_96(Trig96 tg, Split *f1)
             const int k2 = 16 * J + n - 16;
             f = k2 == 0 ? ((c & 1) ? -1.f : 1.f) : -tg.s[(k2 * c) % kN96];
         }
-        put(out, s, f);
+        put(out, s, tg.ap[c] * f);
     }
     f1[q * 64 + l] = out;
 }
@@ -197,8 +204,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();
     const f16v zero = {};
     const float neg1 = opaque_neg1();
-    // apodization of this lane's pixels: rows 32 rt + n, columns 16 kk + 8 h + s
-    const float *__restrict__ apodl = cfg.apod + n + 64 * 8 * h;
 
     const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, 4, wave, njobs);
     if (ww.n <= 0) return;
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             float amx = 0.f;
 #pragma unroll
             for (int q = 0; q < 64; ++q) amx = fmaxf(amx, fabsf(px[q] - mean));
-            const float bound = wave_max(amx);  // >= |x| (apodization <= 1)
+            const float bound = wave_max(amx);  // >= |x| (the apodization, <= 1, is in the tables)
             int ex = 0;
             (void)frexpf(bound, &ex);
             meanI[I] = mean;
@@ -254,17 +259,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 Split f[4];
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) f[kk] = f1[(kk * 3 + J) * 64 + tl];
-                // an opaque zero per iteration: the pixel and apodization
-                // addresses are formed here, not hoisted out of the loop
-                // (128 of them would not fit the registers beside Q)
+                // an opaque zero per iteration: the pixel addresses are
+                // formed here, not hoisted out of the loop (128 of them would
+                // not fit the registers beside Q)
                 int zj;
                 asm volatile("v_mov_b32 %0, 0" : "=v"(zj));
-                const float *ap = apodl + zj;
 #pragma unroll
                 for (int I = 0; I < 2; ++I) {
                     const float *p = pI[I] + zj;
-                    const float mean = meanI[I];
-                    const int e0 = eI[I];
+                    // (p - mean) 2^e0 as one FMA: the scale is a power of two,
+                    // so p 2^e0 - mean 2^e0 rounds exactly as p - mean does
+                    const float sc = ldexpf(1.f, eI[I]), msc = -meanI[I] * sc;
 #pragma unroll
                     for (int rt = 0; rt < 2; ++rt) {
                         Split a1[4];
@@ -272,14 +277,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                         for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
                             for (int s = 0; s < 8; s += 2)
-                                put2(a1[kk], s,
-                                     ldexpf(ap[32 * rt + 64 * (16 * kk + s)] *
-                                                (p[(size_t)(16 * kk + s) * nr + 32 * rt] - mean),
-                                            e0),
-                                     ldexpf(ap[32 * rt + 64 * (16 * kk + s + 1)] *
-                                                (p[(size_t)(16 * kk + s + 1) * nr + 32 * rt] - mean),
-                                            e0),
-                                     neg1);
+                                put2(a1[kk], s, __builtin_fmaf(p[(size_t)(16 * kk + s) * nr + 32 * rt], sc, msc),
+                                     __builtin_fmaf(p[(size_t)(16 * kk + s + 1) * nr + 32 * rt], sc, msc), neg1);
                         f16v d1 = mfma3(a1[0], f[0], zero);
 #pragma unroll
                         for (int kk = 1; kk < 4; ++kk) d1 = mfma3(a1[kk], f[kk], d1);
@@ -430,6 +429,10 @@ static const Trig96 &trig96()
         for (int k = 0; k < kN96; ++k) {
             r.c[k] = (float)std::cos(2.0 * M_PI * k / (double)kN96);
             r.s[k] = (float)std::sin(2.0 * M_PI * k / (double)kN96);
+        }
+        for (int i = 0; i < 64; ++i) {
+            const double d = (i + 1) - 32.5;
+            r.ap[i] = (float)std::exp(-d * d / (2.0 * 16.0 * 16.0));
         }
         return r;
     }();
