@@ -18,25 +18,27 @@
 //   1. D1[I][rt][J] = X_I (rows r of tile rt, K = c in 4 steps) x F1[.][J]:
 //      the column DFT, parts of tile J = Re k2 = 16J..16J+15 (rows 0-15) and
 //      Im of the same k2 (rows 16-31; the Im slot of k2 = 0 carries the real
-//      Nyquist column k2 = 48);
-//   2. per (J, k1 tile nt): D1^T (K = r in 4 steps) x [cos | -sin](r, k1):
+//      Nyquist column k2 = 48), for all three tiles J at once; split as the
+//      stage-2 operands and held for the window (192 registers, partly AGPRs);
+//   2. per (k1 tile nt, J): D1^T (K = r in 4 steps) x [cos | -sin](r, k1):
 //      Z(k1, k2), the cross spectrum P, and
-//   3. at once its contribution to Q[nt][Re | Im][lt] += P (K = Re/Im of k2)
-//      x table: the inverse along k2 for the 51 box columns l2 (2 column tiles
+//   3. at once its contribution to Q[Re | Im][lt] += P (K = Re/Im of k2) x
+//      table: the inverse along k2 for the 51 box columns l2 (2 column tiles
 //      lt), Hermitian weight 2 in the tables; k2 = 0 and 48 enter as
 //      H = Re P - Im P in the K slots of k2 = 0;
-//   4. R[lt1][lt2] = [2cos | -2sin](l1, k1) x [Q_Re; Q_Im](k1, l2) (K = k1 in
-//      2 steps per nt): the tables of stage 3 as the A operand (rows l1), the
-//      stage-3 results as B, R accumulated whole.
-// The inverse tables of stage 3 and the A operands of stage 4 are one set of
-// 2cos / -2sin tables of (k, l) (k = 16 m + the K slot order, l = the box
-// lags by lane), so the LDS holds the stage-2 tables (24 split 32 x 16
-// operands), that set (24) and the k2 = 0 variants of stage 3 (8): 112 KB,
-// beside one 51 x 51 analysis box per wave; the stage-1 tables (12) are read
-// from a device buffer (24 KB, L2-resident). One block of 4 waves per CU (one
-// wave per SIMD, the whole register file: the 12 stage-3 accumulator tiles
-// live in AGPRs); persistent waves walk the windows at a stride
-// (wave_windows).
+//   4. once Q of tile nt is complete, R[lt1][lt2] += [2cos | -2sin](l1, k1) x
+//      [Q_Re; Q_Im](k1, l2) (K = the 32 k1 of the tile in 2 steps): the tables
+//      of stage 3 as the A operand (rows l1), Q as B, R accumulated whole.
+// The separable Gaussian apodization is folded into the tables (its column
+// factor into stage 1, its row factor into stage 2), so the pixels enter as
+// (p - mean) 2^e. The inverse tables of stage 3 and the A operands of stage 4
+// are one set of 2cos / -2sin tables of (k, l) (k = 16 m + the K slot order,
+// l = the box lags by lane), so the LDS holds the stage-2 tables (24 split
+// 32 x 16 operands), that set (24) and the k2 = 0 variants of stage 3 (8):
+// 112 KB, beside one 51 x 51 analysis box per wave; the stage-1 tables (12)
+// are read from a device buffer (24 KB, L2-resident). One block of 4 waves
+// per CU (one wave per SIMD with the whole register file); persistent waves
+// walk the windows at a stride (wave_windows).
 
 #include "hh_internal.h"
 #include "box_peaks.h"
@@ -242,67 +244,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             eI[I] = bound > 0.f ? kS0_96 - ex : 0;
         }
         const int es = eI[0] + eI[1];
-        f16v q3[3][2][2];  // Q[nt][Re | Im][lt]: rows k1 of tile nt, columns l2
+        // ---- 1. column DFT of both windows for every tile J, split as the
+        // stage-2 operand (K = r = 16 kk + rmap): held for the whole window
+        // (some of it in AGPRs) instead of reloading the pixels per tile ----
+        Split a2[2][3][4];  // [I][J][kk]
 #pragma unroll
-        for (int nt = 0; nt < 3; ++nt)
+        for (int I = 0; I < 2; ++I) {
+            const float *p = pI[I];
+            // (p - mean) 2^e0 as one FMA: the scale is a power of two, so
+            // p 2^e0 - mean 2^e0 rounds exactly as p - mean does
+            const float sc = ldexpf(1.f, eI[I]), msc = -meanI[I] * sc;
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                Split a1[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                    for (int s = 0; s < 8; s += 2)
+                        put2(a1[kk], s, __builtin_fmaf(p[(size_t)(16 * kk + s) * nr + 32 * rt], sc, msc),
+                             __builtin_fmaf(p[(size_t)(16 * kk + s + 1) * nr + 32 * rt], sc, msc), neg1);
+#pragma unroll
+                for (int J = 0; J < 3; ++J) {
+                    f16v d1 = mfma3(a1[0], f1[(0 * 3 + J) * 64 + tl], zero);
+#pragma unroll
+                    for (int kk = 1; kk < 4; ++kk) d1 = mfma3(a1[kk], f1[(kk * 3 + J) * 64 + tl], d1);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int s = 0; s < 8; s += 2)
+                            put2(a2[I][J][2 * rt + q], s, d1[8 * q + s], d1[8 * q + s + 1], neg1);
+                }
+            }
+        }
+        f16v r4[2][2];  // R[lt1][lt2]
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) r4[a][b] = zero;
+#pragma unroll
+        for (int nt = 0; nt < 3; ++nt) {
+            f16v q3[2][2];  // Q[Re | Im][lt] of k1 tile nt: rows k1, columns l2
 #pragma unroll
             for (int ri = 0; ri < 2; ++ri)
 #pragma unroll
-                for (int lt = 0; lt < 2; ++lt) q3[nt][ri][lt] = zero;
-#pragma unroll 1
-        for (int J = 0; J < 3; ++J) {
-            // ---- 1. column DFT of both windows, parts of tile J, from the
-            // pixels reloaded (cache-resident) and split again; its result
-            // split as the stage-2 operand (K = r = 16 kk + rmap) ----
-            Split a2[2][4];
-            {
-                Split f[4];
+                for (int lt = 0; lt < 2; ++lt) q3[ri][lt] = zero;
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) f[kk] = f1[(kk * 3 + J) * 64 + tl];
-                // an opaque zero per iteration: the pixel addresses are
-                // formed here, not hoisted out of the loop (128 of them would
-                // not fit the registers beside Q)
-                int zj;
-                asm volatile("v_mov_b32 %0, 0" : "=v"(zj));
-#pragma unroll
-                for (int I = 0; I < 2; ++I) {
-                    const float *p = pI[I] + zj;
-                    // (p - mean) 2^e0 as one FMA: the scale is a power of two,
-                    // so p 2^e0 - mean 2^e0 rounds exactly as p - mean does
-                    const float sc = ldexpf(1.f, eI[I]), msc = -meanI[I] * sc;
-#pragma unroll
-                    for (int rt = 0; rt < 2; ++rt) {
-                        Split a1[4];
-#pragma unroll
-                        for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-                            for (int s = 0; s < 8; s += 2)
-                                put2(a1[kk], s, __builtin_fmaf(p[(size_t)(16 * kk + s) * nr + 32 * rt], sc, msc),
-                                     __builtin_fmaf(p[(size_t)(16 * kk + s + 1) * nr + 32 * rt], sc, msc), neg1);
-                        f16v d1 = mfma3(a1[0], f[0], zero);
-#pragma unroll
-                        for (int kk = 1; kk < 4; ++kk) d1 = mfma3(a1[kk], f[kk], d1);
-#pragma unroll
-                        for (int q = 0; q < 2; ++q)
-#pragma unroll
-                            for (int s = 0; s < 8; s += 2)
-                                put2(a2[I][2 * rt + q], s, d1[8 * q + s], d1[8 * q + s + 1], neg1);
-                    }
-                }
-            }
-            const bool j0 = J == 0;
-#pragma unroll
-            for (int nt = 0; nt < 3; ++nt) {
+            for (int J = 0; J < 3; ++J) {
+                const bool j0 = J == 0;
                 // ---- 2. row DFT: Z(k1 = 32 nt + n, k2 = 16 J + rmap(h, v)) ----
                 float zr[2][9], zi[2][9];
 #pragma unroll
                 for (int I = 0; I < 2; ++I) {
-                    f16v dc = mfma3(a2[I][0], tabs[T96_WC + 0 * 3 + nt][tl], zero);
-                    f16v ds = mfma3(a2[I][0], tabs[T96_WS + 0 * 3 + nt][tl], zero);
+                    f16v dc = mfma3(a2[I][J][0], tabs[T96_WC + 0 * 3 + nt][tl], zero);
+                    f16v ds = mfma3(a2[I][J][0], tabs[T96_WS + 0 * 3 + nt][tl], zero);
 #pragma unroll
                     for (int kk = 1; kk < 4; ++kk) {
-                        dc = mfma3(a2[I][kk], tabs[T96_WC + kk * 3 + nt][tl], dc);
-                        ds = mfma3(a2[I][kk], tabs[T96_WS + kk * 3 + nt][tl], ds);
+                        dc = mfma3(a2[I][J][kk], tabs[T96_WC + kk * 3 + nt][tl], dc);
+                        ds = mfma3(a2[I][J][kk], tabs[T96_WS + kk * 3 + nt][tl], ds);
                     }
 #pragma unroll
                     for (int v = 0; v < 8; ++v) {
@@ -316,7 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     zr[I][8] = dc[8];
                     zi[I][8] = ds[8];
                 }
-                // ---- 3. spectrum; inverse along k2 into Q[nt][.][.] ----
+                // ---- 3. spectrum; inverse along k2 into Q ----
                 float sp_r[9], sp_i[9];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) {
@@ -339,25 +337,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 // k2 = 0 variants of the tables for tile 0
 #pragma unroll
                 for (int lt = 0; lt < 2; ++lt) {
-                    const int cr = j0 ? T96_J0 + 0 + lt : T96_TT + J * 4 + 0 + lt;      // src Re, C
-                    const int sr = j0 ? T96_J0 + 2 + lt : T96_TT + J * 4 + 2 + lt;      // src Re, S
-                    const int ci = j0 ? T96_J0 + 4 + lt : T96_TT + J * 4 + 0 + lt;      // src Im, C
-                    const int si = j0 ? T96_J0 + 6 + lt : T96_TT + J * 4 + 2 + lt;      // src Im, S
-                    q3[nt][0][lt] = mfma3(a3r, tabs[cr][tl], q3[nt][0][lt]);
-                    q3[nt][0][lt] = mfma3(a3i, tabs[si][tl], q3[nt][0][lt]);
-                    q3[nt][1][lt] = mfma3(na3r, tabs[sr][tl], q3[nt][1][lt]);
-                    q3[nt][1][lt] = mfma3(a3i, tabs[ci][tl], q3[nt][1][lt]);
+                    const int cr = j0 ? T96_J0 + 0 + lt : T96_TT + J * 4 + 0 + lt;  // src Re, C
+                    const int sr = j0 ? T96_J0 + 2 + lt : T96_TT + J * 4 + 2 + lt;  // src Re, S
+                    const int ci = j0 ? T96_J0 + 4 + lt : T96_TT + J * 4 + 0 + lt;  // src Im, C
+                    const int si = j0 ? T96_J0 + 6 + lt : T96_TT + J * 4 + 2 + lt;  // src Im, S
+                    q3[0][lt] = mfma3(a3r, tabs[cr][tl], q3[0][lt]);
+                    q3[0][lt] = mfma3(a3i, tabs[si][tl], q3[0][lt]);
+                    q3[1][lt] = mfma3(na3r, tabs[sr][tl], q3[1][lt]);
+                    q3[1][lt] = mfma3(a3i, tabs[ci][tl], q3[1][lt]);
                 }
             }
-        }
-        // ---- 4. R[lt1][lt2] = [2cos | -2sin](l1, k1) x [Q_Re; Q_Im](k1, l2) ----
-        f16v r4[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) r4[a][b] = zero;
-#pragma unroll
-        for (int nt = 0; nt < 3; ++nt)
+            // ---- 4. R[lt1][lt2] += [2cos | -2sin](l1, k1) x [Q_Re; Q_Im](k1, l2)
+            // over the k1 of tile nt ----
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) {
                 Split bq[2][2];  // [Re | Im][lt2]
@@ -367,8 +358,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     for (int lt = 0; lt < 2; ++lt)
 #pragma unroll
                         for (int s = 0; s < 8; s += 2)
-                            put2(bq[ri][lt], s, ldexpf(q3[nt][ri][lt][8 * kk + s], kS4_96),
-                                 ldexpf(q3[nt][ri][lt][8 * kk + s + 1], kS4_96), neg1);
+                            put2(bq[ri][lt], s, ldexpf(q3[ri][lt][8 * kk + s], kS4_96),
+                                 ldexpf(q3[ri][lt][8 * kk + s + 1], kS4_96), neg1);
                 const int m = 2 * nt + kk;
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
@@ -381,6 +372,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     }
                 }
             }
+        }
         // the tables carry the Hermitian weight 2 throughout: R / 2, and the
         // inverse normalization 1 / 96^2
         const int etot = es + kS4_96;
