@@ -347,7 +347,16 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
                     for (int w : {2, 1}) cand.push_back({fm, w, order});
                 else if (f == "mfma96")
                     cand.push_back({fm, 1, order});  // one wave per SIMD (LDS and registers)
-                else if (f.rfind("mfma", 0) == 0)
+                else if (f == "mfma32+uq") {
+                    // the statistics kernel runs beside on the same CUs: few
+                    // correlation waves leave it SIMD slots, and only XCD
+                    // bands are offered, since back-to-back timing here
+                    // favours launch order (about 5 %) where the schedule
+                    // measures it 2 % slower (0.571 against 0.563 ms on the
+                    // 4 MP pair)
+                    if (order == 0)
+                        for (int w : {1, 2, 3, 4}) cand.push_back({fm, w, 0});
+                } else if (f.rfind("mfma", 0) == 0)
                     for (int w : {3, 2, 4, 1}) cand.push_back({fm, w, order});
                 else if (f.rfind("reg", 0) == 0)
                     cand.push_back({fm, 0, order});

@@ -51,6 +51,11 @@ using namespace mfma;
 
 struct Trig {
     float c[32], s[32];  // cos, sin of 2 pi t / 32
+    // the separable Gaussian apodization of a 16-px window (apodization_window,
+    // src/correlators.jl:18-25: exp(-((i - 8.5)^2 + (j - 8.5)^2) / (2 * 4^2))
+    // = ap[i - 1] ap[j - 1]): the column factor goes into the stage-1 table,
+    // the row factor into the stage-2 tables
+    float ap[16];
 };
 
 constexpr int kHW = 7;            // box half-width (Gaussian apodization)
@@ -94,10 +99,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
                 f = (c & 1) ? -1.f : 1.f;
             else
                 f = -tg.s[(c * (n - 16)) & 31];
-            put(F1, s, f);
+            put(F1, s, tg.ap[c] * f);
             const int r = rmap(h, s);  // stage 2: K = r, column k1 = n
-            put(WrT, s, tg.c[(n * r) & 31]);
-            put(WiT, s, -tg.s[(n * r) & 31]);
+            put(WrT, s, tg.ap[r] * tg.c[(n * r) & 31]);
+            put(WiT, s, tg.ap[r] * -tg.s[(n * r) & 31]);
             // stage 3: K = (Re | Im of k2 = rmap(h, s)), column n = (b = n & 15,
             // Re for n < 16 | Im), l2 = b - HW; the K slots of k2 = 0 carry
             // H(k1, 0) (Re table) and H(k1, 16) (Im table)
@@ -143,9 +148,6 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
     const float neg1 = opaque_neg1();
     const int img = n >> 4, row = n & 15;
     const float *src = img ? B : A;
-    float apod8[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) apod8[s] = cfg.apod[row + 16 * (8 * h + s)];
     // overlap gain of this lane's box points (row n, columns rmap(h, v))
     float gain8[8];
 #pragma unroll
@@ -188,27 +190,26 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
         }
         jcur = jnext;
         jnext = jobs4[ww.base + min(k + 2, ww.n - 1) * ww.stride];
-        // ---- 0. window: lane (image, row, 8 columns), mean, apodize ----
+        // ---- 0. window: lane (image, row, 8 columns), mean, scale (the
+        // apodization is in the tables) ----
         float sum = 0.f;
 #pragma unroll
         for (int s = 0; s < 8; ++s) sum += px[s];
         const float mean = img_sum(sum) / 256.f;
-        float x[8], amx = 0.f;
+        float d[8], amx = 0.f;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const float d = px[s] - mean;
-            amx = fmaxf(amx, fabsf(d));
-            x[s] = apod8[s] * d;
+            d[s] = px[s] - mean;
+            amx = fmaxf(amx, fabsf(d[s]));
         }
-        // |x| <= max |px - mean| (apodization <= 1): scale that bound into
-        // [2^-2, 2^-1)
+        // max |px - mean| scaled into [2^-2, 2^-1)
         const float bound = img_max(amx);
         int ex = 0;
         (void)frexpf(bound, &ex);
         const int e0 = bound > 0.f ? -1 - ex : 0;
         Split a1;
 #pragma unroll
-        for (int s = 0; s < 8; s += 2) put2(a1, s, ldexpf(x[s], e0), ldexpf(x[s + 1], e0), neg1);
+        for (int s = 0; s < 8; s += 2) put2(a1, s, ldexpf(d[s], e0), ldexpf(d[s + 1], e0), neg1);
         const int eA = __shfl(e0, 0), eB = __shfl(e0, 16);
 
         // ---- 1. column DFT of both windows ----
@@ -336,6 +337,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, co
 // per window.
 struct Trig64 {
     float c[64], s[64];  // cos, sin of 2 pi t / 64
+    // the separable Gaussian apodization of a 32-px window (exp(-((i - 16.5)^2
+    // + (j - 16.5)^2) / (2 * 8^2)) = ap[i - 1] ap[j - 1]), folded into the
+    // stage-1 (column factor) and stage-2 (row factor) tables
+    float ap[32];
 };
 
 constexpr int kHW64 = 13;                 // box half-width (Gaussian apodization)
@@ -367,12 +372,14 @@ __device__ void build_table64(int q, int l, const Trig64 &tg, Split &out)
                 const int k2 = 16 * J + n - 16;
                 f = k2 == 0 ? ((c & 1) ? -1.f : 1.f) : -tg.s[(k2 * c) & 63];
             }
+            f *= tg.ap[c];
         } else if (q < T64_T3) {  // Wc / Ws[kk][nt]: K = r, column k1
             const bool sn = q >= T64_WS;
             const int i = q - (sn ? T64_WS : T64_WC);
             const int kk = i >> 1, nt = i & 1;
-            const int t = ((32 * nt + n) * (16 * kk + rmap(h, s))) & 63;
-            f = sn ? -tg.s[t] : tg.c[t];
+            const int r = 16 * kk + rmap(h, s);
+            const int t = ((32 * nt + n) * r) & 63;
+            f = tg.ap[r] * (sn ? -tg.s[t] : tg.c[t]);
         } else if (q < T64_C4) {  // T3[J][src][dst]: K = k2, column l2 = b - HW, b = n
             const int i = q - T64_T3;
             const int J = i >> 2, imk = (i >> 1) & 1, imc = i & 1;
@@ -425,9 +432,6 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
     __syncthreads();
     const f16v zero = {};
     const float neg1 = opaque_neg1();
-    // apodization of this lane's pixels (row n, columns 16 kk + 8 h + s), read
-    // per window from the cache rather than held in registers
-    const float *__restrict__ apodl = cfg.apod + n + 32 * 8 * h;
 
     const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, WV, wave, njobs);
     if (ww.n <= 0) return;
@@ -472,12 +476,12 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
                 (void)frexpf(bound, &ex);
                 const int e0 = bound > 0.f ? -3 - ex : 0;  // |x| < 2^-3
                 eI[I] = e0;
+                // the apodization is in the tables
 #pragma unroll
                 for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
                     for (int s = 0; s < 8; s += 2)
-                        put2(a1[I][kk], s, ldexpf(apodl[32 * (16 * kk + s)] * px[I][8 * kk + s], e0),
-                             ldexpf(apodl[32 * (16 * kk + s + 1)] * px[I][8 * kk + s + 1], e0), neg1);
+                        put2(a1[I][kk], s, ldexpf(px[I][8 * kk + s], e0), ldexpf(px[I][8 * kk + s + 1], e0), neg1);
             }
             const int es = eI[0] + eI[1];
             f16v q3[2][2];  // Q[nt][Re | Im]: rows k1 of tile nt, columns l2
@@ -639,6 +643,10 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
                 t.c[k] = (float)std::cos(2.0 * M_PI * k / 64.0);
                 t.s[k] = (float)std::sin(2.0 * M_PI * k / 64.0);
             }
+            for (int i = 0; i < 32; ++i) {
+                const double d = (i + 1) - 16.5;
+                t.ap[i] = (float)std::exp(-d * d / (2.0 * 8.0 * 8.0));
+            }
             return t;
         }();
         // one block per CU (tables and boxes take 84-108 KB of LDS):
@@ -670,6 +678,10 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
         for (int k = 0; k < 32; ++k) {
             t.c[k] = (float)std::cos(2.0 * M_PI * k / 32.0);
             t.s[k] = (float)std::sin(2.0 * M_PI * k / 32.0);
+        }
+        for (int i = 0; i < 16; ++i) {
+            const double d = (i + 1) - 8.5;
+            t.ap[i] = (float)std::exp(-d * d / (2.0 * 4.0 * 4.0));
         }
         return t;
     }();
