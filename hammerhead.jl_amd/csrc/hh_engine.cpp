@@ -380,15 +380,17 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
         const ConfigBufs &cb = get_config(c, P.p);
         hipEvent_t a = get_event(c), b = get_event(c);
         TuneEntry best;
-        float best_score = 1e30f;
-        for (const Cand &cd : cand) {
-            CorrConfig t = base;
-            t.family = cd.family;
-            t.waves_per_simd = cd.w;
-            t.linear_order = cd.order;
-            const std::string f = correlate_family(t, uq);
-            float ms_min = 1e30f;
-            for (int rep = 0; rep < 4; ++rep) {
+        // candidates are timed round-robin, each minimum over rounds 1-4
+        // (round 0 warms caches): the clocks still ramp while tuning runs,
+        // and timing each candidate's launches back to back favoured the
+        // later candidates by 3-5 %
+        std::vector<float> ms_min(cand.size(), 1e30f);
+        for (int rep = 0; rep < 5; ++rep)
+            for (size_t i = 0; i < cand.size(); ++i) {
+                CorrConfig t = base;
+                t.family = cand[i].family;
+                t.waves_per_simd = cand[i].w;
+                t.linear_order = cand[i].order;
                 HH_HIP(hipEventRecord(a, c->stream));
                 const bool fused = launch_correlate(t, E.lay.A, E.lay.B, E.lay.ld, 0, P.all.jobs.p, P.all.njobs, out,
                                                     c->stream, &c->side);
@@ -401,13 +403,21 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
                 HH_HIP(hipEventSynchronize(b));
                 float ms = 0.f;
                 HH_HIP(hipEventElapsedTime(&ms, a, b));
-                if (rep > 0) ms_min = std::min(ms_min, ms);  // the first run warms caches
+                if (rep > 0) ms_min[i] = std::min(ms_min[i], ms);
             }
+        float best_score = 1e30f;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            const Cand &cd = cand[i];
+            CorrConfig t = base;
+            t.family = cd.family;
+            t.waves_per_simd = cd.w;
+            t.linear_order = cd.order;
+            const std::string f = correlate_family(t, uq);
             std::string name = f.rfind("mfma", 0) == 0 ? "waves_per_simd=" + std::to_string(cd.w) + " " : "";
             if (f.rfind("mfma", 0) == 0 || f.rfind("reg", 0) == 0)
                 name += cd.order ? "order=linear" : "order=xcd_bands";
             if (families) name = "family=" + f + (name.empty() ? "" : " " + name);
-            best.ms.push_back({name, ms_min});
+            best.ms.push_back({name, ms_min[i]});
             // a later candidate must win by more than timing noise (2 %);
             // the first listed is the form's own family in XCD bands. Launch
             // order deals neighbouring windows to different XCDs, so each L2
@@ -415,7 +425,7 @@ CorrConfig tune_corr(hh_ctx *c, const PassPlan &P, const CorrConfig &base, bool 
             // 16-px matrix-core kernel, 5x for the statistics kernel), and
             // back-to-back launches here time it more favourably than the
             // schedule does: it must win by 5 % more
-            const float score = ms_min * (cd.order ? 1.05f : 1.f);
+            const float score = ms_min[i] * (cd.order ? 1.05f : 1.f);
             if (score < 0.98f * best_score) {
                 best_score = score;
                 best.family = cd.family;
