@@ -73,10 +73,12 @@ constexpr int kNB = 2 * kHW + 1;  // 15 box rows and columns
 constexpr int kS3P = 12;  // stage-3 operand scale (phase spectrum)
 constexpr int kS4 = -4;   // stage-3 table scale
 
-template <int K, bool PHASE>
-__global__ __launch_bounds__(256) void k_corr_mfma32(CorrConfig cfg, Trig tg, const float *__restrict__ A,
-                                                     const float *__restrict__ B, int nr,
-                                                     const int32_t *__restrict__ jobs, int njobs, CorrOut out)
+// W4: registers capped at 128 per lane for four resident waves per SIMD
+// (a few bytes spill), else the natural 168 (three waves)
+template <int K, bool PHASE, bool W4>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1))) void k_corr_mfma32(
+    CorrConfig cfg, Trig tg, const float *__restrict__ A, const float *__restrict__ B, int nr,
+    const int32_t *__restrict__ jobs, int njobs, CorrOut out)
 {
 #pragma clang fp contract(off)
     constexpr int M = 32, H = 16, LO = H - kHW;
@@ -688,20 +690,20 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
     const int wps = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 3;
     const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 4 * wps));
     const int nblocks = (waves + 3) / 4;
-    if (cfg.method == HH_PHASE) {
-        if (cfg.K == 2)
-            hipLaunchKernelGGL((k_corr_mfma32<2, true>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, out);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs, njobs, out);
+    };
+    const bool ph = cfg.method == HH_PHASE, k2 = cfg.K == 2;
+    if (wps >= 4) {
+        if (ph)
+            k2 ? go(k_corr_mfma32<2, true, true>) : go(k_corr_mfma32<3, true, true>);
         else
-            hipLaunchKernelGGL((k_corr_mfma32<3, true>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, out);
+            k2 ? go(k_corr_mfma32<2, false, true>) : go(k_corr_mfma32<3, false, true>);
     } else {
-        if (cfg.K == 2)
-            hipLaunchKernelGGL((k_corr_mfma32<2, false>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, out);
+        if (ph)
+            k2 ? go(k_corr_mfma32<2, true, false>) : go(k_corr_mfma32<3, true, false>);
         else
-            hipLaunchKernelGGL((k_corr_mfma32<3, false>), dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs,
-                               njobs, out);
+            k2 ? go(k_corr_mfma32<2, false, false>) : go(k_corr_mfma32<3, false, false>);
     }
     HH_HIP(hipGetLastError());
     return true;
