@@ -208,7 +208,7 @@ std::vector<DeformPair> deform_table(hh_ctx *c, const std::vector<int> &pairs, c
         knots(pred.c0, pred.cstep, pred.half, d.gnx, d.x0, d.xsp);
         d.out_off = (long long)L.roff[q] + (long long)L.coff[q] * L.ld;
         d.blk0 = blocks;
-        blocks += (int)(((size_t)d.nr * d.nc + 255) / 256);
+        blocks += deform_strips(d.nr) * d.nc;
         t.push_back(d);
     }
     return t;

@@ -205,6 +205,9 @@ struct DeformPair {
     int blk0;              // first block of the pair in a batched launch
     int pad;
 };
+// Blocks of 256 threads per column of an nr-row image in k_deform (one
+// block per 256-row strip of a column).
+__host__ __device__ inline int deform_strips(int nr) { return (nr + 255) / 256; }
 struct DeformTab {
     const DeformPair *pairs;  // device table, or nullptr: `one`
     int npairs;

@@ -56,7 +56,8 @@ __device__ __forceinline__ void cubw(float d, float w[4])
 }
 
 // The cubic B-spline weights times 6 (the separable sum of a sample is
-// divided by 36 once instead of dividing each of the 8 weights by 6).
+// scaled by 1/36 once, a multiply, instead of dividing each of the 8 weights
+// by 6; within an ulp of the divided form).
 __device__ __forceinline__ void cubw6(float d, float w[4])
 {
     const float d2 = d * d, d3 = d2 * d, e = 1.f - d;
@@ -79,13 +80,14 @@ __device__ __forceinline__ float cubic(const float *__restrict__ C, int nr, int 
     cubw6(yy - (float)iy, wy);
     cubw6(xx - (float)ix, wx);
     const int N1 = nr + 2;
+    const float *col0 = C + (size_t)(ix - 1) * N1 + (iy - 1);  // one 64-bit product, then column steps
     float s = 0.f;
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        const float *col = col0 + n * N1;
         s += wx[n] * (wy[0] * col[0] + wy[1] * col[1] + wy[2] * col[2] + wy[3] * col[3]);
     }
-    return s / 36.f;
+    return s * (1.f / 36.f);
 }
 
 // One axis of a sample position given as an integer pixel k plus a small
@@ -125,13 +127,14 @@ __device__ __forceinline__ float cubic_at(const float *__restrict__ C, int nr, i
     cubw6(fy, wy);
     cubw6(fx, wx);
     const int N1 = nr + 2;
+    const float *col0 = C + (size_t)(ix - 1) * N1 + (iy - 1);  // one 64-bit product, then column steps
     float s = 0.f;
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-        const float *col = C + (size_t)(ix - 1 + n) * N1 + (iy - 1);
+        const float *col = col0 + n * N1;
         s += wx[n] * (wy[0] * col[0] + wy[1] * col[1] + wy[2] * col[2] + wy[3] * col[3]);
     }
-    return s / 36.f;
+    return s * (1.f / 36.f);
 }
 
 // Bilinear sample of an nr x nc image at (r + oy, c + ox) and its partial

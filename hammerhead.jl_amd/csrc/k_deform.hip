@@ -130,7 +130,10 @@ __global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict_
 
 // One warped pixel of each image per thread. Blocks [blk0, blk0 + n) of a
 // batched launch belong to one pair (found by bisection of the pair table);
-// within a pair, consecutive threads take consecutive pixels of a column.
+// within a pair, block b covers rows 256 (b mod S) + 1 .. of column b / S + 1
+// (S = deform_strips(nr) blocks per column), so the pixel's column and row
+// come from the block index (uniform) and the thread index, with no per-lane
+// division.
 __global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__restrict__ wA,
                                                 float *__restrict__ wB)
 {
@@ -154,9 +157,9 @@ __global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__
     else
         P = tab.one;
     const int nr = P.nr, nc = P.nc;
-    const size_t e = (size_t)(blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nr * nc) return;
-    const int r = (int)(e % nr) + 1, c = (int)(e / nr) + 1;
+    const int S = deform_strips(nr), b = (int)blockIdx.x - P.blk0;
+    const int c = b / S + 1, r = (b % S) * 256 + (int)threadIdx.x + 1;
+    if (r > nr || c > nc) return;
     const float R = (float)r, Cc = (float)c;
     int iy, ix;
     float ly, lx;
@@ -223,7 +226,7 @@ void launch_deform(const float *cA, const float *cB, int nr, int nc, const float
     DeformTab t{};
     t.pairs = nullptr;
     t.npairs = 1;
-    t.nblocks = (int)(((size_t)nr * nc + 255) / 256);
+    t.nblocks = deform_strips(nr) * nc;
     t.one = DeformPair{cA, cB, pu, pv, nr, nc, gny, gnx, y0, ysp, x0, xsp, 0, 0, 0};
     launch_deform_batch(t, nr, wA, wB, s);
 }
