@@ -697,10 +697,11 @@ void record(hh_ctx *c, bool dynamic)
                     HH_HIP(hipStreamWaitEvent(s, c->pre_join, 0));
                     pre_pending = false;
                 }
+                // the pass's node values ride along as extra blocks
+                tab.nv = node_values_job(T->grids, dt, P.r0, P.rstep, P.c0, P.cstep, P.half, E.u0.p, E.v0.p);
                 timed(c, "deform", 0, [&] { launch_deform_batch(tab, L.ld, wA, wB, s); });
                 srcA = wA;
                 srcB = wB;
-                launch_node_values_batch(T->grids, dt, P.r0, P.rstep, P.c0, P.cstep, P.half, E.u0.p, E.v0.p, s);
             } else {
                 gfill(c, E.u0.p, E.v0.p, T->grids, 0.f);
                 srcA = L.A;

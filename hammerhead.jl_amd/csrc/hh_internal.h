@@ -208,11 +208,24 @@ struct DeformPair {
 // Blocks of 256 threads per column of an nr-row image in k_deform (one
 // block per 256-row strip of a column).
 __host__ __device__ inline int deform_strips(int nr) { return (nr + 255) / 256; }
+// The predictor node values of a pass (predictor_node_values on the pass's
+// window centres, see launch_node_values_batch), optionally run by extra
+// blocks of the deformation launch: the two read the same predictor and are
+// independent, so one launch serves both.
+struct NodeValuesJob {
+    Grids g;
+    const DeformPair *pairs;  // device table (segment s reads pairs[seg.pair])
+    int r0, rstep, c0, cstep;
+    float half;
+    float *u, *v;
+    int nblocks;              // 0: none
+};
 struct DeformTab {
     const DeformPair *pairs;  // device table, or nullptr: `one`
     int npairs;
     int nblocks;
     DeformPair one;
+    NodeValuesJob nv;
 };
 
 // img: nr x nc with leading dimension ld.
@@ -231,6 +244,10 @@ void launch_node_values(const float *pu, const float *pv, int gny, int gnx, floa
 // + half (0-based i, j); segment s reads the predictor of pairs[seg.pair].
 void launch_node_values_batch(const Grids &g, const DeformPair *pairs, int r0, int rstep, int c0, int cstep,
                               float half, float *u, float *v, hipStream_t s);
+// The same work as a job for extra blocks of a deformation launch
+// (DeformTab::nv).
+NodeValuesJob node_values_job(const Grids &g, const DeformPair *pairs, int r0, int rstep, int c0, int cstep,
+                              float half, float *u, float *v);
 
 struct ValidateArgs {
     Grids grids;

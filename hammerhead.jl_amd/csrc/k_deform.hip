@@ -128,6 +128,18 @@ __global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict_
     }
 }
 
+// Predictor node value of enumerated cell ee of nv.g.
+__device__ __forceinline__ void node_values_cell(const NodeValuesJob &nv, int ee)
+{
+    if (ee >= nv.g.total) return;
+    const Cell k = cell_of(nv.g, ee);
+    const int e = k.b, i = k.r, j = k.c;
+    const DeformPair &P = nv.pairs[k.sg.pair];
+    const float y = (float)(nv.r0 + i * nv.rstep) + nv.half, x = (float)(nv.c0 + j * nv.cstep) + nv.half;
+    nv.u[e] = bilinear(P.pu, P.gny, P.gnx, P.y0, P.ysp, P.x0, P.xsp, y, x);
+    nv.v[e] = bilinear(P.pv, P.gny, P.gnx, P.y0, P.ysp, P.x0, P.xsp, y, x);
+}
+
 // One warped pixel of each image per thread. Blocks [blk0, blk0 + n) of a
 // batched launch belong to one pair (found by bisection of the pair table);
 // within a pair, block b covers rows 256 (b mod S) + 1 .. of column b / S + 1
@@ -137,6 +149,10 @@ __global__ __launch_bounds__(256) void k_prefilter_cols(const float *__restrict_
 __global__ __launch_bounds__(256) void k_deform(DeformTab tab, int ld, float *__restrict__ wA,
                                                 float *__restrict__ wB)
 {
+    if ((int)blockIdx.x >= tab.nblocks) {  // the node-values blocks (uniform branch)
+        node_values_cell(tab.nv, ((int)blockIdx.x - tab.nblocks) * 256 + (int)threadIdx.x);
+        return;
+    }
     int q = 0;
     if (tab.pairs) {
         int lo = 0, hi = tab.npairs - 1;
@@ -192,18 +208,9 @@ __global__ __launch_bounds__(256) void k_node_values(const float *__restrict__ p
 
 // predictor_node_values for the cells of several grids: segment s holds
 // the window centres of pair seg.pair.
-__global__ __launch_bounds__(256) void k_node_values_batch(Grids g, const DeformPair *__restrict__ pairs, int r0,
-                                                           int rstep, int c0, int cstep, float half,
-                                                           float *__restrict__ u, float *__restrict__ v)
+__global__ __launch_bounds__(256) void k_node_values_batch(NodeValuesJob nv)
 {
-    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ee >= g.total) return;
-    const Cell k = cell_of(g, ee);
-    const int e = k.b, i = k.r, j = k.c;
-    const DeformPair &P = pairs[k.sg.pair];
-    const float y = (float)(r0 + i * rstep) + half, x = (float)(c0 + j * cstep) + half;
-    u[e] = bilinear(P.pu, P.gny, P.gnx, P.y0, P.ysp, P.x0, P.xsp, y, x);
-    v[e] = bilinear(P.pv, P.gny, P.gnx, P.y0, P.ysp, P.x0, P.xsp, y, x);
+    node_values_cell(nv, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 }  // namespace
@@ -233,17 +240,24 @@ void launch_deform(const float *cA, const float *cB, int nr, int nc, const float
 
 void launch_deform_batch(const DeformTab &tab, int ld, float *wA, float *wB, hipStream_t s)
 {
-    if (tab.nblocks <= 0) return;
-    hipLaunchKernelGGL(k_deform, dim3((unsigned)tab.nblocks), dim3(256), 0, s, tab, ld, wA, wB);
+    const int nb = tab.nblocks + (tab.nv.g.total > 0 ? tab.nv.nblocks : 0);
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_deform, dim3((unsigned)nb), dim3(256), 0, s, tab, ld, wA, wB);
     HH_HIP(hipGetLastError());
+}
+
+NodeValuesJob node_values_job(const Grids &g, const DeformPair *pairs, int r0, int rstep, int c0, int cstep,
+                              float half, float *u, float *v)
+{
+    return NodeValuesJob{g, pairs, r0, rstep, c0, cstep, half, u, v, (g.total + 255) / 256};
 }
 
 void launch_node_values_batch(const Grids &g, const DeformPair *pairs, int r0, int rstep, int c0, int cstep,
                               float half, float *u, float *v, hipStream_t s)
 {
     if (g.total <= 0) return;
-    hipLaunchKernelGGL(k_node_values_batch, dim3((g.total + 255) / 256), dim3(256), 0, s, g, pairs, r0, rstep, c0,
-                       cstep, half, u, v);
+    const NodeValuesJob nv = node_values_job(g, pairs, r0, rstep, c0, cstep, half, u, v);
+    hipLaunchKernelGGL(k_node_values_batch, dim3((unsigned)nv.nblocks), dim3(256), 0, s, nv);
     HH_HIP(hipGetLastError());
 }
 
