@@ -325,8 +325,10 @@ int hh_graph_replays(hh_ctx *ctx, int64_t *replays);
  * schedule is planned, each pass's correlation kernel is timed over its own
  * job list for every candidate and the fastest is kept per context.
  * Level 1 tunes the launch geometry: resident MFMA waves per SIMD (which
- * sets the windows each persistent wave and workgroup walks), workgroup
- * size of the 32-px matrix-core kernel (4 or 8 waves), window-to-XCD order;
+ * sets the windows each persistent wave and workgroup walks, and for the
+ * 16-px matrix-core kernel its register budget: 3 waves at 168 registers or
+ * 4 at 128), workgroup size of the 32-px matrix-core kernel (4 or 8 waves),
+ * window-to-XCD order;
  * results do not depend on the choice. Level 2 also chooses the kernel
  * family per FFT side among the form's own (the split-FP16 matrix-core DFT
  * in the MFMA form), the register FFT and the LDS radix-2 FFT / direct DFT
