@@ -91,20 +91,55 @@ __device__ __forceinline__ void sort_net(T (&x)[N])
     merge_round<T, N, ceil_log2<N>() - 1>(x);
 }
 
+// Sorts a V-shaped sequence: the first entries non-increasing, then
+// non-decreasing, the rest +inf padding (the residuals |x_i - m| of an
+// ascending x with its padding). Such a sequence is bitonic, and a bitonic
+// merge (half-cleaners of widths P/2, P/4, ..., 1, P the next power of two)
+// sorts it; the comparators that would reach past N only meet the implied
+// +inf padding and are identities. For N = 24: 67 min/max operations on the
+// outputs a median reads, against 221 for the full sorting network.
+template <typename T, int N, int D>
+__device__ __forceinline__ void half_cleaners(T (&x)[N])
+{
+    if constexpr (D >= 1) {
+#pragma unroll
+        for (int i = 0; i + D < N; ++i)
+            if ((i / D) % 2 == 0) {
+                const T a = x[i], b = x[i + D];
+                x[i] = vmin(a, b);
+                x[i + D] = vmax(a, b);
+            }
+        half_cleaners<T, N, D / 2>(x);
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void sort_vshaped(T (&x)[N])
+{
+    half_cleaners<T, N, (1 << ceil_log2<N>()) / 2>(x);
+}
+
 // median! of the first n of N sorted-with-padding registers (padding = +inf
-// placed after the valid values, n <= N). hasnan forces NaN.
+// placed after the valid values, n <= N), in type R (the two middle values
+// converted exactly, then halved and added in R). hasnan forces NaN.
+template <typename R, typename T, int N>
+__device__ __forceinline__ R median_sorted_as(const T (&x)[N], int n, bool hasnan)
+{
+    if (hasnan) return (R)NAN;
+    R lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i <= N / 2; ++i) {
+        if (i == (n - 1) / 2) lo = (R)x[i];
+        if (i == n / 2) hi = (R)x[i];
+    }
+    if (n & 1) return lo;
+    return lo / (R)2 + hi / (R)2;
+}
+
 template <typename T, int N>
 __device__ __forceinline__ T median_sorted(const T (&x)[N], int n, bool hasnan)
 {
-    if (hasnan) return (T)NAN;
-    T lo = 0, hi = 0;
-#pragma unroll
-    for (int i = 0; i <= N / 2; ++i) {
-        if (i == (n - 1) / 2) lo = x[i];
-        if (i == n / 2) hi = x[i];
-    }
-    if (n & 1) return lo;
-    return lo / (T)2 + hi / (T)2;
+    return median_sorted_as<T, T, N>(x, n, hasnan);
 }
 
 template <int NB>
@@ -128,8 +163,7 @@ __global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const 
     const float *__restrict__ ug = u + sg.off;
     const float *__restrict__ vg = v + sg.off;
     const uint8_t *__restrict__ xg = exclude ? exclude + sgoff : nullptr;
-    float nu[N], nv[N], su[N], sv[N];
-    bool ok[N];
+    float su[N], sv[N];
     int n = 0;
     bool nanu = false, nanv = false;
 #pragma unroll
@@ -140,30 +174,32 @@ __global__ __launch_bounds__(256) void k_uod(const float *__restrict__ u, const 
         // clamped addresses: every load is in bounds and unconditional
         const int rc = min(max(r2, 0), ny - 1), cc = min(max(c2, 0), nx - 1);
         const int at = rc + cc * ny;
-        ok[q] = r2 == rc && c2 == cc && !(xg && xg[at]);
-        nu[q] = ok[q] ? ug[at] : INFINITY;
-        nv[q] = ok[q] ? vg[at] : INFINITY;
-        nanu |= ok[q] && isnan(nu[q]);
-        nanv |= ok[q] && isnan(nv[q]);
-        n += ok[q];
-        su[q] = nu[q];
-        sv[q] = nv[q];
+        const bool ok = r2 == rc && c2 == cc && !(xg && xg[at]);
+        su[q] = ok ? ug[at] : INFINITY;
+        sv[q] = ok ? vg[at] : INFINITY;
+        nanu |= ok && isnan(su[q]);
+        nanv |= ok && isnan(sv[q]);
+        n += ok;
     }
     if (n == 0) return;
     sort_net<float, N>(su);
     sort_net<float, N>(sv);
     const float umed = median_sorted<float, N>(su, n, nanu);
     const float vmed = median_sorted<float, N>(sv, n, nanv);
-    bool nanru = false, nanrv = false;
+    // residuals of the sorted values: the first n are the valid values'
+    // (the same multiset as over the neighbours), the padding stays +inf,
+    // and the sequence is V-shaped. A valid residual is NaN exactly when the
+    // median is not finite (NaN among the values, or an infinite median,
+    // which some valid value equals), and then the MAD is NaN whatever the
+    // other entries hold.
+    const bool nanru = !isfinite(umed), nanrv = !isfinite(vmed);
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-        su[q] = ok[q] ? fabsf(nu[q] - umed) : INFINITY;
-        sv[q] = ok[q] ? fabsf(nv[q] - vmed) : INFINITY;
-        nanru |= ok[q] && isnan(su[q]);
-        nanrv |= ok[q] && isnan(sv[q]);
+        su[q] = fabsf(su[q] - umed);
+        sv[q] = fabsf(sv[q] - vmed);
     }
-    sort_net<float, N>(su);
-    sort_net<float, N>(sv);
+    sort_vshaped<float, N>(su);
+    sort_vshaped<float, N>(sv);
     const float madu = median_sorted<float, N>(su, n, nanru);
     const float madv = median_sorted<float, N>(sv, n, nanrv);
     const float du = fabsf(u[e] - umed), dv = fabsf(v[e] - vmed);
@@ -253,8 +289,7 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
     const float *__restrict__ vg = vref + sg.off;
     const uint8_t *__restrict__ fg = flags_in + sg.off;
     const uint8_t *__restrict__ mg = mask ? mask + sg.off : nullptr;
-    double nu[N], nv[N], su[N], sv[N];
-    bool ok[N];
+    float su[N], sv[N];
     int n = 0;
     bool nanu = false, nanv = false;
 #pragma unroll
@@ -269,32 +304,34 @@ __device__ __forceinline__ uint8_t substitute_cell(const float *__restrict__ ure
         const int at = rc + cc * ny;
         const uint8_t fq = fg[at], mq = mg ? mg[at] : (uint8_t)0;
         const float uq = ug[at], vq = vg[at];
-        ok[q] = r2 == rc && c2 == cc && !fq && !mq;
-        nu[q] = ok[q] ? (double)uq : INFINITY;
-        nv[q] = ok[q] ? (double)vq : INFINITY;
-        nanu |= ok[q] && isnan(nu[q]);
-        nanv |= ok[q] && isnan(nv[q]);
-        n += ok[q];
-        su[q] = nu[q];
-        sv[q] = nv[q];
+        const bool ok = r2 == rc && c2 == cc && !fq && !mq;
+        su[q] = ok ? uq : INFINITY;
+        sv[q] = ok ? vq : INFINITY;
+        nanu |= ok && isnan(uq);
+        nanv |= ok && isnan(vq);
+        n += ok;
     }
     if (n < 3) return 1;
-    sort_net<double, N>(su);
-    sort_net<double, N>(sv);
-    const double mu = median_sorted<double, N>(su, n, nanu);
-    const double mv = median_sorted<double, N>(sv, n, nanv);
-    bool nanru = false, nanrv = false;
+    // the values sort as floats (their Float64 images order the same); the
+    // medians and residuals are formed in Float64 as the reference does
+    sort_net<float, N>(su);
+    sort_net<float, N>(sv);
+    const double mu = median_sorted_as<double, float, N>(su, n, nanu);
+    const double mv = median_sorted_as<double, float, N>(sv, n, nanv);
+    // residuals of the sorted values (the valid multiset, then +inf padding,
+    // V-shaped; NaN among the valid ones exactly when the median is not
+    // finite, as in k_uod)
+    double ru[N], rv[N];
+    const bool nanru = !isfinite(mu), nanrv = !isfinite(mv);
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-        su[q] = ok[q] ? fabs(nu[q] - mu) : INFINITY;
-        sv[q] = ok[q] ? fabs(nv[q] - mv) : INFINITY;
-        nanru |= ok[q] && isnan(su[q]);
-        nanrv |= ok[q] && isnan(sv[q]);
+        ru[q] = fabs((double)su[q] - mu);
+        rv[q] = fabs((double)sv[q] - mv);
     }
-    sort_net<double, N>(su);
-    sort_net<double, N>(sv);
-    const double madu = median_sorted<double, N>(su, n, nanru);
-    const double madv = median_sorted<double, N>(sv, n, nanrv);
+    sort_vshaped<double, N>(ru);
+    sort_vshaped<double, N>(rv);
+    const double madu = median_sorted<double, N>(ru, n, nanru);
+    const double madv = median_sorted<double, N>(rv, n, nanrv);
     const size_t ng = (size_t)g.stride;  // alternatives: stride x nalt
     for (int q = 0; q < nalt; ++q) {
         const float au = alt_u[e + q * ng], av = alt_v[e + q * ng];
