@@ -6,6 +6,7 @@ what run_piv gives on that pair alone, since both run the same per-window,
 per-pixel and per-cell arithmetic; a graph replay gives exactly the eager
 launch sequence's result; and a batch pair against the FP64 oracle meets the
 run_piv bars (flags within the near-tie allowance, u/v RMS < 5e-3 px)."""
+import re
 import sys
 
 import numpy as np
@@ -224,3 +225,42 @@ def test_autotune_kernel_families(hh, orc, small_pairs, form):
             c.set_pair(p.A, p.B)
             g = c.run_piv(passes)
         field_parity(g, o, label=f"{form} family {fam}", rms_tol=5e-3)
+
+
+def test_every_launch_geometry_gives_the_same_field(hh, small_pairs):
+    """Every launch-geometry candidate the level-1 tuner times for the
+    matrix-core kernels (resident waves per SIMD, which for the 16-px kernel
+    also selects its 128-register build at 4 waves; window order) gives the
+    untuned field bit for bit: the geometry only changes which wave takes
+    which window."""
+    passes = schedule(hh)
+    p = small_pairs[0]
+    with hh.Context(0) as c:
+        c.set_form(hh.HH_FORM_MFMA16)
+        c.set_autotune(False)
+        c.set_pair(p.A, p.B)
+        base = c.run_piv(passes)
+    with hh.Context(0) as c:
+        c.set_form(hh.HH_FORM_MFMA16)
+        c.set_pair(p.A, p.B)
+        c.run_piv(passes)
+        rep = c.tune_report()
+    assert any(k.startswith("mfma32 ") for k in rep), rep
+    nrun = 0
+    for key, e in rep.items():
+        for name in e["ms"]:
+            m = re.search(r"waves_per_simd=(\d+)", name)
+            if not m:
+                continue
+            w, order = int(m.group(1)), int("order=linear" in name)
+            forced = dict(rep)
+            forced[key] = dict(e, choice=dict(e["choice"], waves_per_simd=w, linear_order=order))
+            with hh.Context(0) as c:
+                c.set_form(hh.HH_FORM_MFMA16)
+                c.tune_load(forced)
+                c.set_pair(p.A, p.B)
+                g = c.run_piv(passes)
+            say(f"{key}: {name}")
+            assert same(base, g), f"{key} {name}: {diff_report(base, g)}"
+            nrun += 1
+    assert nrun >= 8
