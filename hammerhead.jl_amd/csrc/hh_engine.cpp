@@ -766,7 +766,13 @@ void record(hh_ctx *c, bool dynamic)
             va.sflag = E.sflag.p;
             va.count = T->count.p;
             va.mask = P.anyg ? P.gmask.p : nullptr;
-            timed(c, "validate", P.cfg.w, [&] { launch_validate(va, s); });
+            // the next sweep's predictor comes out of the replacement launch
+            // (fixed sweep counts: no pair drops out in between)
+            const bool fuse_pred = it < maxiter && P.fixed && E.smoothing && !P.anyg;
+            va.smooth_u = fuse_pred ? E.pu.p : nullptr;
+            va.smooth_v = fuse_pred ? E.pv.p : nullptr;
+            bool smoothed = false;
+            timed(c, "validate", P.cfg.w, [&] { smoothed = launch_validate(va, s); });
             if (it == maxiter) break;
             if (!P.fixed) {
                 if (!dynamic) throw Error(HH_ESTATE, "internal: iterating pass without a fixed sweep count");
@@ -798,7 +804,7 @@ void record(hh_ctx *c, bool dynamic)
                 }
                 gcopy2(c, E.u.p, E.v.p, E.prevu.p, E.prevv.p, T->grids);
             }
-            predictor(c, T->grids, P);
+            if (!smoothed) predictor(c, T->grids, P);
             have_pred = true;
             pred_self = true;
         }

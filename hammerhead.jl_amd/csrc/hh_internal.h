@@ -264,8 +264,12 @@ struct ValidateArgs {
     // grid mask (nullable, grids.stride bytes): masked cells are excluded
     // from the UOD, never flagged or substituted, and NaN after a replacement
     const uint8_t *mask;
+    // predictor (nullable): with no mask and a replacement stage, the
+    // replacement launch also writes build_predictor's smoothing of the
+    // replaced field here (launch_validate then returns true)
+    float *smooth_u, *smooth_v;
 };
-void launch_validate(const ValidateArgs &a, hipStream_t s);
+bool launch_validate(const ValidateArgs &a, hipStream_t s);
 // u, v, peak ratio and correlation moment of the masked cells set to NaN.
 void launch_mask_nan(const Grids &g, const uint8_t *mask, float *u, float *v, float *pr, float *cm, hipStream_t s);
 // build_predictor's fill (src/pipeline.jl:454-459): the masked cells of

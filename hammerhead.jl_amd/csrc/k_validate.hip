@@ -428,6 +428,76 @@ __global__ __launch_bounds__(256) void k_substitute(const float *__restrict__ ur
     if (count) add_valid(g, ee, valid, count);
 }
 
+// The value replace_vectors! gives an invalid, unmasked cell (r, c) of a
+// segment (0-based): the median of the valid vectors of the smallest box
+// around it (radius 1, 2, ...) holding at least 3, or its own vector when the
+// segment has fewer than 3 valid cells (nval).
+__device__ __forceinline__ float2 replace_invalid(const float *__restrict__ uref, const float *__restrict__ vref,
+                                                  const uint8_t *__restrict__ invalid,
+                                                  const uint8_t *__restrict__ mask, int nval, int ny, int nx, int r,
+                                                  int c)
+{
+    const int e = r + c * ny;
+    auto bad = [&](int at) { return invalid[at] || (mask && mask[at]); };
+    {
+        // radius 1 (the common case) from registers; every neighbour's flag,
+        // mask and vector are loaded together at clamped addresses (a load
+        // behind a test would wait for it)
+        double nu[9], nv[9];
+        int n = 0;
+        bool nanu = false, nanv = false;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int r2 = r + q % 3 - 1, c2 = c + q / 3 - 1;
+            const int rc = min(max(r2, 0), ny - 1), cc = min(max(c2, 0), nx - 1);
+            const int at = rc + cc * ny;
+            const bool bq = invalid[at] || (mask && mask[at]);
+            const float uq = uref[at], vq = vref[at];
+            const bool ok = r2 == rc && c2 == cc && !bq;
+            nu[q] = ok ? (double)uq : INFINITY;
+            nv[q] = ok ? (double)vq : INFINITY;
+            nanu |= ok && isnan(nu[q]);
+            nanv |= ok && isnan(nv[q]);
+            n += ok;
+        }
+        if (nval < 3) return make_float2(uref[e], vref[e]);
+        if (n >= 3) {
+            sort_net<double, 9>(nu);
+            sort_net<double, 9>(nv);
+            return make_float2((float)median_sorted<double, 9>(nu, n, nanu),
+                               (float)median_sorted<double, 9>(nv, n, nanv));
+        }
+    }
+    const int maxr = max(ny, nx) - 1;
+    for (int rad = 2; rad <= maxr; ++rad) {
+        const int r0 = max(0, r - rad), r1 = min(ny - 1, r + rad), c0 = max(0, c - rad), c1 = min(nx - 1, c + rad);
+        int n = 0;
+        for (int c2 = c0; c2 <= c1; ++c2)
+            for (int r2 = r0; r2 <= r1; ++r2) n += !bad(r2 + c2 * ny);
+        if (n < 3) continue;
+        auto gu = [&](int r2, int c2, double &x) {
+            if (bad(r2 + c2 * ny)) return false;
+            x = (double)uref[r2 + c2 * ny];
+            return true;
+        };
+        auto gv = [&](int r2, int c2, double &x) {
+            if (bad(r2 + c2 * ny)) return false;
+            x = (double)vref[r2 + c2 * ny];
+            return true;
+        };
+        return make_float2((float)median_box(gu, r0, r1, c0, c1, n), (float)median_box(gv, r0, r1, c0, c1, n));
+    }
+    return make_float2(uref[e], vref[e]);
+}
+
+// The same as a call (the fused kernel below reaches it from nine places).
+__device__ __noinline__ float2 replace_invalid_call(const float *__restrict__ uref, const float *__restrict__ vref,
+                                                    const uint8_t *__restrict__ invalid, int nval, int ny, int nx,
+                                                    int r, int c)
+{
+    return replace_invalid(uref, vref, invalid, nullptr, nval, ny, nx, r, c);
+}
+
 // flags_out (nullable) receives a copy of invalid. mask (nullable): cells
 // that are invalid as well and end as NaN (validate_and_replace!,
 // src/pipeline.jl:636-642: the invalid set is outliers .| mask, masked
@@ -451,68 +521,73 @@ __global__ __launch_bounds__(256) void k_replace(const float *__restrict__ uref_
         return;
     }
     if (!invalid_all[e]) return;
-    const int ny = sg.ny, nx = sg.nx, r = k.r, c = k.c;
-    const float *__restrict__ uref = uref_all + sg.off;
-    const float *__restrict__ vref = vref_all + sg.off;
-    const uint8_t *__restrict__ invalid = invalid_all + sg.off;
-    const uint8_t *__restrict__ mask = mask_all ? mask_all + sg.off : nullptr;
-    auto bad = [&](int at) { return invalid[at] || (mask && mask[at]); };
-    {
-        // radius 1 (the common case) from registers; the segment's count and
-        // every neighbour's flag, mask and vector are loaded together at
-        // clamped addresses (a load behind a test would wait for it)
-        const int nval = nvalid[k.si];
-        double nu[9], nv[9];
-        int n = 0;
-        bool nanu = false, nanv = false;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const int r2 = r + q % 3 - 1, c2 = c + q / 3 - 1;
-            const int rc = min(max(r2, 0), ny - 1), cc = min(max(c2, 0), nx - 1);
-            const int at = rc + cc * ny;
-            const bool bq = invalid[at] || (mask && mask[at]);
-            const float uq = uref[at], vq = vref[at];
-            const bool ok = r2 == rc && c2 == cc && !bq;
-            nu[q] = ok ? (double)uq : INFINITY;
-            nv[q] = ok ? (double)vq : INFINITY;
-            nanu |= ok && isnan(nu[q]);
-            nanv |= ok && isnan(nv[q]);
-            n += ok;
-        }
-        if (nval < 3) return;
-        if (n >= 3) {
-            sort_net<double, 9>(nu);
-            sort_net<double, 9>(nv);
-            u_out[e] = (float)median_sorted<double, 9>(nu, n, nanu);
-            v_out[e] = (float)median_sorted<double, 9>(nv, n, nanv);
-            return;
-        }
-    }
-    const int maxr = max(ny, nx) - 1;
-    for (int rad = 2; rad <= maxr; ++rad) {
-        const int r0 = max(0, r - rad), r1 = min(ny - 1, r + rad), c0 = max(0, c - rad), c1 = min(nx - 1, c + rad);
-        int n = 0;
-        for (int c2 = c0; c2 <= c1; ++c2)
-            for (int r2 = r0; r2 <= r1; ++r2) n += !bad(r2 + c2 * ny);
-        if (n < 3) continue;
-        auto gu = [&](int r2, int c2, double &x) {
-            if (bad(r2 + c2 * ny)) return false;
-            x = (double)uref[r2 + c2 * ny];
-            return true;
-        };
-        auto gv = [&](int r2, int c2, double &x) {
-            if (bad(r2 + c2 * ny)) return false;
-            x = (double)vref[r2 + c2 * ny];
-            return true;
-        };
-        u_out[e] = (float)median_box(gu, r0, r1, c0, c1, n);
-        v_out[e] = (float)median_box(gv, r0, r1, c0, c1, n);
-        return;
-    }
+    const float2 f = replace_invalid(uref_all + sg.off, vref_all + sg.off, invalid_all + sg.off,
+                                     mask_all ? mask_all + sg.off : nullptr, nvalid[k.si], sg.ny, sg.nx, k.r, k.c);
+    u_out[e] = f.x;
+    v_out[e] = f.y;
 }
 
-// smooth_field of one or two fields (g2/out2 nullable: u and v of a
-// predictor in one launch).
+// The (1, 2, 1) / 4 smoothing of build_predictor (src/pipeline.jl:454-464,
+// src/quality.jl:123-136) at one cell from its 3 x 3 neighbourhood f[dc][dr]
+// (edges replicated).
+__device__ __forceinline__ float smooth3x3(const float (&f)[3][3])
+{
+    const float q = 0.25f;
+    auto tmp = [&](int dr) { return q * ((f[0][dr] + 2.f * f[1][dr]) + f[2][dr]); };
+    return q * ((tmp(0) + 2.f * tmp(1)) + tmp(2));
+}
+
+// k_replace (no mask) followed by the predictor's smoothing of the replaced
+// field in one launch: each cell also forms the replaced values of its 3 x 3
+// neighbourhood (a replacement only reads the input field, so neighbours'
+// results need no exchange; flagged cells are few, and the lanes of a wave
+// that need a neighbour's median run the call together) and writes the
+// smoothed pair to (pu, pv).
+__global__ __launch_bounds__(256) void k_replace_smooth(const float *__restrict__ uref_all,
+                                                        const float *__restrict__ vref_all,
+                                                        const uint8_t *__restrict__ invalid_all, Grids g,
+                                                        const int *__restrict__ nvalid, float *__restrict__ u_out,
+                                                        float *__restrict__ v_out, uint8_t *__restrict__ flags_out,
+                                                        float *__restrict__ pu, float *__restrict__ pv)
+{
+    const int ee = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ee >= g.total) return;
+    const Cell k = cell_of(g, ee);
+    const int e = k.b, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
+    const float *__restrict__ ur = uref_all + k.sg.off;
+    const float *__restrict__ vr = vref_all + k.sg.off;
+    const uint8_t *__restrict__ inv = invalid_all + k.sg.off;
+    const int nval = nvalid[k.si];
+    if (flags_out) flags_out[e] = invalid_all[e];
+    // the nine flags and vectors in flight together, then the (rare)
+    // replacements
+    float fu[3][3], fv[3][3];
+    bool fl[3][3];
+#pragma unroll
+    for (int dc = 0; dc < 3; ++dc)
+#pragma unroll
+        for (int dr = 0; dr < 3; ++dr) {
+            const int at = min(max(r + dr - 1, 0), ny - 1) + min(max(c + dc - 1, 0), nx - 1) * ny;
+            fl[dc][dr] = inv[at];
+            fu[dc][dr] = ur[at];
+            fv[dc][dr] = vr[at];
+        }
+#pragma unroll
+    for (int dc = 0; dc < 3; ++dc)
+#pragma unroll
+        for (int dr = 0; dr < 3; ++dr)
+            if (fl[dc][dr]) {
+                const float2 f = replace_invalid_call(ur, vr, inv, nval, ny, nx, min(max(r + dr - 1, 0), ny - 1),
+                                                      min(max(c + dc - 1, 0), nx - 1));
+                fu[dc][dr] = f.x;
+                fv[dc][dr] = f.y;
+            }
+    u_out[e] = fu[1][1];
+    v_out[e] = fv[1][1];
+    pu[e] = smooth3x3(fu);
+    pv[e] = smooth3x3(fv);
+}
+
 __global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f_all, const float *__restrict__ f2_all,
                                                 Grids g, float *__restrict__ out, float *__restrict__ out2)
 {
@@ -522,10 +597,14 @@ __global__ __launch_bounds__(256) void k_smooth(const float *__restrict__ f_all,
     const int e = k.b, ny = k.sg.ny, nx = k.sg.nx, r = k.r, c = k.c;
     const int cl = c > 0 ? c - 1 : 0, cr = c < nx - 1 ? c + 1 : nx - 1;
     const int rl = r > 0 ? r - 1 : 0, rr = r < ny - 1 ? r + 1 : ny - 1;
-    const float q = 0.25f;
     auto sm = [&](const float *__restrict__ f) {
-        auto tmp = [&](int rw) { return q * ((f[rw + cl * ny] + 2.f * f[rw + c * ny]) + f[rw + cr * ny]); };
-        return q * ((tmp(rl) + 2.f * tmp(r)) + tmp(rr));
+        const int cs[3] = {cl, c, cr}, rs[3] = {rl, r, rr};
+        float n3[3][3];
+#pragma unroll
+        for (int dc = 0; dc < 3; ++dc)
+#pragma unroll
+            for (int dr = 0; dr < 3; ++dr) n3[dc][dr] = f[rs[dr] + cs[dc] * ny];
+        return smooth3x3(n3);
     };
     out[e] = sm(f_all + k.sg.off);
     if (f2_all) out2[e] = sm(f2_all + k.sg.off);
@@ -608,12 +687,13 @@ void uod(const ValidateArgs &a, int nb, double thr, double eps, bool init, hipSt
 
 // Launches: the first UOD writes every flag and clears the counters (else a
 // clear kernel does), the substitution also counts the valid cells the
-// replacement needs (else a count kernel does), then the replacement.
-void launch_validate(const ValidateArgs &a, hipStream_t s)
+// replacement needs (else a count kernel does), then the replacement (with
+// the predictor's smoothing when a.smooth_u is set and there is no mask).
+bool launch_validate(const ValidateArgs &a, hipStream_t s)
 {
     const hh_pass &p = *a.pass;
     const int n = a.grids.total;
-    if (n <= 0) return;
+    if (n <= 0) return false;
     const int ntile = std::max(n, a.grids.nseg);
     const dim3 g((n + 255) / 256), b(256);
     if (p.uod_enable)
@@ -653,20 +733,30 @@ void launch_validate(const ValidateArgs &a, hipStream_t s)
             break;
         }
     }
+    const bool smooth = rep && !a.mask && a.smooth_u && a.smooth_v;
     if (rep) {
         if (sub) {  // substituted field in scratch: replace back into the chain
-            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.grids, (const int *)a.count, a.u,
-                               a.v, a.outliers, a.mask);
+            if (smooth)
+                hipLaunchKernelGGL(k_replace_smooth, g, b, 0, s, a.su, a.sv, a.sflag, a.grids,
+                                   (const int *)a.count, a.u, a.v, a.outliers, a.smooth_u, a.smooth_v);
+            else
+                hipLaunchKernelGGL(k_replace, g, b, 0, s, a.su, a.sv, a.sflag, a.grids, (const int *)a.count, a.u,
+                                   a.v, a.outliers, a.mask);
         } else {
             hipLaunchKernelGGL(k_count_valid, g, b, 0, s, a.outliers, a.mask, a.grids, a.count);
-            hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.grids, (const int *)a.count, a.su,
-                               a.sv, (uint8_t *)nullptr, a.mask);
+            if (smooth)
+                hipLaunchKernelGGL(k_replace_smooth, g, b, 0, s, a.u, a.v, a.outliers, a.grids,
+                                   (const int *)a.count, a.su, a.sv, (uint8_t *)nullptr, a.smooth_u, a.smooth_v);
+            else
+                hipLaunchKernelGGL(k_replace, g, b, 0, s, a.u, a.v, a.outliers, a.grids, (const int *)a.count,
+                                   a.su, a.sv, (uint8_t *)nullptr, a.mask);
         }
     }
     if (sub != rep)  // exactly one stage ran: its result is in scratch
         hipLaunchKernelGGL(k_copy_back, g, b, 0, s, a.su, a.sv, sub ? a.sflag : (const uint8_t *)nullptr, a.u, a.v,
                            a.outliers, a.grids);
     HH_HIP(hipGetLastError());
+    return smooth;
 }
 
 void launch_mask_nan(const Grids &g, const uint8_t *mask, float *u, float *v, float *pr, float *cm, hipStream_t s)
