@@ -145,6 +145,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
         tabs[5][l] = CS4[0];
         tabs[6][l] = CS4[1];
     }
+    // the output pointers are read from LDS where the boxes are analyzed:
+    // held in scalar registers through the loop they spill to VGPR lanes and
+    // cost a v_readlane per use
+    __shared__ CorrOut sout[1];
+    if (tid == 0) sout[0] = out;
     __syncthreads();
     const f16v zero = {};
     const float neg1 = opaque_neg1();
@@ -311,8 +316,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
         const int jb = ww.base + (g + min(slot, ng - 1)) * ww.stride;
         const int4 jj = jobs4[jb];
         float sg = NAN;
-        analyze_box<M, K, kNB, LO, 16, 1, false>(cfg, out, rbox[wave][slot], l & 15, l, nullptr, nullptr, slot < ng,
-                                                  jb, jj.x, jj.y, sg, sg);
+        int zo;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+        analyze_box<M, K, kNB, LO, 16, 1, false>(cfg, sout[zo], rbox[wave][slot], l & 15, l, nullptr, nullptr,
+                                                  slot < ng, jb, jj.x, jj.y, sg, sg);
         wave_sync();
     }
 }
@@ -431,6 +438,8 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
         build_table64(q, l, tg, t);
         tabs[q][l] = t;
     }
+    __shared__ CorrOut sout[1];  // output pointers, read where used (as in k_corr_mfma32)
+    if (tid == 0) sout[0] = out;
     __syncthreads();
     const f16v zero = {};
     const float neg1 = opaque_neg1();
@@ -602,7 +611,9 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
         const int jb = ww.base + (g + min(slot, ng - 1)) * ww.stride;
         const int4 jj = jobs4[jb];
         float sg = NAN;
-        analyze_box<M, K, kNB64, LO, 32, 1, false>(cfg, out, rbox[wave][slot], l & 31, l, nullptr, nullptr,
+        int zo;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+        analyze_box<M, K, kNB64, LO, 32, 1, false>(cfg, sout[zo], rbox[wave][slot], l & 31, l, nullptr, nullptr,
                                                     slot < ng, jb, jj.x, jj.y, sg, sg);
         wave_sync();
     }

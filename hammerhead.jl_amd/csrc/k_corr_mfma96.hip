@@ -203,6 +203,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         build_table96(q, l, tg, t);
         tabs[q][l] = t;
     }
+    // the output pointers, read from LDS where the box is analyzed: held in
+    // scalar registers through the loop they spill to VGPR lanes
+    __shared__ CorrOut sout[1];
+    if (tid == 0) sout[0] = out;
     __syncthreads();
     const f16v zero = {};
     const float neg1 = opaque_neg1();
@@ -395,8 +399,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         }
         wave_sync();
         float sg = NAN;
-        analyze_box<M, K, kNB96, LO, 64, 1, false>(cfg, out, Rb, l, l, nullptr, nullptr, true, jb, jw.x, jw.y, sg,
-                                                   sg);
+        int zo;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+        analyze_box<M, K, kNB96, LO, 64, 1, false>(cfg, sout[zo], Rb, l, l, nullptr, nullptr, true, jb, jw.x, jw.y,
+                                                   sg, sg);
         wave_sync();
     }
 }
