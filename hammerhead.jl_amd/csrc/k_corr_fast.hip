@@ -645,32 +645,36 @@ __device__ void uq_fused16(const float2 (&x)[16], int r, float &sig_u, float &si
 // (hi*hi + hi*lo + lo*hi, FP32 accumulation, ~22-bit operands), after a
 // power-of-two scale that puts max |e| of the (window, component) in
 // [1/2, 1) and is removed exactly afterwards. The diagonal sums over r are
-// FP64, as the cross-row sums of uq_lags_lds.
+// FP32 row by row (each entry is already an FP32 dot product) and FP64 over
+// the lag.
 // One wave per block, 4 windows per wave. The fields are formed as in
-// uq_fused16 (16 lanes per window, one window row per lane; C0, C+, C- the
-// same FP64 sums of FP32 products) and go to LDS as FP32, centred by the
-// FP64 window mean rounded to FP32. Each (window, component) then takes its
-// 15 products; the entries of the nine diagonals |r' - r| <= 4 go to LDS
-// diagonal by diagonal, so that lane k < 41 reads the 16 entries of lag k
-// with four 16-byte loads and sums them in FP64; the lanes (window,
-// component) finish the rings and sigma.
+// uq_fused16 (16 lanes per window, one window row per lane; C0, C+, C- sums
+// of FP32 products, FP32 within a row and FP64 across rows) and go to LDS as
+// FP32, centred by the FP64 window mean rounded to FP32: first the u fields
+// of the four windows, then, in the same place, their v fields. Each
+// (window, component) takes its 15 products; the entries of the nine
+// diagonals |r' - r| <= 4 (r' >= r for the symmetric G_0) go to LDS diagonal by diagonal, so that lane
+// k < 41 reads the 16 entries of lag k with four 16-byte loads; the lanes
+// (window, component) finish the rings and sigma.
 // LDS strides are chosen so that the field writes, the operand reads and the
-// diagonal stores and loads spread over the banks. The kernel is bound by
-// latency at its occupancy (14 KB of LDS and 153 VGPRs per wave), not by the
-// VALU or the matrix pipe: FP32 products on v_mfma_f32_16x16x4_f32 (the same
-// precision as the VALU row partials) ran no faster than the VALU, since
-// they do not co-execute with VALU work and G is twice the band of lags.
+// diagonal stores and loads spread over the banks. One component's fields at
+// a time keep the LDS at 9.2 KB per wave, so four waves fit a SIMD. FP32
+// products on v_mfma_f32_16x16x4_f32 (the same precision as the VALU row
+// partials) ran no faster than the VALU, since they do not co-execute with
+// VALU work and G is twice the band of lags.
 constexpr int kUE = 20;              // e row pitch: k = 0..14, zero pads 15..19
-constexpr int kUC = 320;             // component stride of the fields (16 * kUE)
-constexpr int kUW = 656;             // window stride of the fields
+constexpr int kUW = 320;             // window stride of the fields (16 * kUE)
 constexpr int kUD = 20;              // diagonal pitch (16 entries r = 0..15)
-constexpr int kUDK = 180;            // per dk: diagonals dr + 4 = 0..8
+constexpr int kUDK = 180;            // per dk >= 1: diagonals dr + 4 = 0..8
+constexpr int kUD0 = 5 * kUD;        // dk = 0: diagonals dr = 0..4 (G_0 is symmetric)
 constexpr int kUL = 41;              // lags: S00 and the 40 half-plane offsets
 
 // Lag k: k = 0..4 -> (dr, dk) = (k, 0); k = 5..40 -> dk = 1 + (k - 5) / 9,
 // dr = (k - 5) % 9 - 4. Ring max(|dr|, dk) (0 for S00).
 __host__ __device__ constexpr int uqm_dk(int k) { return k < 5 ? 0 : 1 + (k - 5) / 9; }
 __host__ __device__ constexpr int uqm_dr(int k) { return k < 5 ? k : (k - 5) % 9 - 4; }
+// offset of diagonal (dk, dr) in the diagonal buffer
+__host__ __device__ constexpr int uqm_diag(int dk, int dr) { return dk == 0 ? dr * kUD : kUD0 + (dk - 1) * kUDK + (dr + 4) * kUD; }
 __host__ __device__ constexpr int uqm_ring(int k)
 {
     const int a = uqm_dr(k) < 0 ? -uqm_dr(k) : uqm_dr(k);
@@ -732,33 +736,37 @@ __device__ __forceinline__ int uqm_scale_exp(float amx)
 
 // The u and v fields of uq_component! (src/uncertainty.jl:104-151) for the
 // matrix-core kernel: C0, C+, C- as in uq_fused16; e = the smoothed dC values
-// minus the FP64 window mean rounded to FP32.
+// minus the FP64 window mean rounded to FP32. The reference adds each
+// FP32 product (and each smoothed value of the mean) to a Float64 sum; here a
+// lane first adds its row's 15 or 16 terms in FP32 (relative error ~1e-6 of
+// the row's terms) and the rows are summed in FP64, one conversion per row
+// instead of one per term.
 __device__ __forceinline__ void uqm_field_u(const float2 (&x)[16], double (&C)[3], float (&e)[15])
 {
 #pragma clang fp contract(off)
     const float q = 0.25f;
     float dC[15];
-    double c0 = 0.0, cp = 0.0, cm = 0.0;
+    float c0 = 0.f, cp = 0.f, cm = 0.f;
 #pragma unroll
     for (int c = 0; c < 15; ++c) {
         const float a0 = x[c].x, a1 = x[c + 1].x, b0 = x[c].y, b1 = x[c + 1].y;
         const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
-        c0 += (double)p00;
-        cp += (double)p01;
-        cm += (double)p10;
+        c0 += p00;
+        cp += p01;
+        cm += p10;
         dC[c] = p01 - p10;
     }
-    double mu = 0.0;
+    float mu = 0.f;
 #pragma unroll
     for (int c = 0; c < 15; ++c) {
         const int cl = c > 0 ? c - 1 : 0, cr = c < 14 ? c + 1 : 14;
         e[c] = q * ((dC[cl] + 2.f * dC[c]) + dC[cr]);
-        mu += (double)e[c];
+        mu += e[c];
     }
-    C[0] = row_sum_d(c0);
-    C[1] = row_sum_d(cp);
-    C[2] = row_sum_d(cm);
-    const float mf = (float)(row_sum_d(mu) / 240.0);
+    C[0] = row_sum_d((double)c0);
+    C[1] = row_sum_d((double)cp);
+    C[2] = row_sum_d((double)cm);
+    const float mf = (float)(row_sum_d((double)mu) / 240.0);
 #pragma unroll
     for (int c = 0; c < 15; ++c) e[c] -= mf;
 }
@@ -770,32 +778,30 @@ __device__ __forceinline__ void uqm_field_v(const float2 (&x)[16], int r, double
     const float q = 0.25f;
     const bool live = r < 15;
     float dC[16];
-    double c0 = 0.0, cp = 0.0, cm = 0.0;
+    float c0 = 0.f, cp = 0.f, cm = 0.f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const float a0 = x[j].x, b0 = x[j].y;
         const float a1 = rowsh<1>(x[j].x), b1 = rowsh<1>(x[j].y);
         const float p00 = a0 * b0, p01 = a0 * b1, p10 = a1 * b0;
-        if (live) {
-            c0 += (double)p00;
-            cp += (double)p01;
-            cm += (double)p10;
-        }
+        c0 += p00;
+        cp += p01;
+        cm += p10;
         dC[j] = live ? p01 - p10 : 0.f;
     }
-    double mu = 0.0;
+    float mu = 0.f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const float up = rowsh<-1>(dC[j]), dn = rowsh<1>(dC[j]);
         const float lft = r > 0 ? up : dC[j];
         const float rgt = r < 14 ? dn : dC[j];
         e[j] = live ? q * ((lft + 2.f * dC[j]) + rgt) : 0.f;
-        mu += (double)e[j];
+        mu += e[j];
     }
-    C[0] = row_sum_d(c0);
-    C[1] = row_sum_d(cp);
-    C[2] = row_sum_d(cm);
-    const float mf = (float)(row_sum_d(mu) / 240.0);
+    C[0] = row_sum_d(live ? (double)c0 : 0.0);
+    C[1] = row_sum_d(live ? (double)cp : 0.0);
+    C[2] = row_sum_d(live ? (double)cm : 0.0);
+    const float mf = (float)(row_sum_d((double)mu) / 240.0);
 #pragma unroll
     for (int j = 0; j < 16; ++j) e[j] = live ? e[j] - mf : 0.f;
 }
@@ -806,19 +812,17 @@ __global__ __launch_bounds__(64) void k_uq16_mfma(CorrConfig cfg, const float *_
                                                   CorrOut out)
 {
 #pragma clang fp contract(off)
-    // fields: (window w, component c) e(r, k) at w * kUW + c * kUC + r * kUE + k;
-    // once the products of (w, c) are formed its lag totals (doubles) take
-    // the same place
+    // fields of one component: window w's e(r, k) at w * kUW + r * kUE + k
     __shared__ __align__(16) float ef[4 * kUW];
-    __shared__ __align__(16) float gd[5 * kUDK];  // G_dk(r, r + dr) at dk * kUDK + (dr + 4) * kUD + r
-    __shared__ double cst[4][2][3];               // C0, C+, C-
-    __shared__ int pex[4][2];                     // field scale exponents
-    static_assert(kUC >= 16 * kUE && kUW >= 2 * kUC && kUC % 2 == 0 && kUW % 2 == 0 && kUC / 2 >= kUL,
-                  "field layout");
+    // lag totals per (window, component), with the field scale 2^(2 pex)
+    __shared__ float tot[4][2][kUL];
+    __shared__ __align__(16) float gd[kUD0 + 4 * kUDK];  // G_dk(r, r + dr) at uqm_diag(dk, dr) + r
+    __shared__ int pex[4][2];                            // field scale exponents
+    static_assert(kUW >= 16 * kUE && kUW % 4 == 0, "field layout");
     const int tid = threadIdx.x, line = tid & 15, slot_w = tid >> 4;
     // diagonal entries past the ends of G (r + dr outside 0..15) are never
     // stored: zero once
-    for (int i = tid; i < 5 * kUDK; i += 64) gd[i] = 0.f;
+    for (int i = tid; i < kUD0 + 4 * kUDK; i += 64) gd[i] = 0.f;
     const int b = blockIdx.x;
     const int xcd = b & 7, slot = b >> 3;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
@@ -854,46 +858,6 @@ __global__ __launch_bounds__(64) void k_uq16_mfma(CorrConfig cfg, const float *_
         x[j] = make_float2(a * (ra[j] - meanA), a * (rb[j] - meanB));
     }
     float *ew = ef + slot_w * kUW;
-    {
-        double C[3];
-        float e[15];
-        uqm_field_u(x, C, e);
-        float amx = 0.f;
-#pragma unroll
-        for (int c = 0; c < 15; ++c) amx = fmaxf(amx, fabsf(e[c]));
-        const int p = uqm_scale_exp(amx);
-        float *er = ew + line * kUE;  // r = line, k = c
-#pragma unroll
-        for (int c = 0; c < 15; ++c) er[c] = ldexpf(e[c], p);
-#pragma unroll
-        for (int c = 15; c < kUE; ++c) er[c] = 0.f;
-        if (line == 0) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) cst[slot_w][0][i] = C[i];
-            pex[slot_w][0] = p;
-        }
-    }
-    {
-        double C[3];
-        float e[16];
-        uqm_field_v(x, line, C, e);
-        float amx = 0.f;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) amx = fmaxf(amx, fabsf(e[j]));
-        const int p = uqm_scale_exp(amx);
-        float *ec = ew + kUC;  // r = window column j, k = row i = line (line 15: zeros)
-#pragma unroll
-        for (int j = 0; j < 16; ++j) ec[j * kUE + line] = ldexpf(e[j], p);
-#pragma unroll
-        for (int k = 16; k < kUE; ++k) ec[line * kUE + k] = 0.f;
-        if (line == 0) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) cst[slot_w][1][i] = C[i];
-            pex[slot_w][1] = p;
-        }
-    }
-    lds_wave_sync();
-
     // operands of v_mfma_f32_16x16x16_f16: lane l gives A(i = l % 16, k =
     // 4 (l / 16) + 0..3) and B(k = 4 (l / 16) + 0..3, j = l % 16); it
     // receives D(4 (l / 16) + v, l % 16)
@@ -902,66 +866,104 @@ __global__ __launch_bounds__(64) void k_uq16_mfma(CorrConfig cfg, const float *_
     const float neg1 = -1.f;
     // lag of this lane for the diagonal sums (lanes 41-63 read lag 0's)
     const int lk = l < kUL ? l : 0;
-    const int gb = uqm_dk(lk) * kUDK + (uqm_dr(lk) + 4) * kUD;
+    const int gb = uqm_diag(uqm_dk(lk), uqm_dr(lk));
+    double Cf[3];  // C0, C+, C- of this lane's component in the finalize (u: line 0, v: line 1)
 #pragma unroll 1
-    for (int wc = 0; wc < 8; ++wc) {
-        float *e = ef + (wc >> 1) * kUW + (wc & 1) * kUC;
-        // k = 4 lq + 0..3 (A) and 4 lq + 4..7: two aligned 16-byte loads; the
-        // B operand of dk takes k + dk = 4 lq + dk .. 4 lq + dk + 3 from the
-        // two split quads (the field's zero pads end each row)
-        const f4v *er = reinterpret_cast<const f4v *>(e + eo);
-        const Split4 q0 = split4(er[0], neg1), q1 = split4(er[1], neg1);
-        f4v g[5];
+    for (int comp = 0; comp < 2; ++comp) {
+        double C[3];
+        int p;
+        if (comp == 0) {
+            float e[15];
+            uqm_field_u(x, C, e);
+            float amx = 0.f;
 #pragma unroll
-        for (int dk = 0; dk < 5; ++dk) {
-            Split4 bq;
+            for (int c = 0; c < 15; ++c) amx = fmaxf(amx, fabsf(e[c]));
+            p = uqm_scale_exp(amx);
+            float *er = ew + line * kUE;  // r = line, k = c
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                bq.hi[j] = j + dk < 4 ? q0.hi[j + dk] : q1.hi[j + dk - 4];
-                bq.lo[j] = j + dk < 4 ? q0.lo[j + dk] : q1.lo[j + dk - 4];
-            }
-            g[dk] = mfma3_16(q0, bq, (f4v){0.f, 0.f, 0.f, 0.f});
+            for (int c = 0; c < 15; ++c) er[c] = ldexpf(e[c], p);
+#pragma unroll
+            for (int c = 15; c < kUE; ++c) er[c] = 0.f;
+        } else {
+            // the u fields are consumed (the loop below ended with a sync)
+            float e[16];
+            uqm_field_v(x, line, C, e);
+            float amx = 0.f;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) amx = fmaxf(amx, fabsf(e[j]));
+            p = uqm_scale_exp(amx);
+            // r = window column j, k = row i = line (line 15: zeros)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) ew[j * kUE + line] = ldexpf(e[j], p);
+#pragma unroll
+            for (int k = 16; k < kUE; ++k) ew[line * kUE + k] = 0.f;
         }
-        // entries (r, r') = (4 lq + v, lj) with |r' - r| <= 4, by diagonal
+        if (line == comp) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = 4 * lq + v, dr = lj - r;
-            if (dr >= -4 && dr <= 4) {
-                float *p = gd + (dr + 4) * kUD + r;
-#pragma unroll
-                for (int dk = 0; dk < 5; ++dk) p[dk * kUDK] = g[dk][v];
-            }
+            for (int i = 0; i < 3; ++i) Cf[i] = C[i];
         }
+        if (line == 0) pex[slot_w][comp] = p;
         lds_wave_sync();
-        const f4v *dp = reinterpret_cast<const f4v *>(gd + gb);
-        double s4[4];
+#pragma unroll 1
+        for (int w = 0; w < 4; ++w) {
+            const float *e = ef + w * kUW;
+            // k = 4 lq + 0..3 (A) and 4 lq + 4..7: two aligned 16-byte loads;
+            // the B operand of dk takes k + dk = 4 lq + dk .. 4 lq + dk + 3
+            // from the two split quads (the field's zero pads end each row)
+            const f4v *er = reinterpret_cast<const f4v *>(e + eo);
+            const Split4 q0 = split4(er[0], neg1), q1 = split4(er[1], neg1);
+            f4v g[5];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const f4v q = dp[i];
-            s4[i] = ((double)q[0] + (double)q[1]) + ((double)q[2] + (double)q[3]);
+            for (int dk = 0; dk < 5; ++dk) {
+                Split4 bq;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    bq.hi[j] = j + dk < 4 ? q0.hi[j + dk] : q1.hi[j + dk - 4];
+                    bq.lo[j] = j + dk < 4 ? q0.lo[j + dk] : q1.lo[j + dk - 4];
+                }
+                g[dk] = mfma3_16(q0, bq, (f4v){0.f, 0.f, 0.f, 0.f});
+            }
+            // entries (r, r') = (4 lq + v, lj) with |r' - r| <= 4, by diagonal
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 4 * lq + v, dr = lj - r;
+                if (dr >= -4 && dr <= 4) {
+                    if (dr >= 0) gd[uqm_diag(0, dr) + r] = g[0][v];
+#pragma unroll
+                    for (int dk = 1; dk < 5; ++dk) gd[uqm_diag(dk, dr) + r] = g[dk][v];
+                }
+            }
+            lds_wave_sync();
+            // the 16 entries of the lag's diagonal, added pairwise in FP32
+            // (each entry is already an FP32 dot product; the error of the
+            // sum stays ~1e-7 of S00), then taken to FP64
+            const f4v *dp = reinterpret_cast<const f4v *>(gd + gb);
+            float s4[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const f4v q = dp[i];
+                s4[i] = (q[0] + q[1]) + (q[2] + q[3]);
+            }
+            if (l < kUL) tot[w][comp][l] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+            lds_wave_sync();
         }
-        // the lag total without the field scale (exact)
-        const double tot = ldexp((s4[0] + s4[1]) + (s4[2] + s4[3]), -2 * pex[wc >> 1][wc & 1]);
-        lds_wave_sync();
-        // the field of (w, c) is consumed: its place takes the lag totals
-        if (l < kUL) reinterpret_cast<double *>(e)[l] = tot;
     }
-    lds_wave_sync();
     // lanes (window, component): rings and sigma (finalize_uncertainty,
     // src/uncertainty.jl:162-179); a NaN never replaces a ring maximum
     if (line < 2) {
         UQAcc acc;
         uq_init(acc);
-        const double *s = reinterpret_cast<const double *>(ew + line * kUC);
-        acc.C0 = cst[slot_w][line][0];
-        acc.Cp = cst[slot_w][line][1];
-        acc.Cm = cst[slot_w][line][2];
-        acc.S00 = s[0];
+        const float *s = tot[slot_w][line];
+        const int sc = -2 * pex[slot_w][line];  // the lag totals without the field scale (exact)
+        acc.C0 = Cf[0];
+        acc.Cp = Cf[1];
+        acc.Cm = Cf[2];
+        acc.S00 = ldexp((double)s[0], sc);
 #pragma unroll
         for (int k = 1; k < kUL; ++k) {
             const int q = uqm_ring(k) - 1;
-            const double t = s[k];
-            acc.rmax[q] = t > acc.rmax[q] ? t : acc.rmax[q];
+            const double t = ldexp((double)s[k], sc);
+            acc.rmax[q] = fmax(acc.rmax[q], t);  // a NaN t keeps the maximum
             acc.rsum[q] += t;
         }
         const float sg = uq_sigma(acc);
