@@ -199,9 +199,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
         jnext = jobs4[ww.base + min(k + 2, ww.n - 1) * ww.stride];
         // ---- 0. window: lane (image, row, 8 columns), mean, scale (the
         // apodization is in the tables) ----
-        float sum = 0.f;
+        float sum = px[0];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) sum += px[s];
+        for (int s = 1; s < 8; ++s) sum += px[s];
         const float mean = img_sum(sum) / 256.f;
         float d[8], amx = 0.f;
 #pragma unroll

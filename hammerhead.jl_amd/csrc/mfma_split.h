@@ -88,13 +88,17 @@ __device__ __forceinline__ float img_sum(float v)
     v += rowror_f<1>(v);
     return v + __shfl_xor(v, 32);
 }
+// The maximum of non-negative, non-NaN values (the callers pass max |x|):
+// their bit patterns order as the values, so an unsigned maximum with the
+// DPP operand fused (no NaN canonicalization of fmaxf) gives the same result.
 __device__ __forceinline__ float img_max(float v)
 {
-    v = fmaxf(v, rowror_f<8>(v));
-    v = fmaxf(v, rowror_f<4>(v));
-    v = fmaxf(v, rowror_f<2>(v));
-    v = fmaxf(v, rowror_f<1>(v));
-    return fmaxf(v, __shfl_xor(v, 32));
+    unsigned u = __float_as_uint(v);
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x128, 0xF, 0xF, true));  // row_ror:8
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x124, 0xF, 0xF, true));  // row_ror:4
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x122, 0xF, 0xF, true));  // row_ror:2
+    u = max(u, (unsigned)__builtin_amdgcn_mov_dpp((int)u, 0x121, 0xF, 0xF, true));  // row_ror:1
+    return __uint_as_float(max(u, (unsigned)__shfl_xor((int)u, 32)));
 }
 
 // LDS hand-off between the lanes of one wave (the waves of a block run
