@@ -660,15 +660,17 @@ void record(hh_ctx *c, bool dynamic)
     side.defer_join = true;
     bool uq_pending = false;
     bool have_pred = false, pred_self = false;
+    bool pred_fused = false;  // the previous pass's last replacement launch smoothed the predictor
     for (int k = 0; k < np; ++k) {
         PassPlan &P = *E.plan[k];
         const hh_pass &p = P.p;
         const bool force_replace = k < np - 1;
         if (k > 0) {
-            predictor(c, E.plan[k - 1]->all.grids, *E.plan[k - 1]);
+            if (!pred_fused) predictor(c, E.plan[k - 1]->all.grids, *E.plan[k - 1]);
             have_pred = true;
             pred_self = false;
         }
+        pred_fused = false;
         const int maxiter = p.max_iterations;
         const bool unc = p.uncertainty && !force_replace;
         const bool keep_measured = maxiter > 1 && !(force_replace || p.replace_outliers);
@@ -766,14 +768,18 @@ void record(hh_ctx *c, bool dynamic)
             va.sflag = E.sflag.p;
             va.count = T->count.p;
             va.mask = P.anyg ? P.gmask.p : nullptr;
-            // the next sweep's predictor comes out of the replacement launch
-            // (fixed sweep counts: no pair drops out in between)
-            const bool fuse_pred = it < maxiter && P.fixed && E.smoothing && !P.anyg;
+            // the next sweep's (or the next pass's) predictor comes out of
+            // the replacement launch (fixed sweep counts: no pair drops out
+            // in between, and the last sweep is known in advance)
+            const bool fuse_pred = (it < maxiter || !last) && P.fixed && E.smoothing && !P.anyg;
             va.smooth_u = fuse_pred ? E.pu.p : nullptr;
             va.smooth_v = fuse_pred ? E.pv.p : nullptr;
             bool smoothed = false;
             timed(c, "validate", P.cfg.w, [&] { smoothed = launch_validate(va, s); });
-            if (it == maxiter) break;
+            if (it == maxiter) {
+                pred_fused = smoothed;
+                break;
+            }
             if (!P.fixed) {
                 if (!dynamic) throw Error(HH_ESTATE, "internal: iterating pass without a fixed sweep count");
                 if (it > 1) {
