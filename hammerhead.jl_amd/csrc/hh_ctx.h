@@ -157,6 +157,7 @@ struct Engine {
     DBuf<float> warp;        // warp atlas: A then B
     DBuf<float> ptmp;        // prefilter scratch
     DBuf<float> u, v, pr, cm, altu, altv, uu, uv, u0, v0, su, sv, mu, mv, prevu, prevv, pu, pv, ch;
+    DBuf<float> zero;        // zeros for the first sweep's predictor (cleared when planned, never written)
     DBuf<uint8_t> out, sflag;
     // current plan
     std::vector<uint8_t> key;

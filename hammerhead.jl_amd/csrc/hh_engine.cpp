@@ -567,6 +567,8 @@ void plan(hh_ctx *c, const hh_pass *passes, int np, int smoothing)
     for (DBuf<float> *b : {&E.u, &E.v, &E.pr, &E.cm, &E.uu, &E.uv, &E.u0, &E.v0, &E.su, &E.sv, &E.mu, &E.mv,
                            &E.prevu, &E.prevv, &E.pu, &E.pv, &E.ch})
         b->ensure(std::max<size_t>(E.maxcells, 1));
+    E.zero.ensure(std::max<size_t>(E.maxcells, 1));
+    HH_HIP(hipMemsetAsync(E.zero.p, 0, sizeof(float) * E.zero.n, c->stream));
     E.altu.ensure(std::max<size_t>(E.maxalt, 1));
     E.altv.ensure(std::max<size_t>(E.maxalt, 1));
     E.out.ensure(std::max<size_t>(E.maxcells, 1));
@@ -705,7 +707,6 @@ void record(hh_ctx *c, bool dynamic)
                 srcA = wA;
                 srcB = wB;
             } else {
-                gfill(c, E.u0.p, E.v0.p, T->grids, 0.f);
                 srcA = L.A;
                 srcB = L.B;
             }
@@ -716,8 +717,10 @@ void record(hh_ctx *c, bool dynamic)
             out.cm = E.cm.p;
             out.alt_u = nalt > 0 ? E.altu.p : nullptr;
             out.alt_v = nalt > 0 ? E.altv.p : nullptr;
-            out.u0 = E.u0.p;
-            out.v0 = E.v0.p;
+            // the first sweep has no predictor: zeros (a buffer nothing
+            // writes, so no fill per run)
+            out.u0 = have_pred ? E.u0.p : E.zero.p;
+            out.v0 = have_pred ? E.v0.p : E.zero.p;
             out.ny = P.total;  // jobs carry (cell + 1, 1)
             out.nx = 1;
             // the uncertainty of an iterating pass is taken once on the final
