@@ -959,12 +959,19 @@ __global__ __launch_bounds__(64) void k_uq16_mfma(CorrConfig cfg, const float *_
         acc.Cp = Cf[1];
         acc.Cm = Cf[2];
         acc.S00 = ldexp((double)s[0], sc);
+        // the rings over the scaled totals, the scale removed once per ring
+        // (a power of two: the FP64 sums and maxima scale exactly)
 #pragma unroll
         for (int k = 1; k < kUL; ++k) {
             const int q = uqm_ring(k) - 1;
-            const double t = ldexp((double)s[k], sc);
+            const double t = (double)s[k];
             acc.rmax[q] = fmax(acc.rmax[q], t);  // a NaN t keeps the maximum
             acc.rsum[q] += t;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc.rmax[q] = ldexp(acc.rmax[q], sc);
+            acc.rsum[q] = ldexp(acc.rsum[q], sc);
         }
         const float sg = uq_sigma(acc);
         if (active) {
