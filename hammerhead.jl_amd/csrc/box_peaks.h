@@ -33,7 +33,13 @@ __device__ __forceinline__ void cand_best(float v, int l, float &bv, int &bl)
 {
     unsigned vb = l == INT_MAX ? 0u : __float_as_uint(v);
     auto red = [](unsigned x, auto op) {
-        if constexpr (G >= 16) {
+        if constexpr (G == 8) {
+            // within each half of a DPP row: i <-> 7 - i, then xor 2 and
+            // xor 1 inside the quads (a maximum or minimum in every lane)
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, true));  // row_half_mirror
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+            x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+        } else if constexpr (G >= 16) {
             x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, true));  // row_ror:8
             x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x124, 0xF, 0xF, true));  // row_ror:4
             x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x122, 0xF, 0xF, true));  // row_ror:2
@@ -179,6 +185,54 @@ __device__ __forceinline__ float moment_row(const V &R, float pr, float pc, int 
     return (float)sqrt((sdx + sdy) / sumC);
 }
 
+template <int S>
+__device__ __forceinline__ double row_shl_d(double v)  // lane i gets lane i + S of its DPP row (0 past it)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x100 + S, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x100 + S, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// moment_row for windows of 8 lanes (two per DPP row): lane t takes point t
+// and lane 0 also point 8, which it adds first; the pairs (xor 4, 2, 1) then
+// give lane 0 the sums in the same order as moment_row's rotations over 16
+// lanes (whose lanes 9-15 add exact zeros), so the moment is bit-identical.
+// The result is valid on lane 0 of the window.
+template <int M, typename V>
+__device__ __forceinline__ float moment_row8(const V &R, float pr, float pc, int t)
+{
+    const bool fin = isfinite(pr) && isfinite(pc);
+    const int r0 = fin ? (int)rintf(pr) : 1, c0 = fin ? (int)rintf(pc) : 1;
+    const int rlo = max(1, r0 - 1), rhi = min(M, r0 + 1);
+    const int clo = max(1, c0 - 1), chi = min(M, c0 + 1);
+    double w = 0.0, tx = 0.0, ty = 0.0;
+    auto point = [&](int q) {  // point q of the 3 x 3 neighbourhood, added in
+        const int c = c0 - 1 + q / 3, r = r0 - 1 + q % 3;
+        if (fin && c >= clo && c <= chi && r >= rlo && r <= rhi) {
+            double wq = (double)R(r - 1, c - 1);
+            wq = wq > 0.0 ? wq : 0.0;
+            const float dx = (float)c - pc, dy = (float)r - pr;
+            w += wq;
+            tx += (double)(dx * dx) * wq;
+            ty += (double)(dy * dy) * wq;
+        }
+    };
+    point(t);
+    if (t == 0) point(8);
+    w += row_shl_d<4>(w);
+    tx += row_shl_d<4>(tx);
+    ty += row_shl_d<4>(ty);
+    w += row_shl_d<2>(w);
+    tx += row_shl_d<2>(tx);
+    ty += row_shl_d<2>(ty);
+    w += row_shl_d<1>(w);
+    tx += row_shl_d<1>(tx);
+    ty += row_shl_d<1>(ty);
+    if (!fin || rlo > rhi || clo > chi || w < 1e-9) return NAN;
+    return (float)sqrt((tx + ty) / w);
+}
+
 // Regional-max top-K over the box Rb (NB x NB column-major, plane rows and
 // columns LO .. LO + NB - 1, fftshifted) by the G lanes t = 0 .. G - 1 of
 // one window (NWAVE waves per window, exchanging through mv/ml), then the
@@ -265,6 +319,54 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             const int bi = NB > 32 ? __builtin_ctzll((unsigned long long)okm) : __builtin_ctz((unsigned)okm);
             okm &= okm - 1u;
             insert(Rb[bi + t * NB], (LO + bi) + (LO + t) * M);
+        }
+    } else if constexpr (G == 8 && NB <= 15) {
+        // lane t walks box columns 2t and 2t + 1 down the rows: the left
+        // column of 2t is lane t - 1's second column, the right column of
+        // 2t + 1 lane t + 1's first (DPP row shifts). Two windows share a
+        // 16-lane row; a shift across the boundary reads column 15 of the
+        // window on the left or column 0 of the one on the right only for
+        // lanes whose own column there is past the box (NB <= 15): column
+        // 15 is zero, and lane 7's first column (14) takes its right column
+        // from its own second column. Marked maxima are inserted column by
+        // column, lowest row first (increasing location).
+        auto left = [](float v) {
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xF, 0xF, true));
+        };
+        auto right = [](float v) {
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x101, 0xF, 0xF, true));
+        };
+        const int ca = 2 * t, cb = 2 * t + 1;
+        const bool va = ca < NB, vb = cb < NB;
+        unsigned oka = 0, okb = 0;
+        float pa = 0.f, pb = 0.f;
+        float a0 = va ? Rb[ca * NB] : 0.f, b0 = vb ? Rb[cb * NB] : 0.f;
+        for (int bi = 0; bi < NB; ++bi) {
+            const float an = (va && bi + 1 < NB) ? Rb[bi + 1 + ca * NB] : 0.f;
+            const float bn = (vb && bi + 1 < NB) ? Rb[bi + 1 + cb * NB] : 0.f;
+            const float Va = fmaxf(fmaxf(pa, a0), an), Vb = fmaxf(fmaxf(pb, b0), bn);
+            // column 2t: left = lane t - 1's Vb, right = own Vb
+            const float mea = fmaxf(left(Vb), pa), mla = fmaxf(Vb, an);
+            // column 2t + 1: left = own Va, right = lane t + 1's Va
+            const float meb = fmaxf(Va, pb), mlb = fmaxf(right(Va), bn);
+            const bool oa = a0 > 0.f && !(mea > a0) && !(mla >= a0);
+            const bool ob = b0 > 0.f && !(meb > b0) && !(mlb >= b0);
+            oka |= (oa ? 1u : 0u) << bi;
+            okb |= (ob ? 1u : 0u) << bi;
+            pa = a0;
+            a0 = an;
+            pb = b0;
+            b0 = bn;
+        }
+        while (oka) {
+            const int bi = __builtin_ctz(oka);
+            oka &= oka - 1u;
+            insert(Rb[bi + ca * NB], (LO + bi) + (LO + ca) * M);
+        }
+        while (okb) {
+            const int bi = __builtin_ctz(okb);
+            okb &= okb - 1u;
+            insert(Rb[bi + cb * NB], (LO + bi) + (LO + cb) * M);
         }
     } else {
         for (int p = t; p < NB * NB; p += G) {
@@ -365,7 +467,10 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
         }
     }
     float mom_row = NAN;
-    if constexpr (G >= 16) mom_row = moment_row<M>(R, fits[0].x, fits[0].y, t);
+    if constexpr (G >= 16)
+        mom_row = moment_row<M>(R, fits[0].x, fits[0].y, t);
+    else if constexpr (G == 8)
+        mom_row = moment_row8<M>(R, fits[0].x, fits[0].y, t);
     if (t != 0 || !active) return;
     const size_t g = (size_t)(gi - 1) + (size_t)(gj - 1) * out.ny;
     const size_t ng = (size_t)out.ny * out.nx;
@@ -395,7 +500,7 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
     const float v1 = sv[0];
     const float ratio = found >= 2 ? v1 / sv[1] : (v1 > 0.f ? INFINITY : NAN);
     float mom;
-    if constexpr (G >= 16)
+    if constexpr (G >= 8)
         mom = mom_row;
     else
         mom = moment_b<M>(R, ref.x, ref.y);

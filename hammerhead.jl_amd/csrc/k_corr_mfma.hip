@@ -82,7 +82,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
 {
 #pragma clang fp contract(off)
     constexpr int M = 32, H = 16, LO = H - kHW;
-    __shared__ float rbox[4][4][kNB * kNB + 1];  // [wave][window of the group]
+    // windows analyzed together: 8 (8 lanes each) at up to three waves per
+    // SIMD, 4 (16 lanes each) in the four-wave build, whose LDS must fit
+    // four blocks per CU; the peak search's fixed costs per group (the
+    // candidate merges, fits, moment and output writes) are shared by more
+    // windows in the first
+    constexpr int GW = W4 ? 4 : 8, GL = 64 / GW;
+    __shared__ float rbox[4][GW][kNB * kNB + 1];  // [wave][window of the group]
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int h = l >> 5, n = l & 31;
 
@@ -176,10 +182,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
 #pragma unroll
         for (int s = 0; s < 8; ++s) pnext[s] = p[(size_t)s * nr];
     }
-    // windows in groups of four: the boxes of a group are analyzed together,
-    // 16 lanes per window
-    for (int g = 0; g < ww.n; g += 4) {
-        const int ng = min(4, ww.n - g);
+    // windows in groups of GW: the boxes of a group are analyzed together,
+    // GL lanes per window
+    for (int g = 0; g < ww.n; g += GW) {
+        const int ng = min(GW, ww.n - g);
         for (int j = 0; j < ng; ++j) {
         const int k = g + j;
         float *Rb = rbox[wave][j];
@@ -312,13 +318,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
         }
         }
         wave_sync();
-        const int slot = l >> 4;
+        const int slot = l / GL;
         const int jb = ww.base + (g + min(slot, ng - 1)) * ww.stride;
         const int4 jj = jobs4[jb];
         float sg = NAN;
         int zo;
         asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
-        analyze_box<M, K, kNB, LO, 16, 1, false>(cfg, sout[zo], rbox[wave][slot], l & 15, l, nullptr, nullptr,
+        analyze_box<M, K, kNB, LO, GL, 1, false>(cfg, sout[zo], rbox[wave][slot], l % GL, l, nullptr, nullptr,
                                                   slot < ng, jb, jj.x, jj.y, sg, sg);
         wave_sync();
     }
