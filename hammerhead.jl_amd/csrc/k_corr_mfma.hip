@@ -74,21 +74,21 @@ constexpr int kS3P = 12;  // stage-3 operand scale (phase spectrum)
 constexpr int kS4 = -4;   // stage-3 table scale
 
 // W4: registers capped at 128 per lane for four resident waves per SIMD
-// (a few bytes spill), else the natural 168 (three waves)
+// (a few bytes spill) in blocks of 8 waves (two blocks per CU share their
+// LDS tables), else the natural 168 (three waves) in blocks of 4
 template <int K, bool PHASE, bool W4>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1))) void k_corr_mfma32(
+__global__ __launch_bounds__(W4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1))) void k_corr_mfma32(
     CorrConfig cfg, Trig tg, const float *__restrict__ A, const float *__restrict__ B, int nr,
     const int32_t *__restrict__ jobs, int njobs, CorrOut out)
 {
 #pragma clang fp contract(off)
     constexpr int M = 32, H = 16, LO = H - kHW;
-    // windows analyzed together: 8 (8 lanes each) at up to three waves per
-    // SIMD, 4 (16 lanes each) in the four-wave build, whose LDS must fit
-    // four blocks per CU; the peak search's fixed costs per group (the
-    // candidate merges, fits, moment and output writes) are shared by more
-    // windows in the first
-    constexpr int GW = W4 ? 4 : 8, GL = 64 / GW;
-    __shared__ float rbox[4][GW][kNB * kNB + 1];  // [wave][window of the group]
+    // windows analyzed together: 8, 8 lanes each (16 lanes each would share
+    // the peak search's fixed costs per group, the candidate merges, fits,
+    // moment and output writes, among half as many windows)
+    constexpr int WPB = W4 ? 8 : 4;  // waves per block
+    constexpr int GW = 8, GL = 64 / GW;
+    __shared__ float rbox[WPB][GW][kNB * kNB + 1];  // [wave][window of the group]
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int h = l >> 5, n = l & 31;
 
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W4 ? 4 : 1)
         gain8[v] = (n < kNB && bc < kNB) ? cfg.gain[(LO + n) + (LO + bc) * M] : 0.f;
     }
 
-    const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, 4, wave, njobs);
+    const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, WPB, wave, njobs);
     if (ww.n <= 0) return;
     // software pipeline: the pixels of window k + 1 and the job of k + 2 are
     // in flight while window k is processed
@@ -705,10 +705,11 @@ bool launch_correlate_mfma(const CorrConfig &cfg, const float *A, const float *B
         return t;
     }();
     const int wps = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 3;
+    const int wpb = wps >= 4 ? 8 : 4;  // waves per block of the two builds
     const int waves = std::max(1, std::min((njobs + 3) / 4 * 4, ncu * 4 * wps));
-    const int nblocks = (waves + 3) / 4;
+    const int nblocks = (waves + wpb - 1) / wpb;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, s, cfg, tg, A, B, nr, jobs, njobs, out);
+        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * wpb), 0, s, cfg, tg, A, B, nr, jobs, njobs, out);
     };
     const bool ph = cfg.method == HH_PHASE, k2 = cfg.K == 2;
     if (wps >= 4) {
