@@ -320,16 +320,20 @@ __device__ __forceinline__ void analyze_box(const CorrConfig &cfg, const CorrOut
             okm &= okm - 1u;
             insert(Rb[bi + t * NB], (LO + bi) + (LO + t) * M);
         }
-    } else if constexpr (G == 8 && NB <= 15) {
+    } else if constexpr ((G == 8 && NB <= 15) || (G == 16 && NB <= 31)) {
         // lane t walks box columns 2t and 2t + 1 down the rows: the left
         // column of 2t is lane t - 1's second column, the right column of
-        // 2t + 1 lane t + 1's first (DPP row shifts). Two windows share a
-        // 16-lane row; a shift across the boundary reads column 15 of the
+        // 2t + 1 lane t + 1's first (DPP row shifts). With 16 lanes a window
+        // is one DPP row (shifts past its ends read 0). With 8, two windows
+        // share a row; a shift across the boundary reads column 15 of the
         // window on the left or column 0 of the one on the right only for
         // lanes whose own column there is past the box (NB <= 15): column
         // 15 is zero, and lane 7's first column (14) takes its right column
         // from its own second column. Marked maxima are inserted column by
-        // column, lowest row first (increasing location).
+        // column, lowest row first (increasing location); a lane's top K of
+        // its two columns hold every candidate of the window's top K that
+        // lies there, so the merge gives the same result as one column per
+        // lane.
         auto left = [](float v) {
             return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xF, 0xF, true));
         };

@@ -348,8 +348,8 @@ __global__ __launch_bounds__(W4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(
 //      half): the tables as the A operand and the stage-3 results as B, so
 //      R accumulates whole in one tile.
 // Constant operand tables (28 split 32 x 16 operands, 57 KB) live in LDS,
-// one copy per block; boxes of 2 windows are analyzed together, 32 lanes
-// per window.
+// one copy per block; boxes of 4 windows are analyzed together, 16 lanes
+// (two box columns each) per window.
 struct Trig64 {
     float c[64], s[64];  // cos, sin of 2 pi t / 64
     // the separable Gaussian apodization of a 32-px window (exp(-((i - 16.5)^2
@@ -436,7 +436,8 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
 #pragma clang fp contract(off)
     constexpr int M = 64, H = 32, LO = H - kHW64;
     __shared__ Split tabs[T64_N][64];
-    __shared__ float rbox[WV][2][kNB64 * kNB64 + 1];
+    // boxes of 4 windows analyzed together, 16 lanes (two box columns each) per window
+    __shared__ float rbox[WV][4][kNB64 * kNB64 + 1];
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int h = l >> 5, n = l & 31;
     for (int q = wave; q < T64_N; q += WV) {
@@ -453,8 +454,8 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
     const WaveWindows ww = wave_windows(cfg.linear_order, gridDim.x, blockIdx.x, WV, wave, njobs);
     if (ww.n <= 0) return;
     const int4 *jobs4 = reinterpret_cast<const int4 *>(jobs);
-    for (int g = 0; g < ww.n; g += 2) {
-        const int ng = min(2, ww.n - g);
+    for (int g = 0; g < ww.n; g += 4) {
+        const int ng = min(4, ww.n - g);
         for (int j = 0; j < ng; ++j) {
             const int4 jw = jobs4[ww.base + (g + j) * ww.stride];
             float *Rb = rbox[wave][j];
@@ -613,13 +614,13 @@ __global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(WV / 4,
             }
         }
         wave_sync();
-        const int slot = l >> 5;
+        const int slot = l >> 4;
         const int jb = ww.base + (g + min(slot, ng - 1)) * ww.stride;
         const int4 jj = jobs4[jb];
         float sg = NAN;
         int zo;
         asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
-        analyze_box<M, K, kNB64, LO, 32, 1, false>(cfg, sout[zo], rbox[wave][slot], l & 31, l, nullptr, nullptr,
+        analyze_box<M, K, kNB64, LO, 16, 1, false>(cfg, sout[zo], rbox[wave][slot], l & 15, l, nullptr, nullptr,
                                                     slot < ng, jb, jj.x, jj.y, sg, sg);
         wave_sync();
     }
